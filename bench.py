@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Throughput of the particle half of the PUNC PIC step on one MI355X.
+
+Workload (default): the large single-instance configuration - box [-20,20]^3
+of 64^3 cubes x 6 tetrahedra with the r=2 sphere object removed (1,572,864
+tets before removal), 64M particles (32M electrons q=-1 m=1 vth=1, 32M ions
+q=+1 m=1836 vth=1/sqrt(1836), both drifting (0,0,0.5)), Boris push with
+B=(0,0,1), dt=0.02, Maxwellian flux injection at the exterior boundary, CG1
+deposit each step, cell re-sort every 20 steps.  The field is a frozen
+synthetic CG1 field resident on the GPU (the host field solve is outside this
+path).  One step = push + move + relocation walk with absorption + removal +
+injection + deposit, all on the device.
+
+Prints one JSON line (metric, value in particle-steps/s, roofline of the fused
+particle kernel, CPU baseline of the oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s & %HBM peak, 1 MI355X; batched sweep vs plain loop"
+HBM_PEAK = 8.0e12
+
+CONFIGS = {
+    # name: (n, L, r, particles per species, B, description)
+    "large": (64, 20.0, 2.0, 32 * 1024 * 1024,
+              "box [-20,20]^3, 64^3 cubes x 6 tets, r=2 sphere removed, 64M particles, Boris B=(0,0,1), dt=0.02"),
+    "probe": (32, 10.0, 1.0, 2 * 1024 * 1024,
+              "box [-10,10]^3, 32^3 cubes x 6 tets, r=1 sphere removed, 4M particles, Boris B=(0,0,1), dt=0.02"),
+    "small": (16, 10.0, 1.0, 256 * 1024,
+              "box [-10,10]^3, 16^3 cubes x 6 tets, r=1 sphere removed, 512K particles (smoke size)"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def synthetic_field(hm, seed=1):
+    rng = np.random.default_rng(seed)
+    xyz = np.zeros((hm.ndof, 3))
+    xyz[hm.vdof] = hm.xyz
+    r2 = np.maximum((xyz ** 2).sum(1), 1.0)
+    # a weak probe-like radial field plus noise
+    E = 0.05 * xyz / r2[:, None] ** 1.5 + 0.01 * rng.normal(size=(hm.ndof, 3))
+    return np.ascontiguousarray(E)
+
+
+def run_gpu(cfg, steps, warmup, sort_every, seed=0):
+    import puncpp_amd as P
+    from puncpp_amd import _native as N
+
+    n, L, r, nps, desc = CONFIGS[cfg]
+    t0 = time.time()
+    hm = P.box_mesh(n, [-L] * 3, [L] * 3, sphere_r=r)
+    mesh = P.Mesh(hm)
+    log(f"[bench] mesh {cfg}: {mesh.num_cells} cells, {mesh.ndof} dofs, {mesh.num_exterior} exterior facets "
+        f"({time.time() - t0:.1f}s)")
+    vth = [1.0, 1.0 / np.sqrt(1836.0)]
+    vd = [0.0, 0.0, 0.5]
+    qm = [(-1.0, 1.0), (1.0, 1836.0)]
+    density = nps / mesh.volume()
+    species = [P.Species(q, m, density, 1.0, P.ParticleAmountType.phys_per_sim, mesh,
+                         P.UniformPosition(mesh), P.Maxwellian(vth[s], vd))
+               for s, (q, m) in enumerate(qm)]
+    cap = int(2 * nps * 1.05) + 1_000_000
+    pop = P.Population(mesh, qm, capacity=cap)
+    for s in range(2):
+        N.check(N.lib().punc_gpu_pop_load(pop._h, s, nps, seed, 0.0, 0.0, 0, vth[s], N.dptr(np.array(vd))))
+    pop.sort()
+    P.punc._register_injectors(pop, species)
+    E = synthetic_field(hm)
+    N.check(N.lib().punc_gpu_set_field(pop._h, N.dptr(E)))
+    dv = P.element_volume(mesh)
+    N.check(N.lib().punc_gpu_distribute(pop._h, N.dptr(dv), None))
+    log(f"[bench] loaded {pop.size()} particles ({time.time() - t0:.1f}s)")
+    B = np.array([0.0, 0.0, 1.0])
+    dt = 0.02
+    lib = N.lib()
+
+    def step(k):
+        push_dt = (1.0 - 0.5 * (k == 0)) * dt
+        N.check(lib.punc_gpu_step(pop._h, 2, push_dt, N.dptr(B), dt, None, 1, seed, k))
+        if sort_every and (k + 1) % sort_every == 0:
+            N.check(lib.punc_gpu_sort(pop._h))
+
+    for k in range(warmup):
+        step(k)
+    N.check(lib.punc_gpu_synchronize(pop._h))
+    n_start = pop.size()
+    log(f"[bench] warmup done, {n_start} particles")
+    N.check(lib.punc_gpu_timing(pop._h, 1))
+    t1 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        step(k)
+    N.check(lib.punc_gpu_synchronize(pop._h))
+    t2 = time.perf_counter()
+    ms_tot = N.C.c_double()
+    nl = N.C.c_int64()
+    N.check(lib.punc_gpu_timing_read(pop._h, N.C.byref(ms_tot), N.C.byref(nl)))
+    n_end = pop.size()
+    ke = N.C.c_double()
+    ab = np.zeros((2, 64), dtype=np.int64)
+    kin = N.C.c_int64()
+    N.check(lib.punc_gpu_step_results(pop._h, N.C.byref(ke), N.i64ptr(ab), N.C.byref(kin)))
+    elapsed = t2 - t1
+    n_avg = 0.5 * (n_start + n_end)
+    value = n_avg * steps / elapsed
+    kern_ms = ms_tot.value / max(nl.value, 1)
+    b_launch = 104.0 * n_avg + 128.0 * mesh.num_cells + 32.0 * mesh.ndof
+    achieved = b_launch / (kern_ms * 1e-3)
+    log(f"[bench] {steps} steps in {elapsed:.3f}s, {value:.4g} particle-steps/s, fused kernel {kern_ms:.3f} ms "
+        f"({achieved / 1e9:.0f} GB/s), injected last step {kin.value}, absorbed {ab.sum()}")
+    return {
+        "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
+        "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
+        "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density,
+    }
+
+
+def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
+    """Oracle (serial, 1 core) on a bounded sample: the same mesh and species,
+    1M particles per species, `steps` full particle steps."""
+    import oracle as O
+
+    hm = res["hm"]
+    t0 = time.time()
+    o = O.OraclePop(hm, [-1.0, 1.0], [1.0, 1836.0])
+    for s in range(2):
+        o.load(s, sample_per_species, seed, 0.0, 0.0, 0, res["vth"][s], res["vd"])
+        o.set_injector(s, res["vth"][s], res["vd"], res["density"], 5.0)
+    E = synthetic_field(hm)
+    dv = o.element_volume()
+    B = np.array([0.0, 0.0, 1.0])
+    dt = 0.02
+    log(f"[bench] oracle set up ({time.time() - t0:.1f}s)")
+    t1 = time.perf_counter()
+    ps = 0
+    for k in range(steps):
+        ps += o.size()
+        o.boris(E, B, dt)
+        o.move(dt)
+        o.update()
+        o.inject(dt, seed, k)
+        o.distribute(dv)
+    t2 = time.perf_counter()
+    v = ps / (t2 - t1)
+    log(f"[bench] oracle {ps} particle-steps in {t2 - t1:.2f}s -> {v:.4g}/s")
+    return {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (serial C++ restatement, g++ -O2) on the same mesh with "
+                      f"{2 * sample_per_species} particles, {steps} steps of boris+move+update+inject+distribute; "
+                      f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cores visible"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="large", choices=sorted(CONFIGS))
+    ap.add_argument("--sort-every", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              f"sharding of this path is a design note, not a build target.", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args.config, args.steps, args.warmup, args.sort_every)
+    cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
+    out = {
+        "metric": METRIC,
+        "value": res["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {res['desc']}", "cells": res["cells"], "dofs": res["ndof"],
+                   "particles": int(res["n_avg"]), "sort_every": args.sort_every,
+                   "field": "frozen synthetic CG1 field on device"},
+        "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK, "traffic": None,
+                     "kernel": "k_particles<boris,move,locate,deposit>",
+                     "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * C ABI of the MI355X particle path for PUNC++ (libpunc_gpu.so).
+ *
+ * The reference (PUNC++) is a header-only C++ template library over DOLFIN
+ * (punc/include/punc/{population,pusher,distributor,injector}.h).  This ABI is
+ * what its particle operators bind to when they run on the GPU: plain
+ * pointers and sizes, host arrays in and out, device state behind opaque
+ * handles.  The C++ drop-in API in include/punc/ sits on top of it.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative value on error; the
+ *    message is available from punc_gpu_last_error() (thread-local).  Nothing
+ *    throws across the ABI.
+ *  - Vectors of points are row-major n x 3 doubles.
+ *  - Cells are tetrahedra with ascending vertex ids; local facet i is opposite
+ *    local vertex i.  A facet neighbour < 0 is -(boundary id): 1 = exterior,
+ *    2.. = objects (reference mesh.h:61-71).
+ *  - A population handle is bound to one HIP stream; calls on one handle are
+ *    not thread-safe, calls on different handles are.
+ */
+#ifndef PUNC_GPU_H
+#define PUNC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct punc_hostmesh_s punc_hostmesh_t;
+typedef struct punc_mesh_s punc_mesh_t;
+typedef struct punc_pop_s punc_pop_t;
+
+#define PUNC_GPU_MAX_BND 64
+
+/* ---- errors ------------------------------------------------------------ */
+/* Last error message of the calling thread ("" if none). */
+const char* punc_gpu_last_error(void);
+/* ABI version (increments on incompatible changes). */
+int punc_gpu_abi_version(void);
+
+/* ---- synthetic meshes (host) ------------------------------------------- */
+/* Box [lo,hi]^3 of n^3 cubes x 6 Kuhn tetrahedra; cells with centroid inside
+ * radius sphere_r (>0) are removed and form object boundary 2; exterior is 1.
+ * periodic != 0 maps opposite vertices to one CG1 dof.  Stands in for the
+ * reference's Mesh(fname) file loading (mesh.cpp:41-92), which needs DOLFIN. */
+int punc_gpu_meshgen_box(int n, const double lo[3], const double hi[3], double sphere_r,
+                         int periodic, punc_hostmesh_t** out);
+int punc_gpu_hostmesh_sizes(const punc_hostmesh_t* m, int64_t* nv, int64_t* nc, int64_t* ndof);
+int punc_gpu_hostmesh_copy(const punc_hostmesh_t* m, double* xyz, int32_t* cells,
+                           int32_t* facet_bnd, int32_t* vdof);
+void punc_gpu_hostmesh_free(punc_hostmesh_t* m);
+
+/* ---- mesh / localizer -------------------------------------------------- */
+/* Builds the localizer (facet neighbours, facet planes, barycentric matrices)
+ * and uploads the packed cell records.  Replaces Population<3>::Population and
+ * init_localizer (population.h:310-389) and Cell<3>::init_barycentric_matrix
+ * (population.cpp:181-240).  facet_bnd[c*4+i] gives the boundary id of local
+ * facet i when that facet has one adjacent cell.  vdof maps vertex -> CG1 dof. */
+int punc_gpu_mesh_create(int64_t nv, const double* xyz, int64_t nc, const int32_t* cells,
+                         const int32_t* facet_bnd, const int32_t* vdof, int64_t ndof,
+                         punc_mesh_t** out);
+void punc_gpu_mesh_destroy(punc_mesh_t* m);
+/* Sizes, number of exterior facets and the mesh volume (Mesh::volume, mesh.cpp:147-164). */
+int punc_gpu_mesh_info(const punc_mesh_t* m, int64_t* nv, int64_t* nc, int64_t* ndof,
+                       int64_t* n_exterior, double* volume);
+/* Host copies of the localizer: adj nc*4 (Cell::facet_adjacents), planes nc*16
+ * (Cell::facet_plane_coeffs), bary nc*12 (barycentric_matrix rows 0..2),
+ * vcoords nc*12 (Cell::vertex_coordinates), dofs nc*4.  Any pointer may be NULL. */
+int punc_gpu_mesh_localizer(const punc_mesh_t* m, int64_t* adj, double* planes, double* bary,
+                            double* vcoords, int32_t* dofs);
+/* element_volume(V, voronoi) (distributor.cpp:35-69): per dof (D+1)/patch volume
+ * (voronoi != 0) or 1/patch volume. */
+int punc_gpu_element_volume(const punc_mesh_t* m, int voronoi, double* dv_inv);
+/* Exterior facets (Mesh::exterior_boundaries, mesh.cpp:166-236): area, 9
+ * vertex coordinates, inward unit normal, 9-entry basis, and the owning cell. */
+int punc_gpu_exterior_facets(const punc_mesh_t* m, double* area, double* vertices,
+                             double* normal, double* basis, int32_t* cell);
+/* Point location (punc::locate, population.cpp:24-39): lowest-id cell whose
+ * four facet planes all give proj <= 0, else -1. */
+int punc_gpu_mesh_locate(const punc_mesh_t* m, int64_t n, const double* xs, int64_t* cells);
+
+/* Maxwellian flux tables for create_flux (injector.cpp:104-171 with
+ * Maxwellian::flux_num_particles / flux_max, distributions.cpp:52-72):
+ * num[j] = expected particles per unit density per unit time through exterior
+ * facet j, pdf_max[j] = maximum of the flux pdf for facet j's normal. */
+int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const double vd[3], double* num,
+                             double* pdf_max);
+
+/* ---- population -------------------------------------------------------- */
+/* A population of nspecies species with weighted charge q[s] and mass m[s]
+ * (Species::q, Species::m, population.cpp:61-64), room for capacity particles.
+ * stream: a hipStream_t, or NULL to let the library create one. */
+int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q, const double* mass,
+                        int64_t capacity, void* stream, punc_pop_t** out);
+void punc_gpu_pop_destroy(punc_pop_t* p);
+/* add_particles(xs, vs, q, m) (population.h:411-435): locates each point and
+ * appends it as species s; points outside the mesh are skipped (the reference
+ * indexes out of bounds there) and *n_added tells how many were kept. */
+int punc_gpu_pop_add(punc_pop_t* p, int species, int64_t n, const double* xs, const double* vs,
+                     int64_t* n_added);
+/* Appends particles with known species and cells (restart from a saved
+ * state); cells must be valid cell ids. */
+int punc_gpu_pop_add_located(punc_pop_t* p, int64_t n, const double* xs, const double* vs,
+                             const int32_t* species, const int64_t* cells);
+/* load_particles (injector.h:244-278) on the device for species s: N particles
+ * with positions by rejection against inside(x)*(1+amp*cos(kw*x[axis]))
+ * (amp = 0: UniformPosition) and Maxwellian velocities by inverse CDF. */
+int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp, double kw,
+                      int axis, double vth, const double vd[3]);
+int punc_gpu_pop_size(punc_pop_t* p, int64_t* n);
+/* Particles per species (num_of_particles/positives/negatives derive from it,
+ * population.h:701-744). */
+int punc_gpu_pop_counts(punc_pop_t* p, int64_t* per_species);
+/* Download all particles: xs, vs n*3, species n, cells n (any may be NULL). */
+int punc_gpu_pop_get(punc_pop_t* p, double* xs, double* vs, int32_t* species, int64_t* cells);
+
+/* ---- the particle operators -------------------------------------------- */
+/* Upload the CG1 electric field, ndof*3 doubles (E_x,E_y,E_z per dof). */
+int punc_gpu_set_field(punc_pop_t* p, const double* E);
+/* accel_cg1(pop, E, dt) (pusher.h:154-201); returns the mid-step KE. */
+int punc_gpu_accel(punc_pop_t* p, double dt, double* KE);
+/* boris_cg1(pop, E, B, dt) (pusher.h:282-372); returns 0.5*m*|v-|^2 summed. */
+int punc_gpu_boris(punc_pop_t* p, const double B[3], double dt, double* KE);
+/* move(pop, dt) (pusher.h:648-662). */
+int punc_gpu_move(punc_pop_t* p, double dt);
+/* move_periodic(pop, dt, Ld) (pusher.h:666-681). */
+int punc_gpu_move_periodic(punc_pop_t* p, double dt, const double Ld[3]);
+/* Population::update (population.h:641-699): relocation walk, removal of
+ * particles that left the mesh.  absorbed[s*PUNC_GPU_MAX_BND + b] receives the
+ * number of species-s particles that left through boundary b this call
+ * (may be NULL).  Object charge/current bookkeeping is done by the caller. */
+int punc_gpu_update(punc_pop_t* p, int64_t* absorbed);
+/* distribute_cg1(pop, rho, dv_inv) (distributor.h:168-210).  dv_inv (ndof) is
+ * uploaded when non-NULL and kept for later calls; rho (ndof) may be NULL to
+ * keep the result on the device. */
+int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* rho);
+/* kinetic_energy(pop) (diagnostics.h:248-265). */
+int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE);
+/* Re-sort the particles by cell (counting sort, order within a cell arbitrary). */
+int punc_gpu_sort(punc_pop_t* p);
+
+/* ---- injection --------------------------------------------------------- */
+/* Per species Maxwellian flux source: thermal speed, drift, density n
+ * (Species::n) and rejection box half width in units of vth (Pdf::vdf_range). */
+int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
+                          double density, double vrange);
+/* inject_particles(pop, species, facets, dt) (injector.h:157-232) with a
+ * counter-based stream keyed by (seed, step).  n_injected may be NULL. */
+int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected);
+
+/* ---- fused step -------------------------------------------------------- */
+/* One particle half-step as the reference time loop orders it
+ * (interaction.cpp:384-422): push (pusher 1 = accel_cg1, 2 = boris_cg1, with
+ * push_dt), move by dt (periodic when Ld != NULL), update, inject (when
+ * inject != 0), and the CG1 deposit of the resulting population for the next
+ * distribute_cg1, all on the device.  Results stay on the device until read
+ * by punc_gpu_step_results / punc_gpu_get_rho. */
+int punc_gpu_step(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                  const double* Ld, int inject, uint64_t seed, uint64_t step);
+/* KE of the last push, absorbed counts since the last read (nspecies*64), and
+ * particles injected by the last step.  Any pointer may be NULL. */
+int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t* n_injected);
+/* rho (ndof) from the last deposit. */
+int punc_gpu_get_rho(punc_pop_t* p, double* rho);
+
+/* ---- launch control, timing -------------------------------------------- */
+int punc_gpu_set_launch(punc_pop_t* p, int grid, int block);
+/* Record HIP events around every launch of the main particle kernel. */
+int punc_gpu_timing(punc_pop_t* p, int enable);
+int punc_gpu_timing_read(punc_pop_t* p, double* ms_total, int64_t* launches);
+int punc_gpu_synchronize(punc_pop_t* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUNC_GPU_H */

@@ -1,0 +1,10 @@
+"""MI355X-native particle half of the PUNC++ PIC step.
+
+libpunc_gpu.so (HIP kernels for gfx950 behind a C ABI, include/punc_gpu.h)
+holds the product; this package is its Python mirror of the reference's
+operator API (puncpp_amd.punc) and the build recipe (puncpp_amd.build).
+"""
+from .punc import (CosinePosition, ExteriorFacet, HostMesh, Maxwellian, Mesh, Object,  # noqa: F401
+                   ParticleAmountType, Population, Species, UniformPosition, accel_cg1, boris_cg1,
+                   box_mesh, create_flux, distribute_cg1, element_volume, inject_particles,
+                   kinetic_energy, load_particles, move, move_periodic)

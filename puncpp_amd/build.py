@@ -1,0 +1,56 @@
+"""Build recipe for libpunc_gpu.so (HIP kernels for gfx950 + host setup + C ABI).
+
+    python -m puncpp_amd.build        # or puncpp_amd.build.build()
+
+hipcc compiles the device code for gfx950 only.  -ffp-contract=off keeps every
+floating-point expression exactly as written (no FMA contraction), which is
+what makes positions, velocities and cell decisions bit-identical to the
+serial CPU path.  Packed-FP32 instruction selection is disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libpunc_gpu.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+          "-Wno-unused-function", "-Wno-unused-variable"]
+DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+SOURCES = ["kernels.hip", "capi.hip", "mesh_host.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(BUILD, src + ".o")
+        if src.endswith(".hip"):
+            cmd = [HIPCC] + DEVICE + COMMON + ["-c", path, "-o", obj]
+        else:
+            cmd = [HIPCC, "-x", "c++"] + COMMON + ["-c", path, "-o", obj]
+        _run(cmd, verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)

@@ -1,0 +1,1111 @@
+// C ABI of libpunc_gpu.so: handles, device memory, and the operator entry
+// points declared in include/punc_gpu.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/punc_gpu.h"
+#include "device.h"
+#include "kernels.h"
+#include "mesh_host.h"
+
+using namespace punc_gpu;
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(const std::string& msg) {
+    g_err = msg;
+    return -1;
+}
+static int hipfail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -2;
+}
+#define HIPCHK(call)                                  \
+    do {                                              \
+        hipError_t e__ = (call);                      \
+        if (e__ != hipSuccess) return hipfail(e__, #call); \
+    } while (0)
+
+extern "C" const char* punc_gpu_last_error(void) { return g_err.c_str(); }
+extern "C" int punc_gpu_abi_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// Handles
+// ---------------------------------------------------------------------------
+struct punc_hostmesh_s {
+    HostMesh m;
+};
+
+struct punc_mesh_s {
+    int device = 0;
+    HostMesh host;
+    MeshTopo topo;
+    CellRec* d_rec = nullptr;
+    CellPlanes* d_planes = nullptr;
+    int32_t* d_bstart = nullptr;
+    int32_t* d_bcells = nullptr;
+    // exterior facets for injection
+    double* d_fverts = nullptr;
+    double* d_fnormal = nullptr;
+    int32_t* d_fcell = nullptr;
+    DevMesh dm;
+};
+
+struct CoefEntry {
+    int kind;
+    double dt;
+    double B[3];
+    SpCoef* d;
+};
+
+struct punc_pop_s {
+    punc_mesh_s* mesh = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int nsp = 0;
+    std::vector<double> q, m;
+    int cell_bits = 0;
+    int64_t capacity = 0;
+    int grid = 0, block = 256;
+
+    DevState* d_state = nullptr;
+    void* bufs[2][7] = {};
+    int32_t* d_holes = nullptr;
+    int32_t* d_H = nullptr;
+    int32_t* d_S = nullptr;
+    unsigned long long* d_absorbed = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    unsigned long long* d_errs = nullptr;  // [0] load failures, [1] injection failures
+    double4* d_E = nullptr;
+    bool field_set = false;
+    double* d_rho_raw = nullptr;
+    double* d_rho = nullptr;
+    double* d_dv_inv = nullptr;
+    bool dv_set = false;
+    double* d_ke_part = nullptr;
+    double* d_ke = nullptr;
+    int ke_np = 0;
+    std::vector<CoefEntry> coef_cache;
+    // scan / sort scratch
+    int64_t* d_hist = nullptr;
+    int64_t* d_off = nullptr;
+    int64_t* d_bsum = nullptr;
+    int64_t* d_total = nullptr;  // [0] scan total, [1] injection candidates, [2] kept
+    int64_t bsum_len = 0;
+    // injection
+    std::vector<int> inj_on;
+    std::vector<SpInject> spi;
+    std::vector<double> flux_num, flux_max;  // nsp*nf (per unit density and time)
+    SpInject* d_spi = nullptr;
+    double* d_num = nullptr;
+    double* d_pmax = nullptr;
+    double inj_dt = -1.0;
+    int64_t cand_cap = 0;
+    int64_t* d_fcount = nullptr;
+    int64_t* d_foff = nullptr;
+    double* d_cx = nullptr;
+    int32_t* d_ccs = nullptr;
+    int64_t* d_keep = nullptr;
+    int64_t* d_pos = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+};
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic mesh
+// ---------------------------------------------------------------------------
+extern "C" int punc_gpu_meshgen_box(int n, const double lo[3], const double hi[3], double sphere_r,
+                                    int periodic, punc_hostmesh_t** out) {
+    if (!out || !lo || !hi) return fail("meshgen_box: null argument");
+    auto* h = new punc_hostmesh_s();
+    std::string err;
+    if (!meshgen_box(n, lo, hi, sphere_r, periodic, h->m, err)) {
+        delete h;
+        return fail(err);
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int punc_gpu_hostmesh_sizes(const punc_hostmesh_t* m, int64_t* nv, int64_t* nc,
+                                       int64_t* ndof) {
+    if (!m) return fail("hostmesh_sizes: null mesh");
+    if (nv) *nv = m->m.nv;
+    if (nc) *nc = m->m.nc;
+    if (ndof) *ndof = m->m.ndof;
+    return 0;
+}
+
+extern "C" int punc_gpu_hostmesh_copy(const punc_hostmesh_t* m, double* xyz, int32_t* cells,
+                                      int32_t* facet_bnd, int32_t* vdof) {
+    if (!m) return fail("hostmesh_copy: null mesh");
+    if (xyz) memcpy(xyz, m->m.xyz.data(), m->m.xyz.size() * sizeof(double));
+    if (cells) memcpy(cells, m->m.cells.data(), m->m.cells.size() * sizeof(int32_t));
+    if (facet_bnd) memcpy(facet_bnd, m->m.facet_bnd.data(), m->m.facet_bnd.size() * sizeof(int32_t));
+    if (vdof) memcpy(vdof, m->m.vdof.data(), m->m.vdof.size() * sizeof(int32_t));
+    return 0;
+}
+
+extern "C" void punc_gpu_hostmesh_free(punc_hostmesh_t* m) { delete m; }
+
+// ---------------------------------------------------------------------------
+// Mesh
+// ---------------------------------------------------------------------------
+static void mesh_free_device(punc_mesh_s* M) {
+    hipFree(M->d_rec);
+    hipFree(M->d_planes);
+    hipFree(M->d_bstart);
+    hipFree(M->d_bcells);
+    hipFree(M->d_fverts);
+    hipFree(M->d_fnormal);
+    hipFree(M->d_fcell);
+}
+
+extern "C" int punc_gpu_mesh_create(int64_t nv, const double* xyz, int64_t nc, const int32_t* cells,
+                                    const int32_t* facet_bnd, const int32_t* vdof, int64_t ndof,
+                                    punc_mesh_t** out) {
+    if (!out || !xyz || !cells || !facet_bnd) return fail("mesh_create: null argument");
+    if (nv <= 0 || nc <= 0) return fail("mesh_create: empty mesh");
+    auto* M = new punc_mesh_s();
+    M->host.nv = nv;
+    M->host.nc = nc;
+    M->host.xyz.assign(xyz, xyz + nv * 3);
+    M->host.cells.assign(cells, cells + nc * 4);
+    M->host.facet_bnd.assign(facet_bnd, facet_bnd + nc * 4);
+    if (vdof) {
+        M->host.vdof.assign(vdof, vdof + nv);
+        M->host.ndof = ndof;
+    } else {
+        M->host.vdof.resize(nv);
+        for (int64_t v = 0; v < nv; ++v) M->host.vdof[v] = (int32_t)v;
+        M->host.ndof = nv;
+    }
+    std::string err;
+    if (!build_topology(M->host, M->topo, err)) {
+        delete M;
+        return fail("mesh_create: " + err);
+    }
+    *out = M;
+    return 0;
+}
+
+// Device copies of the mesh are made on first use by a population, so the
+// host-side localizer can be built and inspected without a GPU.
+static int mesh_ensure_device(punc_mesh_s* M) {
+    if (M->d_rec) return 0;
+    HIPCHK(hipGetDevice(&M->device));
+    const MeshTopo& T = M->topo;
+    const int64_t nc = T.nc;
+    const size_t nf = T.ext.size();
+    std::vector<double> fv(nf * 9 + 1), fn(nf * 3 + 1);
+    std::vector<int32_t> fc(nf + 1);
+    for (size_t j = 0; j < nf; ++j) {
+        memcpy(&fv[j * 9], T.ext[j].vertices, 9 * sizeof(double));
+        memcpy(&fn[j * 3], T.ext[j].normal, 3 * sizeof(double));
+        fc[j] = T.ext[j].cell;
+    }
+#define MCHK(call)                     \
+    do {                               \
+        hipError_t e2 = (call);        \
+        if (e2 != hipSuccess) {        \
+            mesh_free_device(M);       \
+            M->d_rec = nullptr;        \
+            return hipfail(e2, #call); \
+        }                              \
+    } while (0)
+    MCHK(dalloc(&M->d_rec, nc));
+    MCHK(dalloc(&M->d_planes, nc));
+    MCHK(dalloc(&M->d_bstart, T.bucket_start.size()));
+    MCHK(dalloc(&M->d_bcells, T.bucket_cells.size()));
+    MCHK(dalloc(&M->d_fverts, fv.size()));
+    MCHK(dalloc(&M->d_fnormal, fn.size()));
+    MCHK(dalloc(&M->d_fcell, fc.size()));
+    MCHK(hipMemcpy(M->d_rec, T.rec.data(), nc * sizeof(CellRec), hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(M->d_planes, T.planes.data(), nc * sizeof(CellPlanes), hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(M->d_bstart, T.bucket_start.data(), T.bucket_start.size() * 4, hipMemcpyHostToDevice));
+    if (!T.bucket_cells.empty())
+        MCHK(hipMemcpy(M->d_bcells, T.bucket_cells.data(), T.bucket_cells.size() * 4, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(M->d_fverts, fv.data(), fv.size() * 8, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(M->d_fnormal, fn.data(), fn.size() * 8, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(M->d_fcell, fc.data(), fc.size() * 4, hipMemcpyHostToDevice));
+#undef MCHK
+    DevMesh& D = M->dm;
+    D.rec = M->d_rec;
+    D.planes = M->d_planes;
+    D.bstart = M->d_bstart;
+    D.bcells = M->d_bcells;
+    D.nc = nc;
+    D.ndof = M->host.ndof;
+    for (int d = 0; d < 3; ++d) {
+        D.gn[d] = T.grid_n[d];
+        D.glo[d] = T.grid_lo[d];
+        D.gh[d] = T.grid_h[d];
+    }
+    return 0;
+}
+
+extern "C" void punc_gpu_mesh_destroy(punc_mesh_t* m) {
+    if (!m) return;
+    mesh_free_device(m);
+    delete m;
+}
+
+extern "C" int punc_gpu_mesh_info(const punc_mesh_t* m, int64_t* nv, int64_t* nc, int64_t* ndof,
+                                  int64_t* n_exterior, double* volume) {
+    if (!m) return fail("mesh_info: null mesh");
+    if (nv) *nv = m->topo.nv;
+    if (nc) *nc = m->topo.nc;
+    if (ndof) *ndof = m->topo.ndof;
+    if (n_exterior) *n_exterior = (int64_t)m->topo.ext.size();
+    if (volume) *volume = m->topo.volume;
+    return 0;
+}
+
+extern "C" int punc_gpu_mesh_localizer(const punc_mesh_t* m, int64_t* adj, double* planes,
+                                       double* bary, double* vcoords, int32_t* dofs) {
+    if (!m) return fail("mesh_localizer: null mesh");
+    const MeshTopo& T = m->topo;
+    for (int64_t c = 0; c < T.nc; ++c) {
+        if (adj)
+            for (int i = 0; i < 4; ++i) adj[c * 4 + i] = T.adj[c * 4 + i];
+        if (planes) memcpy(planes + c * 16, T.planes[c].p, 16 * sizeof(double));
+        if (bary) memcpy(bary + c * 12, T.rec[c].bary, 12 * sizeof(double));
+        if (vcoords) memcpy(vcoords + c * 12, &T.vcoords[c * 12], 12 * sizeof(double));
+        if (dofs)
+            for (int i = 0; i < 4; ++i) dofs[c * 4 + i] = T.rec[c].dof[i];
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_element_volume(const punc_mesh_t* m, int voronoi, double* dv_inv) {
+    if (!m || !dv_inv) return fail("element_volume: null argument");
+    std::vector<double> dv;
+    element_volume(m->host, m->topo, voronoi != 0, dv);
+    memcpy(dv_inv, dv.data(), dv.size() * sizeof(double));
+    return 0;
+}
+
+extern "C" int punc_gpu_exterior_facets(const punc_mesh_t* m, double* area, double* vertices,
+                                        double* normal, double* basis, int32_t* cell) {
+    if (!m) return fail("exterior_facets: null mesh");
+    const auto& ext = m->topo.ext;
+    for (size_t j = 0; j < ext.size(); ++j) {
+        if (area) area[j] = ext[j].area;
+        if (vertices) memcpy(vertices + 9 * j, ext[j].vertices, 9 * sizeof(double));
+        if (normal) memcpy(normal + 3 * j, ext[j].normal, 3 * sizeof(double));
+        if (basis) memcpy(basis + 9 * j, ext[j].basis, 9 * sizeof(double));
+        if (cell) cell[j] = ext[j].cell;
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_mesh_locate(const punc_mesh_t* m, int64_t n, const double* xs, int64_t* cells) {
+    if (!m || (n > 0 && (!xs || !cells))) return fail("mesh_locate: null argument");
+    for (int64_t i = 0; i < n; ++i) cells[i] = locate_host(m->topo, xs + 3 * i);
+    return 0;
+}
+
+extern "C" int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const double vd[3],
+                                        double* num, double* pdf_max) {
+    if (!m || !vd) return fail("maxwellian_flux: null argument");
+    const auto& ext = m->topo.ext;
+    for (size_t j = 0; j < ext.size(); ++j) {
+        if (num) num[j] = maxwellian_flux_num(vth, vd, ext[j].normal, ext[j].area);
+        if (pdf_max) pdf_max[j] = maxwellian_flux_max(vth, vd, ext[j].normal);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Population
+// ---------------------------------------------------------------------------
+static void pop_free(punc_pop_s* P) {
+    if (P->d_state) hipFree(P->d_state);
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 7; ++k) hipFree(P->bufs[b][k]);
+    hipFree(P->d_holes);
+    hipFree(P->d_H);
+    hipFree(P->d_S);
+    hipFree(P->d_absorbed);
+    hipFree(P->d_cnt);
+    hipFree(P->d_errs);
+    hipFree(P->d_E);
+    hipFree(P->d_rho_raw);
+    hipFree(P->d_rho);
+    hipFree(P->d_dv_inv);
+    hipFree(P->d_ke_part);
+    hipFree(P->d_ke);
+    for (auto& c : P->coef_cache) hipFree(c.d);
+    hipFree(P->d_hist);
+    hipFree(P->d_off);
+    hipFree(P->d_bsum);
+    hipFree(P->d_total);
+    hipFree(P->d_spi);
+    hipFree(P->d_num);
+    hipFree(P->d_pmax);
+    hipFree(P->d_fcount);
+    hipFree(P->d_foff);
+    hipFree(P->d_cx);
+    hipFree(P->d_ccs);
+    hipFree(P->d_keep);
+    hipFree(P->d_pos);
+    for (auto& e : P->ev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    if (P->own_stream && P->stream) hipStreamDestroy(P->stream);
+}
+
+extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q, const double* mass,
+                                   int64_t capacity, void* stream, punc_pop_t** out) {
+    if (!m || !q || !mass || !out) return fail("pop_create: null argument");
+    {
+        int r0 = mesh_ensure_device(m);
+        if (r0) return r0;
+    }
+    if (nspecies < 1 || nspecies > 128) return fail("pop_create: nspecies must be in [1, 128]");
+    if (capacity < 1 || capacity >= ((int64_t)1 << 31)) return fail("pop_create: capacity must be in [1, 2^31)");
+    int cell_bits = 1;
+    while (((int64_t)1 << cell_bits) < m->topo.nc) ++cell_bits;
+    int sp_bits = 0;
+    while ((1 << sp_bits) < nspecies) ++sp_bits;
+    if (cell_bits + sp_bits > 31) return fail("pop_create: too many cells x species for the packed cell field");
+    auto* P = new punc_pop_s();
+    P->mesh = m;
+    P->device = m->device;
+    P->nsp = nspecies;
+    P->q.assign(q, q + nspecies);
+    P->m.assign(mass, mass + nspecies);
+    P->cell_bits = cell_bits;
+    P->capacity = capacity;
+#define PCHK(call)                     \
+    do {                               \
+        hipError_t e2 = (call);        \
+        if (e2 != hipSuccess) {        \
+            pop_free(P);               \
+            delete P;                  \
+            return hipfail(e2, #call); \
+        }                              \
+    } while (0)
+    PCHK(hipSetDevice(P->device));
+    if (stream) {
+        P->stream = (hipStream_t)stream;
+    } else {
+        PCHK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+        P->own_stream = true;
+    }
+    hipDeviceProp_t prop;
+    PCHK(hipGetDeviceProperties(&prop, P->device));
+    P->grid = prop.multiProcessorCount * 8;
+    P->ke_np = 65536;
+    for (int b = 0; b < 2; ++b) {
+        for (int k = 0; k < 6; ++k) PCHK(dalloc((double**)&P->bufs[b][k], capacity));
+        PCHK(dalloc((int32_t**)&P->bufs[b][6], capacity));
+    }
+    PCHK(dalloc(&P->d_state, 1));
+    PCHK(dalloc(&P->d_holes, capacity));
+    PCHK(dalloc(&P->d_H, capacity));
+    PCHK(dalloc(&P->d_S, capacity));
+    PCHK(dalloc(&P->d_absorbed, (size_t)nspecies * MAX_BND));
+    PCHK(dalloc(&P->d_cnt, nspecies));
+    PCHK(dalloc(&P->d_errs, 4));
+    const int64_t ndof = m->topo.ndof;
+    PCHK(dalloc(&P->d_E, ndof));
+    PCHK(dalloc(&P->d_rho_raw, ndof));
+    PCHK(dalloc(&P->d_rho, ndof));
+    PCHK(dalloc(&P->d_dv_inv, ndof));
+    PCHK(dalloc(&P->d_ke_part, P->ke_np));
+    PCHK(dalloc(&P->d_ke, 4));
+    const int64_t nc = m->topo.nc;
+    PCHK(dalloc(&P->d_hist, nc));
+    PCHK(dalloc(&P->d_off, nc));
+    P->bsum_len = std::max<int64_t>(nc / 1024 + 2, 4096);
+    PCHK(dalloc(&P->d_bsum, P->bsum_len));
+    PCHK(dalloc(&P->d_total, 8));
+    DevState s;
+    memset(&s, 0, sizeof(s));
+    PopArrays* sets[2] = {&s.cur, &s.alt};
+    for (int b = 0; b < 2; ++b) {
+        sets[b]->x = (double*)P->bufs[b][0];
+        sets[b]->y = (double*)P->bufs[b][1];
+        sets[b]->z = (double*)P->bufs[b][2];
+        sets[b]->vx = (double*)P->bufs[b][3];
+        sets[b]->vy = (double*)P->bufs[b][4];
+        sets[b]->vz = (double*)P->bufs[b][5];
+        sets[b]->cs = (int32_t*)P->bufs[b][6];
+    }
+    s.capacity = capacity;
+    PCHK(hipMemcpy(P->d_state, &s, sizeof(s), hipMemcpyHostToDevice));
+    PCHK(hipMemset(P->d_absorbed, 0, (size_t)nspecies * MAX_BND * 8));
+    PCHK(hipMemset(P->d_errs, 0, 4 * 8));
+    PCHK(hipMemset(P->d_E, 0, ndof * sizeof(double4)));
+    PCHK(hipMemset(P->d_rho_raw, 0, ndof * 8));
+    PCHK(hipMemset(P->d_rho, 0, ndof * 8));
+    PCHK(hipMemset(P->d_ke, 0, 4 * 8));
+    PCHK(hipMemset(P->d_total, 0, 8 * 8));
+    P->inj_on.assign(nspecies, 0);
+    P->spi.assign(nspecies, SpInject{});
+    const size_t nf = m->topo.ext.size();
+    P->flux_num.assign((size_t)nspecies * nf, 0.0);
+    P->flux_max.assign((size_t)nspecies * nf, 0.0);
+#undef PCHK
+    *out = P;
+    return 0;
+}
+
+extern "C" void punc_gpu_pop_destroy(punc_pop_t* p) {
+    if (!p) return;
+    hipSetDevice(p->device);
+    if (p->stream) hipStreamSynchronize(p->stream);
+    pop_free(p);
+    delete p;
+}
+
+static int read_state(punc_pop_s* P, DevState& s) {
+    HIPCHK(hipMemcpyAsync(&s, P->d_state, sizeof(DevState), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return 0;
+}
+
+static int check_errors(punc_pop_s* P, const char* what) {
+    DevState s;
+    int r = read_state(P, s);
+    if (r) return r;
+    unsigned long long errs[4];
+    HIPCHK(hipMemcpy(errs, P->d_errs, sizeof(errs), hipMemcpyDeviceToHost));
+    if (s.n_lost > 0)
+        return fail(std::string(what) + ": " + std::to_string(s.n_lost) +
+                    " particle(s) exceeded the relocation hop cap and were removed");
+    if (s.overflow > 0)
+        return fail(std::string(what) + ": population capacity exceeded (" +
+                    std::to_string(s.overflow) + " particle(s) dropped)");
+    if (errs[0] || errs[1])
+        return fail(std::string(what) + ": rejection sampler did not converge");
+    return 0;
+}
+
+static int append_host(punc_pop_s* P, int64_t n, const double* xs, const double* vs,
+                       const int32_t* cs_packed) {
+    DevState s;
+    int r = read_state(P, s);
+    if (r) return r;
+    if (s.n + n > P->capacity) return fail("population capacity exceeded");
+    std::vector<double> col((size_t)n);
+    double* dst[6] = {s.cur.x, s.cur.y, s.cur.z, s.cur.vx, s.cur.vy, s.cur.vz};
+    for (int k = 0; k < 6; ++k) {
+        const double* src = k < 3 ? xs : vs;
+        for (int64_t i = 0; i < n; ++i) col[i] = src[i * 3 + (k % 3)];
+        HIPCHK(hipMemcpy(dst[k] + s.n, col.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(s.cur.cs + s.n, cs_packed, n * 4, hipMemcpyHostToDevice));
+    int64_t nn = s.n + n;
+    HIPCHK(hipMemcpy((char*)P->d_state + offsetof(DevState, n), &nn, 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int punc_gpu_pop_add(punc_pop_t* p, int species, int64_t n, const double* xs,
+                                const double* vs, int64_t* n_added) {
+    if (!p || (n > 0 && (!xs || !vs))) return fail("pop_add: null argument");
+    if (species < 0 || species >= p->nsp) return fail("pop_add: species out of range");
+    HIPCHK(hipSetDevice(p->device));
+    std::vector<double> kx, kv;
+    std::vector<int32_t> kc;
+    kx.reserve(n * 3);
+    kv.reserve(n * 3);
+    kc.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c = locate_host(p->mesh->topo, xs + 3 * i);
+        if (c < 0) continue;
+        kx.insert(kx.end(), xs + 3 * i, xs + 3 * i + 3);
+        kv.insert(kv.end(), vs + 3 * i, vs + 3 * i + 3);
+        kc.push_back(pack_cs(species, (int32_t)c, p->cell_bits));
+    }
+    const int64_t k = (int64_t)kc.size();
+    if (k > 0) {
+        int r = append_host(p, k, kx.data(), kv.data(), kc.data());
+        if (r) return r;
+    }
+    if (n_added) *n_added = k;
+    return 0;
+}
+
+extern "C" int punc_gpu_pop_add_located(punc_pop_t* p, int64_t n, const double* xs, const double* vs,
+                                        const int32_t* species, const int64_t* cells) {
+    if (!p || (n > 0 && (!xs || !vs || !species || !cells))) return fail("pop_add_located: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    std::vector<int32_t> cs((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (species[i] < 0 || species[i] >= p->nsp) return fail("pop_add_located: species out of range");
+        if (cells[i] < 0 || cells[i] >= p->mesh->topo.nc) return fail("pop_add_located: cell out of range");
+        cs[i] = pack_cs(species[i], (int32_t)cells[i], p->cell_bits);
+    }
+    if (n == 0) return 0;
+    return append_host(p, n, xs, vs, cs.data());
+}
+
+extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp,
+                                 double kw, int axis, double vth, const double vd[3]) {
+    if (!p || !vd) return fail("pop_load: null argument");
+    if (species < 0 || species >= p->nsp) return fail("pop_load: species out of range");
+    if (axis < 0 || axis > 2) return fail("pop_load: axis must be 0, 1 or 2");
+    if (N < 0) return fail("pop_load: negative count");
+    HIPCHK(hipSetDevice(p->device));
+    DevState s;
+    int r = read_state(p, s);
+    if (r) return r;
+    if (s.n + N > p->capacity) return fail("pop_load: population capacity exceeded");
+    const MeshTopo& T = p->mesh->topo;
+    HIPCHK(launch_load(p->mesh->dm, p->d_state, s.n, N, species, p->cell_bits, seed, T.coord_min,
+                       T.coord_max, amp, kw, axis, vth, vd, p->d_errs, p->stream));
+    int64_t nn = s.n + N;
+    HIPCHK(hipMemcpyAsync((char*)p->d_state + offsetof(DevState, n), &nn, 8, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return check_errors(p, "pop_load");
+}
+
+extern "C" int punc_gpu_pop_size(punc_pop_t* p, int64_t* n) {
+    if (!p || !n) return fail("pop_size: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    DevState s;
+    int r = read_state(p, s);
+    if (r) return r;
+    *n = s.n;
+    return 0;
+}
+
+extern "C" int punc_gpu_pop_counts(punc_pop_t* p, int64_t* per_species) {
+    if (!p || !per_species) return fail("pop_counts: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(launch_fill_u64(p->d_cnt, p->nsp, p->stream));
+    HIPCHK(launch_count_species(p->d_state, p->cell_bits, p->nsp, p->d_cnt, p->grid, p->stream));
+    HIPCHK(hipMemcpyAsync(per_species, p->d_cnt, p->nsp * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_pop_get(punc_pop_t* p, double* xs, double* vs, int32_t* species, int64_t* cells) {
+    if (!p) return fail("pop_get: null population");
+    HIPCHK(hipSetDevice(p->device));
+    DevState s;
+    int r = read_state(p, s);
+    if (r) return r;
+    const int64_t n = s.n;
+    if (n == 0) return 0;
+    std::vector<double> col((size_t)n);
+    const double* src[6] = {s.cur.x, s.cur.y, s.cur.z, s.cur.vx, s.cur.vy, s.cur.vz};
+    for (int k = 0; k < 6; ++k) {
+        double* dst = k < 3 ? xs : vs;
+        if (!dst) continue;
+        HIPCHK(hipMemcpy(col.data(), src[k], n * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) dst[i * 3 + (k % 3)] = col[i];
+    }
+    if (species || cells) {
+        std::vector<int32_t> cs((size_t)n);
+        HIPCHK(hipMemcpy(cs.data(), s.cur.cs, n * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            if (species) species[i] = cs_species(cs[i], p->cell_bits);
+            if (cells) cells[i] = cs_cell(cs[i], p->cell_bits);
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Operators
+// ---------------------------------------------------------------------------
+// Per-species coefficients for (kind, dt, B): kind 1 accel, 2 boris, 0 none.
+static int get_coef(punc_pop_s* P, int kind, double dt, const double* B, const SpCoef** out) {
+    double Bv[3] = {0, 0, 0};
+    if (B)
+        for (int i = 0; i < 3; ++i) Bv[i] = B[i];
+    for (auto& c : P->coef_cache)
+        if (c.kind == kind && c.dt == dt && c.B[0] == Bv[0] && c.B[1] == Bv[1] && c.B[2] == Bv[2]) {
+            *out = c.d;
+            return 0;
+        }
+    std::vector<SpCoef> h((size_t)P->nsp);
+    for (int s = 0; s < P->nsp; ++s) {
+        SpCoef& K = h[s];
+        memset(&K, 0, sizeof(K));
+        const double q = P->q[s], m = P->m[s];
+        K.q = q;
+        K.m = m;
+        K.qm = q / m;
+        K.half_m = 0.5 * m;
+        if (kind == 1) {
+            K.a = dt * (q / m);  // pusher.h:194
+        } else if (kind == 2) {
+            K.a = 0.5 * dt * (q / m);  // pusher.h:345,367
+            double t_mag2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                K.t[i] = tan((dt * q / (2.0 * m)) * Bv[i]);  // pusher.h:334
+                t_mag2 += K.t[i] * K.t[i];
+            }
+            for (int i = 0; i < 3; ++i) K.s[i] = 2 * K.t[i] / (1 + t_mag2);  // pusher.h:340
+        }
+    }
+    if (P->coef_cache.size() >= 8) {
+        hipStreamSynchronize(P->stream);
+        for (auto& c : P->coef_cache) hipFree(c.d);
+        P->coef_cache.clear();
+    }
+    CoefEntry e;
+    e.kind = kind;
+    e.dt = dt;
+    for (int i = 0; i < 3; ++i) e.B[i] = Bv[i];
+    HIPCHK(dalloc(&e.d, P->nsp));
+    HIPCHK(hipMemcpy(e.d, h.data(), P->nsp * sizeof(SpCoef), hipMemcpyHostToDevice));
+    P->coef_cache.push_back(e);
+    *out = e.d;
+    return 0;
+}
+
+static ParticleLaunch base_launch(punc_pop_s* P) {
+    ParticleLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.mesh = P->mesh->dm;
+    L.state = P->d_state;
+    L.E = P->d_E;
+    L.cell_bits = P->cell_bits;
+    L.rho_raw = P->d_rho_raw;
+    L.ke_part = P->d_ke_part;
+    L.holes = P->d_holes;
+    L.absorbed = P->d_absorbed;
+    L.grid = P->grid;
+    L.block = P->block;
+    return L;
+}
+
+static hipError_t timed_launch(punc_pop_s* P, const ParticleLaunch& L) {
+    if (!P->timing) return launch_particles(L, P->stream);
+    if (P->ev_used == P->ev.size()) {
+        std::pair<hipEvent_t, hipEvent_t> pr;
+        hipError_t e = hipEventCreate(&pr.first);
+        if (e != hipSuccess) return e;
+        e = hipEventCreate(&pr.second);
+        if (e != hipSuccess) return e;
+        P->ev.push_back(pr);
+    }
+    auto& pr = P->ev[P->ev_used++];
+    hipError_t e = hipEventRecord(pr.first, P->stream);
+    if (e != hipSuccess) return e;
+    e = launch_particles(L, P->stream);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(pr.second, P->stream);
+}
+
+static int push_common(punc_pop_s* P, int kind, double dt, const double* B, double* KE) {
+    if (!P->field_set) return fail("push: no electric field set (punc_gpu_set_field)");
+    const SpCoef* coef;
+    int r = get_coef(P, kind, dt, B, &coef);
+    if (r) return r;
+    ParticleLaunch L = base_launch(P);
+    L.coef = coef;
+    L.push = kind;
+    HIPCHK(timed_launch(P, L));
+    HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
+    if (KE) {
+        HIPCHK(hipMemcpyAsync(KE, P->d_ke, 8, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_set_field(punc_pop_t* p, const double* E) {
+    if (!p || !E) return fail("set_field: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    const int64_t ndof = p->mesh->topo.ndof;
+    HIPCHK(hipMemcpy2DAsync(p->d_E, sizeof(double4), E, 3 * sizeof(double), 3 * sizeof(double), ndof,
+                            hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->field_set = true;
+    return 0;
+}
+
+extern "C" int punc_gpu_accel(punc_pop_t* p, double dt, double* KE) {
+    if (!p) return fail("accel: null population");
+    HIPCHK(hipSetDevice(p->device));
+    return push_common(p, 1, dt, nullptr, KE);
+}
+
+extern "C" int punc_gpu_boris(punc_pop_t* p, const double B[3], double dt, double* KE) {
+    if (!p || !B) return fail("boris: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    return push_common(p, 2, dt, B, KE);
+}
+
+static int move_common(punc_pop_s* P, double dt, const double* Ld) {
+    const SpCoef* coef;
+    int r = get_coef(P, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    ParticleLaunch L = base_launch(P);
+    L.coef = coef;
+    L.dt = dt;
+    L.move = Ld ? 2 : 1;
+    if (Ld)
+        for (int i = 0; i < 3; ++i) L.Ld[i] = Ld[i];
+    HIPCHK(timed_launch(P, L));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_move(punc_pop_t* p, double dt) {
+    if (!p) return fail("move: null population");
+    HIPCHK(hipSetDevice(p->device));
+    return move_common(p, dt, nullptr);
+}
+
+extern "C" int punc_gpu_move_periodic(punc_pop_t* p, double dt, const double Ld[3]) {
+    if (!p || !Ld) return fail("move_periodic: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    return move_common(p, dt, Ld);
+}
+
+static int read_absorbed(punc_pop_s* P, int64_t* absorbed) {
+    if (absorbed)
+        HIPCHK(hipMemcpyAsync(absorbed, P->d_absorbed, (size_t)P->nsp * MAX_BND * 8,
+                              hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(launch_fill_u64(P->d_absorbed, (int64_t)P->nsp * MAX_BND, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_update(punc_pop_t* p, int64_t* absorbed) {
+    if (!p) return fail("update: null population");
+    HIPCHK(hipSetDevice(p->device));
+    const SpCoef* coef;
+    int r = get_coef(p, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    HIPCHK(launch_fill_u64(p->d_absorbed, (int64_t)p->nsp * MAX_BND, p->stream));
+    ParticleLaunch L = base_launch(p);
+    L.coef = coef;
+    L.locate = true;
+    HIPCHK(timed_launch(p, L));
+    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_H, p->d_S, p->grid, p->stream));
+    r = read_absorbed(p, absorbed);
+    if (r) return r;
+    return check_errors(p, "update");
+}
+
+static int upload_dv(punc_pop_s* P, const double* dv_inv) {
+    if (dv_inv) {
+        HIPCHK(hipMemcpyAsync(P->d_dv_inv, dv_inv, P->mesh->topo.ndof * 8, hipMemcpyHostToDevice, P->stream));
+        P->dv_set = true;
+    }
+    if (!P->dv_set) {
+        std::vector<double> dv;
+        element_volume(P->mesh->host, P->mesh->topo, true, dv);
+        HIPCHK(hipMemcpy(P->d_dv_inv, dv.data(), dv.size() * 8, hipMemcpyHostToDevice));
+        P->dv_set = true;
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* rho) {
+    if (!p) return fail("distribute: null population");
+    HIPCHK(hipSetDevice(p->device));
+    int r = upload_dv(p, dv_inv);
+    if (r) return r;
+    const SpCoef* coef;
+    r = get_coef(p, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    const int64_t ndof = p->mesh->topo.ndof;
+    HIPCHK(launch_fill_f64(p->d_rho_raw, ndof, 0.0, p->stream));
+    ParticleLaunch L = base_launch(p);
+    L.coef = coef;
+    L.deposit = true;
+    HIPCHK(timed_launch(p, L));
+    HIPCHK(launch_scale_rho(p->d_rho_raw, p->d_dv_inv, p->d_rho, ndof, p->stream));
+    if (rho) HIPCHK(hipMemcpyAsync(rho, p->d_rho, ndof * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
+    if (!p || !KE) return fail("kinetic_energy: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    const SpCoef* coef;
+    int r = get_coef(p, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    HIPCHK(launch_kinetic(p->d_state, coef, p->cell_bits, p->d_ke_part, p->grid, p->stream));
+    HIPCHK(launch_sum_partials(p->d_ke_part, p->grid, p->d_ke + 1, p->stream));
+    HIPCHK(hipMemcpyAsync(KE, p->d_ke + 1, 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_sort(punc_pop_t* p) {
+    if (!p) return fail("sort: null population");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(launch_sort(p->d_state, p->cell_bits, p->mesh->topo.nc, p->d_hist, p->d_off, p->d_bsum,
+                       p->d_total, p->grid, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Injection
+// ---------------------------------------------------------------------------
+extern "C" int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
+                                     double density, double vrange) {
+    if (!p || !vd) return fail("injector_set: null argument");
+    if (species < 0 || species >= p->nsp) return fail("injector_set: species out of range");
+    if (!(vth > 0)) return fail("injector_set: vth must be positive");
+    SpInject& S = p->spi[species];
+    memset(&S, 0, sizeof(S));
+    S.vth = vth;
+    for (int i = 0; i < 3; ++i) S.vd[i] = vd[i];
+    S.vth2 = vth * vth;
+    S.factor = (1.0 / (pow(sqrt(2. * M_PI * S.vth2), 3)));  // distributions.cpp:28
+    S.vrange = vrange;
+    S.q = p->q[species];
+    S.m = p->m[species];
+    p->inj_on[species] = 1;
+    const auto& ext = p->mesh->topo.ext;
+    const size_t nf = ext.size();
+    for (size_t j = 0; j < nf; ++j) {
+        // create_flux (injector.cpp:127-152); density is applied per call
+        p->flux_num[species * nf + j] = maxwellian_flux_num(vth, vd, ext[j].normal, ext[j].area);
+        p->flux_max[species * nf + j] = maxwellian_flux_max(vth, vd, ext[j].normal);
+    }
+    // density kept in SpInject.pad[0]
+    S.pad[0] = density;
+    p->inj_dt = -1.0;  // force re-upload
+    return 0;
+}
+
+static int inject_prepare(punc_pop_s* P, double dt) {
+    if (P->inj_dt == dt && P->d_num) return 0;
+    const size_t nf = P->mesh->topo.ext.size();
+    const int nsp = P->nsp;
+    std::vector<double> num((size_t)nsp * nf + 1, 0.0), pmax((size_t)nsp * nf + 1, 0.0);
+    int64_t cap = 0;
+    for (int s = 0; s < nsp; ++s) {
+        if (!P->inj_on[s]) continue;
+        const double nd = P->spi[s].pad[0] * dt;  // injector.h:174: num_i *= n*dt
+        for (size_t j = 0; j < nf; ++j) {
+            double v = P->flux_num[s * nf + j];
+            v *= nd;
+            num[s * nf + j] = v;
+            pmax[s * nf + j] = P->flux_max[s * nf + j];
+            cap += (int64_t)v + 1;
+        }
+    }
+    cap = std::max<int64_t>(cap, 1024);
+    HIPCHK(hipStreamSynchronize(P->stream));
+    hipFree(P->d_num);
+    hipFree(P->d_pmax);
+    hipFree(P->d_spi);
+    HIPCHK(dalloc(&P->d_num, num.size()));
+    HIPCHK(dalloc(&P->d_pmax, pmax.size()));
+    HIPCHK(dalloc(&P->d_spi, nsp));
+    HIPCHK(hipMemcpy(P->d_num, num.data(), num.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_pmax, pmax.data(), pmax.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_spi, P->spi.data(), nsp * sizeof(SpInject), hipMemcpyHostToDevice));
+    if (cap > P->cand_cap || !P->d_fcount) {
+        hipFree(P->d_fcount);
+        hipFree(P->d_foff);
+        hipFree(P->d_cx);
+        hipFree(P->d_ccs);
+        hipFree(P->d_keep);
+        hipFree(P->d_pos);
+        P->cand_cap = cap;
+        HIPCHK(dalloc(&P->d_fcount, (size_t)nsp * nf + 1));
+        HIPCHK(dalloc(&P->d_foff, (size_t)nsp * nf + 1));
+        HIPCHK(dalloc(&P->d_cx, (size_t)cap * 6));
+        HIPCHK(dalloc(&P->d_ccs, cap));
+        HIPCHK(dalloc(&P->d_keep, cap));
+        HIPCHK(dalloc(&P->d_pos, cap));
+        const int64_t need = std::max<int64_t>(((int64_t)nsp * nf) / 1024, cap / 1024) + 2;
+        if (need > P->bsum_len) {
+            hipFree(P->d_bsum);
+            P->bsum_len = need;
+            HIPCHK(dalloc(&P->d_bsum, P->bsum_len));
+        }
+    }
+    P->inj_dt = dt;
+    return 0;
+}
+
+static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, uint64_t step, double* rho_raw) {
+    int r = inject_prepare(P, dt);
+    if (r) return r;
+    InjectLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.mesh = P->mesh->dm;
+    L.inj.fverts = P->mesh->d_fverts;
+    L.inj.fnormal = P->mesh->d_fnormal;
+    L.inj.fcell = P->mesh->d_fcell;
+    L.inj.num = P->d_num;
+    L.inj.pmax = P->d_pmax;
+    L.inj.nf = (int64_t)P->mesh->topo.ext.size();
+    L.state = P->d_state;
+    L.spi = P->d_spi;
+    const SpCoef* coef;
+    r = get_coef(P, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    L.coef = coef;
+    L.nsp = P->nsp;
+    L.seed = seed;
+    L.step = step;
+    L.dt = dt;
+    L.cell_bits = P->cell_bits;
+    L.cand_cap = P->cand_cap;
+    L.fcount = P->d_fcount;
+    L.foff = P->d_foff;
+    L.bsum = P->d_bsum;
+    L.ntot = P->d_total + 1;
+    L.nkept = P->d_total + 2;
+    L.cx = P->d_cx;
+    L.ccs = P->d_ccs;
+    L.keep = P->d_keep;
+    L.pos = P->d_pos;
+    L.rho_raw = rho_raw;
+    L.errs = P->d_errs + 1;
+    if (L.inj.nf == 0) return 0;
+    HIPCHK(launch_inject(L, P->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected) {
+    if (!p) return fail("inject: null population");
+    HIPCHK(hipSetDevice(p->device));
+    int r = inject_enqueue(p, dt, seed, step, nullptr);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    r = check_errors(p, "inject");
+    if (r) return r;
+    if (n_injected) {
+        DevState s;
+        r = read_state(p, s);
+        if (r) return r;
+        *n_injected = p->mesh->topo.ext.empty() ? 0 : s.inj_kept;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fused step
+// ---------------------------------------------------------------------------
+extern "C" int punc_gpu_step(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                             const double* Ld, int inject, uint64_t seed, uint64_t step) {
+    if (!p) return fail("step: null population");
+    if (pusher != 1 && pusher != 2) return fail("step: pusher must be 1 (accel_cg1) or 2 (boris_cg1)");
+    if (pusher == 2 && !B) return fail("step: boris needs B");
+    if (!p->field_set) return fail("step: no electric field set (punc_gpu_set_field)");
+    HIPCHK(hipSetDevice(p->device));
+    int r = upload_dv(p, nullptr);
+    if (r) return r;
+    const SpCoef* coef;
+    r = get_coef(p, pusher, push_dt, pusher == 2 ? B : nullptr, &coef);
+    if (r) return r;
+    const int64_t ndof = p->mesh->topo.ndof;
+    HIPCHK(launch_fill_f64(p->d_rho_raw, ndof, 0.0, p->stream));
+    ParticleLaunch L = base_launch(p);
+    L.coef = coef;
+    L.push = pusher;
+    L.dt = dt;
+    L.move = Ld ? 2 : 1;
+    if (Ld)
+        for (int i = 0; i < 3; ++i) L.Ld[i] = Ld[i];
+    L.locate = true;
+    L.deposit = true;
+    HIPCHK(timed_launch(p, L));
+    HIPCHK(launch_sum_partials(p->d_ke_part, L.grid, p->d_ke, p->stream));
+    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_H, p->d_S, p->grid, p->stream));
+    if (inject) {
+        r = inject_enqueue(p, dt, seed, step, p->d_rho_raw);
+        if (r) return r;
+    }
+    HIPCHK(launch_scale_rho(p->d_rho_raw, p->d_dv_inv, p->d_rho, ndof, p->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t* n_injected) {
+    if (!p) return fail("step_results: null population");
+    HIPCHK(hipSetDevice(p->device));
+    if (KE) HIPCHK(hipMemcpyAsync(KE, p->d_ke, 8, hipMemcpyDeviceToHost, p->stream));
+    int r = read_absorbed(p, absorbed);
+    if (r) return r;
+    r = check_errors(p, "step");
+    if (r) return r;
+    if (n_injected) {
+        DevState s;
+        r = read_state(p, s);
+        if (r) return r;
+        *n_injected = s.inj_kept;
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_get_rho(punc_pop_t* p, double* rho) {
+    if (!p || !rho) return fail("get_rho: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipMemcpyAsync(rho, p->d_rho, p->mesh->topo.ndof * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch control and timing
+// ---------------------------------------------------------------------------
+extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block) {
+    if (!p) return fail("set_launch: null population");
+    if (grid < 1 || grid > p->ke_np) return fail("set_launch: grid out of range");
+    if (block != 64 && block != 128 && block != 256) return fail("set_launch: block must be 64, 128 or 256");
+    p->grid = grid;
+    p->block = block;
+    return 0;
+}
+
+extern "C" int punc_gpu_timing(punc_pop_t* p, int enable) {
+    if (!p) return fail("timing: null population");
+    p->timing = enable != 0;
+    p->ev_used = 0;
+    p->timed_ms = 0.0;
+    p->timed_launches = 0;
+    return 0;
+}
+
+extern "C" int punc_gpu_timing_read(punc_pop_t* p, double* ms_total, int64_t* launches) {
+    if (!p) return fail("timing_read: null population");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (size_t k = 0; k < p->ev_used; ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev[k].first, p->ev[k].second));
+        p->timed_ms += ms;
+        p->timed_launches += 1;
+    }
+    p->ev_used = 0;
+    if (ms_total) *ms_total = p->timed_ms;
+    if (launches) *launches = p->timed_launches;
+    return 0;
+}
+
+extern "C" int punc_gpu_synchronize(punc_pop_t* p) {
+    if (!p) return fail("synchronize: null population");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}

@@ -1,0 +1,85 @@
+// Device-side state shared by the kernels (kernels.hip) and the C ABI (capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "punc_types.h"
+
+namespace punc_gpu {
+
+constexpr int MAX_BND = 64;  // boundary ids 1..63
+
+// Particle storage: FP64 structure of arrays plus one packed int32 cell field.
+struct PopArrays {
+    double* x;
+    double* y;
+    double* z;
+    double* vx;
+    double* vy;
+    double* vz;
+    int32_t* cs;
+};
+
+// Population state in device memory.  Kernels read the array pointers and the
+// counters through a pointer, so the cell re-sort can swap buffers and the
+// counts can change inside a captured graph without host involvement.
+struct DevState {
+    PopArrays cur;
+    PopArrays alt;
+    int64_t n;         // live particles
+    int64_t capacity;  // allocated slots per buffer
+    int64_t n_holes;   // particles removed by the last relocation (hole list length)
+    int64_t n_lost;    // relocation walks that exceeded the hop cap
+    int64_t nH;        // compaction: holes below the new count
+    int64_t nS;        // compaction: survivors above the new count
+    int64_t inj_cand;  // injection: candidates this call
+    int64_t inj_kept;  // injection: kept candidates this call
+    int64_t overflow;  // particles dropped because capacity was full
+    int64_t pad[7];
+};
+
+// Per-species coefficients for one call (depends on dt and B).
+struct SpCoef {
+    double q, m;
+    double qm;      // q / m
+    double half_m;  // 0.5 * m
+    double a;       // accel: dt*(q/m); boris: (0.5*dt)*(q/m)
+    double t[3];    // boris: tan(((dt*q)/(2.0*m))*B_i)
+    double s[3];    // boris: 2*t_i/(1+|t|^2)
+    double pad;
+};
+
+// Mesh data on the device.
+struct DevMesh {
+    const CellRec* rec;
+    const CellPlanes* planes;
+    const int32_t* bstart;
+    const int32_t* bcells;
+    int64_t nc;
+    int64_t ndof;
+    int gn[3];
+    double glo[3];
+    double gh[3];
+};
+
+// Exterior-facet injection table for one species set.
+struct DevInject {
+    const double* fverts;   // nf*9 facet vertices
+    const double* fnormal;  // nf*3 inward normals
+    const int32_t* fcell;   // nf owning cell
+    const double* num;      // nsp*nf expected count per facet per step (flux*n*dt)
+    const double* pmax;     // nsp*nf flux pdf maximum
+    int64_t nf;
+};
+
+struct SpInject {
+    double vth;
+    double vd[3];
+    double factor;  // 1/(sqrt(2*pi*vth^2))^3
+    double vth2;
+    double vrange;  // rejection box half width in units of vth
+    double q, m;
+    double pad[7];
+};
+
+}  // namespace punc_gpu

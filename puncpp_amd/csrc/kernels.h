@@ -1,0 +1,76 @@
+// Launch wrappers for the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace punc_gpu {
+
+struct ParticleLaunch {
+    DevMesh mesh;
+    DevState* state;
+    const SpCoef* coef;
+    const double4* E;
+    double dt;
+    double Ld[3];
+    int cell_bits;
+    double* rho_raw;
+    double* ke_part;
+    int32_t* holes;
+    unsigned long long* absorbed;
+    int push;  // 0 none, 1 accel, 2 boris
+    int move;  // 0 none, 1 plain, 2 periodic
+    bool locate;
+    bool deposit;
+    int grid;
+    int block;
+};
+
+struct InjectLaunch {
+    DevMesh mesh;
+    DevInject inj;
+    DevState* state;
+    const SpInject* spi;
+    const SpCoef* coef;
+    int nsp;
+    uint64_t seed;
+    uint64_t step;
+    double dt;
+    int cell_bits;
+    int64_t cand_cap;
+    int64_t* fcount;
+    int64_t* foff;
+    int64_t* bsum;
+    int64_t* ntot;
+    int64_t* nkept;
+    double* cx;
+    int32_t* ccs;
+    int64_t* keep;
+    int64_t* pos;
+    double* rho_raw;  // nullable: deposit the kept particles
+    unsigned long long* errs;
+};
+
+hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st);
+hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double* ke_part, int grid,
+                          hipStream_t st);
+hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t st);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_scale_rho(const double* raw, const double* dv_inv, double* rho, int64_t n,
+                            hipStream_t st);
+hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* H, int32_t* Sv, int grid,
+                          hipStream_t st);
+hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned long long* cnt,
+                                int grid, hipStream_t st);
+hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
+                       hipStream_t st);
+hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
+                       int64_t* bsum, int64_t* total, int grid, hipStream_t st);
+hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int cell_bits,
+                       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis,
+                       double vth, const double vd[3], unsigned long long* errs, hipStream_t st);
+hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
+
+}  // namespace punc_gpu

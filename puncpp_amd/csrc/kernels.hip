@@ -1,0 +1,778 @@
+// HIP kernels for the particle half of the PUNC PIC step on MI355X (gfx950).
+//
+// All particle data is FP64 structure-of-arrays in HBM; one lane handles one
+// particle.  Blocks of the persistent kernels take contiguous particle chunks,
+// and the chunk order is arranged so that each XCD (blocks are dispatched
+// round-robin over the 8 XCDs) works on one contiguous range of the
+// cell-sorted population, keeping a region's cell records and field dofs in
+// that XCD's L2.  The library is built with -ffp-contract=off and every
+// expression follows the reference's evaluation order, so positions,
+// velocities and cell decisions are bit-identical to the CPU path.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace punc_gpu {
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+__device__ inline int64_t logical_block() {
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    if (G % 8 == 0) return (b % 8) * (G / 8) + (b / 8);
+    return b;
+}
+
+__device__ inline void chunk_range(int64_t n, int64_t& s, int64_t& e) {
+    const int64_t G = gridDim.x;
+    int64_t per = (n + G - 1) / G;
+    per = (per + blockDim.x - 1) / blockDim.x * blockDim.x;
+    s = logical_block() * per;
+    e = s + per < n ? s + per : n;
+}
+
+// Barycentric coordinates (reference population.h:885-893).
+__device__ inline void bary4(const double* A, double x, double y, double z, double l[4]) {
+    l[0] = ((A[0] + A[1] * x) + A[2] * y) + A[3] * z;
+    l[1] = ((A[4] + A[5] * x) + A[6] * y) + A[7] * z;
+    l[2] = ((A[8] + A[9] * x) + A[10] * y) + A[11] * z;
+    l[3] = ((1 - l[0]) - l[1]) - l[2];
+}
+
+// Visibility walk (reference relocate_fast, population.h:539-569): from cell c,
+// move through the FIRST facet (in local order) whose plane gives proj > 0;
+// return the cell where no facet fires, or -bnd_id when the walk leaves the
+// mesh.  Iterative with a hop cap instead of the reference's recursion.
+__device__ inline int32_t walk(const DevMesh& M, int32_t c, double x, double y, double z) {
+    for (int h = 0; h < WALK_MAX_HOPS; ++h) {
+        const double* p = M.planes[c].p;
+        const double p0 = plane_proj(p, x, y, z);
+        const double p1 = plane_proj(p + 4, x, y, z);
+        const double p2 = plane_proj(p + 8, x, y, z);
+        const double p3 = plane_proj(p + 12, x, y, z);
+        const int f = p0 > 0 ? 0 : (p1 > 0 ? 1 : (p2 > 0 ? 2 : (p3 > 0 ? 3 : -1)));
+        if (f < 0) return c;
+        const int32_t nb = M.rec[c].nbr[f];
+        if (nb < 0) return nb;
+        c = nb;
+    }
+    return WALK_FAILED;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int k = 0; k < nw; ++k) r += sh[k];
+    }
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// The particle kernel.  PUSH: 0 none, 1 accel_cg1, 2 boris_cg1.  MOVE: 0 none,
+// 1 move, 2 move_periodic.  LOCATE: relocation walk with removal bookkeeping.
+// DEPOSIT: CG1 charge deposit q*lambda_i at the (new) cell.
+// ---------------------------------------------------------------------------
+template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
+__global__ void __launch_bounds__(256)
+k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
+            const double4* __restrict__ E, const double dt, const double Lx, const double Ly,
+            const double Lz, const int cell_bits, double* __restrict__ rho_raw,
+            double* __restrict__ ke_part, int32_t* __restrict__ holes,
+            unsigned long long* __restrict__ absorbed) {
+    __shared__ double sh[8];
+    const PopArrays A = S->cur;
+    const int64_t n = S->n;
+    int64_t s, e;
+    chunk_range(n, s, e);
+    const int32_t cmask = (1 << cell_bits) - 1;
+    double ke = 0.0;
+    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+        const int32_t cs = A.cs[i];
+        const int sp = cs >> cell_bits;
+        int32_t c = cs & cmask;
+        double x = A.x[i], y = A.y[i], z = A.z[i];
+        double vx = 0, vy = 0, vz = 0;
+        if (PUSH != 0) {
+            vx = A.vx[i];
+            vy = A.vy[i];
+            vz = A.vz[i];
+        }
+        if (MOVE != 0 && PUSH == 0) {
+            vx = A.vx[i];
+            vy = A.vy[i];
+            vz = A.vz[i];
+        }
+        if (PUSH != 0) {
+            const CellRec& R = M.rec[c];
+            double l[4];
+            bary4(R.bary, x, y, z, l);
+            const double4 e0 = E[R.dof[0]], e1 = E[R.dof[1]], e2 = E[R.dof[2]], e3 = E[R.dof[3]];
+            // Ei = sum_i coeffs[i]*values[j*4+i] starting from 0.0 (pusher.h:188-192, 324-328)
+            double Ex = 0.0, Ey = 0.0, Ez = 0.0;
+            Ex += l[0] * e0.x; Ex += l[1] * e1.x; Ex += l[2] * e2.x; Ex += l[3] * e3.x;
+            Ey += l[0] * e0.y; Ey += l[1] * e1.y; Ey += l[2] * e2.y; Ey += l[3] * e3.y;
+            Ez += l[0] * e0.z; Ez += l[1] * e1.z; Ez += l[2] * e2.z; Ez += l[3] * e3.z;
+            const SpCoef& K = coef[sp];
+            if (PUSH == 1) {
+                // accel_cg1 (pusher.h:194-196): Ei *= dt*(q/m); KE += 0.5*m*v*(v+Ei); v += Ei
+                Ex = Ex * K.a; Ey = Ey * K.a; Ez = Ez * K.a;
+                ke += (K.half_m * vx) * (vx + Ex);
+                ke += (K.half_m * vy) * (vy + Ey);
+                ke += (K.half_m * vz) * (vz + Ez);
+                vx += Ex; vy += Ey; vz += Ez;
+            } else {
+                // boris_cg1 (pusher.h:343-368)
+                const double mx = vx + K.a * Ex, my = vy + K.a * Ey, mz = vz + K.a * Ez;
+                ke += (K.half_m * mx) * mx;
+                ke += (K.half_m * my) * my;
+                ke += (K.half_m * mz) * mz;
+                const double cx = my * K.t[2] - mz * K.t[1];
+                const double cy = -mx * K.t[2] + mz * K.t[0];
+                const double cz = mx * K.t[1] - my * K.t[0];
+                const double px = mx + cx, py = my + cy, pz = mz + cz;
+                const double dx = py * K.s[2] - pz * K.s[1];
+                const double dy = -px * K.s[2] + pz * K.s[0];
+                const double dz = px * K.s[1] - py * K.s[0];
+                vx = (mx + dx) + K.a * Ex;
+                vy = (my + dy) + K.a * Ey;
+                vz = (mz + dz) + K.a * Ez;
+            }
+            A.vx[i] = vx;
+            A.vy[i] = vy;
+            A.vz[i] = vz;
+        }
+        if (MOVE != 0) {
+            // move / move_periodic (pusher.h:658, 676-677)
+            x += dt * vx;
+            y += dt * vy;
+            z += dt * vz;
+            if (MOVE == 2) {
+                x -= Lx * floor(x / Lx);
+                y -= Ly * floor(y / Ly);
+                z -= Lz * floor(z / Lz);
+            }
+            A.x[i] = x;
+            A.y[i] = y;
+            A.z[i] = z;
+        }
+        bool alive = true;
+        if (LOCATE) {
+            const int32_t c2 = walk(M, c, x, y, z);
+            if (c2 < 0) {
+                alive = false;
+                if (c2 == WALK_FAILED)
+                    atomicAdd((unsigned long long*)&S->n_lost, 1ull);
+                else
+                    atomicAdd(&absorbed[sp * MAX_BND + (-c2)], 1ull);
+                const unsigned long long slot = atomicAdd((unsigned long long*)&S->n_holes, 1ull);
+                holes[slot] = (int32_t)i;
+                A.cs[i] = c2;
+            } else if (c2 != c) {
+                c = c2;
+                A.cs[i] = (sp << cell_bits) | c2;
+            }
+        }
+        if (DEPOSIT && alive) {
+            // distribute_cg1 (distributor.h:189-203): accum_i += q*lambda_i
+            const CellRec& R = M.rec[c];
+            double l[4];
+            bary4(R.bary, x, y, z, l);
+            const double q = coef[sp].q;
+            unsafeAtomicAdd(&rho_raw[R.dof[0]], q * l[0]);
+            unsafeAtomicAdd(&rho_raw[R.dof[1]], q * l[1]);
+            unsafeAtomicAdd(&rho_raw[R.dof[2]], q * l[2]);
+            unsafeAtomicAdd(&rho_raw[R.dof[3]], q * l[3]);
+        }
+    }
+    if (PUSH != 0) {
+        const double t = block_sum(ke, sh);
+        if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
+    }
+}
+
+// Kinetic energy sum 0.5*m*v.v (reference diagnostics.h:248-265).
+__global__ void __launch_bounds__(256)
+k_kinetic(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
+          double* __restrict__ ke_part) {
+    __shared__ double sh[8];
+    const PopArrays A = S->cur;
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    double ke = 0.0;
+    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+        const int sp = A.cs[i] >> cell_bits;
+        const double hm = coef[sp].half_m;
+        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+        ke += (hm * vx) * vx;
+        ke += (hm * vy) * vy;
+        ke += (hm * vz) * vz;
+    }
+    const double t = block_sum(ke, sh);
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
+}
+
+// Deterministic final sum of the per-block partials (one block).
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ part, int np,
+                                                      double* __restrict__ out) {
+    __shared__ double sh[8];
+    double v = 0.0;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) v += part[k];
+    const double t = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// rho_j = dv_inv_j * raw_j (distributor.h:205-208).
+__global__ void k_scale_rho(const double* __restrict__ raw, const double* __restrict__ dv_inv,
+                            double* __restrict__ rho, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        rho[i] = raw[i] * dv_inv[i];
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Removal of particles that left the mesh: in-place hole filling.  With L
+// removed particles and n' = n - L survivors, each hole below n' receives one
+// surviving particle from [n', n).  Mirrors the effect of the reference's
+// swap-with-back deletion (population.h:686-692) on the particle set.
+// ---------------------------------------------------------------------------
+__global__ void k_compact_collect(DevState* __restrict__ S, const int32_t* __restrict__ holes,
+                                  int32_t* __restrict__ H, int32_t* __restrict__ Sv) {
+    const int64_t n = S->n, L = S->n_holes, nn = n - L;
+    const int32_t* cs = S->cur.cs;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t k = t0; k < L; k += stride) {
+        const int32_t h = holes[k];
+        if (h < nn) H[atomicAdd((unsigned long long*)&S->nH, 1ull)] = h;
+    }
+    for (int64_t j = nn + t0; j < n; j += stride)
+        if (cs[j] >= 0) Sv[atomicAdd((unsigned long long*)&S->nS, 1ull)] = (int32_t)j;
+}
+
+__global__ void k_compact_move(DevState* __restrict__ S, const int32_t* __restrict__ H,
+                               const int32_t* __restrict__ Sv) {
+    const PopArrays A = S->cur;
+    const int64_t m = S->nH;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = H[k], s = Sv[k];
+        A.x[d] = A.x[s];
+        A.y[d] = A.y[s];
+        A.z[d] = A.z[s];
+        A.vx[d] = A.vx[s];
+        A.vy[d] = A.vy[s];
+        A.vz[d] = A.vz[s];
+        A.cs[d] = A.cs[s];
+    }
+}
+
+__global__ void k_compact_finish(DevState* __restrict__ S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->n -= S->n_holes;
+        S->n_holes = 0;
+        S->nH = 0;
+        S->nS = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Counts per species (reference num_of_particles/positives/negatives,
+// population.h:701-744, derive from these).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_count_species(DevState* __restrict__ S, int cell_bits, int nsp, unsigned long long* __restrict__ cnt) {
+    __shared__ unsigned int c[256];
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x) c[k] = 0;
+    __syncthreads();
+    const int32_t* cs = S->cur.cs;
+    const int64_t n = S->n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&c[cs[i] >> cell_bits], 1u);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x)
+        if (c[k]) atomicAdd(&cnt[k], (unsigned long long)c[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int64 values (three phases, 1024 elements per block).
+// ---------------------------------------------------------------------------
+__device__ inline int64_t block_excl_scan_1024(int64_t v, int64_t* sh, int64_t* total) {
+    // 256 threads x 4 elements handled by caller; here a generic 256-thread scan.
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        int64_t a = (t >= o) ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    const int64_t incl = sh[t];
+    *total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict__ out, int64_t len,
+             int64_t* __restrict__ bsum) {
+    __shared__ int64_t sh[256];
+    const int64_t base = blockIdx.x * 1024ll + threadIdx.x * 4ll;
+    int64_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < len) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan_1024(s, sh, &tot);
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < len) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256)
+k_scan_bsums(int64_t* __restrict__ bsum, int64_t nb, int64_t* __restrict__ total) {
+    __shared__ int64_t sh[256];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += 256) {
+        const int64_t i = b0 + threadIdx.x;
+        const int64_t v = i < nb ? bsum[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_excl_scan_1024(v, sh, &tot);
+        if (i < nb) bsum[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t len, const int64_t* __restrict__ bsum) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < len) out[i] += bsum[i / 1024];
+}
+
+// ---------------------------------------------------------------------------
+// Cell re-sort: counting sort of the population by cell id into the alternate
+// buffers, then a pointer swap.  Order within a cell is arbitrary.
+// ---------------------------------------------------------------------------
+__global__ void k_sort_hist(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ hist) {
+    const int32_t* cs = S->cur.cs;
+    const int64_t n = S->n;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd((unsigned long long*)&hist[cs[i] & cmask], 1ull);
+}
+
+__global__ void k_sort_scatter(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ cursor) {
+    const PopArrays A = S->cur, B = S->alt;
+    const int64_t n = S->n;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t cs = A.cs[i];
+        const int64_t d = (int64_t)atomicAdd((unsigned long long*)&cursor[cs & cmask], 1ull);
+        B.x[d] = A.x[i];
+        B.y[d] = A.y[i];
+        B.z[d] = A.z[i];
+        B.vx[d] = A.vx[i];
+        B.vy[d] = A.vy[i];
+        B.vz[d] = A.vz[i];
+        B.cs[d] = cs;
+    }
+}
+
+__global__ void k_swap_buffers(DevState* __restrict__ S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const PopArrays t = S->cur;
+        S->cur = S->alt;
+        S->alt = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial loading (reference load_particles, injector.h:244-278): positions by
+// rejection in the box [cmin, cmax]^3 against the position pdf
+// (inside(x) * (1 + amp*cos(kw*x[axis])), max 1+amp; amp = 0 is
+// UniformPosition), velocities by the Maxwellian inverse CDF
+// v = vd - sqrt(2)*vth*erfc_inv(2u) (distributions.cpp:41-50).  Slot k of
+// species sp draws from Philox counter (k, attempt, sp, 0), key (seed, LOAD).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int sp, int cell_bits,
+       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis, double vth,
+       double vd0, double vd1, double vd2, unsigned long long* __restrict__ errs) {
+    const PopArrays A = S->cur;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double p[3];
+        int32_t c = -1;
+        bool ok = false;
+        for (uint64_t a = 0; a < 1000000ull; ++a) {
+            U64x4 ctr = {{(uint64_t)k, a, (uint64_t)sp, 0ull}};
+            const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
+            for (int d = 0; d < 3; ++d) p[d] = cmin + (cmax - cmin) * u01(r.v[d]);
+            c = locate_in_grid(M.planes, M.bstart, M.bcells, M.gn, M.glo, M.gh, p[0], p[1], p[2]);
+            const double pdf = (c >= 0) * (1.0 + amp * cos(kw * p[axis]));
+            if (u01(r.v[3]) * (1.0 + amp) < pdf) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) {
+            atomicAdd(errs, 1ull);
+            c = 0;
+        }
+        U64x4 ctr = {{(uint64_t)k, 0xFFFFFFFFull, (uint64_t)sp, 0ull}};
+        const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
+        const double vd[3] = {vd0, vd1, vd2};
+        double v[3];
+        for (int d = 0; d < 3; ++d) v[d] = vd[d] - sqrt(2.0) * vth * erfc_inv_approx(2 * u01(r.v[d]));
+        const int64_t i = base + k;
+        A.x[i] = p[0];
+        A.y[i] = p[1];
+        A.z[i] = p[2];
+        A.vx[i] = v[0];
+        A.vy[i] = v[1];
+        A.vz[i] = v[2];
+        A.cs[i] = (sp << cell_bits) | c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary injection (reference inject_particles, injector.h:157-232).
+// ---------------------------------------------------------------------------
+// Per (species, facet): N = floor(num) + [U < frac] (injector.h:186-191).
+__global__ void k_inject_count(const DevInject I, int nsp, uint64_t seed, uint64_t step,
+                               int64_t* __restrict__ fcount) {
+    const int64_t tot = I.nf * nsp;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = idx / I.nf, j = idx - s * I.nf;
+        const double num = I.num[idx];
+        int64_t N = (int64_t)num;
+        U64x4 ctr = {{step, (uint64_t)j, (uint64_t)s, 0ull}};
+        const U64x4 r = philox4x64_10(ctr, seed, TAG_INJECT_COUNT);
+        if (u01(r.v[0]) < (num - N)) N += 1;
+        fcount[idx] = N;
+    }
+}
+
+// One lane per candidate: facet point (injector.cpp:78-102), flux-Maxwellian
+// velocity by rejection (injector.cpp:51-76, distributions.h:93-102), push by
+// U*dt*v (injector.h:212-216), location by a walk from the facet's cell.
+__global__ void __launch_bounds__(256)
+k_inject_gen(const DevMesh M, const DevInject I, const SpInject* __restrict__ spi, int nsp,
+             const int64_t* __restrict__ foff, const int64_t* __restrict__ ntot, uint64_t seed,
+             uint64_t step, double dt, int cell_bits, int64_t cand_cap, double* __restrict__ cx,
+             int32_t* __restrict__ ccs, int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
+    const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
+    const int64_t nidx = I.nf * nsp;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < cand_cap;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        if (k >= total) {
+            keep[k] = 0;
+            continue;
+        }
+        // Facet of candidate k: last idx with foff[idx] <= k.
+        int64_t lo = 0, hi = nidx - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (foff[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const int64_t idx = lo, s = idx / I.nf, j = idx - s * I.nf, slot = k - foff[idx];
+        const double* V = I.fverts + 9 * j;
+        const double* nrm = I.fnormal + 3 * j;
+        const SpInject P = spi[s];
+        const uint64_t sj = ((uint64_t)s << 32) | (uint64_t)j;
+        U64x4 c0 = {{step, sj, (uint64_t)slot, 0ull}};
+        const U64x4 r0 = philox4x64_10(c0, seed, TAG_INJECT);
+        double x[3];
+        double r = u01(r0.v[0]);
+        for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
+        r = sqrt(u01(r0.v[1]));
+        for (int d = 0; d < 3; ++d) x[d] = r * x[d] + (1.0 - r) * V[6 + d];
+        const double pmax = I.pmax[idx];
+        double v[3];
+        bool ok = false;
+        for (uint64_t a = 1; a < 1000000ull; ++a) {
+            U64x4 ca = {{step, sj, (uint64_t)slot, a}};
+            const U64x4 ra = philox4x64_10(ca, seed, TAG_INJECT);
+            for (int d = 0; d < 3; ++d) {
+                const double lo_d = P.vd[d] - P.vrange * P.vth;
+                const double hi_d = P.vd[d] + P.vrange * P.vth;
+                v[d] = lo_d + (hi_d - lo_d) * u01(ra.v[d]);
+            }
+            double vn = 0.0;
+            for (int d = 0; d < 3; ++d) vn += v[d] * nrm[d];
+            double vs = 0.0;
+            for (int d = 0; d < 3; ++d) vs += (v[d] - P.vd[d]) * (v[d] - P.vd[d]);
+            const double f = (vn > 0.0) * vn * (P.factor * exp(-0.5 * vs / P.vth2));
+            if (u01(ra.v[3]) * pmax < f) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) atomicAdd(errs, 1ull);
+        const double rp = u01(r0.v[2]);
+        for (int d = 0; d < 3; ++d) x[d] = x[d] + rp * dt * v[d];
+        const int32_t c = ok ? walk(M, I.fcell[j], x[0], x[1], x[2]) : -1;
+        const bool kept = c >= 0;
+        keep[k] = kept ? 1 : 0;
+        cx[k] = x[0];
+        cx[cand_cap + k] = x[1];
+        cx[2 * cand_cap + k] = x[2];
+        cx[3 * cand_cap + k] = v[0];
+        cx[4 * cand_cap + k] = v[1];
+        cx[5 * cand_cap + k] = v[2];
+        ccs[k] = kept ? (int32_t)((s << cell_bits) | c) : -1;
+    }
+}
+
+// Append kept candidates at the end of the population, in candidate order,
+// and deposit their charge when rho_raw is given.
+__global__ void __launch_bounds__(256)
+k_inject_append(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
+                int64_t cand_cap, const int64_t* __restrict__ ntot, const double* __restrict__ cx,
+                const int32_t* __restrict__ ccs, const int64_t* __restrict__ pos, int cell_bits,
+                double* __restrict__ rho_raw) {
+    const PopArrays A = S->cur;
+    const int64_t n0 = S->n, cap = S->capacity;
+    const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t cs = ccs[k];
+        if (cs < 0) continue;
+        const int64_t d = n0 + pos[k];
+        if (d >= cap) {
+            atomicAdd((unsigned long long*)&S->overflow, 1ull);
+            continue;
+        }
+        const double x = cx[k], y = cx[cand_cap + k], z = cx[2 * cand_cap + k];
+        A.x[d] = x;
+        A.y[d] = y;
+        A.z[d] = z;
+        A.vx[d] = cx[3 * cand_cap + k];
+        A.vy[d] = cx[4 * cand_cap + k];
+        A.vz[d] = cx[5 * cand_cap + k];
+        A.cs[d] = cs;
+        if (rho_raw) {
+            const CellRec& R = M.rec[cs & cmask];
+            double l[4];
+            bary4(R.bary, x, y, z, l);
+            const double q = coef[cs >> cell_bits].q;
+            unsafeAtomicAdd(&rho_raw[R.dof[0]], q * l[0]);
+            unsafeAtomicAdd(&rho_raw[R.dof[1]], q * l[1]);
+            unsafeAtomicAdd(&rho_raw[R.dof[2]], q * l[2]);
+            unsafeAtomicAdd(&rho_raw[R.dof[3]], q * l[3]);
+        }
+    }
+}
+
+__global__ void k_inject_finish(DevState* __restrict__ S, const int64_t* __restrict__ ntot,
+                                const int64_t* __restrict__ nkept, int64_t cand_cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t total = *ntot;
+        int64_t kept = *nkept;
+        S->inj_cand = total;
+        if (total > cand_cap) S->overflow += total - cand_cap;
+        const int64_t room = S->capacity - S->n;
+        if (kept > room) kept = room;
+        S->inj_kept = kept;
+        S->n += kept;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+#define LAUNCH_CHECK() \
+    do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return e_; } while (0)
+
+template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
+static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT>), dim3(L.grid), dim3(L.block), 0, st,
+                       L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
+                       L.rho_raw, L.ke_part, L.holes, L.absorbed);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int PUSH, int MOVE>
+static hipError_t launch_pm(const ParticleLaunch& L, hipStream_t st) {
+    if (L.locate) {
+        if (L.deposit) return launch_particles_t<PUSH, MOVE, true, true>(L, st);
+        return launch_particles_t<PUSH, MOVE, true, false>(L, st);
+    }
+    if (L.deposit) return launch_particles_t<PUSH, MOVE, false, true>(L, st);
+    return launch_particles_t<PUSH, MOVE, false, false>(L, st);
+}
+
+hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st) {
+    switch (L.push * 3 + L.move) {
+        case 0: return launch_pm<0, 0>(L, st);
+        case 1: return launch_pm<0, 1>(L, st);
+        case 2: return launch_pm<0, 2>(L, st);
+        case 3: return launch_pm<1, 0>(L, st);
+        case 4: return launch_pm<1, 1>(L, st);
+        case 5: return launch_pm<1, 2>(L, st);
+        case 6: return launch_pm<2, 0>(L, st);
+        case 7: return launch_pm<2, 1>(L, st);
+        case 8: return launch_pm<2, 2>(L, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double* ke_part, int grid,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_kinetic, dim3(grid), dim3(256), 0, st, S, coef, cell_bits, ke_part);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, np, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+static int fill_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(fill_grid(n)), dim3(256), 0, st, p, n, v);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, st, p, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(n)), dim3(256), 0, st, p, n, v);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_scale_rho(const double* raw, const double* dv_inv, double* rho, int64_t n,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_rho, dim3(fill_grid(n)), dim3(256), 0, st, raw, dv_inv, rho, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* H, int32_t* Sv, int grid,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_collect, dim3(grid), dim3(256), 0, st, S, holes, H, Sv);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_compact_move, dim3(grid), dim3(256), 0, st, S, H, Sv);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(64), 0, st, S);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned long long* cnt,
+                                int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_species, dim3(grid), dim3(256), 0, st, S, cell_bits, nsp, cnt);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
+                       hipStream_t st) {
+    const int64_t nb = (len + 1023) / 1024;
+    if (nb == 0) {
+        return launch_fill_u64((unsigned long long*)total, 1, st);
+    }
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(256), 0, st, in, out, len, bsum);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(256), 0, st, bsum, nb, total);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, out, len, bsum);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
+                       int64_t* bsum, int64_t* total, int grid, hipStream_t st) {
+    hipError_t e = launch_fill_u64((unsigned long long*)hist, nc, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sort_hist, dim3(grid), dim3(256), 0, st, S, cell_bits, hist);
+    LAUNCH_CHECK();
+    e = launch_scan(hist, off, nc, bsum, total, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, st, S, cell_bits, off);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_swap_buffers, dim3(1), dim3(64), 0, st, S);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int cell_bits,
+                       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis,
+                       double vth, const double vd[3], unsigned long long* errs, hipStream_t st) {
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_load, dim3(fill_grid(N)), dim3(256), 0, st, M, S, base, N, sp, cell_bits, seed,
+                       cmin, cmax, amp, kw, axis, vth, vd[0], vd[1], vd[2], errs);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
+    const int64_t nidx = L.inj.nf * L.nsp;
+    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(nidx)), dim3(256), 0, st, L.inj, L.nsp, L.seed,
+                       L.step, L.fcount);
+    LAUNCH_CHECK();
+    hipError_t e = launch_scan(L.fcount, L.foff, nidx, L.bsum, L.ntot, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_inject_gen, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.inj,
+                       L.spi, L.nsp, L.foff, L.ntot, L.seed, L.step, L.dt, L.cell_bits, L.cand_cap,
+                       L.cx, L.ccs, L.keep, L.errs);
+    LAUNCH_CHECK();
+    e = launch_scan(L.keep, L.pos, L.cand_cap, L.bsum, L.nkept, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_inject_append, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.state,
+                       L.coef, L.cand_cap, L.ntot, L.cx, L.ccs, L.pos, L.cell_bits, L.rho_raw);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_inject_finish, dim3(1), dim3(64), 0, st, L.state, L.ntot, L.nkept, L.cand_cap);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+}  // namespace punc_gpu

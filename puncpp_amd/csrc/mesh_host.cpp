@@ -1,0 +1,486 @@
+// Host-side mesh setup: generator, localizer, patch volumes, exterior facets,
+// point-location grid and Maxwellian flux tables.  See mesh_host.h.
+#include "mesh_host.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+
+namespace punc_gpu {
+
+namespace {
+
+struct FacetKey {
+    int32_t a, b, c;
+    int64_t slot;  // cell*4 + local facet
+};
+
+// Local facet i of a tetrahedron is opposite local vertex i; its vertices are
+// the other three, kept in ascending local order.
+inline void facet_locals(int i, int out[3]) {
+    int k = 0;
+    for (int j = 0; j < 4; ++j)
+        if (j != i) out[k++] = j;
+}
+
+// Facet-to-cell adjacency: adj[c*4+i] = neighbour cell, or -1 if the facet has
+// one adjacent cell.  Returns false if a facet is shared by more than two cells.
+bool facet_adjacency(const std::vector<int32_t>& cells, int64_t nc, std::vector<int64_t>& adj,
+                     std::string& err) {
+    std::vector<FacetKey> keys((size_t)nc * 4);
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t* v = &cells[(size_t)c * 4];
+        for (int i = 0; i < 4; ++i) {
+            int l[3];
+            facet_locals(i, l);
+            keys[(size_t)c * 4 + i] = FacetKey{v[l[0]], v[l[1]], v[l[2]], c * 4 + i};
+        }
+    }
+    std::sort(keys.begin(), keys.end(), [](const FacetKey& x, const FacetKey& y) {
+        if (x.a != y.a) return x.a < y.a;
+        if (x.b != y.b) return x.b < y.b;
+        if (x.c != y.c) return x.c < y.c;
+        return x.slot < y.slot;
+    });
+    adj.assign((size_t)nc * 4, -1);
+    size_t k = 0;
+    while (k < keys.size()) {
+        size_t e = k + 1;
+        while (e < keys.size() && keys[e].a == keys[k].a && keys[e].b == keys[k].b &&
+               keys[e].c == keys[k].c)
+            ++e;
+        if (e - k > 2) {
+            err = "non-manifold mesh: a facet is shared by more than two cells";
+            return false;
+        }
+        if (e - k == 2) {
+            adj[keys[k].slot] = keys[k + 1].slot / 4;
+            adj[keys[k + 1].slot] = keys[k].slot / 4;
+        }
+        k = e;
+    }
+    return true;
+}
+
+inline void cross3(const double* a, const double* b, double* r) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Outward unit normal of local facet i and the facet midpoint.  Convention of
+// this library (the reference takes DOLFIN's Cell::normal and
+// Facet::midpoint, population.h:375-382): with P0 the opposite vertex and
+// P1<P2<P3 the facet vertices, n = (P2-P1) x (P3-P1) / |.|, flipped so that
+// n . (P0-P1) <= 0; midpoint = ((P1+P2)+P3)/3.
+void facet_normal_midpoint(const double* vc, int i, double n[3], double mid[3], double* area) {
+    int l[3];
+    facet_locals(i, l);
+    const double* P0 = vc + 3 * i;
+    const double* P1 = vc + 3 * l[0];
+    const double* P2 = vc + 3 * l[1];
+    const double* P3 = vc + 3 * l[2];
+    double V0[3], V1[3], V2[3];
+    for (int d = 0; d < 3; ++d) {
+        V0[d] = P0[d] - P1[d];
+        V1[d] = P2[d] - P1[d];
+        V2[d] = P3[d] - P1[d];
+    }
+    cross3(V1, V2, n);
+    double norm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int d = 0; d < 3; ++d) n[d] = n[d] / norm;
+    if (n[0] * V0[0] + n[1] * V0[1] + n[2] * V0[2] > 0.0)
+        for (int d = 0; d < 3; ++d) n[d] = -n[d];
+    for (int d = 0; d < 3; ++d) mid[d] = ((P1[d] + P2[d]) + P3[d]) / 3.0;
+    if (area) *area = 0.5 * norm;
+}
+
+// Barycentric matrix rows 0..2 (reference population.cpp:181-240).
+void barycentric_matrix(const double* vc, double* A) {
+    double x1 = vc[0], y1 = vc[1], z1 = vc[2];
+    double x2 = vc[3], y2 = vc[4], z2 = vc[5];
+    double x3 = vc[6], y3 = vc[7], z3 = vc[8];
+    double x4 = vc[9], y4 = vc[10], z4 = vc[11];
+    double x13 = x1 - x3, y13 = y1 - y3, z13 = z1 - z3;
+    double x14 = x1 - x4, y14 = y1 - y4, z14 = z1 - z4;
+    double x24 = x2 - x4, y24 = y2 - y4, z24 = z2 - z4;
+    double x31 = x3 - x1, y31 = y3 - y1, z31 = z3 - z1;
+    double x32 = x3 - x2, y32 = y3 - y2, z32 = z3 - z2;
+    double x34 = x3 - x4, y34 = y3 - y4, z34 = z3 - z4;
+    double x42 = x4 - x2, y42 = y4 - y2, z42 = z4 - z2;
+    double x43 = x4 - x3, y43 = y4 - y3, z43 = z4 - z3;
+    double V01 = (x2 * (y3 * z4 - y4 * z3) + x3 * (y4 * z2 - y2 * z4) + x4 * (y2 * z3 - y3 * z2)) / 6.0;
+    double V02 = (x1 * (y4 * z3 - y3 * z4) + x3 * (y1 * z4 - y4 * z1) + x4 * (y3 * z1 - y1 * z3)) / 6.0;
+    double V03 = (x1 * (y2 * z4 - y4 * z2) + x2 * (y4 * z1 - y1 * z4) + x4 * (y1 * z2 - y2 * z1)) / 6.0;
+    double V04 = (x1 * (y3 * z2 - y2 * z3) + x2 * (y1 * z3 - y3 * z1) + x3 * (y2 * z1 - y1 * z2)) / 6.0;
+    double V = V01 + V02 + V03 + V04;
+    double V6 = 6 * V;
+    A[0] = V01 / V;
+    A[4] = V02 / V;
+    A[8] = V03 / V;
+    A[1] = (y42 * z32 - y32 * z42) / V6;
+    A[5] = (y31 * z43 - y34 * z13) / V6;
+    A[9] = (y24 * z14 - y14 * z24) / V6;
+    A[2] = (x32 * z42 - x42 * z32) / V6;
+    A[6] = (x43 * z31 - x13 * z34) / V6;
+    A[10] = (x14 * z24 - x24 * z14) / V6;
+    A[3] = (x42 * y32 - x32 * y42) / V6;
+    A[7] = (x31 * y43 - x34 * y13) / V6;
+    A[11] = (x24 * y14 - x14 * y24) / V6;
+}
+
+// |det(P2-P1, P3-P1, P4-P1)| / 6.
+double tet_volume(const double* vc) {
+    double a[3], b[3], c[3], bc[3];
+    for (int d = 0; d < 3; ++d) {
+        a[d] = vc[3 + d] - vc[d];
+        b[d] = vc[6 + d] - vc[d];
+        c[d] = vc[9 + d] - vc[d];
+    }
+    cross3(b, c, bc);
+    double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    return fabs(det) / 6.0;
+}
+
+}  // namespace
+
+bool meshgen_box(int n, const double lo[3], const double hi[3], double sphere_r, int periodic,
+                 HostMesh& out, std::string& err) {
+    if (n < 1 || n > 1024) {
+        err = "meshgen_box: n must be in [1, 1024]";
+        return false;
+    }
+    if (periodic && sphere_r > 0) {
+        err = "meshgen_box: a periodic box cannot hold an object";
+        return false;
+    }
+    const int64_t n1 = n + 1;
+    const int64_t nv_full = n1 * n1 * n1;
+    double h[3];
+    for (int d = 0; d < 3; ++d) h[d] = (hi[d] - lo[d]) / n;
+    std::vector<double> xyz((size_t)nv_full * 3);
+    for (int64_t k = 0; k < n1; ++k)
+        for (int64_t j = 0; j < n1; ++j)
+            for (int64_t i = 0; i < n1; ++i) {
+                int64_t v = i + n1 * (j + n1 * k);
+                xyz[v * 3 + 0] = (i == n) ? hi[0] : lo[0] + i * h[0];
+                xyz[v * 3 + 1] = (j == n) ? hi[1] : lo[1] + j * h[1];
+                xyz[v * 3 + 2] = (k == n) ? hi[2] : lo[2] + k * h[2];
+            }
+    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    std::vector<int32_t> cells;
+    cells.reserve((size_t)n * n * n * 24);
+    const double r2 = sphere_r * sphere_r;
+    for (int64_t k = 0; k < n; ++k)
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i)
+                for (int p = 0; p < 6; ++p) {
+                    int64_t ijk[3] = {i, j, k};
+                    int32_t tv[4];
+                    tv[0] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
+                    for (int s = 0; s < 2; ++s) {
+                        ijk[perms[p][s]] += 1;
+                        tv[s + 1] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
+                    }
+                    tv[3] = (int32_t)((i + 1) + n1 * ((j + 1) + n1 * (k + 1)));
+                    std::sort(tv, tv + 4);
+                    if (sphere_r > 0) {
+                        double c[3] = {0, 0, 0};
+                        for (int q = 0; q < 4; ++q)
+                            for (int d = 0; d < 3; ++d) c[d] += xyz[(size_t)tv[q] * 3 + d];
+                        for (int d = 0; d < 3; ++d) c[d] /= 4.0;
+                        if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] < r2) continue;
+                    }
+                    cells.insert(cells.end(), tv, tv + 4);
+                }
+    const int64_t nc = (int64_t)cells.size() / 4;
+    // Compact away vertices that no cell uses (the sphere interior).
+    std::vector<int32_t> remap((size_t)nv_full, -1);
+    for (int32_t v : cells) remap[v] = 0;
+    int64_t nv = 0;
+    std::vector<int64_t> lattice;  // kept vertex -> lattice index
+    for (int64_t v = 0; v < nv_full; ++v)
+        if (remap[v] == 0) {
+            remap[v] = (int32_t)nv++;
+            lattice.push_back(v);
+        }
+    out.nv = nv;
+    out.nc = nc;
+    out.xyz.resize((size_t)nv * 3);
+    for (int64_t v = 0; v < nv; ++v)
+        for (int d = 0; d < 3; ++d) out.xyz[v * 3 + d] = xyz[lattice[v] * 3 + d];
+    out.cells.resize(cells.size());
+    for (size_t q = 0; q < cells.size(); ++q) out.cells[q] = remap[cells[q]];
+    // Boundary markers: facets with one adjacent cell are exterior (1) when
+    // their three vertices lie on one box face, object (2) otherwise.
+    std::vector<int64_t> adj;
+    if (!facet_adjacency(out.cells, nc, adj, err)) return false;
+    out.facet_bnd.assign((size_t)nc * 4, 0);
+    for (int64_t c = 0; c < nc; ++c)
+        for (int f = 0; f < 4; ++f) {
+            if (adj[c * 4 + f] >= 0) continue;
+            int l[3];
+            facet_locals(f, l);
+            bool on_box = false;
+            for (int d = 0; d < 3 && !on_box; ++d) {
+                for (int side = 0; side < 2 && !on_box; ++side) {
+                    int64_t want = side ? n : 0;
+                    bool all = true;
+                    for (int q = 0; q < 3; ++q) {
+                        int64_t L = lattice[out.cells[c * 4 + l[q]]];
+                        int64_t idx[3] = {L % n1, (L / n1) % n1, L / (n1 * n1)};
+                        all = all && idx[d] == want;
+                    }
+                    on_box = all;
+                }
+            }
+            out.facet_bnd[c * 4 + f] = on_box ? 1 : 2;
+        }
+    out.vdof.resize((size_t)nv);
+    if (periodic) {
+        for (int64_t v = 0; v < nv; ++v) {
+            int64_t L = lattice[v];
+            int64_t i = (L % n1) % n, j = ((L / n1) % n1) % n, k = (L / (n1 * n1)) % n;
+            out.vdof[v] = (int32_t)(i + n * (j + n * k));
+        }
+        out.ndof = (int64_t)n * n * n;
+    } else {
+        for (int64_t v = 0; v < nv; ++v) out.vdof[v] = (int32_t)v;
+        out.ndof = nv;
+    }
+    return true;
+}
+
+bool build_topology(const HostMesh& m, MeshTopo& t, std::string& err) {
+    const int64_t nc = m.nc, nv = m.nv;
+    if (nc <= 0 || nv <= 0) {
+        err = "empty mesh";
+        return false;
+    }
+    if ((int64_t)m.cells.size() != nc * 4 || (int64_t)m.xyz.size() != nv * 3 ||
+        (int64_t)m.facet_bnd.size() != nc * 4 || (int64_t)m.vdof.size() != nv) {
+        err = "mesh arrays have inconsistent sizes";
+        return false;
+    }
+    if (nc >= (int64_t)1 << 30) {
+        err = "too many cells (limit 2^30)";
+        return false;
+    }
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t* v = &m.cells[c * 4];
+        for (int i = 0; i < 4; ++i)
+            if (v[i] < 0 || v[i] >= nv) {
+                err = "cell vertex id out of range";
+                return false;
+            }
+        if (!(v[0] < v[1] && v[1] < v[2] && v[2] < v[3])) {
+            err = "cell vertex ids must be strictly ascending (UFC ordering)";
+            return false;
+        }
+    }
+    for (int64_t v = 0; v < nv; ++v)
+        if (m.vdof[v] < 0 || m.vdof[v] >= m.ndof) {
+            err = "vertex dof id out of range";
+            return false;
+        }
+    t.nc = nc;
+    t.nv = nv;
+    t.ndof = m.ndof;
+    std::vector<int64_t> adj;
+    if (!facet_adjacency(m.cells, nc, adj, err)) return false;
+    t.adj.resize((size_t)nc * 4);
+    t.rec.resize((size_t)nc);
+    t.planes.resize((size_t)nc);
+    t.vcoords.resize((size_t)nc * 12);
+    t.ext.clear();
+    std::vector<char> seen_bnd(64, 0);
+    t.volume = 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+        double* vc = &t.vcoords[c * 12];
+        for (int i = 0; i < 4; ++i)
+            for (int d = 0; d < 3; ++d) vc[i * 3 + d] = m.xyz[(size_t)m.cells[c * 4 + i] * 3 + d];
+        CellRec& r = t.rec[c];
+        barycentric_matrix(vc, r.bary);
+        CellPlanes& P = t.planes[c];
+        for (int i = 0; i < 4; ++i) {
+            r.dof[i] = m.vdof[m.cells[c * 4 + i]];
+            int64_t a = adj[c * 4 + i];
+            if (a < 0) {
+                int32_t b = m.facet_bnd[c * 4 + i];
+                if (b < 1 || b > 63) {
+                    err = "boundary facet without a boundary id in [1, 63]";
+                    return false;
+                }
+                seen_bnd[b] = 1;
+                a = -(int64_t)b;
+            }
+            t.adj[c * 4 + i] = a;
+            r.nbr[i] = (int32_t)a;
+            double n[3], mid[3], area;
+            facet_normal_midpoint(vc, i, n, mid, &area);
+            double dot = 0;
+            for (int d = 0; d < 3; ++d) dot += mid[d] * n[d];
+            P.p[4 * i + 0] = -dot;
+            for (int d = 0; d < 3; ++d) P.p[4 * i + 1 + d] = n[d];
+            if (a == -1) {
+                // Exterior facet (reference mesh.cpp:166-236): inward normal,
+                // facet vertices in ascending global order, orthonormal basis.
+                ExteriorFacetH ef;
+                ef.area = area;
+                ef.cell = (int32_t)c;
+                int l[3];
+                facet_locals(i, l);
+                for (int q = 0; q < 3; ++q)
+                    for (int d = 0; d < 3; ++d) ef.vertices[q * 3 + d] = vc[l[q] * 3 + d];
+                for (int d = 0; d < 3; ++d) {
+                    ef.normal[d] = -1 * n[d];
+                    ef.basis[d * 3] = ef.normal[d];
+                }
+                double e[3], norm = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    e[d] = ef.vertices[d] - ef.vertices[3 + d];
+                    norm += e[d] * e[d];
+                }
+                for (int d = 0; d < 3; ++d) {
+                    e[d] /= sqrt(norm);
+                    ef.basis[d * 3 + 1] = e[d];
+                }
+                ef.basis[2] = ef.normal[1] * e[2] - ef.normal[2] * e[1];
+                ef.basis[5] = ef.normal[2] * e[0] - ef.normal[0] * e[2];
+                ef.basis[8] = ef.normal[0] * e[1] - ef.normal[1] * e[0];
+                t.ext.push_back(ef);
+            }
+        }
+        t.volume += tet_volume(vc);
+    }
+    t.bnd_ids.clear();
+    for (int b = 1; b < 64; ++b)
+        if (seen_bnd[b]) t.bnd_ids.push_back(b);
+    // Bounding box and the UniformPosition domain (min/max over all coordinates,
+    // reference distributions.h:196-208).
+    for (int d = 0; d < 3; ++d) {
+        t.bbox_lo[d] = m.xyz[d];
+        t.bbox_hi[d] = m.xyz[d];
+    }
+    t.coord_min = m.xyz[0];
+    t.coord_max = m.xyz[0];
+    for (int64_t v = 0; v < nv; ++v)
+        for (int d = 0; d < 3; ++d) {
+            double x = m.xyz[v * 3 + d];
+            t.bbox_lo[d] = std::min(t.bbox_lo[d], x);
+            t.bbox_hi[d] = std::max(t.bbox_hi[d], x);
+            t.coord_min = std::min(t.coord_min, x);
+            t.coord_max = std::max(t.coord_max, x);
+        }
+    // Point-location grid: about six cells per bucket.
+    double ext[3], bvol = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        ext[d] = t.bbox_hi[d] - t.bbox_lo[d];
+        if (ext[d] <= 0) ext[d] = 1.0;
+        bvol *= ext[d];
+    }
+    double hb = cbrt(bvol * 6.0 / (double)nc);
+    int64_t nb = 1;
+    for (int d = 0; d < 3; ++d) {
+        int g = (int)llround(ext[d] / hb);
+        g = std::max(1, std::min(g, 512));
+        t.grid_n[d] = g;
+        t.grid_lo[d] = t.bbox_lo[d];
+        t.grid_h[d] = ext[d] / g;
+        nb *= g;
+    }
+    std::vector<int32_t> cnt((size_t)nb + 1, 0);
+    auto brange = [&](int64_t c, int lo_i[3], int hi_i[3]) {
+        const double* vc = &t.vcoords[c * 12];
+        for (int d = 0; d < 3; ++d) {
+            double mn = vc[d], mx = vc[d];
+            for (int i = 1; i < 4; ++i) {
+                mn = std::min(mn, vc[i * 3 + d]);
+                mx = std::max(mx, vc[i * 3 + d]);
+            }
+            double a = floor((mn - t.grid_lo[d]) / t.grid_h[d] - 1e-7);
+            double b = floor((mx - t.grid_lo[d]) / t.grid_h[d] + 1e-7);
+            lo_i[d] = (int)std::max(0.0, std::min(a, (double)t.grid_n[d] - 1));
+            hi_i[d] = (int)std::max(0.0, std::min(b, (double)t.grid_n[d] - 1));
+        }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int32_t> fill;
+        if (pass == 1) {
+            t.bucket_start.assign((size_t)nb + 1, 0);
+            for (int64_t b = 0; b < nb; ++b) t.bucket_start[b + 1] = t.bucket_start[b] + cnt[b];
+            t.bucket_cells.assign((size_t)t.bucket_start[nb], 0);
+            fill.assign(t.bucket_start.begin(), t.bucket_start.end() - 1);
+        }
+        for (int64_t c = 0; c < nc; ++c) {
+            int a[3], b[3];
+            brange(c, a, b);
+            for (int k = a[2]; k <= b[2]; ++k)
+                for (int j = a[1]; j <= b[1]; ++j)
+                    for (int i = a[0]; i <= b[0]; ++i) {
+                        int64_t bi = i + (int64_t)t.grid_n[0] * (j + (int64_t)t.grid_n[1] * k);
+                        if (pass == 0)
+                            cnt[bi]++;
+                        else
+                            t.bucket_cells[fill[bi]++] = (int32_t)c;
+                    }
+        }
+    }
+    return true;
+}
+
+void element_volume(const HostMesh& m, const MeshTopo& t, bool voronoi, std::vector<double>& dv) {
+    // Vertex -> incident cells in ascending cell order, then per vertex in
+    // vertex order add each incident cell volume to the vertex's dof
+    // (reference distributor.cpp:44-54).
+    std::vector<int64_t> start((size_t)m.nv + 1, 0);
+    for (int32_t v : m.cells) start[v + 1]++;
+    for (int64_t v = 0; v < m.nv; ++v) start[v + 1] += start[v];
+    std::vector<int64_t> pos(start.begin(), start.end() - 1);
+    std::vector<int32_t> inc((size_t)start[m.nv]);
+    for (int64_t c = 0; c < m.nc; ++c)
+        for (int i = 0; i < 4; ++i) inc[pos[m.cells[c * 4 + i]]++] = (int32_t)c;
+    std::vector<double> cvol((size_t)m.nc);
+    for (int64_t c = 0; c < m.nc; ++c) cvol[c] = tet_volume(&t.vcoords[c * 12]);
+    dv.assign((size_t)m.ndof, 0.0);
+    for (int64_t v = 0; v < m.nv; ++v)
+        for (int64_t k = start[v]; k < start[v + 1]; ++k) dv[m.vdof[v]] += cvol[inc[k]];
+    const double g_dim = 3;
+    for (int64_t i = 0; i < m.ndof; ++i) dv[i] = voronoi ? (g_dim + 1.0) / dv[i] : 1.0 / dv[i];
+}
+
+int64_t locate_host(const MeshTopo& t, const double* x) {
+    return locate_in_grid(t.planes.data(), t.bucket_start.data(), t.bucket_cells.data(), t.grid_n,
+                          t.grid_lo, t.grid_h, x[0], x[1], x[2]);
+}
+
+double maxwellian_flux_num(double vth, const double vd[3], const double n[3], double S) {
+    double vdn = 0.0;
+    for (int i = 0; i < 3; ++i) vdn = vdn + n[i] * vd[i];
+    return S * (vth / (sqrt(2 * M_PI)) * exp(-0.5 * (vdn / vth) * (vdn / vth)) +
+                0.5 * vdn * (1. + erf(vdn / (sqrt(2) * vth))));
+}
+
+double maxwellian_pdf(double vth, const double vd[3], const double v[3]) {
+    double vth2 = vth * vth;
+    double factor = (1.0 / (pow(sqrt(2. * M_PI * vth2), 3)));
+    double v_sqrt = 0.0;
+    for (int i = 0; i < 3; ++i) v_sqrt += (v[i] - vd[i]) * (v[i] - vd[i]);
+    return factor * exp(-0.5 * v_sqrt / vth2);
+}
+
+double maxwellian_flux_max(double vth, const double vd[3], const double n[3]) {
+    double vdn = 0.0;
+    for (int i = 0; i < 3; ++i) vdn = vdn + n[i] * vd[i];
+    double vth2 = vth * vth;
+    double tmp[3];
+    for (int i = 0; i < 3; ++i)
+        tmp[i] = vd[i] - 0.5 * n[i] * vdn + 0.5 * n[i] * sqrt(4 * vth2 + vdn * vdn);
+    double vn = 0.0;
+    for (int i = 0; i < 3; ++i) vn += tmp[i] * n[i];
+    return (vn > 0.0) * vn * maxwellian_pdf(vth, vd, tmp);
+}
+
+}  // namespace punc_gpu

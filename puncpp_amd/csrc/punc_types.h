@@ -1,0 +1,171 @@
+// Shared host/device definitions for the PUNC particle path on MI355X (gfx950).
+//
+// Everything here is compiled twice: by hipcc for the device kernels and by the
+// host pass for the setup code in mesh.cpp.  All floating-point expressions are
+// written in the evaluation order of the PUNC++ particle code and the library is
+// built with -ffp-contract=off, so host and device produce identical bits for
+// +, -, *, / and sqrt.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PUNC_HD __host__ __device__ inline
+#else
+#define PUNC_HD inline
+#endif
+
+namespace punc_gpu {
+
+// ----------------------------------------------------------------------------
+// Packed per-cell mesh data.
+//
+// CellRec (128 B, one cache line): the barycentric matrix rows 0..2 of
+// Cell<3>::init_barycentric_matrix (reference population.cpp:181-240, row 3 is
+// implicit: lambda3 = 1 - l0 - l1 - l2, population.h:885-893), the four CG1 dof
+// ids of the cell's vertices (local vertex order), and the four facet
+// neighbours (cell id, or -bnd_id on a boundary facet; population.h:363-373).
+//
+// CellPlanes (128 B): the facet planes (c0, nx, ny, nz) per local facet,
+// c0 = -(midpoint . n) with n the outward unit normal (population.h:375-382).
+// Local facet i is opposite local vertex i.
+// ----------------------------------------------------------------------------
+struct alignas(128) CellRec {
+    double bary[12];
+    int32_t dof[4];
+    int32_t nbr[4];
+};
+struct alignas(128) CellPlanes {
+    double p[16];
+};
+static_assert(sizeof(CellRec) == 128, "CellRec must be one 128-byte line");
+static_assert(sizeof(CellPlanes) == 128, "CellPlanes must be one 128-byte line");
+
+// Particle cell field: bits [0, cell_bits) hold the cell id, the bits above it
+// hold the species index; a negative value marks a particle that left through
+// boundary -value during the last relocation.
+PUNC_HD int32_t pack_cs(int32_t species, int32_t cell, int cell_bits) {
+    return (species << cell_bits) | cell;
+}
+PUNC_HD int32_t cs_cell(int32_t cs, int cell_bits) { return cs & ((1 << cell_bits) - 1); }
+PUNC_HD int32_t cs_species(int32_t cs, int cell_bits) { return cs >> cell_bits; }
+
+// Relocation walk outcome when the hop cap is exceeded (never a valid -bnd id).
+constexpr int32_t WALK_FAILED = -(1 << 30);
+constexpr int WALK_MAX_HOPS = 100000;
+
+// ----------------------------------------------------------------------------
+// Counter-based RNG: Philox4x64-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11).  Replaces the reference's unseeded
+// std::mt19937_64 (injector.h:161) so that every draw is addressable by
+// (seed, step, species, facet, slot, attempt) and identical on CPU and GPU.
+// ----------------------------------------------------------------------------
+struct U64x4 { uint64_t v[4]; };
+
+PUNC_HD void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    *lo = a * b;
+    *hi = __umul64hi(a, b);
+#else
+    unsigned __int128 p = (unsigned __int128)a * b;
+    *lo = (uint64_t)p;
+    *hi = (uint64_t)(p >> 64);
+#endif
+}
+
+PUNC_HD U64x4 philox4x64_10(U64x4 c, uint64_t k0, uint64_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ull, c.v[0], &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ull, c.v[2], &hi1, &lo1);
+        U64x4 n;
+        n.v[0] = hi1 ^ c.v[1] ^ k0;
+        n.v[1] = lo1;
+        n.v[2] = hi0 ^ c.v[3] ^ k1;
+        n.v[3] = lo0;
+        c = n;
+        k0 += 0x9E3779B97F4A7C15ull;
+        k1 += 0xBB67AE8584CAA73Bull;
+    }
+    return c;
+}
+
+// Uniform double strictly inside (0, 1): ((x >> 12) + 1/2) * 2^-52.
+PUNC_HD double u01(uint64_t x) {
+    return ((double)(x >> 12) * 2.0 + 1.0) * 0x1.0p-53;
+}
+
+// Stream tags (second key word) keep the draws of different phases disjoint.
+constexpr uint64_t TAG_INJECT_COUNT = 0x494e4a43ull;  // per (species, facet) count rounding
+constexpr uint64_t TAG_INJECT = 0x494e4a50ull;        // per injected candidate
+constexpr uint64_t TAG_LOAD = 0x4c4f4144ull;          // initial loading
+
+// ----------------------------------------------------------------------------
+// Inverse complementary error function.  Initial guess from Abramowitz &
+// Stegun 26.2.23 (normal quantile, |err| < 4.5e-4), then three Halley steps on
+// erfc(x) = y, which converge cubically to full double precision.  Boost's
+// erfc_inv (distributions.cpp:47) is not available here.
+// ----------------------------------------------------------------------------
+PUNC_HD double erfc_inv_approx(double y) {
+    // erfc(-x) = 2 - erfc(x): map y > 1 to 2 - y (an exact subtraction).
+    const double sgn = y > 1.0 ? -1.0 : 1.0;
+    if (y > 1.0) y = 2.0 - y;
+    // p = y/2 is the lower-tail probability of a standard normal at -sqrt(2) x.
+    double p = 0.5 * y;
+    double pp = p < 0.5 ? p : 1.0 - p;
+    double t = sqrt(-2.0 * log(pp));
+    double z = t - (2.515517 + 0.802853 * t + 0.010328 * t * t) /
+                       (1.0 + 1.432788 * t + 0.189269 * t * t + 0.001308 * t * t * t);
+    // z approximates the upper quantile for pp; erfc(x)=y means x = z/sqrt(2) when p<0.5.
+    double x = (p < 0.5 ? z : -z) * 0.70710678118654752440;
+    const double two_over_sqrtpi = 1.12837916709551257390;
+    // Near y = 1 solve erf(x) = 1 - y instead (exact subtraction, full
+    // relative accuracy for small x).
+    const bool central = y >= 0.5 && y <= 1.5;
+    const double zc = 1.0 - y;
+    for (int it = 0; it < 3; ++it) {
+        double f = central ? zc - erf(x) : erfc(x) - y;
+        double fp = -two_over_sqrtpi * exp(-x * x);
+        double d = f / fp;
+        x = x - d / (1.0 + x * d);
+    }
+    return sgn * x;
+}
+
+}  // namespace punc_gpu
+
+namespace punc_gpu {
+
+// Facet-plane projection in the reference's evaluation order
+// (population.h:547-552): proj = ((c0 + c1*x) + c2*y) + c3*z.
+PUNC_HD double plane_proj(const double* p, double x, double y, double z) {
+    return ((p[0] + p[1] * x) + p[2] * y) + p[3] * z;
+}
+
+// Point location through a uniform bucket grid: the lowest-id candidate cell
+// whose four facet planes all give proj <= 0, or -1.  Bucket lists are in
+// ascending cell order and hold every cell whose bounding box meets the bucket,
+// so the result equals a brute-force scan over all cells in id order.
+PUNC_HD int32_t locate_in_grid(const CellPlanes* planes, const int32_t* bucket_start,
+                               const int32_t* bucket_cells, const int* gn, const double* glo,
+                               const double* gh, double x, double y, double z) {
+    double q[3] = {x, y, z};
+    int b[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = floor((q[d] - glo[d]) / gh[d]);
+        int i = (f < 0.0) ? 0 : (f >= (double)gn[d] ? gn[d] - 1 : (int)f);
+        b[d] = i;
+    }
+    int64_t bi = (int64_t)b[0] + (int64_t)gn[0] * ((int64_t)b[1] + (int64_t)gn[1] * b[2]);
+    for (int32_t k = bucket_start[bi]; k < bucket_start[bi + 1]; ++k) {
+        int32_t c = bucket_cells[k];
+        const double* p = planes[c].p;
+        bool inside = true;
+        for (int f = 0; f < 4; ++f) inside = inside && !(plane_proj(p + 4 * f, x, y, z) > 0.0);
+        if (inside) return c;
+    }
+    return -1;
+}
+
+}  // namespace punc_gpu

@@ -1,0 +1,467 @@
+"""Python mirror of the PUNC++ particle-path operator API over libpunc_gpu.so.
+
+Names, argument meaning and error behaviour follow the reference's C++ API
+(punc/include/punc/{population,pusher,distributor,injector,distributions,
+mesh,object,diagnostics}.h); DOLFIN functions become numpy arrays of CG1 dofs
+(E: ndof x 3, rho: ndof).  Every operator runs on the GPU through the C ABI;
+there is no CPU fallback.
+"""
+import enum
+import math
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, dptr, i32ptr, i64ptr, vec3
+
+
+# ---------------------------------------------------------------------------
+# Meshes
+# ---------------------------------------------------------------------------
+class HostMesh:
+    """Raw tetrahedral mesh arrays: xyz (nv,3), cells (nc,4) ascending vertex
+    ids, facet_bnd (nc,4) boundary id of local facet i (opposite vertex i),
+    vdof (nv,) vertex -> CG1 dof, ndof."""
+
+    def __init__(self, xyz, cells, facet_bnd, vdof=None, ndof=None):
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        self.facet_bnd = np.ascontiguousarray(facet_bnd, dtype=np.int32).reshape(-1, 4)
+        nv = self.xyz.shape[0]
+        if vdof is None:
+            vdof = np.arange(nv, dtype=np.int32)
+            ndof = nv
+        self.vdof = np.ascontiguousarray(vdof, dtype=np.int32)
+        self.ndof = int(ndof)
+
+    @property
+    def nv(self):
+        return self.xyz.shape[0]
+
+    @property
+    def nc(self):
+        return self.cells.shape[0]
+
+
+def box_mesh(n, lo, hi, sphere_r=0.0, periodic=False):
+    """n^3 cubes x 6 Kuhn tetrahedra on [lo,hi]; cells with centroid within
+    sphere_r of the origin removed (object boundary 2; exterior is 1)."""
+    L = N.lib()
+    h = N.vp()
+    lo3, hi3 = vec3(lo), vec3(hi)
+    check(L.punc_gpu_meshgen_box(int(n), dptr(lo3), dptr(hi3), float(sphere_r), int(bool(periodic)),
+                                 N.C.byref(h)))
+    try:
+        nv, nc, nd = N.C.c_int64(), N.C.c_int64(), N.C.c_int64()
+        check(L.punc_gpu_hostmesh_sizes(h, N.C.byref(nv), N.C.byref(nc), N.C.byref(nd)))
+        xyz = np.empty((nv.value, 3))
+        cells = np.empty((nc.value, 4), dtype=np.int32)
+        fb = np.empty((nc.value, 4), dtype=np.int32)
+        vdof = np.empty(nv.value, dtype=np.int32)
+        check(L.punc_gpu_hostmesh_copy(h, dptr(xyz), i32ptr(cells), i32ptr(fb), i32ptr(vdof)))
+    finally:
+        L.punc_gpu_hostmesh_free(h)
+    return HostMesh(xyz, cells, fb, vdof, nd.value)
+
+
+class ExteriorFacet:
+    """Exterior facet (reference mesh.h:38-44) plus its owning cell."""
+
+    def __init__(self, area, vertices, normal, basis, cell):
+        self.area = area
+        self.vertices = vertices
+        self.normal = normal
+        self.basis = basis
+        self.cell = cell
+
+
+class Mesh:
+    """Device-resident mesh and localizer (reference Mesh, mesh.h:49-97, plus
+    the localizer part of Population<3>, population.h:310-389)."""
+
+    dim = 3
+    ext_bnd_id = 1
+
+    def __init__(self, host):
+        self.host = host
+        L = N.lib()
+        h = N.vp()
+        check(L.punc_gpu_mesh_create(host.nv, dptr(host.xyz), host.nc, i32ptr(host.cells),
+                                     i32ptr(host.facet_bnd), i32ptr(host.vdof), host.ndof,
+                                     N.C.byref(h)))
+        self._h = h
+        nv, nc, nd, ne = (N.C.c_int64() for _ in range(4))
+        vol = N.C.c_double()
+        check(L.punc_gpu_mesh_info(h, N.C.byref(nv), N.C.byref(nc), N.C.byref(nd), N.C.byref(ne),
+                                   N.C.byref(vol)))
+        self.num_vertices, self.num_cells, self.ndof = nv.value, nc.value, nd.value
+        self.num_exterior = ne.value
+        self._volume = vol.value
+        bnd = set(np.unique(host.facet_bnd[self._boundary_mask()]).tolist())
+        self.num_objects = len([b for b in bnd if b >= 2])
+        self._facets = None
+
+    def _boundary_mask(self):
+        adj = self.localizer()["adj"]
+        return adj < 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().punc_gpu_mesh_destroy(h)
+            self._h = None
+
+    def volume(self):
+        return self._volume
+
+    def domain_size(self):
+        # Mesh::domain_size (mesh.cpp:124-145): per axis the largest coordinate >= 0
+        return [max(0.0, float(self.host.xyz[:, d].max())) for d in range(3)]
+
+    def localizer(self):
+        nc = self.num_cells
+        out = {"adj": np.empty((nc, 4), dtype=np.int64), "planes": np.empty((nc, 16)),
+               "bary": np.empty((nc, 12)), "vcoords": np.empty((nc, 12)),
+               "dofs": np.empty((nc, 4), dtype=np.int32)}
+        check(N.lib().punc_gpu_mesh_localizer(self._h, i64ptr(out["adj"]), dptr(out["planes"]),
+                                              dptr(out["bary"]), dptr(out["vcoords"]),
+                                              i32ptr(out["dofs"])))
+        return out
+
+    def exterior_facet_arrays(self):
+        ne = self.num_exterior
+        a = {"area": np.empty(ne), "vertices": np.empty((ne, 9)), "normal": np.empty((ne, 3)),
+             "basis": np.empty((ne, 9)), "cell": np.empty(ne, dtype=np.int32)}
+        check(N.lib().punc_gpu_exterior_facets(self._h, dptr(a["area"]), dptr(a["vertices"]),
+                                               dptr(a["normal"]), dptr(a["basis"]), i32ptr(a["cell"])))
+        return a
+
+    @property
+    def exterior_facets(self):
+        if self._facets is None:
+            a = self.exterior_facet_arrays()
+            self._facets = [ExteriorFacet(a["area"][j], a["vertices"][j], a["normal"][j],
+                                          a["basis"][j], int(a["cell"][j]))
+                            for j in range(self.num_exterior)]
+        return self._facets
+
+    def locate(self, xs):
+        xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+        out = np.empty(xs.shape[0], dtype=np.int64)
+        check(N.lib().punc_gpu_mesh_locate(self._h, xs.shape[0], dptr(xs), i64ptr(out)))
+        return out
+
+
+def element_volume(mesh, voronoi=True):
+    """Reference distributor.cpp:35-69: (D+1)/patch volume per CG1 dof."""
+    out = np.empty(mesh.ndof)
+    check(N.lib().punc_gpu_element_volume(mesh._h, int(bool(voronoi)), dptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Distributions and species
+# ---------------------------------------------------------------------------
+class UniformPosition:
+    """Reference distributions.h:186-221."""
+
+    amp = 0.0
+    kw = 0.0
+    axis = 0
+
+    def __init__(self, mesh=None):
+        self.mesh = mesh
+
+    def max(self):
+        return 1.0
+
+
+class CosinePosition(UniformPosition):
+    """inside(x) * (1 + amp*cos(kw*x[axis])): a density-perturbed uniform
+    position pdf (the Langmuir-wave pattern of demo/langmuir/langmuir.cpp:17-54)."""
+
+    def __init__(self, mesh=None, amp=0.01, kw=1.0, axis=0):
+        super().__init__(mesh)
+        self.amp = float(amp)
+        self.kw = float(kw)
+        self.axis = int(axis)
+
+    def max(self):
+        return 1.0 + self.amp
+
+
+class Maxwellian:
+    """Reference distributions.h:233-249, distributions.cpp:23-77."""
+
+    def __init__(self, vth, vd, vdf_range=5.0):
+        self._vth = float(vth)
+        self._vd = vec3(vd)
+        self.vdf_range = float(vdf_range)
+        self.dim = 3
+        self.num_particles = None
+        self.pdf_max = None
+
+    def vth(self):
+        return self._vth
+
+    def vd(self):
+        return self._vd.copy()
+
+
+class ParticleAmountType(enum.Enum):
+    per_cell = 0
+    in_total = 1
+    per_volume = 2
+    phys_per_sim = 3
+
+
+class Species:
+    """Reference population.h:99-125, population.cpp:41-70 (the amount-type
+    fall-through converting the amount into a statistical weight)."""
+
+    def __init__(self, charge, mass, density, amount, type, mesh, pdf, vdf, eps0=1.0):
+        t = ParticleAmountType(type)
+        if t == ParticleAmountType.per_cell:
+            amount *= mesh.num_cells
+        if t in (ParticleAmountType.per_cell, ParticleAmountType.in_total):
+            amount /= mesh.volume()
+        if t in (ParticleAmountType.per_cell, ParticleAmountType.in_total,
+                 ParticleAmountType.per_volume):
+            amount = density / amount
+        self.weight = amount
+        self.q = charge * amount
+        self.m = mass * amount
+        self.n = density / amount
+        self.num = int(self.n * mesh.volume())
+        self.pdf = pdf
+        self.vdf = vdf
+        self.debye = math.sqrt(eps0 * self.m / (self.n * self.q * self.q)) * vdf.vth()
+
+
+def create_flux(species, mesh):
+    """Reference injector.cpp:104-171 (Maxwellian: closed-form flux number and
+    flux-pdf maximum per exterior facet)."""
+    for s in species:
+        ne = mesh.num_exterior
+        num, pmax = np.empty(ne), np.empty(ne)
+        vd = s.vdf.vd()
+        check(N.lib().punc_gpu_maxwellian_flux(mesh._h, s.vdf.vth(), dptr(vd), dptr(num), dptr(pmax)))
+        s.vdf.num_particles = num
+        s.vdf.pdf_max = pmax
+
+
+class Object:
+    """Reference object.h:80-130 (charge/current bookkeeping only)."""
+
+    def __init__(self, bnd_id):
+        self.bnd_id = int(bnd_id)
+        self.charge = 0.0
+        self.current = 0.0
+        self.potential = 0.0
+
+    def set_potential(self, p):
+        self.potential = p
+
+    def get_potential(self):
+        return self.potential
+
+
+# ---------------------------------------------------------------------------
+# Population
+# ---------------------------------------------------------------------------
+class Population:
+    """Device-resident population (reference Population<3>,
+    population.h:236-308).  species: list of Species (or of (q, m) pairs)."""
+
+    g_dim = 3
+
+    def __init__(self, mesh, species, capacity=None, stream=None):
+        self.mesh = mesh
+        qs, ms = [], []
+        for s in species:
+            q, m = (s.q, s.m) if hasattr(s, "q") else s
+            qs.append(float(q))
+            ms.append(float(m))
+        self.q = np.array(qs)
+        self.m = np.array(ms)
+        self.nspecies = len(qs)
+        if capacity is None:
+            capacity = 1 << 20
+        self.capacity = int(capacity)
+        h = N.vp()
+        check(N.lib().punc_gpu_pop_create(mesh._h, self.nspecies, dptr(self.q), dptr(self.m),
+                                          self.capacity, stream, N.C.byref(h)))
+        self._h = h
+        self._inj_set = set()
+        self.dv_inv_uploaded = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().punc_gpu_pop_destroy(h)
+            self._h = None
+
+    def species_index(self, q, m):
+        for s in range(self.nspecies):
+            if self.q[s] == q and self.m[s] == m:
+                return s
+        raise ValueError(f"no species with q={q}, m={m} in this population")
+
+    def add_particles(self, xs, vs, q, m):
+        """Reference population.h:411-435; points outside the mesh are skipped.
+        Returns the number added."""
+        s = self.species_index(q, m)
+        xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+        vs = np.ascontiguousarray(vs, dtype=np.float64).reshape(-1, 3)
+        if xs.shape != vs.shape:
+            raise ValueError("xs and vs differ in shape")
+        k = N.C.c_int64()
+        check(N.lib().punc_gpu_pop_add(self._h, s, xs.shape[0], dptr(xs), dptr(vs), N.C.byref(k)))
+        return k.value
+
+    def add_located(self, xs, vs, species, cells):
+        xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+        vs = np.ascontiguousarray(vs, dtype=np.float64).reshape(-1, 3)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        cl = np.ascontiguousarray(cells, dtype=np.int64)
+        check(N.lib().punc_gpu_pop_add_located(self._h, xs.shape[0], dptr(xs), dptr(vs), i32ptr(sp),
+                                               i64ptr(cl)))
+
+    def locate(self, x):
+        return int(self.mesh.locate(np.asarray(x, dtype=np.float64).reshape(1, 3))[0])
+
+    def size(self):
+        n = N.C.c_int64()
+        check(N.lib().punc_gpu_pop_size(self._h, N.C.byref(n)))
+        return n.value
+
+    def counts(self):
+        out = np.empty(self.nspecies, dtype=np.int64)
+        check(N.lib().punc_gpu_pop_counts(self._h, i64ptr(out)))
+        return out
+
+    def num_of_particles(self):
+        return int(self.counts().sum())
+
+    def num_of_positives(self):
+        c = self.counts()
+        return int(c[self.q > 0].sum())
+
+    def num_of_negatives(self):
+        c = self.counts()
+        return int(c[self.q < 0].sum())
+
+    def get_particles(self):
+        n = self.size()
+        xs, vs = np.empty((n, 3)), np.empty((n, 3))
+        sp = np.empty(n, dtype=np.int32)
+        cl = np.empty(n, dtype=np.int64)
+        check(N.lib().punc_gpu_pop_get(self._h, dptr(xs), dptr(vs), i32ptr(sp), i64ptr(cl)))
+        return {"x": xs, "v": vs, "species": sp, "cell": cl}
+
+    def update(self, objects=(), dt=1.0):
+        """Reference population.h:641-699: relocate, remove leavers, and per
+        object current = sum q of absorbed particles; charge += current;
+        current /= dt.  Returns absorbed counts (nspecies x 64)."""
+        ab = np.zeros((self.nspecies, N_MAX_BND), dtype=np.int64)
+        check(N.lib().punc_gpu_update(self._h, i64ptr(ab)))
+        _objects_update(objects, ab, self.q, dt)
+        return ab
+
+    def sort(self):
+        check(N.lib().punc_gpu_sort(self._h))
+
+    def set_launch(self, grid, block=256):
+        check(N.lib().punc_gpu_set_launch(self._h, int(grid), int(block)))
+
+    def synchronize(self):
+        check(N.lib().punc_gpu_synchronize(self._h))
+
+
+N_MAX_BND = 64
+
+
+def _objects_update(objects, absorbed, q, dt):
+    for o in objects:
+        o.current = 0.0
+    for o in objects:
+        o.current = float(np.dot(absorbed[:, o.bnd_id].astype(np.float64), q))
+        o.charge += o.current
+        o.current /= dt
+
+
+# ---------------------------------------------------------------------------
+# Operators (reference pusher.h, distributor.h, injector.h, diagnostics.h)
+# ---------------------------------------------------------------------------
+def _set_field(pop, E):
+    E = np.ascontiguousarray(E, dtype=np.float64).reshape(pop.mesh.ndof, 3)
+    check(N.lib().punc_gpu_set_field(pop._h, dptr(E)))
+
+
+def accel_cg1(pop, E, dt):
+    """pusher.h:154-201.  E: CG1 dofs (ndof, 3).  Returns the mid-step KE."""
+    _set_field(pop, E)
+    ke = N.C.c_double()
+    check(N.lib().punc_gpu_accel(pop._h, float(dt), N.C.byref(ke)))
+    return ke.value
+
+
+def boris_cg1(pop, E, B, dt):
+    """pusher.h:282-372."""
+    _set_field(pop, E)
+    ke = N.C.c_double()
+    check(N.lib().punc_gpu_boris(pop._h, dptr(vec3(B)), float(dt), N.C.byref(ke)))
+    return ke.value
+
+
+def move(pop, dt):
+    """pusher.h:648-662."""
+    check(N.lib().punc_gpu_move(pop._h, float(dt)))
+
+
+def move_periodic(pop, dt, Ld):
+    """pusher.h:666-681."""
+    check(N.lib().punc_gpu_move_periodic(pop._h, float(dt), dptr(vec3(Ld))))
+
+
+def distribute_cg1(pop, dv_inv, rho=None):
+    """distributor.h:168-210.  Returns rho (ndof); fills `rho` when given."""
+    out = rho if rho is not None else np.empty(pop.mesh.ndof)
+    dv = np.ascontiguousarray(dv_inv, dtype=np.float64)
+    check(N.lib().punc_gpu_distribute(pop._h, dptr(dv), dptr(out)))
+    return out
+
+
+def kinetic_energy(pop):
+    """diagnostics.h:248-265."""
+    ke = N.C.c_double()
+    check(N.lib().punc_gpu_kinetic_energy(pop._h, N.C.byref(ke)))
+    return ke.value
+
+
+def _register_injectors(pop, species):
+    for s in species:
+        idx = pop.species_index(s.q, s.m)
+        if idx in pop._inj_set:
+            continue
+        check(N.lib().punc_gpu_injector_set(pop._h, idx, s.vdf.vth(), dptr(s.vdf.vd()), float(s.n),
+                                            float(s.vdf.vdf_range)))
+        pop._inj_set.add(idx)
+
+
+def inject_particles(pop, species, facets=None, dt=1.0, seed=0, step=0):
+    """injector.h:157-232 with a counter-based stream keyed by (seed, step).
+    Returns the number of particles added."""
+    _register_injectors(pop, species)
+    k = N.C.c_int64()
+    check(N.lib().punc_gpu_inject(pop._h, float(dt), int(seed), int(step), N.C.byref(k)))
+    return k.value
+
+
+def load_particles(pop, species, seed=0):
+    """injector.h:244-278: species.num particles per species."""
+    for s in species:
+        idx = pop.species_index(s.q, s.m)
+        pdf = s.pdf
+        check(N.lib().punc_gpu_pop_load(pop._h, idx, int(s.num), int(seed), float(pdf.amp),
+                                        float(pdf.kw), int(pdf.axis), s.vdf.vth(), dptr(s.vdf.vd())))

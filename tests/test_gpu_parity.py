@@ -1,0 +1,237 @@
+"""GPU parity: every particle operator through the C ABI against the oracle on
+the same seeded inputs.  Cells, counts and absorbed counts must be bit-exact;
+positions and velocities within 1e-12 relative (expected: bit-identical, the
+library is built without FMA contraction); KE within 1e-11 relative; rho and
+object charge within 1e-11 of a magnitude scale (atomic summation order)."""
+import numpy as np
+import pytest
+
+import puncpp_amd as P
+from helpers import (MI, ME, QE, QI, assert_same_sets, loaded_pair, near_face, periodic_mesh,
+                     rho_scale, sphere_mesh, synthetic_field)
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def test_initial_state_roundtrip(gpu):
+    _print_case("roundtrip")
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (1500, 1500))
+    assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what="roundtrip")
+    c = pop.counts()
+    assert c.tolist() == [1500, 1500]
+    assert pop.num_of_negatives() == o.num_of_negatives()
+    assert pop.num_of_positives() == o.num_of_positives()
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_accel_cg1(gpu, half):
+    _print_case("accel", half=half)
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (3000, 3000))
+    E = synthetic_field(hm)
+    dt = 0.05 * (0.5 if half else 1.0)
+    ke_g = P.accel_cg1(pop, E, dt)
+    ke_o = o.accel(E, dt)
+    a, b = pop.get_particles(), o.get_particles()
+    # accel does not reorder: arrays correspond element-wise
+    np.testing.assert_array_equal(a["v"], b["v"])
+    np.testing.assert_array_equal(a["x"], b["x"])
+    assert abs(ke_g - ke_o) <= 1e-11 * abs(ke_o)
+
+
+def test_boris_cg1(gpu):
+    _print_case("boris")
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (3000, 3000))
+    E = synthetic_field(hm)
+    B = [0.0, 0.0, 1.0]
+    ke_g = P.boris_cg1(pop, E, B, 0.02)
+    ke_o = o.boris(E, B, 0.02)
+    a, b = pop.get_particles(), o.get_particles()
+    np.testing.assert_array_equal(a["v"], b["v"])
+    assert abs(ke_g - ke_o) <= 1e-11 * abs(ke_o)
+    # oblique B exercises every component of t and s
+    B2 = [0.3, -0.7, 1.1]
+    ke_g = P.boris_cg1(pop, E, B2, 0.03)
+    ke_o = o.boris(E, B2, 0.03)
+    np.testing.assert_array_equal(pop.get_particles()["v"], o.get_particles()["v"])
+    assert abs(ke_g - ke_o) <= 1e-11 * abs(ke_o)
+
+
+def test_kinetic_energy(gpu):
+    _print_case("kinetic_energy")
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (2000, 2500))
+    ke_g, ke_o = P.kinetic_energy(pop), o.kinetic_energy()
+    assert abs(ke_g - ke_o) <= 1e-11 * abs(ke_o)
+
+
+@pytest.mark.parametrize("steps", [1, 4])
+def test_move_update(gpu, steps):
+    _print_case("move_update", steps=steps)
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (4000, 4000), vth=(1.0, 0.3))
+    objects = [P.Object(2)]
+    total = np.zeros((2, 64), dtype=np.int64)
+    for k in range(steps):
+        P.move(pop, 0.2)
+        o.move(0.2)
+        ab_g = pop.update(objects, 0.2)
+        ab_o, ch_o, lost = o.update()
+        assert lost == 0
+        np.testing.assert_array_equal(ab_g, ab_o)
+        total += ab_o
+        a, b = pop.get_particles(), o.get_particles()
+        assert_same_sets(a, b, rel=0.0, what=f"step {k}")
+    assert total[:, 1].sum() > 0, "test should see exterior leavers"
+    assert total[:, 2].sum() > 0, "test should see object absorption"
+    print(f"near-face particles (<1e-12): {near_face(o, o.get_particles())}", flush=True)
+    # object charge bookkeeping (object.h:118-121, population.h:695-698)
+    assert objects[0].charge == pytest.approx(float(total[0, 2] * QE + total[1, 2] * QI), rel=1e-12)
+
+
+def test_move_periodic_update(gpu):
+    _print_case("move_periodic")
+    hm = periodic_mesh(6)
+    o, mesh, pop = loaded_pair(hm, (3000, 3000), vth=(1.0, 0.5), vd=(0.0, 0.0, 0.0),
+                               amp=0.01, kw=1.0)
+    Ld = [2 * np.pi] * 3
+    for k in range(3):
+        P.move_periodic(pop, 0.3, Ld)
+        o.move_periodic(0.3, Ld)
+        ab_g = pop.update([], 0.3)
+        ab_o, _, lost = o.update()
+        assert lost == 0
+        assert ab_g.sum() == 0 and ab_o.sum() == 0
+        assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what=f"periodic step {k}")
+
+
+def test_distribute_cg1(gpu):
+    _print_case("distribute")
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (5000, 5000))
+    dv = P.element_volume(mesh)
+    rho_g = P.distribute_cg1(pop, dv)
+    rho_o = o.distribute(dv)
+    scale = rho_scale(o, hm, dv)
+    err = np.abs(rho_g - rho_o)
+    assert np.all(err <= 1e-11 * np.maximum(scale, 1e-300)), err.max()
+    # charge conservation: sum rho_j / dv_inv_j = sum q
+    tot = np.sum(rho_g / dv)
+    assert tot == pytest.approx(5000 * QE + 5000 * QI, abs=1e-9)
+
+
+def test_distribute_periodic(gpu):
+    _print_case("distribute_periodic")
+    hm = periodic_mesh(5)
+    o, mesh, pop = loaded_pair(hm, (4000, 3000), vd=(0.0, 0.0, 0.0))
+    dv = P.element_volume(mesh)
+    rho_g, rho_o = P.distribute_cg1(pop, dv), o.distribute(dv)
+    scale = rho_scale(o, hm, dv)
+    assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
+
+
+def test_inject(gpu):
+    _print_case("inject")
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    o, mesh, pop = loaded_pair(hm, (10, 10))
+    vth = [1.0, 1.0 / np.sqrt(1836.0)]
+    vd = [0.0, 0.0, 0.5]
+    n_dens = [40.0, 40.0]
+    species = []
+    for s, (q, m) in enumerate([(QE, ME), (QI, MI)]):
+        sp = P.Species(q, m, n_dens[s], 1.0, P.ParticleAmountType.phys_per_sim, mesh,
+                       P.UniformPosition(mesh), P.Maxwellian(vth[s], vd))
+        species.append(sp)
+        o.set_injector(s, vth[s], vd, sp.n, 5.0)
+    dt = 0.1
+    for step in range(3):
+        k_g = P.inject_particles(pop, species, None, dt, seed=7, step=step)
+        k_o = o.inject(dt, 7, step)
+        assert k_g == k_o, (k_g, k_o)
+        assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what=f"inject {step}")
+    assert k_o > 0
+
+
+def test_device_load_matches_oracle(gpu):
+    _print_case("device_load")
+    hm = sphere_mesh()
+    import oracle as O
+    o = O.OraclePop(hm, [QE, QI], [ME, MI])
+    mesh = P.Mesh(hm)
+    pop = P.Population(mesh, [(QE, ME), (QI, MI)], capacity=20000)
+    vth = [1.0, 0.05]
+    for s in range(2):
+        assert o.load(s, 3000, 11, 0.01, 1.0, 0, vth[s], [0.0, 0.0, 0.5]) == 0
+        P.punc._native.check(P.punc._native.lib().punc_gpu_pop_load(
+            pop._h, s, 3000, 11, 0.01, 1.0, 0, vth[s], P.punc.dptr(np.array([0.0, 0.0, 0.5]))))
+    a, b = pop.get_particles(), o.get_particles()
+    # positions are exact; velocities go through erfc_inv (1-2 ulp between implementations)
+    ca, cb = a, b
+    order_a = np.lexsort((ca["x"][:, 2], ca["x"][:, 1], ca["x"][:, 0], ca["species"]))
+    order_b = np.lexsort((cb["x"][:, 2], cb["x"][:, 1], cb["x"][:, 0], cb["species"]))
+    np.testing.assert_array_equal(ca["x"][order_a], cb["x"][order_b])
+    np.testing.assert_array_equal(ca["cell"][order_a], cb["cell"][order_b])
+    va, vb = ca["v"][order_a], cb["v"][order_b]
+    assert np.max(np.abs(va - vb) / np.maximum(np.abs(vb), 1e-300)) <= 1e-12
+
+
+def test_sort_preserves_set(gpu):
+    _print_case("sort")
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (3000, 3000))
+    before = pop.get_particles()
+    pop.sort()
+    after = pop.get_particles()
+    assert_same_sets(after, before, rel=0.0, what="sort")
+    assert np.all(np.diff(after["cell"]) >= 0)
+
+
+@pytest.mark.parametrize("pusher", [1, 2])
+def test_fused_step_matches_sequence(gpu, pusher):
+    """punc_gpu_step (push+move+update+inject+deposit) against the oracle's
+    reference-ordered sequence (interaction.cpp:384-422, then distribute)."""
+    _print_case("fused_step", pusher=pusher)
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    o, mesh, pop = loaded_pair(hm, (4000, 4000))
+    E = synthetic_field(hm, scale=0.2)
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    vth = [1.0, 1.0 / np.sqrt(1836.0)]
+    vd = [0.0, 0.0, 0.5]
+    species = [P.Species(q, m, 30.0, 1.0, P.ParticleAmountType.phys_per_sim, mesh,
+                         P.UniformPosition(mesh), P.Maxwellian(vth[s], vd))
+               for s, (q, m) in enumerate([(QE, ME), (QI, MI)])]
+    P.punc._register_injectors(pop, species)
+    for s in range(2):
+        o.set_injector(s, vth[s], vd, species[s].n, 5.0)
+    L = P.punc._native
+    L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
+    L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
+    dt = 0.05
+    for n in range(4):
+        push_dt = (1.0 - 0.5 * (n == 0)) * dt
+        L.check(L.lib().punc_gpu_step(pop._h, pusher, push_dt, L.dptr(B), dt, None, 1, 5, n))
+        ke = L.C.c_double()
+        ab = np.zeros((2, 64), dtype=np.int64)
+        kin = L.C.c_int64()
+        L.check(L.lib().punc_gpu_step_results(pop._h, L.C.byref(ke), L.i64ptr(ab), L.C.byref(kin)))
+        ke_o = o.accel(E, push_dt) if pusher == 1 else o.boris(E, B, push_dt)
+        o.move(dt)
+        ab_o, _, lost = o.update()
+        k_o = o.inject(dt, 5, n)
+        assert lost == 0
+        assert abs(ke.value - ke_o) <= 1e-11 * abs(ke_o)
+        np.testing.assert_array_equal(ab, ab_o)
+        assert kin.value == k_o
+        assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what=f"fused step {n}")
+        rho_g = np.empty(hm.ndof)
+        L.check(L.lib().punc_gpu_get_rho(pop._h, L.dptr(rho_g)))
+        rho_o = o.distribute(dv)
+        scale = rho_scale(o, hm, dv)
+        assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
