@@ -1,0 +1,137 @@
+// Drop-in for punc/include/punc/mesh.h (PUNC++) on the MI355X particle path.
+//
+// punc::Mesh holds a tetrahedral mesh and its device-side localizer.  The
+// reference loads a DOLFIN mesh from file (mesh.cpp:41-92); DOLFIN is not part
+// of this library, so a Mesh is built from plain arrays (vertex coordinates,
+// ascending cell vertex ids, per-(cell, local facet) boundary ids, vertex ->
+// CG1 dof map) or by the structured box generator.  Boundary ids follow the
+// reference (mesh.h:61-71): 0 interior, 1 exterior, 2.. objects.
+#ifndef PUNC_GPU_MESH_H
+#define PUNC_GPU_MESH_H
+
+#include <punc_gpu.h>
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace punc {
+
+//! Throws std::runtime_error with the library's message on a nonzero status.
+inline void check(int rc) {
+    if (rc != 0) throw std::runtime_error(punc_gpu_last_error());
+}
+
+//! Exterior facet (reference mesh.h:38-44) plus its owning cell.
+struct ExteriorFacet {
+    double area;                   ///< Area of the facet
+    std::vector<double> vertices;  ///< 3x3 vertex coordinates, ascending vertex ids
+    std::vector<double> normal;    ///< Inward unit normal
+    std::vector<double> basis;     ///< Orthonormal basis, first column = normal
+    int cell;                      ///< The adjacent cell
+};
+
+class Mesh {
+public:
+    using size_t = std::size_t;
+
+    size_t dim = 3;          ///< Geometric dimension (3 only)
+    size_t ext_bnd_id = 1;   ///< Id of the exterior boundary
+    size_t num_objects = 0;  ///< Number of objects
+    std::vector<ExteriorFacet> exterior_facets;
+
+    std::vector<double> xyz;          ///< nv*3
+    std::vector<int32_t> cells;       ///< nc*4
+    std::vector<int32_t> facet_bnd;   ///< nc*4
+    std::vector<int32_t> vdof;        ///< nv
+    int64_t ndof = 0;
+
+    Mesh(std::vector<double> xyz_, std::vector<int32_t> cells_, std::vector<int32_t> facet_bnd_,
+         std::vector<int32_t> vdof_ = {}, int64_t ndof_ = -1)
+        : xyz(std::move(xyz_)), cells(std::move(cells_)), facet_bnd(std::move(facet_bnd_)),
+          vdof(std::move(vdof_)), ndof(ndof_) {
+        const int64_t nv = (int64_t)xyz.size() / 3;
+        if (vdof.empty()) {
+            vdof.resize(nv);
+            for (int64_t v = 0; v < nv; ++v) vdof[v] = (int32_t)v;
+            ndof = nv;
+        }
+        punc_mesh_t* h = nullptr;
+        check(punc_gpu_mesh_create(nv, xyz.data(), (int64_t)cells.size() / 4, cells.data(),
+                                   facet_bnd.data(), vdof.data(), ndof, &h));
+        handle_.reset(h, punc_gpu_mesh_destroy);
+        init_info();
+    }
+
+    //! Structured box of n^3 cubes x 6 tetrahedra, optional spherical object.
+    static Mesh box(int n, std::array<double, 3> lo, std::array<double, 3> hi, double sphere_r = 0.0,
+                    bool periodic = false) {
+        punc_hostmesh_t* hm = nullptr;
+        check(punc_gpu_meshgen_box(n, lo.data(), hi.data(), sphere_r, periodic ? 1 : 0, &hm));
+        int64_t nv, nc, nd;
+        punc_gpu_hostmesh_sizes(hm, &nv, &nc, &nd);
+        std::vector<double> x(nv * 3);
+        std::vector<int32_t> c(nc * 4), b(nc * 4), d(nv);
+        punc_gpu_hostmesh_copy(hm, x.data(), c.data(), b.data(), d.data());
+        punc_gpu_hostmesh_free(hm);
+        return Mesh(std::move(x), std::move(c), std::move(b), std::move(d), nd);
+    }
+
+    int64_t num_vertices() const { return (int64_t)xyz.size() / 3; }
+    int64_t num_cells() const { return (int64_t)cells.size() / 4; }
+
+    //! Mesh::volume (mesh.cpp:147-164).
+    double volume() const { return volume_; }
+
+    //! Mesh::domain_size (mesh.cpp:124-145).
+    std::vector<double> domain_size() const {
+        std::vector<double> Ld(3, 0.0);
+        for (size_t i = 0; i < xyz.size(); ++i)
+            if (xyz[i] >= Ld[i % 3]) Ld[i % 3] = xyz[i];
+        return Ld;
+    }
+
+    punc_mesh_t* handle() const { return handle_.get(); }
+
+private:
+    std::shared_ptr<punc_mesh_t> handle_;
+    double volume_ = 0.0;
+
+    void init_info() {
+        int64_t nv, nc, nd, ne;
+        check(punc_gpu_mesh_info(handle(), &nv, &nc, &nd, &ne, &volume_));
+        std::vector<double> area(ne), verts(ne * 9), normal(ne * 3), basis(ne * 9);
+        std::vector<int32_t> cell(ne);
+        check(punc_gpu_exterior_facets(handle(), area.data(), verts.data(), normal.data(), basis.data(),
+                                       cell.data()));
+        exterior_facets.clear();
+        for (int64_t j = 0; j < ne; ++j)
+            exterior_facets.push_back(ExteriorFacet{area[j],
+                                                    {verts.begin() + 9 * j, verts.begin() + 9 * j + 9},
+                                                    {normal.begin() + 3 * j, normal.begin() + 3 * j + 3},
+                                                    {basis.begin() + 9 * j, basis.begin() + 9 * j + 9},
+                                                    cell[j]});
+        std::vector<int64_t> adj(nc * 4);
+        check(punc_gpu_mesh_localizer(handle(), adj.data(), nullptr, nullptr, nullptr, nullptr));
+        std::vector<char> seen(64, 0);
+        for (int64_t a : adj)
+            if (a < -1 && a > -64) seen[-a] = 1;
+        num_objects = 0;
+        for (char s : seen) num_objects += s;
+    }
+};
+
+//! A CG1 function on a mesh: ndof values per component (the stand-in for
+//! df::Function on the particle path; E has 3 components, rho has 1).
+class Function {
+public:
+    Function(const Mesh& mesh, int value_dim) : value_dim(value_dim), values(mesh.ndof * value_dim, 0.0) {}
+    int value_dim;
+    std::vector<double> values;  ///< dof-major: values[dof*value_dim + component]
+};
+
+}  // namespace punc
+
+#endif

@@ -90,10 +90,15 @@ int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const double vd[3
 /* ---- population -------------------------------------------------------- */
 /* A population of nspecies species with weighted charge q[s] and mass m[s]
  * (Species::q, Species::m, population.cpp:61-64), room for capacity particles.
+ * nspecies may be 0; species can be added later (up to 128).
  * stream: a hipStream_t, or NULL to let the library create one. */
 int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q, const double* mass,
                         int64_t capacity, void* stream, punc_pop_t** out);
 void punc_gpu_pop_destroy(punc_pop_t* p);
+/* Registers species (q, mass) and returns its index (the existing index when
+ * already registered).  Population<3> registers species on first use, as the
+ * reference's particles carry q and m (population.h:74-94). */
+int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, int* index);
 /* add_particles(xs, vs, q, m) (population.h:411-435): locates each point and
  * appends it as species s; points outside the mesh are skipped (the reference
  * indexes out of bounds there) and *n_added tells how many were kept. */
@@ -145,6 +150,9 @@ int punc_gpu_sort(punc_pop_t* p);
  * (Species::n) and rejection box half width in units of vth (Pdf::vdf_range). */
 int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
                           double density, double vrange);
+/* Restricts species s to inject through exterior facets [begin, end) (one
+ * instance of a batched mesh); by default a species uses every exterior facet. */
+int punc_gpu_injector_facets(punc_pop_t* p, int species, int64_t begin, int64_t end);
 /* inject_particles(pop, species, facets, dt) (injector.h:157-232) with a
  * counter-based stream keyed by (seed, step).  n_injected may be NULL. */
 int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected);
@@ -158,6 +166,13 @@ int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int6
  * by punc_gpu_step_results / punc_gpu_get_rho. */
 int punc_gpu_step(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
                   const double* Ld, int inject, uint64_t seed, uint64_t step);
+/* Captures one punc_gpu_step (same arguments) into a HIP graph; each replay
+ * advances the injection step index, starting at first_step.  All buffers are
+ * prepared before the capture; the graph holds kernel nodes only. */
+int punc_gpu_step_graph(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                        const double* Ld, int inject, uint64_t seed, uint64_t first_step);
+/* Replays the captured step count times on the population's stream. */
+int punc_gpu_graph_launch(punc_pop_t* p, int count);
 /* KE of the last push, absorbed counts since the last read (nspecies*64), and
  * particles injected by the last step.  Any pointer may be NULL. */
 int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t* n_injected);

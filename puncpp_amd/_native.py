@@ -37,6 +37,7 @@ _SIGS = {
     "punc_gpu_maxwellian_flux": (C.c_int, [vp, C.c_double, d_p, d_p, d_p]),
     "punc_gpu_pop_create": (C.c_int, [vp, C.c_int, d_p, d_p, C.c_int64, vp, C.POINTER(vp)]),
     "punc_gpu_pop_destroy": (None, [vp]),
+    "punc_gpu_pop_add_species": (C.c_int, [vp, C.c_double, C.c_double, i32_p]),
     "punc_gpu_pop_add": (C.c_int, [vp, C.c_int, C.c_int64, d_p, d_p, i64_p]),
     "punc_gpu_pop_add_located": (C.c_int, [vp, C.c_int64, d_p, d_p, i32_p, i64_p]),
     "punc_gpu_pop_load": (C.c_int, [vp, C.c_int, C.c_int64, C.c_uint64, C.c_double, C.c_double,
@@ -54,7 +55,11 @@ _SIGS = {
     "punc_gpu_kinetic_energy": (C.c_int, [vp, d_p]),
     "punc_gpu_sort": (C.c_int, [vp]),
     "punc_gpu_injector_set": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
+    "punc_gpu_injector_facets": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int64]),
     "punc_gpu_inject": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_uint64, i64_p]),
+    "punc_gpu_step_graph": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, d_p, C.c_int,
+                                      C.c_uint64, C.c_uint64]),
+    "punc_gpu_graph_launch": (C.c_int, [vp, C.c_int]),
     "punc_gpu_step": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, d_p, C.c_int, C.c_uint64,
                                 C.c_uint64]),
     "punc_gpu_step_results": (C.c_int, [vp, d_p, i64_p, i64_p]),

@@ -73,6 +73,7 @@ struct punc_pop_s {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int nsp = 0;
+    int max_sp = 0;
     std::vector<double> q, m;
     int cell_bits = 0;
     int64_t capacity = 0;
@@ -102,8 +103,15 @@ struct punc_pop_s {
     int64_t* d_bsum = nullptr;
     int64_t* d_total = nullptr;  // [0] scan total, [1] injection candidates, [2] kept
     int64_t bsum_len = 0;
+    // captured step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     // injection
     std::vector<int> inj_on;
+    std::vector<int64_t> inj_range;  // per species [begin, end) of exterior facets, -1: all
+    int32_t* d_pair_sp = nullptr;
+    int32_t* d_pair_f = nullptr;
+    int64_t npairs = 0;
     std::vector<SpInject> spi;
     std::vector<double> flux_num, flux_max;  // nsp*nf (per unit density and time)
     SpInject* d_spi = nullptr;
@@ -339,7 +347,12 @@ extern "C" int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const 
 // ---------------------------------------------------------------------------
 // Population
 // ---------------------------------------------------------------------------
+static void graph_free(punc_pop_s* P);
+
 static void pop_free(punc_pop_s* P) {
+    graph_free(P);
+    hipFree(P->d_pair_sp);
+    hipFree(P->d_pair_f);
     if (P->d_state) hipFree(P->d_state);
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 7; ++k) hipFree(P->bufs[b][k]);
@@ -378,22 +391,23 @@ static void pop_free(punc_pop_s* P) {
 
 extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q, const double* mass,
                                    int64_t capacity, void* stream, punc_pop_t** out) {
-    if (!m || !q || !mass || !out) return fail("pop_create: null argument");
+    if (!m || !out || (nspecies > 0 && (!q || !mass))) return fail("pop_create: null argument");
     {
         int r0 = mesh_ensure_device(m);
         if (r0) return r0;
     }
-    if (nspecies < 1 || nspecies > 128) return fail("pop_create: nspecies must be in [1, 128]");
+    if (nspecies < 0 || nspecies > MAX_SPECIES) return fail("pop_create: nspecies must be in [0, 128]");
     if (capacity < 1 || capacity >= ((int64_t)1 << 31)) return fail("pop_create: capacity must be in [1, 2^31)");
     int cell_bits = 1;
     while (((int64_t)1 << cell_bits) < m->topo.nc) ++cell_bits;
-    int sp_bits = 0;
-    while ((1 << sp_bits) < nspecies) ++sp_bits;
-    if (cell_bits + sp_bits > 31) return fail("pop_create: too many cells x species for the packed cell field");
+    // The packed cell field keeps 31 - cell_bits bits for the species index.
+    const int max_sp = (int)std::min<int64_t>(MAX_SPECIES, (int64_t)1 << std::max(0, 31 - cell_bits));
+    if (nspecies > max_sp) return fail("pop_create: too many cells x species for the packed cell field");
     auto* P = new punc_pop_s();
     P->mesh = m;
     P->device = m->device;
     P->nsp = nspecies;
+    P->max_sp = max_sp;
     P->q.assign(q, q + nspecies);
     P->m.assign(mass, mass + nspecies);
     P->cell_bits = cell_bits;
@@ -426,8 +440,8 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_holes, capacity));
     PCHK(dalloc(&P->d_H, capacity));
     PCHK(dalloc(&P->d_S, capacity));
-    PCHK(dalloc(&P->d_absorbed, (size_t)nspecies * MAX_BND));
-    PCHK(dalloc(&P->d_cnt, nspecies));
+    PCHK(dalloc(&P->d_absorbed, (size_t)MAX_SPECIES * MAX_BND));
+    PCHK(dalloc(&P->d_cnt, MAX_SPECIES));
     PCHK(dalloc(&P->d_errs, 4));
     const int64_t ndof = m->topo.ndof;
     PCHK(dalloc(&P->d_E, ndof));
@@ -456,7 +470,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     }
     s.capacity = capacity;
     PCHK(hipMemcpy(P->d_state, &s, sizeof(s), hipMemcpyHostToDevice));
-    PCHK(hipMemset(P->d_absorbed, 0, (size_t)nspecies * MAX_BND * 8));
+    PCHK(hipMemset(P->d_absorbed, 0, (size_t)MAX_SPECIES * MAX_BND * 8));
     PCHK(hipMemset(P->d_errs, 0, 4 * 8));
     PCHK(hipMemset(P->d_E, 0, ndof * sizeof(double4)));
     PCHK(hipMemset(P->d_rho_raw, 0, ndof * 8));
@@ -464,6 +478,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipMemset(P->d_ke, 0, 4 * 8));
     PCHK(hipMemset(P->d_total, 0, 8 * 8));
     P->inj_on.assign(nspecies, 0);
+    P->inj_range.assign(2 * nspecies, -1);
     P->spi.assign(nspecies, SpInject{});
     const size_t nf = m->topo.ext.size();
     P->flux_num.assign((size_t)nspecies * nf, 0.0);
@@ -479,6 +494,33 @@ extern "C" void punc_gpu_pop_destroy(punc_pop_t* p) {
     if (p->stream) hipStreamSynchronize(p->stream);
     pop_free(p);
     delete p;
+}
+
+extern "C" int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, int* index) {
+    if (!p || !index) return fail("pop_add_species: null argument");
+    if (!(mass != 0.0)) return fail("pop_add_species: mass must be nonzero");
+    for (int s = 0; s < p->nsp; ++s)
+        if (p->q[s] == q && p->m[s] == mass) {
+            *index = s;
+            return 0;
+        }
+    if (p->nsp >= p->max_sp) return fail("pop_add_species: species limit of this population reached");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->q.push_back(q);
+    p->m.push_back(mass);
+    p->nsp += 1;
+    for (auto& c : p->coef_cache) hipFree(c.d);
+    p->coef_cache.clear();
+    const size_t nf = p->mesh->topo.ext.size();
+    p->inj_on.push_back(0);
+    p->inj_range.resize(2 * p->nsp, -1);
+    p->spi.push_back(SpInject{});
+    p->flux_num.resize((size_t)p->nsp * nf, 0.0);
+    p->flux_max.resize((size_t)p->nsp * nf, 0.0);
+    p->inj_dt = -1.0;
+    *index = p->nsp - 1;
+    return 0;
 }
 
 static int read_state(punc_pop_s* P, DevState& s) {
@@ -880,75 +922,110 @@ extern "C" int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, con
     S.vrange = vrange;
     S.q = p->q[species];
     S.m = p->m[species];
+    S.density = density;
     p->inj_on[species] = 1;
     const auto& ext = p->mesh->topo.ext;
     const size_t nf = ext.size();
     for (size_t j = 0; j < nf; ++j) {
-        // create_flux (injector.cpp:127-152); density is applied per call
+        // create_flux (injector.cpp:127-152); density and dt are applied per call
         p->flux_num[species * nf + j] = maxwellian_flux_num(vth, vd, ext[j].normal, ext[j].area);
         p->flux_max[species * nf + j] = maxwellian_flux_max(vth, vd, ext[j].normal);
     }
-    // density kept in SpInject.pad[0]
-    S.pad[0] = density;
-    p->inj_dt = -1.0;  // force re-upload
+    p->inj_dt = -1.0;  // rebuild the pair table
     return 0;
 }
 
+extern "C" int punc_gpu_injector_facets(punc_pop_t* p, int species, int64_t begin, int64_t end) {
+    if (!p) return fail("injector_facets: null population");
+    if (species < 0 || species >= p->nsp) return fail("injector_facets: species out of range");
+    const int64_t nf = (int64_t)p->mesh->topo.ext.size();
+    if (begin < 0 || end > nf || begin > end) return fail("injector_facets: bad facet range");
+    if ((int64_t)p->inj_range.size() < 2 * p->nsp) p->inj_range.resize(2 * p->nsp, -1);
+    p->inj_range[2 * species] = begin;
+    p->inj_range[2 * species + 1] = end;
+    p->inj_dt = -1.0;
+    return 0;
+}
+
+// Builds the (species, facet) pair table for dt: species-major, facets
+// ascending, one entry per facet of each injecting species.
 static int inject_prepare(punc_pop_s* P, double dt) {
     if (P->inj_dt == dt && P->d_num) return 0;
-    const size_t nf = P->mesh->topo.ext.size();
-    const int nsp = P->nsp;
-    std::vector<double> num((size_t)nsp * nf + 1, 0.0), pmax((size_t)nsp * nf + 1, 0.0);
+    const int64_t nf = (int64_t)P->mesh->topo.ext.size();
+    std::vector<int32_t> psp, pf;
+    std::vector<double> num, pmax;
     int64_t cap = 0;
-    for (int s = 0; s < nsp; ++s) {
+    for (int s = 0; s < P->nsp; ++s) {
         if (!P->inj_on[s]) continue;
-        const double nd = P->spi[s].pad[0] * dt;  // injector.h:174: num_i *= n*dt
-        for (size_t j = 0; j < nf; ++j) {
+        int64_t fb = 0, fe = nf;
+        if ((int64_t)P->inj_range.size() >= 2 * (s + 1) && P->inj_range[2 * s] >= 0) {
+            fb = P->inj_range[2 * s];
+            fe = P->inj_range[2 * s + 1];
+        }
+        const double nd = P->spi[s].density * dt;
+        for (int64_t j = fb; j < fe; ++j) {
             double v = P->flux_num[s * nf + j];
-            v *= nd;
-            num[s * nf + j] = v;
-            pmax[s * nf + j] = P->flux_max[s * nf + j];
+            v *= nd;  // injector.h:174: num_i *= n*dt
+            psp.push_back(s);
+            pf.push_back((int32_t)j);
+            num.push_back(v);
+            pmax.push_back(P->flux_max[s * nf + j]);
             cap += (int64_t)v + 1;
         }
     }
+    const int64_t np = (int64_t)psp.size();
     cap = std::max<int64_t>(cap, 1024);
     HIPCHK(hipStreamSynchronize(P->stream));
     hipFree(P->d_num);
     hipFree(P->d_pmax);
     hipFree(P->d_spi);
-    HIPCHK(dalloc(&P->d_num, num.size()));
-    HIPCHK(dalloc(&P->d_pmax, pmax.size()));
-    HIPCHK(dalloc(&P->d_spi, nsp));
-    HIPCHK(hipMemcpy(P->d_num, num.data(), num.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P->d_pmax, pmax.data(), pmax.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P->d_spi, P->spi.data(), nsp * sizeof(SpInject), hipMemcpyHostToDevice));
-    if (cap > P->cand_cap || !P->d_fcount) {
-        hipFree(P->d_fcount);
-        hipFree(P->d_foff);
+    hipFree(P->d_pair_sp);
+    hipFree(P->d_pair_f);
+    hipFree(P->d_fcount);
+    hipFree(P->d_foff);
+    P->d_num = P->d_pmax = nullptr;
+    HIPCHK(dalloc(&P->d_num, np));
+    HIPCHK(dalloc(&P->d_pmax, np));
+    HIPCHK(dalloc(&P->d_pair_sp, np));
+    HIPCHK(dalloc(&P->d_pair_f, np));
+    HIPCHK(dalloc(&P->d_fcount, np));
+    HIPCHK(dalloc(&P->d_foff, np));
+    HIPCHK(dalloc(&P->d_spi, P->nsp));
+    if (np) {
+        HIPCHK(hipMemcpy(P->d_num, num.data(), np * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P->d_pmax, pmax.data(), np * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P->d_pair_sp, psp.data(), np * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P->d_pair_f, pf.data(), np * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(P->d_spi, P->spi.data(), P->nsp * sizeof(SpInject), hipMemcpyHostToDevice));
+    P->npairs = np;
+    if (cap > P->cand_cap) {
         hipFree(P->d_cx);
         hipFree(P->d_ccs);
         hipFree(P->d_keep);
         hipFree(P->d_pos);
         P->cand_cap = cap;
-        HIPCHK(dalloc(&P->d_fcount, (size_t)nsp * nf + 1));
-        HIPCHK(dalloc(&P->d_foff, (size_t)nsp * nf + 1));
         HIPCHK(dalloc(&P->d_cx, (size_t)cap * 6));
         HIPCHK(dalloc(&P->d_ccs, cap));
         HIPCHK(dalloc(&P->d_keep, cap));
         HIPCHK(dalloc(&P->d_pos, cap));
-        const int64_t need = std::max<int64_t>(((int64_t)nsp * nf) / 1024, cap / 1024) + 2;
-        if (need > P->bsum_len) {
-            hipFree(P->d_bsum);
-            P->bsum_len = need;
-            HIPCHK(dalloc(&P->d_bsum, P->bsum_len));
-        }
+    }
+    const int64_t need = std::max<int64_t>(np / 1024, P->cand_cap / 1024) + 2;
+    if (need > P->bsum_len) {
+        hipFree(P->d_bsum);
+        P->bsum_len = need;
+        HIPCHK(dalloc(&P->d_bsum, P->bsum_len));
     }
     P->inj_dt = dt;
     return 0;
 }
 
-static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, uint64_t step, double* rho_raw) {
-    int r = inject_prepare(P, dt);
+// Enqueues one injection (no host synchronization); the step index is read
+// from the device state.
+static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, double* rho_raw) {
+    if (P->npairs == 0) return 0;
+    const SpCoef* coef;
+    int r = get_coef(P, 0, 0.0, nullptr, &coef);
     if (r) return r;
     InjectLaunch L;
     memset(&L, 0, sizeof(L));
@@ -956,18 +1033,15 @@ static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, uint64_t step
     L.inj.fverts = P->mesh->d_fverts;
     L.inj.fnormal = P->mesh->d_fnormal;
     L.inj.fcell = P->mesh->d_fcell;
+    L.inj.pair_sp = P->d_pair_sp;
+    L.inj.pair_f = P->d_pair_f;
     L.inj.num = P->d_num;
     L.inj.pmax = P->d_pmax;
-    L.inj.nf = (int64_t)P->mesh->topo.ext.size();
+    L.inj.npairs = P->npairs;
     L.state = P->d_state;
     L.spi = P->d_spi;
-    const SpCoef* coef;
-    r = get_coef(P, 0, 0.0, nullptr, &coef);
-    if (r) return r;
     L.coef = coef;
-    L.nsp = P->nsp;
     L.seed = seed;
-    L.step = step;
     L.dt = dt;
     L.cell_bits = P->cell_bits;
     L.cand_cap = P->cand_cap;
@@ -982,7 +1056,6 @@ static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, uint64_t step
     L.pos = P->d_pos;
     L.rho_raw = rho_raw;
     L.errs = P->d_errs + 1;
-    if (L.inj.nf == 0) return 0;
     HIPCHK(launch_inject(L, P->stream));
     return 0;
 }
@@ -990,54 +1063,168 @@ static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, uint64_t step
 extern "C" int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected) {
     if (!p) return fail("inject: null population");
     HIPCHK(hipSetDevice(p->device));
-    int r = inject_enqueue(p, dt, seed, step, nullptr);
+    int r = inject_prepare(p, dt);
     if (r) return r;
+    HIPCHK(launch_fill_u64((unsigned long long*)(p->d_total + 1), 2, p->stream));
+    HIPCHK(launch_set_step(p->d_state, step, p->stream));
+    r = inject_enqueue(p, dt, seed, nullptr);
+    if (r) return r;
+    HIPCHK(launch_set_step(p->d_state, step, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     r = check_errors(p, "inject");
     if (r) return r;
     if (n_injected) {
-        DevState s;
-        r = read_state(p, s);
-        if (r) return r;
-        *n_injected = p->mesh->topo.ext.empty() ? 0 : s.inj_kept;
+        if (p->npairs == 0) {
+            *n_injected = 0;
+        } else {
+            DevState s;
+            r = read_state(p, s);
+            if (r) return r;
+            *n_injected = s.inj_kept;
+        }
     }
     return 0;
 }
 
 // ---------------------------------------------------------------------------
-// Fused step
+// Fused step and its graph
 // ---------------------------------------------------------------------------
+struct StepArgs {
+    int pusher;
+    double push_dt;
+    double B[3];
+    double dt;
+    bool periodic;
+    double Ld[3];
+    bool inject;
+    uint64_t seed;
+};
+
+static int step_check(punc_pop_s* P, int pusher, const double* B) {
+    if (pusher != 1 && pusher != 2) return fail("step: pusher must be 1 (accel_cg1) or 2 (boris_cg1)");
+    if (pusher == 2 && !B) return fail("step: boris needs B");
+    if (!P->field_set) return fail("step: no electric field set (punc_gpu_set_field)");
+    return 0;
+}
+
+// Everything that allocates or copies from the host, done before enqueueing.
+static int step_prepare(punc_pop_s* P, const StepArgs& a, const SpCoef** coef) {
+    int r = upload_dv(P, nullptr);
+    if (r) return r;
+    r = get_coef(P, a.pusher, a.push_dt, a.pusher == 2 ? a.B : nullptr, coef);
+    if (r) return r;
+    const SpCoef* c0;
+    r = get_coef(P, 0, 0.0, nullptr, &c0);
+    if (r) return r;
+    if (a.inject) {
+        r = inject_prepare(P, a.dt);
+        if (r) return r;
+    }
+    return 0;
+}
+
+// Kernel launches only: safe to capture into a graph.
+static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bool advance_step) {
+    const int64_t ndof = P->mesh->topo.ndof;
+    HIPCHK(launch_fill_f64(P->d_rho_raw, ndof, 0.0, P->stream));
+    ParticleLaunch L = base_launch(P);
+    L.coef = coef;
+    L.push = a.pusher;
+    L.dt = a.dt;
+    L.move = a.periodic ? 2 : 1;
+    for (int i = 0; i < 3; ++i) L.Ld[i] = a.Ld[i];
+    L.locate = true;
+    L.deposit = true;
+    HIPCHK(timed_launch(P, L));
+    HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
+    HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_H, P->d_S, P->grid, P->stream));
+    if (a.inject) {
+        int r = inject_enqueue(P, a.dt, a.seed, P->d_rho_raw);
+        if (r) return r;
+    }
+    HIPCHK(launch_scale_rho(P->d_rho_raw, P->d_dv_inv, P->d_rho, ndof, P->stream));
+    if (advance_step) HIPCHK(launch_next_step(P->d_state, P->stream));
+    return 0;
+}
+
+static StepArgs make_args(int pusher, double push_dt, const double* B, double dt, const double* Ld,
+                          int inject, uint64_t seed) {
+    StepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.pusher = pusher;
+    a.push_dt = push_dt;
+    if (B)
+        for (int i = 0; i < 3; ++i) a.B[i] = B[i];
+    a.dt = dt;
+    a.periodic = Ld != nullptr;
+    if (Ld)
+        for (int i = 0; i < 3; ++i) a.Ld[i] = Ld[i];
+    a.inject = inject != 0;
+    a.seed = seed;
+    return a;
+}
+
 extern "C" int punc_gpu_step(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
                              const double* Ld, int inject, uint64_t seed, uint64_t step) {
     if (!p) return fail("step: null population");
-    if (pusher != 1 && pusher != 2) return fail("step: pusher must be 1 (accel_cg1) or 2 (boris_cg1)");
-    if (pusher == 2 && !B) return fail("step: boris needs B");
-    if (!p->field_set) return fail("step: no electric field set (punc_gpu_set_field)");
+    int r = step_check(p, pusher, B);
+    if (r) return r;
     HIPCHK(hipSetDevice(p->device));
-    int r = upload_dv(p, nullptr);
-    if (r) return r;
+    StepArgs a = make_args(pusher, push_dt, B, dt, Ld, inject, seed);
     const SpCoef* coef;
-    r = get_coef(p, pusher, push_dt, pusher == 2 ? B : nullptr, &coef);
+    r = step_prepare(p, a, &coef);
     if (r) return r;
-    const int64_t ndof = p->mesh->topo.ndof;
-    HIPCHK(launch_fill_f64(p->d_rho_raw, ndof, 0.0, p->stream));
-    ParticleLaunch L = base_launch(p);
-    L.coef = coef;
-    L.push = pusher;
-    L.dt = dt;
-    L.move = Ld ? 2 : 1;
-    if (Ld)
-        for (int i = 0; i < 3; ++i) L.Ld[i] = Ld[i];
-    L.locate = true;
-    L.deposit = true;
-    HIPCHK(timed_launch(p, L));
-    HIPCHK(launch_sum_partials(p->d_ke_part, L.grid, p->d_ke, p->stream));
-    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_H, p->d_S, p->grid, p->stream));
-    if (inject) {
-        r = inject_enqueue(p, dt, seed, step, p->d_rho_raw);
-        if (r) return r;
+    HIPCHK(launch_set_step(p->d_state, step, p->stream));
+    return step_enqueue(p, a, coef, false);
+}
+
+static void graph_free(punc_pop_s* P) {
+    if (P->graph_exec) hipGraphExecDestroy(P->graph_exec);
+    if (P->graph) hipGraphDestroy(P->graph);
+    P->graph_exec = nullptr;
+    P->graph = nullptr;
+}
+
+extern "C" int punc_gpu_step_graph(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                                   const double* Ld, int inject, uint64_t seed, uint64_t first_step) {
+    if (!p) return fail("step_graph: null population");
+    int r = step_check(p, pusher, B);
+    if (r) return r;
+    HIPCHK(hipSetDevice(p->device));
+    StepArgs a = make_args(pusher, push_dt, B, dt, Ld, inject, seed);
+    const SpCoef* coef;
+    r = step_prepare(p, a, &coef);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    graph_free(p);
+    HIPCHK(launch_set_step(p->d_state, first_step, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    const bool was_timing = p->timing;
+    p->timing = false;  // no event records inside the graph
+    HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    r = step_enqueue(p, a, coef, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->stream, &g);
+    p->timing = was_timing;
+    if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
     }
-    HIPCHK(launch_scale_rho(p->d_rho_raw, p->d_dv_inv, p->d_rho, ndof, p->stream));
+    if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+    p->graph = g;
+    e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        graph_free(p);
+        return hipfail(e, "hipGraphInstantiate");
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_graph_launch(punc_pop_t* p, int count) {
+    if (!p) return fail("graph_launch: null population");
+    if (!p->graph_exec) return fail("graph_launch: no captured step (punc_gpu_step_graph)");
+    HIPCHK(hipSetDevice(p->device));
+    for (int k = 0; k < count; ++k) HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
     return 0;
 }
 

@@ -7,7 +7,8 @@
 
 namespace punc_gpu {
 
-constexpr int MAX_BND = 64;  // boundary ids 1..63
+constexpr int MAX_BND = 64;       // boundary ids 1..63
+constexpr int MAX_SPECIES = 128;  // species per population
 
 // Particle storage: FP64 structure of arrays plus one packed int32 cell field.
 struct PopArrays {
@@ -35,7 +36,8 @@ struct DevState {
     int64_t inj_cand;  // injection: candidates this call
     int64_t inj_kept;  // injection: kept candidates this call
     int64_t overflow;  // particles dropped because capacity was full
-    int64_t pad[7];
+    uint64_t step;     // injection stream step index
+    int64_t pad[6];
 };
 
 // Per-species coefficients for one call (depends on dt and B).
@@ -67,9 +69,11 @@ struct DevInject {
     const double* fverts;   // nf*9 facet vertices
     const double* fnormal;  // nf*3 inward normals
     const int32_t* fcell;   // nf owning cell
-    const double* num;      // nsp*nf expected count per facet per step (flux*n*dt)
-    const double* pmax;     // nsp*nf flux pdf maximum
-    int64_t nf;
+    const int32_t* pair_sp; // npairs species of each (species, facet) pair
+    const int32_t* pair_f;  // npairs exterior facet of each pair
+    const double* num;      // npairs expected count per step (flux*n*dt)
+    const double* pmax;     // npairs flux pdf maximum
+    int64_t npairs;
 };
 
 struct SpInject {
@@ -79,7 +83,8 @@ struct SpInject {
     double vth2;
     double vrange;  // rejection box half width in units of vth
     double q, m;
-    double pad[7];
+    double density;  // Species::n
+    double pad[6];
 };
 
 }  // namespace punc_gpu

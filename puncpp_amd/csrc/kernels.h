@@ -32,9 +32,7 @@ struct InjectLaunch {
     DevState* state;
     const SpInject* spi;
     const SpCoef* coef;
-    int nsp;
     uint64_t seed;
-    uint64_t step;
     double dt;
     int cell_bits;
     int64_t cand_cap;
@@ -72,5 +70,7 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
                        uint64_t seed, double cmin, double cmax, double amp, double kw, int axis,
                        double vth, const double vd[3], unsigned long long* errs, hipStream_t st);
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
+hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
+hipError_t launch_next_step(DevState* S, hipStream_t st);
 
 }  // namespace punc_gpu

@@ -467,19 +467,24 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
 // ---------------------------------------------------------------------------
 // Boundary injection (reference inject_particles, injector.h:157-232).
 // ---------------------------------------------------------------------------
-// Per (species, facet): N = floor(num) + [U < frac] (injector.h:186-191).
-__global__ void k_inject_count(const DevInject I, int nsp, uint64_t seed, uint64_t step,
+// Injection works on a table of (species, facet) pairs, species-major and
+// facets ascending within a species, the order of the reference's loops
+// (injector.h:168-205).  The step index comes from the device state so that a
+// captured graph advances the stream on every replay.
+//
+// Per pair: N = floor(num) + [U < frac] (injector.h:186-191).
+__global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S, uint64_t seed,
                                int64_t* __restrict__ fcount) {
-    const int64_t tot = I.nf * nsp;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = idx / I.nf, j = idx - s * I.nf;
-        const double num = I.num[idx];
+    const uint64_t step = S->step;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < I.npairs;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = I.pair_sp[p], j = I.pair_f[p];
+        const double num = I.num[p];
         int64_t N = (int64_t)num;
         U64x4 ctr = {{step, (uint64_t)j, (uint64_t)s, 0ull}};
         const U64x4 r = philox4x64_10(ctr, seed, TAG_INJECT_COUNT);
         if (u01(r.v[0]) < (num - N)) N += 1;
-        fcount[idx] = N;
+        fcount[p] = N;
     }
 }
 
@@ -487,25 +492,26 @@ __global__ void k_inject_count(const DevInject I, int nsp, uint64_t seed, uint64
 // velocity by rejection (injector.cpp:51-76, distributions.h:93-102), push by
 // U*dt*v (injector.h:212-216), location by a walk from the facet's cell.
 __global__ void __launch_bounds__(256)
-k_inject_gen(const DevMesh M, const DevInject I, const SpInject* __restrict__ spi, int nsp,
-             const int64_t* __restrict__ foff, const int64_t* __restrict__ ntot, uint64_t seed,
-             uint64_t step, double dt, int cell_bits, int64_t cand_cap, double* __restrict__ cx,
-             int32_t* __restrict__ ccs, int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
+k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
+             const SpInject* __restrict__ spi, const int64_t* __restrict__ foff,
+             const int64_t* __restrict__ ntot, uint64_t seed, double dt, int cell_bits,
+             int64_t cand_cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
+             int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
+    const uint64_t step = S->step;
     const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
-    const int64_t nidx = I.nf * nsp;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < cand_cap;
          k += (int64_t)gridDim.x * blockDim.x) {
         if (k >= total) {
             keep[k] = 0;
             continue;
         }
-        // Facet of candidate k: last idx with foff[idx] <= k.
-        int64_t lo = 0, hi = nidx - 1;
+        // Pair of candidate k: the last p with foff[p] <= k (its count is > 0).
+        int64_t lo = 0, hi = I.npairs - 1;
         while (lo < hi) {
             const int64_t mid = (lo + hi + 1) >> 1;
             if (foff[mid] <= k) lo = mid; else hi = mid - 1;
         }
-        const int64_t idx = lo, s = idx / I.nf, j = idx - s * I.nf, slot = k - foff[idx];
+        const int64_t p = lo, s = I.pair_sp[p], j = I.pair_f[p], slot = k - foff[p];
         const double* V = I.fverts + 9 * j;
         const double* nrm = I.fnormal + 3 * j;
         const SpInject P = spi[s];
@@ -517,7 +523,7 @@ k_inject_gen(const DevMesh M, const DevInject I, const SpInject* __restrict__ sp
         for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
         r = sqrt(u01(r0.v[1]));
         for (int d = 0; d < 3; ++d) x[d] = r * x[d] + (1.0 - r) * V[6 + d];
-        const double pmax = I.pmax[idx];
+        const double pmax = I.pmax[p];
         double v[3];
         bool ok = false;
         for (uint64_t a = 1; a < 1000000ull; ++a) {
@@ -607,6 +613,13 @@ __global__ void k_inject_finish(DevState* __restrict__ S, const int64_t* __restr
         S->inj_kept = kept;
         S->n += kept;
     }
+}
+
+__global__ void k_set_step(DevState* __restrict__ S, uint64_t step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) S->step = step;
+}
+__global__ void k_next_step(DevState* __restrict__ S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) S->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -755,14 +768,14 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
 }
 
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
-    const int64_t nidx = L.inj.nf * L.nsp;
-    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(nidx)), dim3(256), 0, st, L.inj, L.nsp, L.seed,
-                       L.step, L.fcount);
+    const int64_t np = L.inj.npairs;
+    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(np)), dim3(256), 0, st, L.inj, L.state, L.seed,
+                       L.fcount);
     LAUNCH_CHECK();
-    hipError_t e = launch_scan(L.fcount, L.foff, nidx, L.bsum, L.ntot, st);
+    hipError_t e = launch_scan(L.fcount, L.foff, np, L.bsum, L.ntot, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_inject_gen, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.inj,
-                       L.spi, L.nsp, L.foff, L.ntot, L.seed, L.step, L.dt, L.cell_bits, L.cand_cap,
+                       L.state, L.spi, L.foff, L.ntot, L.seed, L.dt, L.cell_bits, L.cand_cap,
                        L.cx, L.ccs, L.keep, L.errs);
     LAUNCH_CHECK();
     e = launch_scan(L.keep, L.pos, L.cand_cap, L.bsum, L.nkept, st);
@@ -775,4 +788,17 @@ hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
     return hipSuccess;
 }
 
+}  // namespace punc_gpu
+
+namespace punc_gpu {
+hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, S, step);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+hipError_t launch_next_step(DevState* S, hipStream_t st) {
+    hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, st, S);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
 }  // namespace punc_gpu

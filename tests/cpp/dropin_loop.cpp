@@ -1,0 +1,61 @@
+// The reference time loop (interaction/interaction.cpp:342-423) written
+// against the drop-in C++ API (include/punc.h): distribute, [host field solve
+// replaced by a fixed field], count, push (half step at n == 0), move,
+// update, inject.  Prints one line per step and checks particle-number
+// bookkeeping: N(n+1) = N(n) - absorbed + injected.
+#include <punc.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+using namespace punc;
+
+int main(int argc, char** argv) {
+    int steps = argc > 1 ? atoi(argv[1]) : 5;
+    Mesh mesh = Mesh::box(8, {-4.0, -4.0, -4.0}, {4.0, 4.0, 4.0}, 1.0);
+    std::vector<double> vd{0.0, 0.0, 0.5};
+    auto pdf = std::make_shared<UniformPosition>(mesh);
+    std::vector<Species> species;
+    species.emplace_back(-1.0, 1.0, 20.0, 1.0, ParticleAmountType::phys_per_sim, mesh, pdf,
+                         std::make_shared<Maxwellian>(1.0, vd), 1.0);
+    species.emplace_back(1.0, 1836.0, 20.0, 1.0, ParticleAmountType::phys_per_sim, mesh, pdf,
+                         std::make_shared<Maxwellian>(1.0 / std::sqrt(1836.0), vd), 1.0);
+    create_flux(mesh, species);
+    Population<3> pop(mesh, 1 << 20);
+    pop.set_seed(42);
+    load_particles(pop, species);
+    auto dv_inv = element_volume(mesh);
+    Function rho(mesh, 1), E(mesh, 3);
+    for (int64_t d = 0; d < mesh.ndof; ++d) E.values[3 * d + 2] = 0.01;
+    ObjectVector objects{std::make_shared<Object>(2)};
+    std::vector<double> B{0.0, 0.0, 1.0};
+    const double dt = 0.05;
+    size_t n_prev = pop.num_of_particles();
+    int bad = 0;
+    for (int n = 0; n <= steps; ++n) {
+        distribute_cg1(pop, rho, dv_inv);
+        double qsum = 0.0;
+        for (int64_t d = 0; d < mesh.ndof; ++d) qsum += rho.values[d] / dv_inv[d];
+        size_t num_e = pop.num_of_negatives(), num_i = pop.num_of_positives();
+        double KE = boris_cg1(pop, E, B, (1.0 - 0.5 * (n == 0)) * dt);
+        if (n == 0) KE = kinetic_energy(pop);
+        move(pop, dt);
+        size_t before = pop.num_of_particles();
+        double q_before = objects[0]->charge;
+        pop.update(objects, dt);
+        size_t after_update = pop.num_of_particles();
+        inject_particles(pop, species, mesh.exterior_facets, dt);
+        size_t after = pop.num_of_particles();
+        double qexp = (double)num_i - (double)num_e;
+        printf("step %d ne %zu ni %zu KE %.6e Q_rho %.6e (expect %.0f) removed %zu injected %zu obj_charge %.1f\n",
+               n, num_e, num_i, KE, qsum, qexp, before - after_update, after - after_update,
+               objects[0]->charge);
+        if (std::fabs(qsum - qexp) > 1e-8 * (num_e + num_i)) bad++;
+        if (num_e + num_i != n_prev) bad++;
+        if (objects[0]->charge - q_before > (double)(before - after_update) + 1e-9) bad++;
+        n_prev = after;
+    }
+    printf(bad ? "FAIL\n" : "OK\n");
+    return bad ? 1 : 0;
+}

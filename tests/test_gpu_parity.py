@@ -235,3 +235,43 @@ def test_fused_step_matches_sequence(gpu, pusher):
         rho_o = o.distribute(dv)
         scale = rho_scale(o, hm, dv)
         assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
+
+
+def test_graph_replay_matches_eager(gpu):
+    """A captured step replayed one step at a time (with host work and
+    synchronization between replays) gives what the eager step gives."""
+    _print_case("graph_replay")
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    o, mesh, popA = loaded_pair(hm, (3000, 3000))
+    st = o.get_particles()
+    popB = P.Population(mesh, [(QE, ME), (QI, MI)], capacity=100000)
+    popB.add_located(st["x"], st["v"], st["species"], st["cell"])
+    E = synthetic_field(hm, scale=0.2)
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    vd = [0.0, 0.0, 0.5]
+    species = [P.Species(q, m, 30.0, 1.0, P.ParticleAmountType.phys_per_sim, mesh,
+                         P.UniformPosition(mesh), P.Maxwellian(v, vd))
+               for (q, m), v in zip([(QE, ME), (QI, MI)], [1.0, 0.05])]
+    L = P.punc._native
+    for pop in (popA, popB):
+        P.punc._register_injectors(pop, species)
+        L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
+        L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
+    dt = 0.05
+    L.check(L.lib().punc_gpu_step_graph(popB._h, 2, dt, L.dptr(B), dt, None, 1, 9, 0))
+    for n in range(4):
+        L.check(L.lib().punc_gpu_step(popA._h, 2, dt, L.dptr(B), dt, None, 1, 9, n))
+        L.check(L.lib().punc_gpu_graph_launch(popB._h, 1))
+        L.check(L.lib().punc_gpu_synchronize(popB._h))
+        ka, kb = L.C.c_double(), L.C.c_double()
+        aba, abb = np.zeros((2, 64), np.int64), np.zeros((2, 64), np.int64)
+        L.check(L.lib().punc_gpu_step_results(popA._h, L.C.byref(ka), L.i64ptr(aba), None))
+        L.check(L.lib().punc_gpu_step_results(popB._h, L.C.byref(kb), L.i64ptr(abb), None))
+        np.testing.assert_array_equal(aba, abb)
+        assert ka.value == pytest.approx(kb.value, rel=1e-12)
+        assert_same_sets(popB.get_particles(), popA.get_particles(), rel=0.0, what=f"graph step {n}")
+        ra, rb = np.empty(hm.ndof), np.empty(hm.ndof)
+        L.check(L.lib().punc_gpu_get_rho(popA._h, L.dptr(ra)))
+        L.check(L.lib().punc_gpu_get_rho(popB._h, L.dptr(rb)))
+        assert np.allclose(ra, rb, rtol=1e-11, atol=1e-11 * np.abs(ra).max())
