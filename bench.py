@@ -86,27 +86,37 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0):
     dt = 0.02
     lib = N.lib()
 
-    def step(k):
-        push_dt = (1.0 - 0.5 * (k == 0)) * dt
-        N.check(lib.punc_gpu_step(pop._h, 2, push_dt, N.dptr(B), dt, None, 1, seed, k))
-        if sort_every and (k + 1) % sort_every == 0:
-            N.check(lib.punc_gpu_sort(pop._h))
+    # Step 0 is the half push (interaction.cpp:386,388); later steps replay a
+    # captured graph of the full step; the cell re-sort runs between replays.
+    N.check(lib.punc_gpu_step(pop._h, 2, 0.5 * dt, N.dptr(B), dt, None, 1, seed, 0))
+    N.check(lib.punc_gpu_step_graph(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, 1))
+    k_done = [1]
 
-    for k in range(warmup):
-        step(k)
+    def run(nsteps):
+        for _ in range(nsteps):
+            N.check(lib.punc_gpu_graph_launch(pop._h, 1))
+            k_done[0] += 1
+            if sort_every and k_done[0] % sort_every == 0:
+                N.check(lib.punc_gpu_sort(pop._h))
+
+    run(max(warmup - 1, 0))
     N.check(lib.punc_gpu_synchronize(pop._h))
     n_start = pop.size()
     log(f"[bench] warmup done, {n_start} particles")
-    N.check(lib.punc_gpu_timing(pop._h, 1))
     t1 = time.perf_counter()
-    for k in range(warmup, warmup + steps):
-        step(k)
+    run(steps)
     N.check(lib.punc_gpu_synchronize(pop._h))
     t2 = time.perf_counter()
+    n_end = pop.size()
+    # Kernel timing pass: eager steps with HIP events around the fused particle
+    # kernel on the population's stream.
+    N.check(lib.punc_gpu_timing(pop._h, 1))
+    for k in range(10):
+        N.check(lib.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, k_done[0] + k))
     ms_tot = N.C.c_double()
     nl = N.C.c_int64()
     N.check(lib.punc_gpu_timing_read(pop._h, N.C.byref(ms_tot), N.C.byref(nl)))
-    n_end = pop.size()
+    N.check(lib.punc_gpu_timing(pop._h, 0))
     ke = N.C.c_double()
     ab = np.zeros((2, 64), dtype=np.int64)
     kin = N.C.c_int64()

@@ -150,9 +150,16 @@ int punc_gpu_sort(punc_pop_t* p);
  * (Species::n) and rejection box half width in units of vth (Pdf::vdf_range). */
 int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
                           double density, double vrange);
-/* Restricts species s to inject through exterior facets [begin, end) (one
- * instance of a batched mesh); by default a species uses every exterior facet. */
-int punc_gpu_injector_facets(punc_pop_t* p, int species, int64_t begin, int64_t end);
+/* Batched instances.  Binds species s to one instance of a concatenated mesh:
+ * its particles live in cells [cell_begin, cell_end) (loading and host
+ * location stay there), it injects through exterior facets [facet_begin,
+ * facet_end), and its random streams use `seed`, species id rng_species and
+ * facet ids counted from facet_begin - so the instance draws exactly what it
+ * draws as a mesh of its own with that seed and species index.  No
+ * counterpart in the reference (one instance per process there). */
+int punc_gpu_species_instance(punc_pop_t* p, int species, int rng_species, uint64_t seed,
+                              int64_t cell_begin, int64_t cell_end, int64_t facet_begin,
+                              int64_t facet_end);
 /* inject_particles(pop, species, facets, dt) (injector.h:157-232) with a
  * counter-based stream keyed by (seed, step).  n_injected may be NULL. */
 int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected);

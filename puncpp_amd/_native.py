@@ -55,7 +55,8 @@ _SIGS = {
     "punc_gpu_kinetic_energy": (C.c_int, [vp, d_p]),
     "punc_gpu_sort": (C.c_int, [vp]),
     "punc_gpu_injector_set": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
-    "punc_gpu_injector_facets": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int64]),
+    "punc_gpu_species_instance": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int64,
+                                            C.c_int64, C.c_int64]),
     "punc_gpu_inject": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_uint64, i64_p]),
     "punc_gpu_step_graph": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, d_p, C.c_int,
                                       C.c_uint64, C.c_uint64]),

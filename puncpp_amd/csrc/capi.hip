@@ -60,6 +60,14 @@ struct punc_mesh_s {
     DevMesh dm;
 };
 
+// Binding of a species to one instance of a batched mesh.
+struct SpInstance {
+    bool bound = false;
+    int rng_sp = 0;
+    uint64_t seed = 0;
+    int64_t c0 = 0, c1 = 0, f0 = 0, f1 = 0;
+};
+
 struct CoefEntry {
     int kind;
     double dt;
@@ -108,7 +116,7 @@ struct punc_pop_s {
     hipGraphExec_t graph_exec = nullptr;
     // injection
     std::vector<int> inj_on;
-    std::vector<int64_t> inj_range;  // per species [begin, end) of exterior facets, -1: all
+    std::vector<SpInstance> inst;    // per species instance binding
     int32_t* d_pair_sp = nullptr;
     int32_t* d_pair_f = nullptr;
     int64_t npairs = 0;
@@ -478,7 +486,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipMemset(P->d_ke, 0, 4 * 8));
     PCHK(hipMemset(P->d_total, 0, 8 * 8));
     P->inj_on.assign(nspecies, 0);
-    P->inj_range.assign(2 * nspecies, -1);
+    P->inst.assign(nspecies, SpInstance{});
     P->spi.assign(nspecies, SpInject{});
     const size_t nf = m->topo.ext.size();
     P->flux_num.assign((size_t)nspecies * nf, 0.0);
@@ -514,7 +522,7 @@ extern "C" int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, in
     p->coef_cache.clear();
     const size_t nf = p->mesh->topo.ext.size();
     p->inj_on.push_back(0);
-    p->inj_range.resize(2 * p->nsp, -1);
+    p->inst.push_back(SpInstance{});
     p->spi.push_back(SpInject{});
     p->flux_num.resize((size_t)p->nsp * nf, 0.0);
     p->flux_max.resize((size_t)p->nsp * nf, 0.0);
@@ -575,8 +583,13 @@ extern "C" int punc_gpu_pop_add(punc_pop_t* p, int species, int64_t n, const dou
     kx.reserve(n * 3);
     kv.reserve(n * 3);
     kc.reserve(n);
+    const SpInstance& I = p->inst[species];
+    const MeshTopo& T = p->mesh->topo;
+    const int32_t c0 = I.bound ? (int32_t)I.c0 : 0, c1 = I.bound ? (int32_t)I.c1 : (int32_t)T.nc;
     for (int64_t i = 0; i < n; ++i) {
-        const int64_t c = locate_host(p->mesh->topo, xs + 3 * i);
+        const double* x = xs + 3 * i;
+        const int64_t c = locate_in_grid(T.planes.data(), T.bucket_start.data(), T.bucket_cells.data(),
+                                         T.grid_n, T.grid_lo, T.grid_h, x[0], x[1], x[2], c0, c1);
         if (c < 0) continue;
         kx.insert(kx.end(), xs + 3 * i, xs + 3 * i + 3);
         kv.insert(kv.end(), vs + 3 * i, vs + 3 * i + 3);
@@ -617,8 +630,25 @@ extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t
     if (r) return r;
     if (s.n + N > p->capacity) return fail("pop_load: population capacity exceeded");
     const MeshTopo& T = p->mesh->topo;
-    HIPCHK(launch_load(p->mesh->dm, p->d_state, s.n, N, species, p->cell_bits, seed, T.coord_min,
-                       T.coord_max, amp, kw, axis, vth, vd, p->d_errs, p->stream));
+    const SpInstance& I = p->inst[species];
+    double cmin = T.coord_min, cmax = T.coord_max;
+    int32_t c0 = 0, c1 = (int32_t)T.nc;
+    int rng_sp = species;
+    if (I.bound) {
+        // The instance's own UniformPosition domain (distributions.h:196-208).
+        c0 = (int32_t)I.c0;
+        c1 = (int32_t)I.c1;
+        cmin = T.vcoords[c0 * 12];
+        cmax = cmin;
+        for (int64_t k = c0 * 12; k < (int64_t)c1 * 12; ++k) {
+            cmin = std::min(cmin, T.vcoords[k]);
+            cmax = std::max(cmax, T.vcoords[k]);
+        }
+        rng_sp = I.rng_sp;
+        seed = I.seed;
+    }
+    HIPCHK(launch_load(p->mesh->dm, p->d_state, s.n, N, species, rng_sp, c0, c1, p->cell_bits, seed, cmin,
+                       cmax, amp, kw, axis, vth, vd, p->d_errs, p->stream));
     int64_t nn = s.n + N;
     HIPCHK(hipMemcpyAsync((char*)p->d_state + offsetof(DevState, n), &nn, 8, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -935,14 +965,24 @@ extern "C" int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, con
     return 0;
 }
 
-extern "C" int punc_gpu_injector_facets(punc_pop_t* p, int species, int64_t begin, int64_t end) {
-    if (!p) return fail("injector_facets: null population");
-    if (species < 0 || species >= p->nsp) return fail("injector_facets: species out of range");
-    const int64_t nf = (int64_t)p->mesh->topo.ext.size();
-    if (begin < 0 || end > nf || begin > end) return fail("injector_facets: bad facet range");
-    if ((int64_t)p->inj_range.size() < 2 * p->nsp) p->inj_range.resize(2 * p->nsp, -1);
-    p->inj_range[2 * species] = begin;
-    p->inj_range[2 * species + 1] = end;
+extern "C" int punc_gpu_species_instance(punc_pop_t* p, int species, int rng_species, uint64_t seed,
+                                         int64_t cell_begin, int64_t cell_end, int64_t facet_begin,
+                                         int64_t facet_end) {
+    if (!p) return fail("species_instance: null population");
+    if (species < 0 || species >= p->nsp) return fail("species_instance: species out of range");
+    const int64_t nf = (int64_t)p->mesh->topo.ext.size(), nc = p->mesh->topo.nc;
+    if (facet_begin < 0 || facet_end > nf || facet_begin > facet_end)
+        return fail("species_instance: bad exterior facet range");
+    if (cell_begin < 0 || cell_end > nc || cell_begin >= cell_end) return fail("species_instance: bad cell range");
+    if (rng_species < 0) return fail("species_instance: negative stream species id");
+    SpInstance& I = p->inst[species];
+    I.bound = true;
+    I.rng_sp = rng_species;
+    I.seed = seed;
+    I.c0 = cell_begin;
+    I.c1 = cell_end;
+    I.f0 = facet_begin;
+    I.f1 = facet_end;
     p->inj_dt = -1.0;
     return 0;
 }
@@ -957,11 +997,13 @@ static int inject_prepare(punc_pop_s* P, double dt) {
     int64_t cap = 0;
     for (int s = 0; s < P->nsp; ++s) {
         if (!P->inj_on[s]) continue;
-        int64_t fb = 0, fe = nf;
-        if ((int64_t)P->inj_range.size() >= 2 * (s + 1) && P->inj_range[2 * s] >= 0) {
-            fb = P->inj_range[2 * s];
-            fe = P->inj_range[2 * s + 1];
-        }
+        const SpInstance& I = P->inst[s];
+        const int64_t fb = I.bound ? I.f0 : 0, fe = I.bound ? I.f1 : nf;
+        SpInject& S = P->spi[s];
+        S.use_seed = I.bound ? 1 : 0;
+        S.seed = I.seed;
+        S.rng_sp = I.bound ? I.rng_sp : s;
+        S.facet_base = fb;
         const double nd = P->spi[s].density * dt;
         for (int64_t j = fb; j < fe; ++j) {
             double v = P->flux_num[s * nf + j];

@@ -84,7 +84,15 @@ struct SpInject {
     double vrange;  // rejection box half width in units of vth
     double q, m;
     double density;  // Species::n
-    double pad[6];
+    // Random stream identity: draws use key (seed, tag) and the species id
+    // rng_sp, and facet ids relative to facet_base, so an instance of a batched
+    // mesh draws exactly what it draws as a mesh of its own.
+    uint64_t seed;
+    int32_t use_seed;  // 0: the seed given to the call
+    int32_t rng_sp;
+    int64_t facet_base;
+    double pad[3];
 };
+static_assert(sizeof(SpInject) == 128, "SpInject layout");
 
 }  // namespace punc_gpu

@@ -66,9 +66,10 @@ hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bs
                        hipStream_t st);
 hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
                        int64_t* bsum, int64_t* total, int grid, hipStream_t st);
-hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int cell_bits,
-                       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis,
-                       double vth, const double vd[3], unsigned long long* errs, hipStream_t st);
+hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
+                       int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
+                       double cmax, double amp, double kw, int axis, double vth, const double vd[3],
+                       unsigned long long* errs, hipStream_t st);
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
 hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
 hipError_t launch_next_step(DevState* S, hipStream_t st);

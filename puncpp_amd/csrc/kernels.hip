@@ -424,9 +424,10 @@ __global__ void k_swap_buffers(DevState* __restrict__ S) {
 // species sp draws from Philox counter (k, attempt, sp, 0), key (seed, LOAD).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int sp, int cell_bits,
-       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis, double vth,
-       double vd0, double vd1, double vd2, unsigned long long* __restrict__ errs) {
+k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int sp, int rng_sp,
+       int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin, double cmax,
+       double amp, double kw, int axis, double vth, double vd0, double vd1, double vd2,
+       unsigned long long* __restrict__ errs) {
     const PopArrays A = S->cur;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -434,10 +435,11 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
         int32_t c = -1;
         bool ok = false;
         for (uint64_t a = 0; a < 1000000ull; ++a) {
-            U64x4 ctr = {{(uint64_t)k, a, (uint64_t)sp, 0ull}};
+            U64x4 ctr = {{(uint64_t)k, a, (uint64_t)rng_sp, 0ull}};
             const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
             for (int d = 0; d < 3; ++d) p[d] = cmin + (cmax - cmin) * u01(r.v[d]);
-            c = locate_in_grid(M.planes, M.bstart, M.bcells, M.gn, M.glo, M.gh, p[0], p[1], p[2]);
+            c = locate_in_grid(M.planes, M.bstart, M.bcells, M.gn, M.glo, M.gh, p[0], p[1], p[2],
+                               cell_lo, cell_hi);
             const double pdf = (c >= 0) * (1.0 + amp * cos(kw * p[axis]));
             if (u01(r.v[3]) * (1.0 + amp) < pdf) {
                 ok = true;
@@ -448,7 +450,7 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
             atomicAdd(errs, 1ull);
             c = 0;
         }
-        U64x4 ctr = {{(uint64_t)k, 0xFFFFFFFFull, (uint64_t)sp, 0ull}};
+        U64x4 ctr = {{(uint64_t)k, 0xFFFFFFFFull, (uint64_t)rng_sp, 0ull}};
         const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
         const double vd[3] = {vd0, vd1, vd2};
         double v[3];
@@ -473,16 +475,19 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
 // captured graph advances the stream on every replay.
 //
 // Per pair: N = floor(num) + [U < frac] (injector.h:186-191).
-__global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S, uint64_t seed,
+__global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S,
+                               const SpInject* __restrict__ spi, uint64_t seed,
                                int64_t* __restrict__ fcount) {
     const uint64_t step = S->step;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < I.npairs;
          p += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = I.pair_sp[p], j = I.pair_f[p];
+        const SpInject& P = spi[s];
+        const uint64_t key = P.use_seed ? P.seed : seed;
         const double num = I.num[p];
         int64_t N = (int64_t)num;
-        U64x4 ctr = {{step, (uint64_t)j, (uint64_t)s, 0ull}};
-        const U64x4 r = philox4x64_10(ctr, seed, TAG_INJECT_COUNT);
+        U64x4 ctr = {{step, (uint64_t)(j - P.facet_base), (uint64_t)P.rng_sp, 0ull}};
+        const U64x4 r = philox4x64_10(ctr, key, TAG_INJECT_COUNT);
         if (u01(r.v[0]) < (num - N)) N += 1;
         fcount[p] = N;
     }
@@ -515,9 +520,10 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
         const double* V = I.fverts + 9 * j;
         const double* nrm = I.fnormal + 3 * j;
         const SpInject P = spi[s];
-        const uint64_t sj = ((uint64_t)s << 32) | (uint64_t)j;
+        const uint64_t key = P.use_seed ? P.seed : seed;
+        const uint64_t sj = ((uint64_t)P.rng_sp << 32) | (uint64_t)(j - P.facet_base);
         U64x4 c0 = {{step, sj, (uint64_t)slot, 0ull}};
-        const U64x4 r0 = philox4x64_10(c0, seed, TAG_INJECT);
+        const U64x4 r0 = philox4x64_10(c0, key, TAG_INJECT);
         double x[3];
         double r = u01(r0.v[0]);
         for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
@@ -528,7 +534,7 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
         bool ok = false;
         for (uint64_t a = 1; a < 1000000ull; ++a) {
             U64x4 ca = {{step, sj, (uint64_t)slot, a}};
-            const U64x4 ra = philox4x64_10(ca, seed, TAG_INJECT);
+            const U64x4 ra = philox4x64_10(ca, key, TAG_INJECT);
             for (int d = 0; d < 3; ++d) {
                 const double lo_d = P.vd[d] - P.vrange * P.vth;
                 const double hi_d = P.vd[d] + P.vrange * P.vth;
@@ -757,20 +763,21 @@ hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, in
     return hipSuccess;
 }
 
-hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int cell_bits,
-                       uint64_t seed, double cmin, double cmax, double amp, double kw, int axis,
-                       double vth, const double vd[3], unsigned long long* errs, hipStream_t st) {
+hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
+                       int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
+                       double cmax, double amp, double kw, int axis, double vth, const double vd[3],
+                       unsigned long long* errs, hipStream_t st) {
     if (N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_load, dim3(fill_grid(N)), dim3(256), 0, st, M, S, base, N, sp, cell_bits, seed,
-                       cmin, cmax, amp, kw, axis, vth, vd[0], vd[1], vd[2], errs);
+    hipLaunchKernelGGL(k_load, dim3(fill_grid(N)), dim3(256), 0, st, M, S, base, N, sp, rng_sp, cell_lo,
+                       cell_hi, cell_bits, seed, cmin, cmax, amp, kw, axis, vth, vd[0], vd[1], vd[2], errs);
     LAUNCH_CHECK();
     return hipSuccess;
 }
 
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
     const int64_t np = L.inj.npairs;
-    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(np)), dim3(256), 0, st, L.inj, L.state, L.seed,
-                       L.fcount);
+    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(np)), dim3(256), 0, st, L.inj, L.state, L.spi,
+                       L.seed, L.fcount);
     LAUNCH_CHECK();
     hipError_t e = launch_scan(L.fcount, L.foff, np, L.bsum, L.ntot, st);
     if (e != hipSuccess) return e;

@@ -149,7 +149,8 @@ PUNC_HD double plane_proj(const double* p, double x, double y, double z) {
 // so the result equals a brute-force scan over all cells in id order.
 PUNC_HD int32_t locate_in_grid(const CellPlanes* planes, const int32_t* bucket_start,
                                const int32_t* bucket_cells, const int* gn, const double* glo,
-                               const double* gh, double x, double y, double z) {
+                               const double* gh, double x, double y, double z,
+                               int32_t cell_lo = 0, int32_t cell_hi = 0x7fffffff) {
     double q[3] = {x, y, z};
     int b[3];
     for (int d = 0; d < 3; ++d) {
@@ -160,6 +161,7 @@ PUNC_HD int32_t locate_in_grid(const CellPlanes* planes, const int32_t* bucket_s
     int64_t bi = (int64_t)b[0] + (int64_t)gn[0] * ((int64_t)b[1] + (int64_t)gn[1] * b[2]);
     for (int32_t k = bucket_start[bi]; k < bucket_start[bi + 1]; ++k) {
         int32_t c = bucket_cells[k];
+        if (c < cell_lo || c >= cell_hi) continue;
         const double* p = planes[c].p;
         bool inside = true;
         for (int f = 0; f < 4; ++f) inside = inside && !(plane_proj(p + 4 * f, x, y, z) > 0.0);
