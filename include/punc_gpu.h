@@ -186,12 +186,37 @@ int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t*
 /* rho (ndof) from the last deposit. */
 int punc_gpu_get_rho(punc_pop_t* p, double* rho);
 
+/* ---- adjoint ----------------------------------------------------------- */
+/* Reverse mode through K steps of boris_cg1 + move (+ the relocation walk)
+ * and the CG1 deposit of the final positions, along the forward run's cell
+ * path, with step checkpoints.  No counterpart in the reference.
+ * n particles with per-particle species (q, mass per species). */
+typedef struct punc_adj_s punc_adj_t;
+const char* punc_gpu_adjoint_last_error(void);
+int punc_gpu_adjoint_create(punc_mesh_t* m, int nspecies, const double* q, const double* mass, int64_t n,
+                            const int32_t* species, punc_adj_t** out);
+void punc_gpu_adjoint_destroy(punc_adj_t* a);
+/* Forward K steps from (x0, cell0, v0) in the fixed field E (ndof*3), with B,
+ * dt and the periodic box Ld (NULL: not periodic), keeping a checkpoint every
+ * `every` steps.  w (ndof, may be NULL) receives w_j = sum_p q_p lambda_j(x_K)
+ * over particles still in the mesh; xK, vK, cellK (may be NULL) the final state. */
+int punc_gpu_adjoint_forward(punc_adj_t* a, const double* x0, const int64_t* cell0, const double* v0,
+                             const double* E, const double B[3], double dt, const double* Ld, int K,
+                             int every, double* w, double* xK, double* vK, int64_t* cellK);
+/* Given g = dJ/dw (ndof), returns dJ/dE (ndof*3), dJ/dv0 and dJ/dx0 (n*3);
+ * any output may be NULL. */
+int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, double* dJdv0, double* dJdx0);
+
 /* ---- launch control, timing -------------------------------------------- */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block);
 /* Record HIP events around every launch of the main particle kernel. */
 int punc_gpu_timing(punc_pop_t* p, int enable);
 int punc_gpu_timing_read(punc_pop_t* p, double* ms_total, int64_t* launches);
 int punc_gpu_synchronize(punc_pop_t* p);
+/* Streaming-copy reference kernel: copies nbytes between two fresh buffers
+ * reps times and returns the mean time of one copy (achievable bandwidth,
+ * and the scale for HBM byte counters).  No counterpart in the reference. */
+int punc_gpu_copy_probe(int64_t nbytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -1338,3 +1338,56 @@ extern "C" int punc_gpu_synchronize(punc_pop_t* p) {
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }
+
+extern "C" int punc_gpu_copy_probe(int64_t nbytes, int reps, double* ms_per_copy) {
+    if (nbytes < 16 || reps < 1 || !ms_per_copy) return fail("copy_probe: bad argument");
+    const int64_t n16 = nbytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n16 * 16));
+    hipError_t e = hipMalloc(&b, n16 * 16);
+    if (e != hipSuccess) {
+        hipFree(a);
+        return hipfail(e, "hipMalloc");
+    }
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    hipGetDeviceProperties(&prop, dev);
+    const int grid = prop.multiProcessorCount * 8;
+    hipMemsetAsync(a, 0, n16 * 16, st);
+    e = launch_copy16(a, b, n16, grid, st);  // warm-up
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy16(a, b, n16, grid, st);
+    hipEventRecord(e1, st);
+    hipStreamSynchronize(st);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return hipfail(e, "copy kernel");
+    *ms_per_copy = ms / reps;
+    return 0;
+}
+
+// Device view of a mesh for the other translation units of the library.
+int punc_internal_mesh_view(punc_mesh_t* m, DevMesh* out, int64_t* ndof, std::string& err) {
+    if (!m) {
+        err = "null mesh";
+        return -1;
+    }
+    if (mesh_ensure_device(m)) {
+        err = g_err;
+        return -1;
+    }
+    *out = m->dm;
+    *ndof = m->topo.ndof;
+    return 0;
+}

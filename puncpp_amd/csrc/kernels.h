@@ -73,5 +73,6 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
 hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
 hipError_t launch_next_step(DevState* S, hipStream_t st);
+hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStream_t st);
 
 }  // namespace punc_gpu

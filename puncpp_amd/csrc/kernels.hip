@@ -809,3 +809,19 @@ hipError_t launch_next_step(DevState* S, hipStream_t st) {
     return hipSuccess;
 }
 }  // namespace punc_gpu
+
+// ---------------------------------------------------------------------------
+// Streaming copy of 16-byte vectors: the reference point for achievable HBM
+// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+// ---------------------------------------------------------------------------
+namespace punc_gpu {
+__global__ void __launch_bounds__(256) k_copy16(const double2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (const double2*)a, (double2*)b, n16);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+}  // namespace punc_gpu

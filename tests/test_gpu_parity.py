@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import puncpp_amd as P
+from puncpp_amd import _native as N
 from helpers import (MI, ME, QE, QI, assert_same_sets, loaded_pair, near_face, periodic_mesh,
                      rho_scale, sphere_mesh, synthetic_field)
 
@@ -168,8 +169,8 @@ def test_device_load_matches_oracle(gpu):
     vth = [1.0, 0.05]
     for s in range(2):
         assert o.load(s, 3000, 11, 0.01, 1.0, 0, vth[s], [0.0, 0.0, 0.5]) == 0
-        P.punc._native.check(P.punc._native.lib().punc_gpu_pop_load(
-            pop._h, s, 3000, 11, 0.01, 1.0, 0, vth[s], P.punc.dptr(np.array([0.0, 0.0, 0.5]))))
+        N.check(N.lib().punc_gpu_pop_load(
+            pop._h, s, 3000, 11, 0.01, 1.0, 0, vth[s], N.dptr(np.array([0.0, 0.0, 0.5]))))
     a, b = pop.get_particles(), o.get_particles()
     # positions are exact; velocities go through erfc_inv (1-2 ulp between implementations)
     ca, cb = a, b
@@ -210,7 +211,7 @@ def test_fused_step_matches_sequence(gpu, pusher):
     P.punc._register_injectors(pop, species)
     for s in range(2):
         o.set_injector(s, vth[s], vd, species[s].n, 5.0)
-    L = P.punc._native
+    L = N
     L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
     L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
     dt = 0.05
@@ -253,7 +254,7 @@ def test_graph_replay_matches_eager(gpu):
     species = [P.Species(q, m, 30.0, 1.0, P.ParticleAmountType.phys_per_sim, mesh,
                          P.UniformPosition(mesh), P.Maxwellian(v, vd))
                for (q, m), v in zip([(QE, ME), (QI, MI)], [1.0, 0.05])]
-    L = P.punc._native
+    L = N
     for pop in (popA, popB):
         P.punc._register_injectors(pop, species)
         L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
