@@ -208,7 +208,11 @@ int punc_gpu_adjoint_forward(punc_adj_t* a, const double* x0, const int64_t* cel
 int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, double* dJdv0, double* dJdx0);
 
 /* ---- launch control, timing -------------------------------------------- */
-int punc_gpu_set_launch(punc_pop_t* p, int grid, int block);
+/* Launch shape of the particle kernels: persistent grid size, block size
+ * (64, 128 or 256) and deposit mode (0: four FP64 atomics per particle;
+ * 1: wavefront runs of equal cells are summed across lanes first). */
+int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
+int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
 /* Record HIP events around every launch of the main particle kernel. */
 int punc_gpu_timing(punc_pop_t* p, int enable);
 int punc_gpu_timing_read(punc_pop_t* p, double* ms_total, int64_t* launches);

@@ -86,6 +86,7 @@ struct punc_pop_s {
     int cell_bits = 0;
     int64_t capacity = 0;
     int grid = 0, block = 256;
+    int dep_mode = 1;
 
     DevState* d_state = nullptr;
     void* bufs[2][7] = {};
@@ -765,6 +766,7 @@ static ParticleLaunch base_launch(punc_pop_s* P) {
     L.absorbed = P->d_absorbed;
     L.grid = P->grid;
     L.block = P->block;
+    L.dep_mode = P->dep_mode;
     return L;
 }
 
@@ -1298,12 +1300,23 @@ extern "C" int punc_gpu_get_rho(punc_pop_t* p, double* rho) {
 // ---------------------------------------------------------------------------
 // Launch control and timing
 // ---------------------------------------------------------------------------
-extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block) {
+extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode) {
     if (!p) return fail("set_launch: null population");
     if (grid < 1 || grid > p->ke_np) return fail("set_launch: grid out of range");
     if (block != 64 && block != 128 && block != 256) return fail("set_launch: block must be 64, 128 or 256");
+    if (deposit_mode != 0 && deposit_mode != 1) return fail("set_launch: deposit_mode must be 0 or 1");
+    graph_free(p);  // a captured graph holds the old launch shape
     p->grid = grid;
     p->block = block;
+    p->dep_mode = deposit_mode;
+    return 0;
+}
+
+extern "C" int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode) {
+    if (!p) return fail("get_launch: null population");
+    if (grid) *grid = p->grid;
+    if (block) *block = p->block;
+    if (deposit_mode) *deposit_mode = p->dep_mode;
     return 0;
 }
 

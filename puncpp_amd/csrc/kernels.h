@@ -22,6 +22,7 @@ struct ParticleLaunch {
     int move;  // 0 none, 1 plain, 2 periodic
     bool locate;
     bool deposit;
+    int dep_mode;  // 0: per-particle atomics, 1: wavefront run reduction first
     int grid;
     int block;
 };

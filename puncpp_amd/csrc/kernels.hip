@@ -60,6 +60,48 @@ __device__ inline int32_t walk(const DevMesh& M, int32_t c, double x, double y, 
     return WALK_FAILED;
 }
 
+// CG1 deposit of one particle per lane (distribute_cg1, distributor.h:189-203:
+// accum_i += q*lambda_i, then rho[dof_i] += accum_i).  key = cell, or -1 for
+// lanes with nothing to deposit.  mode 0: four FP64 atomics per particle.
+// mode 1: lanes of a wavefront that hold the same cell (runs of equal keys,
+// which cell-sorted particles form) first sum their contributions with a
+// segmented reduction across lanes, then the run's first lane issues the four
+// atomics.  Every lane of the wavefront must call this together.
+__device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_raw, int32_t key, double x,
+                                   double y, double z, double q, int mode) {
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    int32_t dof[4] = {0, 0, 0, 0};
+    if (key >= 0) {
+        const CellRec& R = M.rec[key];
+        double l[4];
+        bary4(R.bary, x, y, z, l);
+        for (int k = 0; k < 4; ++k) {
+            d[k] = q * l[k];
+            dof[k] = R.dof[k];
+        }
+    }
+    if (mode == 1) {
+        // A run is a maximal stretch of consecutive lanes with one key; the
+        // same key may reappear in a later run, which is summed separately.
+        const int lane = threadIdx.x & 63;
+        const int32_t kprev = __shfl_up(key, 1, 64);
+        const bool head = lane == 0 || kprev != key;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const int run_end = later ? __ffsll((long long)later) - 1 : 64;
+        for (int o = 1; o < 64; o <<= 1) {
+            double t[4];
+            for (int k = 0; k < 4; ++k) t[k] = __shfl_down(d[k], o, 64);
+            if (lane + o < run_end)
+                for (int k = 0; k < 4; ++k) d[k] += t[k];
+        }
+        if (key < 0 || !head) return;
+    } else if (key < 0) {
+        return;
+    }
+    for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&rho_raw[dof[k]], d[k]);
+}
+
 __device__ inline double block_sum(double v, double* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -85,7 +127,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             const double4* __restrict__ E, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes,
-            unsigned long long* __restrict__ absorbed) {
+            unsigned long long* __restrict__ absorbed, const int dep_mode) {
     __shared__ double sh[8];
     const PopArrays A = S->cur;
     const int64_t n = S->n;
@@ -93,7 +135,14 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     chunk_range(n, s, e);
     const int32_t cmask = (1 << cell_bits) - 1;
     double ke = 0.0;
-    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+    // Block-uniform trip count so that whole wavefronts reach the deposit's
+    // cross-lane reduction together; lanes past the chunk end are inactive.
+    for (int64_t base = s; base < e; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool active = i < e;
+        int32_t dkey = -1;  // deposit cell of this lane (-1: nothing to deposit)
+        double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
+        if (active) {
         const int32_t cs = A.cs[i];
         const int sp = cs >> cell_bits;
         int32_t c = cs & cmask;
@@ -180,16 +229,14 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             }
         }
         if (DEPOSIT && alive) {
-            // distribute_cg1 (distributor.h:189-203): accum_i += q*lambda_i
-            const CellRec& R = M.rec[c];
-            double l[4];
-            bary4(R.bary, x, y, z, l);
-            const double q = coef[sp].q;
-            unsafeAtomicAdd(&rho_raw[R.dof[0]], q * l[0]);
-            unsafeAtomicAdd(&rho_raw[R.dof[1]], q * l[1]);
-            unsafeAtomicAdd(&rho_raw[R.dof[2]], q * l[2]);
-            unsafeAtomicAdd(&rho_raw[R.dof[3]], q * l[3]);
+            dkey = c;
+            dx_ = x;
+            dy_ = y;
+            dz_ = z;
+            dq = coef[sp].q;
         }
+        }  // active
+        if (DEPOSIT) deposit_cg1(M, rho_raw, dkey, dx_, dy_, dz_, dq, dep_mode);
     }
     if (PUSH != 0) {
         const double t = block_sum(ke, sh);
@@ -638,7 +685,7 @@ template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
 static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
     hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT>), dim3(L.grid), dim3(L.block), 0, st,
                        L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
-                       L.rho_raw, L.ke_part, L.holes, L.absorbed);
+                       L.rho_raw, L.ke_part, L.holes, L.absorbed, L.dep_mode);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -371,8 +371,13 @@ class Population:
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
 
-    def set_launch(self, grid, block=256):
-        check(N.lib().punc_gpu_set_launch(self._h, int(grid), int(block)))
+    def set_launch(self, grid, block=256, deposit_mode=1):
+        check(N.lib().punc_gpu_set_launch(self._h, int(grid), int(block), int(deposit_mode)))
+
+    def get_launch(self):
+        g, b, d = N.C.c_int32(), N.C.c_int32(), N.C.c_int32()
+        check(N.lib().punc_gpu_get_launch(self._h, N.C.byref(g), N.C.byref(b), N.C.byref(d)))
+        return g.value, b.value, d.value
 
     def synchronize(self):
         check(N.lib().punc_gpu_synchronize(self._h))

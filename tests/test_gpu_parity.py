@@ -127,6 +127,27 @@ def test_distribute_cg1(gpu):
     assert tot == pytest.approx(5000 * QE + 5000 * QI, abs=1e-9)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_distribute_unsorted_both_modes(gpu, mode):
+    """Particles in random order: equal cells recur in non-adjacent lanes."""
+    _print_case("distribute_unsorted", mode=mode)
+    hm = sphere_mesh(n=4, L=2.0, r=0.7)
+    import oracle as O
+    o = O.OraclePop(hm, [QE, QI], [ME, MI])
+    o.load(0, 6000, 4, 0.0, 0.0, 0, 1.0, [0, 0, 0])
+    o.load(1, 6000, 4, 0.0, 0.0, 0, 1.0, [0, 0, 0])
+    st = o.get_particles()
+    perm = np.random.default_rng(0).permutation(st["x"].shape[0])
+    mesh = P.Mesh(hm)
+    pop = P.Population(mesh, [(QE, ME), (QI, MI)], capacity=20000)
+    pop.add_located(st["x"][perm], st["v"][perm], st["species"][perm], st["cell"][perm])
+    pop.set_launch(pop.get_launch()[0], 256, mode)
+    dv = P.element_volume(mesh)
+    rho_g, rho_o = P.distribute_cg1(pop, dv), o.distribute(dv)
+    scale = rho_scale(o, hm, dv)
+    assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
+
+
 def test_distribute_periodic(gpu):
     _print_case("distribute_periodic")
     hm = periodic_mesh(5)
