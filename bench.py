@@ -54,7 +54,7 @@ def synthetic_field(hm, seed=1):
     return np.ascontiguousarray(E)
 
 
-def run_gpu(cfg, steps, warmup, sort_every, seed=0):
+def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
     import puncpp_amd as P
     from puncpp_amd import _native as N
 
@@ -86,6 +86,14 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0):
     dt = 0.02
     lib = N.lib()
 
+    # Launch shape from the autotuner (runs on a snapshot; the state is kept).
+    tune = None
+    if autotune:
+        g, b, md, ms = N.C.c_int32(), N.C.c_int32(), N.C.c_int32(), N.C.c_double()
+        N.check(lib.punc_gpu_autotune(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, 3, N.C.byref(g),
+                                      N.C.byref(b), N.C.byref(md), N.C.byref(ms)))
+        tune = {"grid": g.value, "block": b.value, "deposit_mode": md.value, "step_ms": ms.value}
+        log(f"[bench] autotune: {tune}")
     # Step 0 is the half push (interaction.cpp:386,388); later steps replay a
     # captured graph of the full step; the cell re-sort runs between replays.
     N.check(lib.punc_gpu_step(pop._h, 2, 0.5 * dt, N.dptr(B), dt, None, 1, seed, 0))
@@ -132,7 +140,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0):
     return {
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
-        "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density,
+        "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune,
     }
 
 
@@ -170,6 +178,99 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
                       f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cores visible"}
 
 
+def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1):
+    """Bias sweep: n_inst independent sphere-probe instances, meshes n^3 x 6
+    tets with n ~ U{16,20,24,28,32}, bias linspace(-5, 5) shaping each
+    instance's (frozen, synthetic) probe field, density scale ~ U[0.5, 2].
+    Boris dt=0.02 with injection.  The plain loop steps each instance with
+    its own population and stream; the batched path steps all instances in
+    one population (one launch per phase, graph-replayed)."""
+    import puncpp_amd as P
+    from puncpp_amd import _native as N
+    from puncpp_amd.batch import BatchedPopulation
+
+    rng = np.random.default_rng(seed)
+    ns = rng.choice([16, 20, 24, 28, 32], size=n_inst)
+    bias = np.linspace(-5.0, 5.0, n_inst)
+    scale = rng.uniform(0.5, 2.0, size=n_inst)
+    L, r, dt = 10.0, 1.0, 0.02
+    vd = [0.0, 0.0, 0.5]
+    B = np.array([0.0, 0.0, 1.0])
+    lib = N.lib()
+    t0 = time.time()
+    hms, meshes, species, fields = [], [], [], []
+    for i in range(n_inst):
+        hm = P.box_mesh(int(ns[i]), [-L] * 3, [L] * 3, sphere_r=r)
+        mesh = P.Mesh(hm)
+        dens = scale[i] * base_particles / 2 / mesh.volume()
+        sp = [P.Species(q, m, dens, 1.0, P.ParticleAmountType.phys_per_sim, mesh, P.UniformPosition(mesh),
+                        P.Maxwellian(v, vd))
+              for (q, m), v in zip([(-1.0, 1.0), (1.0, 1836.0)], [1.0, 1.0 / np.sqrt(1836.0)])]
+        xyz = np.zeros((hm.ndof, 3))
+        xyz[hm.vdof] = hm.xyz
+        r2 = np.maximum((xyz ** 2).sum(1), r * r)
+        E = np.ascontiguousarray(0.05 * bias[i] * xyz / r2[:, None] ** 1.5)
+        hms.append(hm)
+        meshes.append(mesh)
+        species.append(sp)
+        fields.append(E)
+    total = sum(int(s.num) for sp in species for s in sp)
+    log(f"[bench] batched: {n_inst} instances, {total} particles, meshes built ({time.time() - t0:.1f}s)")
+    # plain: one population per instance
+    plain = []
+    for i in range(n_inst):
+        pop = P.Population(meshes[i], [(-1.0, 1.0), (1.0, 1836.0)], capacity=int(sum(s.num for s in species[i]) * 1.1) + 100000)
+        for s, S in enumerate(species[i]):
+            N.check(lib.punc_gpu_pop_load(pop._h, s, int(S.num), 1000 + i, 0.0, 0.0, 0, S.vdf.vth(), N.dptr(S.vdf.vd())))
+        pop.sort()
+        P.punc._register_injectors(pop, species[i])
+        N.check(lib.punc_gpu_set_field(pop._h, N.dptr(fields[i])))
+        N.check(lib.punc_gpu_distribute(pop._h, N.dptr(P.element_volume(meshes[i])), None))
+        plain.append(pop)
+    bp = BatchedPopulation(hms, species, [1000 + i for i in range(n_inst)], capacity=int(total * 1.1) + 1_000_000)
+    bp.load()
+    bp.pop.sort()
+    bp.register_injectors()
+    N.check(lib.punc_gpu_set_field(bp.pop._h, N.dptr(np.concatenate(fields))))
+    N.check(lib.punc_gpu_distribute(bp.pop._h, N.dptr(P.element_volume(bp.mesh)), None))
+    log(f"[bench] batched: populations loaded ({time.time() - t0:.1f}s)")
+
+    def plain_steps(k0, k):
+        for n in range(k0, k0 + k):
+            for i, pop in enumerate(plain):
+                N.check(lib.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 1, 1000 + i, n))
+        for pop in plain:
+            N.check(lib.punc_gpu_synchronize(pop._h))
+
+    plain_steps(0, warmup)
+    n0 = sum(p.size() for p in plain)
+    t1 = time.perf_counter()
+    plain_steps(warmup, steps)
+    t2 = time.perf_counter()
+    n1 = sum(p.size() for p in plain)
+    plain_rate = 0.5 * (n0 + n1) * steps / (t2 - t1)
+    N.check(lib.punc_gpu_step_graph(bp.pop._h, 2, dt, N.dptr(B), dt, None, 1, 0, 0))
+    N.check(lib.punc_gpu_graph_launch(bp.pop._h, warmup))
+    N.check(lib.punc_gpu_synchronize(bp.pop._h))
+    m0 = bp.pop.size()
+    t3 = time.perf_counter()
+    N.check(lib.punc_gpu_graph_launch(bp.pop._h, steps))
+    N.check(lib.punc_gpu_synchronize(bp.pop._h))
+    t4 = time.perf_counter()
+    m1 = bp.pop.size()
+    batched_rate = 0.5 * (m0 + m1) * steps / (t4 - t3)
+    # identical evolution: per-instance particle counts after the same steps
+    counts_b = np.bincount(bp.pop.get_particles()["species"] // 2, minlength=n_inst)
+    counts_p = np.array([p.size() for p in plain])
+    same = bool(np.array_equal(counts_b, counts_p))
+    log(f"[bench] batched {batched_rate:.4g} vs plain {plain_rate:.4g} particle-steps/s "
+        f"({batched_rate / plain_rate:.2f}x), per-instance counts identical: {same}")
+    return {"instances": n_inst, "particles": int(0.5 * (m0 + m1)), "steps": steps,
+            "value": batched_rate, "plain_loop_value": plain_rate, "speedup": batched_rate / plain_rate,
+            "unit": "particle-steps/s", "per_instance_counts_identical": same,
+            "ms_per_step_batched": 1e3 * (t4 - t3) / steps, "ms_per_step_plain": 1e3 * (t2 - t1) / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,12 +279,27 @@ def main():
     ap.add_argument("--config", default="large", choices=sorted(CONFIGS))
     ap.add_argument("--sort-every", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batched-instances", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the streaming-copy reference kernel (1 GiB, 20 copies)")
     args = ap.parse_args()
+    if args.copy_probe:
+        from puncpp_amd import _native as N
+        ms = N.C.c_double()
+        nbytes = 1 << 30
+        N.check(N.lib().punc_gpu_copy_probe(nbytes, 20, N.C.byref(ms)))
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value,
+                          "GBps_read_plus_write": 2 * nbytes / ms.value / 1e6}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               f"sharding of this path is a design note, not a build target.", file=sys.stderr)
         sys.exit(2)
-    res = run_gpu(args.config, args.steps, args.warmup, args.sort_every)
+    res = run_gpu(args.config, args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune)
+    batched = None if args.no_batched else run_batched(args.batched_instances, args.batched_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
     out = {
         "metric": METRIC,
@@ -199,13 +315,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {res['desc']}", "cells": res["cells"], "dofs": res["ndof"],
-                   "particles": int(res["n_avg"]), "sort_every": args.sort_every,
+                   "particles": int(res["n_avg"]), "sort_every": args.sort_every, "launch": res["tune"],
                    "field": "frozen synthetic CG1 field on device"},
         "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK, "traffic": None,
                      "kernel": "k_particles<boris,move,locate,deposit>",
                      "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

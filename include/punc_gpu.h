@@ -213,6 +213,14 @@ int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, doub
  * 1: wavefront runs of equal cells are summed across lanes first). */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
+/* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every
+ * launch shape (grid 2/4/8/16 x CUs, block 128/256, deposit mode 0/1) on a
+ * snapshot of the population, restores the snapshot (particles, counts,
+ * absorbed counters, rho, KE), and keeps the fastest shape.  A captured step
+ * graph is dropped.  No counterpart in the reference. */
+int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                      const double* Ld, int inject, uint64_t seed, int reps, int* best_grid,
+                      int* best_block, int* best_mode, double* best_ms);
 /* Record HIP events around every launch of the main particle kernel. */
 int punc_gpu_timing(punc_pop_t* p, int enable);
 int punc_gpu_timing_read(punc_pop_t* p, double* ms_total, int64_t* launches);

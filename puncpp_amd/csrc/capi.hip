@@ -1304,7 +1304,7 @@ extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int depos
     if (!p) return fail("set_launch: null population");
     if (grid < 1 || grid > p->ke_np) return fail("set_launch: grid out of range");
     if (block != 64 && block != 128 && block != 256) return fail("set_launch: block must be 64, 128 or 256");
-    if (deposit_mode != 0 && deposit_mode != 1) return fail("set_launch: deposit_mode must be 0 or 1");
+    if (deposit_mode < 0 || deposit_mode > 2) return fail("set_launch: deposit_mode must be 0, 1 or 2");
     graph_free(p);  // a captured graph holds the old launch shape
     p->grid = grid;
     p->block = block;
@@ -1387,6 +1387,136 @@ extern "C" int punc_gpu_copy_probe(int64_t nbytes, int reps, double* ms_per_copy
     hipFree(b);
     if (e != hipSuccess) return hipfail(e, "copy kernel");
     *ms_per_copy = ms / reps;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times the fused step for each launch shape on a snapshot of the
+// population and restores the snapshot afterwards, so tuning leaves the
+// simulation state untouched.
+// ---------------------------------------------------------------------------
+extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
+                                 const double* Ld, int inject, uint64_t seed, int reps, int* best_grid,
+                                 int* best_block, int* best_mode, double* best_ms) {
+    if (!p) return fail("autotune: null population");
+    if (reps < 1) return fail("autotune: reps must be >= 1");
+    int r = step_check(p, pusher, B);
+    if (r) return r;
+    HIPCHK(hipSetDevice(p->device));
+    StepArgs a = make_args(pusher, push_dt, B, dt, Ld, inject, seed);
+    const SpCoef* coef;
+    r = step_prepare(p, a, &coef);
+    if (r) return r;
+    DevState s0;
+    r = read_state(p, s0);
+    if (r) return r;
+    const int64_t n = s0.n;
+    const size_t nab = (size_t)MAX_SPECIES * MAX_BND;
+    // snapshot: particle arrays, device state, absorbed counters
+    std::vector<void*> snap(7, nullptr);
+    void* snap_ab = nullptr;
+    auto release = [&]() {
+        for (void* q : snap) hipFree(q);
+        hipFree(snap_ab);
+    };
+    const void* src[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
+    for (int k = 0; k < 7; ++k) {
+        const size_t bytes = (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4);
+        hipError_t e = hipMalloc(&snap[k], bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(snap[k], src[k], bytes, hipMemcpyDeviceToDevice, p->stream);
+        if (e != hipSuccess) {
+            release();
+            return hipfail(e, "autotune snapshot");
+        }
+    }
+    const int64_t ndof = p->mesh->topo.ndof;
+    hipError_t e = hipMalloc(&snap_ab, nab * 8 + ndof * 8 + 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(snap_ab, p->d_absorbed, nab * 8, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((char*)snap_ab + nab * 8, p->d_rho, ndof * 8, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((char*)snap_ab + nab * 8 + ndof * 8, p->d_ke, 8, hipMemcpyDeviceToDevice, p->stream);
+    if (e != hipSuccess) {
+        release();
+        return hipfail(e, "autotune snapshot");
+    }
+    hipDeviceProp_t prop;
+    hipGetDeviceProperties(&prop, p->device);
+    const int cus = prop.multiProcessorCount;
+    const int g0 = p->grid, b0 = p->block, m0 = p->dep_mode;
+    int bg = g0, bb = b0, bm = m0;
+    double bms = 1e300;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const int grids[] = {2, 4, 8, 16};
+    const int blocks[] = {128, 256};
+    const int modes[] = {0, 1, 2};
+    int rc = 0;
+    for (int gm : grids) {
+        for (int bl : blocks) {
+            for (int md : modes) {
+                const int grid = std::min(cus * gm, p->ke_np);
+                p->grid = grid;
+                p->block = bl;
+                p->dep_mode = md;
+                // restore the snapshot, one untimed step, then reps timed steps
+                for (int k = 0; k < 7 && !rc; ++k) {
+                    void* dst[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
+                    if (hipMemcpyAsync(dst[k], snap[k], (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4),
+                                       hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+                        rc = fail("autotune: restore failed");
+                }
+                if (!rc && hipMemcpyAsync(p->d_state, &s0, sizeof(DevState), hipMemcpyHostToDevice, p->stream) != hipSuccess)
+                    rc = fail("autotune: restore failed");
+                if (!rc) rc = step_enqueue(p, a, coef, true);
+                if (!rc && hipEventRecord(e0, p->stream) != hipSuccess) rc = fail("autotune: event");
+                for (int it = 0; it < reps && !rc; ++it) rc = step_enqueue(p, a, coef, true);
+                if (!rc && hipEventRecord(e1, p->stream) != hipSuccess) rc = fail("autotune: event");
+                if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = fail("autotune: step failed");
+                if (rc) break;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                const double per = ms / reps;
+                if (per < bms) {
+                    bms = per;
+                    bg = grid;
+                    bb = bl;
+                    bm = md;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // restore the state the caller handed in
+    const void* back[7] = {snap[0], snap[1], snap[2], snap[3], snap[4], snap[5], snap[6]};
+    void* dst[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
+    for (int k = 0; k < 7; ++k)
+        hipMemcpyAsync(dst[k], back[k], (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4), hipMemcpyDeviceToDevice,
+                       p->stream);
+    hipMemcpyAsync(p->d_state, &s0, sizeof(DevState), hipMemcpyHostToDevice, p->stream);
+    hipMemcpyAsync(p->d_absorbed, snap_ab, nab * 8, hipMemcpyDeviceToDevice, p->stream);
+    hipMemcpyAsync(p->d_rho, (char*)snap_ab + nab * 8, ndof * 8, hipMemcpyDeviceToDevice, p->stream);
+    hipMemcpyAsync(p->d_ke, (char*)snap_ab + nab * 8 + ndof * 8, 8, hipMemcpyDeviceToDevice, p->stream);
+    hipStreamSynchronize(p->stream);
+    release();
+    graph_free(p);
+    if (rc) {
+        p->grid = g0;
+        p->block = b0;
+        p->dep_mode = m0;
+        return rc;
+    }
+    p->grid = bg;
+    p->block = bb;
+    p->dep_mode = bm;
+    if (best_grid) *best_grid = bg;
+    if (best_block) *best_block = bb;
+    if (best_mode) *best_mode = bm;
+    if (best_ms) *best_ms = bms;
     return 0;
 }
 

@@ -67,6 +67,106 @@ __device__ inline int32_t walk(const DevMesh& M, int32_t c, double x, double y, 
 // which cell-sorted particles form) first sum their contributions with a
 // segmented reduction across lanes, then the run's first lane issues the four
 // atomics.  Every lane of the wavefront must call this together.
+// Removal bookkeeping for the particles of one wavefront that left the mesh
+// (key >= 0): one atomic per wavefront reserves their slots in the hole list,
+// and one atomic per distinct (species, boundary) adds to the absorbed
+// counts.  Device-scope atomics on a single address serialize, so per-lane
+// atomics here would cost a memory round trip per leaving particle.
+// Every lane of the wavefront must call this together.
+constexpr int32_t ABS_LOST = 0x7fffffff;
+
+__device__ inline void record_leavers(DevState* __restrict__ S, int32_t* __restrict__ holes,
+                                      unsigned long long* __restrict__ absorbed, int32_t key, int64_t i) {
+    const unsigned long long lmask = __ballot(key >= 0);
+    if (lmask == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)lmask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd((unsigned long long*)&S->n_holes, (unsigned long long)__popcll(lmask));
+    base = __shfl(base, leader, 64);
+    if (key >= 0) holes[base + __popcll(lmask & ((1ull << lane) - 1ull))] = (int32_t)i;
+    unsigned long long left = lmask;
+    while (left) {
+        const int l0 = __ffsll((long long)left) - 1;
+        const int32_t k0 = __shfl(key, l0, 64);
+        const unsigned long long same = __ballot(key == k0) & left;
+        if (lane == l0) {
+            if (k0 == ABS_LOST)
+                atomicAdd((unsigned long long*)&S->n_lost, (unsigned long long)__popcll(same));
+            else
+                atomicAdd(&absorbed[k0], (unsigned long long)__popcll(same));
+        }
+        left &= ~same;
+    }
+}
+
+// Block-local accumulation table in LDS (deposit mode 2): open addressing on
+// the dof id, values summed with LDS FP64 atomics, flushed to HBM with one
+// global atomic per occupied slot.
+constexpr int LDS_SLOTS = 2048;
+constexpr int32_t LDS_EMPTY = -1;
+constexpr size_t LDS_TABLE_BYTES = LDS_SLOTS * (sizeof(double) + sizeof(int32_t)) + 16;
+
+struct LdsTable {
+    double* val;
+    int32_t* key;
+    int32_t* count;
+};
+
+__device__ inline LdsTable lds_table() {
+    extern __shared__ double dyn_smem[];
+    LdsTable t;
+    t.val = dyn_smem;
+    t.key = (int32_t*)(dyn_smem + LDS_SLOTS);
+    t.count = t.key + LDS_SLOTS;
+    return t;
+}
+
+__device__ inline void lds_clear(LdsTable t) {
+    for (int k = threadIdx.x; k < LDS_SLOTS; k += blockDim.x) {
+        t.key[k] = LDS_EMPTY;
+        t.val[k] = 0.0;
+    }
+    if (threadIdx.x == 0) *t.count = 0;
+}
+
+__device__ inline void lds_add(LdsTable t, double* __restrict__ rho_raw, int32_t dof, double v) {
+    uint32_t h = ((uint32_t)dof * 0x9E3779B1u) >> (32 - 11);  // 11 = log2(LDS_SLOTS)
+    for (int probe = 0; probe < 16; ++probe) {
+        const int32_t k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == dof) {
+            unsafeAtomicAdd(&t.val[h], v);
+            return;
+        }
+        if (k == LDS_EMPTY) {
+            const int32_t old = atomicCAS(&t.key[h], LDS_EMPTY, dof);
+            if (old == LDS_EMPTY || old == dof) {
+                unsafeAtomicAdd(&t.val[h], v);
+                if (old == LDS_EMPTY) atomicAdd(t.count, 1);
+                return;
+            }
+        }
+        h = (h + 1) & (LDS_SLOTS - 1);
+    }
+    unsafeAtomicAdd(&rho_raw[dof], v);  // table region full: straight to HBM
+}
+
+// Every thread of the block must call this (it synchronizes).
+__device__ inline void lds_flush(LdsTable t, double* __restrict__ rho_raw) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < LDS_SLOTS; k += blockDim.x) {
+        const int32_t d = t.key[k];
+        if (d != LDS_EMPTY) {
+            unsafeAtomicAdd(&rho_raw[d], t.val[k]);
+            t.key[k] = LDS_EMPTY;
+            t.val[k] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *t.count = 0;
+    __syncthreads();
+}
+
 __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_raw, int32_t key, double x,
                                    double y, double z, double q, int mode) {
     double d[4] = {0.0, 0.0, 0.0, 0.0};
@@ -80,7 +180,7 @@ __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_ra
             dof[k] = R.dof[k];
         }
     }
-    if (mode == 1) {
+    if (mode >= 1) {
         // A run is a maximal stretch of consecutive lanes with one key; the
         // same key may reappear in a later run, which is summed separately.
         const int lane = threadIdx.x & 63;
@@ -96,6 +196,11 @@ __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_ra
                 for (int k = 0; k < 4; ++k) d[k] += t[k];
         }
         if (key < 0 || !head) return;
+        if (mode == 2) {
+            const LdsTable t = lds_table();
+            for (int k = 0; k < 4; ++k) lds_add(t, rho_raw, dof[k], d[k]);
+            return;
+        }
     } else if (key < 0) {
         return;
     }
@@ -135,6 +240,11 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     chunk_range(n, s, e);
     const int32_t cmask = (1 << cell_bits) - 1;
     double ke = 0.0;
+    const bool use_lds = DEPOSIT && dep_mode == 2;
+    if (use_lds) {
+        lds_clear(lds_table());
+        __syncthreads();
+    }
     // Block-uniform trip count so that whole wavefronts reach the deposit's
     // cross-lane reduction together; lanes past the chunk end are inactive.
     for (int64_t base = s; base < e; base += blockDim.x) {
@@ -142,6 +252,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         const bool active = i < e;
         int32_t dkey = -1;  // deposit cell of this lane (-1: nothing to deposit)
         double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
+        int32_t leave_key = -1;  // particle left: species*MAX_BND + bnd, or ABS_LOST
         if (active) {
         const int32_t cs = A.cs[i];
         const int sp = cs >> cell_bits;
@@ -216,12 +327,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             const int32_t c2 = walk(M, c, x, y, z);
             if (c2 < 0) {
                 alive = false;
-                if (c2 == WALK_FAILED)
-                    atomicAdd((unsigned long long*)&S->n_lost, 1ull);
-                else
-                    atomicAdd(&absorbed[sp * MAX_BND + (-c2)], 1ull);
-                const unsigned long long slot = atomicAdd((unsigned long long*)&S->n_holes, 1ull);
-                holes[slot] = (int32_t)i;
+                leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
                 A.cs[i] = c2;
             } else if (c2 != c) {
                 c = c2;
@@ -236,8 +342,14 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             dq = coef[sp].q;
         }
         }  // active
+        if (LOCATE) record_leavers(S, holes, absorbed, leave_key, i);
         if (DEPOSIT) deposit_cg1(M, rho_raw, dkey, dx_, dy_, dz_, dq, dep_mode);
+        if (use_lds) {
+            __syncthreads();
+            if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
+        }
     }
+    if (use_lds) lds_flush(lds_table(), rho_raw);
     if (PUSH != 0) {
         const double t = block_sum(ke, sh);
         if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
@@ -427,23 +539,59 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t len, const int64_t
 // Cell re-sort: counting sort of the population by cell id into the alternate
 // buffers, then a pointer swap.  Order within a cell is arbitrary.
 // ---------------------------------------------------------------------------
+// Runs of equal keys in consecutive lanes of a wavefront: the run's first
+// lane, this lane's offset in its run, and the run length.  Keys < 0 are
+// runs too (callers skip them).  Every lane of the wavefront must call this.
+struct RunInfo {
+    int head_lane;
+    int rank;
+    int len;
+};
+__device__ inline RunInfo lane_run(int32_t key) {
+    const int lane = threadIdx.x & 63;
+    const int32_t kprev = __shfl_up(key, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || kprev != key);
+    const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int head = 63 - __clzll((long long)upto);
+    const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+    const int next_head = later ? __ffsll((long long)later) - 1 : 64;
+    RunInfo r;
+    r.head_lane = head;
+    r.rank = lane - head;
+    r.len = next_head - head;
+    return r;
+}
+
+// Counting sort by cell.  Within a wavefront, runs of particles in one cell
+// take their slots with one atomic per run (the population is re-sorted
+// often, so runs are long).
 __global__ void k_sort_hist(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ hist) {
     const int32_t* cs = S->cur.cs;
     const int64_t n = S->n;
     const int32_t cmask = (1 << cell_bits) - 1;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd((unsigned long long*)&hist[cs[i] & cmask], 1ull);
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const int32_t key = i < n ? (cs[i] & cmask) : -1;
+        const RunInfo r = lane_run(key);
+        if (key >= 0 && r.rank == 0) atomicAdd((unsigned long long*)&hist[key], (unsigned long long)r.len);
+    }
 }
 
 __global__ void k_sort_scatter(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ cursor) {
     const PopArrays A = S->cur, B = S->alt;
     const int64_t n = S->n;
     const int32_t cmask = (1 << cell_bits) - 1;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t cs = A.cs[i];
-        const int64_t d = (int64_t)atomicAdd((unsigned long long*)&cursor[cs & cmask], 1ull);
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const int32_t cs = active ? A.cs[i] : 0;
+        const int32_t key = active ? (cs & cmask) : -1;
+        const RunInfo r = lane_run(key);
+        unsigned long long slot = 0;
+        if (key >= 0 && r.rank == 0) slot = atomicAdd((unsigned long long*)&cursor[key], (unsigned long long)r.len);
+        slot = __shfl(slot, r.head_lane, 64);
+        if (!active) continue;
+        const int64_t d = (int64_t)slot + r.rank;
         B.x[d] = A.x[i];
         B.y[d] = A.y[i];
         B.z[d] = A.z[i];
@@ -540,9 +688,17 @@ __global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S
     }
 }
 
-// One lane per candidate: facet point (injector.cpp:78-102), flux-Maxwellian
-// velocity by rejection (injector.cpp:51-76, distributions.h:93-102), push by
-// U*dt*v (injector.h:212-216), location by a walk from the facet's cell.
+// Candidates are processed 64 per wavefront.  The velocity rejection sampler
+// (injector.cpp:51-76 against the flux pdf, distributions.h:93-102) runs
+// cooperatively: for one candidate at a time the 64 lanes evaluate attempts
+// 1+64r+lane of round r and the lowest accepting attempt wins - exactly the
+// draw the sequential loop accepts, without a lane idling while the slowest
+// candidate of its wavefront keeps rejecting (acceptance is about 1% for a
+// non-drifting Maxwellian in the vd +- 5 vth box).  Then every lane finishes
+// its own candidate: facet point (injector.cpp:78-102), push by U*dt*v
+// (injector.h:212-216), location by a walk from the facet's cell.
+constexpr uint64_t INJECT_MAX_ATTEMPTS = 1000000ull;
+
 __global__ void __launch_bounds__(256)
 k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
              const SpInject* __restrict__ spi, const int64_t* __restrict__ foff,
@@ -551,24 +707,82 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
              int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
     const uint64_t step = S->step;
     const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < cand_cap;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        if (k >= total) {
-            keep[k] = 0;
-            continue;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t g0 = wave * 64; g0 < cand_cap; g0 += nwaves * 64) {
+        const int64_t k = g0 + lane;
+        const bool mine = k < total;
+        // This lane's candidate: its pair (last p with foff[p] <= k; count > 0)
+        int64_t p = 0, s = 0, j = 0, slot = 0;
+        if (mine) {
+            int64_t lo = 0, hi = I.npairs - 1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (foff[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            p = lo;
+            s = I.pair_sp[p];
+            j = I.pair_f[p];
+            slot = k - foff[p];
         }
-        // Pair of candidate k: the last p with foff[p] <= k (its count is > 0).
-        int64_t lo = 0, hi = I.npairs - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (foff[mid] <= k) lo = mid; else hi = mid - 1;
-        }
-        const int64_t p = lo, s = I.pair_sp[p], j = I.pair_f[p], slot = k - foff[p];
-        const double* V = I.fverts + 9 * j;
-        const double* nrm = I.fnormal + 3 * j;
         const SpInject P = spi[s];
         const uint64_t key = P.use_seed ? P.seed : seed;
         const uint64_t sj = ((uint64_t)P.rng_sp << 32) | (uint64_t)(j - P.facet_base);
+        const double* nrm = I.fnormal + 3 * j;
+        const double n0 = nrm[0], n1 = nrm[1], n2 = nrm[2];
+        const double pmax = I.pmax[p];
+        double v[3] = {0.0, 0.0, 0.0};
+        bool ok = false;
+        const int64_t ng = total - g0 < 64 ? total - g0 : 64;  // wavefront-uniform
+        for (int c = 0; c < ng; ++c) {
+            // broadcast candidate c's parameters
+            const uint64_t key_c = __shfl(key, c, 64), sj_c = __shfl(sj, c, 64);
+            const uint64_t slot_c = (uint64_t)__shfl(slot, c, 64);
+            const double vth = __shfl(P.vth, c, 64), vrange = __shfl(P.vrange, c, 64);
+            const double vd0 = __shfl(P.vd[0], c, 64), vd1 = __shfl(P.vd[1], c, 64), vd2 = __shfl(P.vd[2], c, 64);
+            const double fac = __shfl(P.factor, c, 64), vth2 = __shfl(P.vth2, c, 64);
+            const double m0 = __shfl(n0, c, 64), m1 = __shfl(n1, c, 64), m2 = __shfl(n2, c, 64);
+            const double pm = __shfl(pmax, c, 64);
+            const double lo0 = vd0 - vrange * vth, hi0 = vd0 + vrange * vth;
+            const double lo1 = vd1 - vrange * vth, hi1 = vd1 + vrange * vth;
+            const double lo2 = vd2 - vrange * vth, hi2 = vd2 + vrange * vth;
+            for (uint64_t a0 = 1; a0 < INJECT_MAX_ATTEMPTS; a0 += 64) {
+                U64x4 ca = {{step, sj_c, slot_c, a0 + (uint64_t)lane}};
+                const U64x4 ra = philox4x64_10(ca, key_c, TAG_INJECT);
+                const double w0 = lo0 + (hi0 - lo0) * u01(ra.v[0]);
+                const double w1 = lo1 + (hi1 - lo1) * u01(ra.v[1]);
+                const double w2 = lo2 + (hi2 - lo2) * u01(ra.v[2]);
+                double vn = 0.0;
+                vn += w0 * m0;
+                vn += w1 * m1;
+                vn += w2 * m2;
+                double vs = 0.0;
+                vs += (w0 - vd0) * (w0 - vd0);
+                vs += (w1 - vd1) * (w1 - vd1);
+                vs += (w2 - vd2) * (w2 - vd2);
+                const double f = (vn > 0.0) * vn * (fac * exp(-0.5 * vs / vth2));
+                const unsigned long long acc = __ballot(u01(ra.v[3]) * pm < f);
+                if (acc) {
+                    const int win = __ffsll((long long)acc) - 1;
+                    const double a = __shfl(w0, win, 64), b = __shfl(w1, win, 64), e = __shfl(w2, win, 64);
+                    if (lane == c) {
+                        v[0] = a;
+                        v[1] = b;
+                        v[2] = e;
+                        ok = true;
+                    }
+                    break;
+                }
+            }
+        }
+        if (k >= cand_cap) continue;
+        if (!mine) {
+            keep[k] = 0;
+            continue;
+        }
+        if (!ok) atomicAdd(errs, 1ull);
+        const double* V = I.fverts + 9 * j;
         U64x4 c0 = {{step, sj, (uint64_t)slot, 0ull}};
         const U64x4 r0 = philox4x64_10(c0, key, TAG_INJECT);
         double x[3];
@@ -576,28 +790,6 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
         for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
         r = sqrt(u01(r0.v[1]));
         for (int d = 0; d < 3; ++d) x[d] = r * x[d] + (1.0 - r) * V[6 + d];
-        const double pmax = I.pmax[p];
-        double v[3];
-        bool ok = false;
-        for (uint64_t a = 1; a < 1000000ull; ++a) {
-            U64x4 ca = {{step, sj, (uint64_t)slot, a}};
-            const U64x4 ra = philox4x64_10(ca, key, TAG_INJECT);
-            for (int d = 0; d < 3; ++d) {
-                const double lo_d = P.vd[d] - P.vrange * P.vth;
-                const double hi_d = P.vd[d] + P.vrange * P.vth;
-                v[d] = lo_d + (hi_d - lo_d) * u01(ra.v[d]);
-            }
-            double vn = 0.0;
-            for (int d = 0; d < 3; ++d) vn += v[d] * nrm[d];
-            double vs = 0.0;
-            for (int d = 0; d < 3; ++d) vs += (v[d] - P.vd[d]) * (v[d] - P.vd[d]);
-            const double f = (vn > 0.0) * vn * (P.factor * exp(-0.5 * vs / P.vth2));
-            if (u01(ra.v[3]) * pmax < f) {
-                ok = true;
-                break;
-            }
-        }
-        if (!ok) atomicAdd(errs, 1ull);
         const double rp = u01(r0.v[2]);
         for (int d = 0; d < 3; ++d) x[d] = x[d] + rp * dt * v[d];
         const int32_t c = ok ? walk(M, I.fcell[j], x[0], x[1], x[2]) : -1;
@@ -683,7 +875,8 @@ __global__ void k_next_step(DevState* __restrict__ S) {
 
 template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
 static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
-    hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT>), dim3(L.grid), dim3(L.block), 0, st,
+    const size_t shmem = (DEPOSIT && L.dep_mode == 2) ? LDS_TABLE_BYTES : 0;
+    hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT>), dim3(L.grid), dim3(L.block), shmem, st,
                        L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
                        L.rho_raw, L.ke_part, L.holes, L.absorbed, L.dep_mode);
     LAUNCH_CHECK();
