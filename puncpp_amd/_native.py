@@ -34,6 +34,7 @@ _SIGS = {
     "punc_gpu_element_volume": (C.c_int, [vp, C.c_int, d_p]),
     "punc_gpu_exterior_facets": (C.c_int, [vp, d_p, d_p, d_p, d_p, i32_p]),
     "punc_gpu_mesh_locate": (C.c_int, [vp, C.c_int64, d_p, i64_p]),
+    "punc_internal_inside_tol": (C.c_int, [vp, d_p]),
     "punc_gpu_maxwellian_flux": (C.c_int, [vp, C.c_double, d_p, d_p, d_p]),
     "punc_gpu_pop_create": (C.c_int, [vp, C.c_int, d_p, d_p, C.c_int64, vp, C.POINTER(vp)]),
     "punc_gpu_pop_destroy": (None, [vp]),

@@ -273,6 +273,7 @@ static int mesh_ensure_device(punc_mesh_s* M) {
     D.bcells = M->d_bcells;
     D.nc = nc;
     D.ndof = M->host.ndof;
+    D.inside_tol = T.inside_tol;
     for (int d = 0; d < 3; ++d) {
         D.gn[d] = T.grid_n[d];
         D.glo[d] = T.grid_lo[d];
@@ -1532,5 +1533,12 @@ int punc_internal_mesh_view(punc_mesh_t* m, DevMesh* out, int64_t* ndof, std::st
     }
     *out = m->dm;
     *ndof = m->topo.ndof;
+    return 0;
+}
+
+// Host value of the mesh's "certainly inside" margin (0: shortcut off).
+extern "C" int punc_internal_inside_tol(const punc_mesh_t* m, double* tol) {
+    if (!m || !tol) return -1;
+    *tol = m->topo.inside_tol;
     return 0;
 }

@@ -59,6 +59,7 @@ struct DevMesh {
     const int32_t* bcells;
     int64_t nc;
     int64_t ndof;
+    double inside_tol;  // 0: no "certainly inside" shortcut
     int gn[3];
     double glo[3];
     double gh[3];

@@ -167,17 +167,27 @@ __device__ inline void lds_flush(LdsTable t, double* __restrict__ rho_raw) {
     __syncthreads();
 }
 
+// ready: lpre/dpre already hold lambda at (x, y, z) in cell `key` and its dofs.
+template <int mode>
 __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_raw, int32_t key, double x,
-                                   double y, double z, double q, int mode) {
+                                   double y, double z, double q, bool ready, const double* lpre,
+                                   const int32_t* dpre) {
     double d[4] = {0.0, 0.0, 0.0, 0.0};
     int32_t dof[4] = {0, 0, 0, 0};
     if (key >= 0) {
-        const CellRec& R = M.rec[key];
-        double l[4];
-        bary4(R.bary, x, y, z, l);
-        for (int k = 0; k < 4; ++k) {
-            d[k] = q * l[k];
-            dof[k] = R.dof[k];
+        if (ready) {
+            for (int k = 0; k < 4; ++k) {
+                d[k] = q * lpre[k];
+                dof[k] = dpre[k];
+            }
+        } else {
+            const CellRec& R = M.rec[key];
+            double l[4];
+            bary4(R.bary, x, y, z, l);
+            for (int k = 0; k < 4; ++k) {
+                d[k] = q * l[k];
+                dof[k] = R.dof[k];
+            }
         }
     }
     if (mode >= 1) {
@@ -226,13 +236,13 @@ __device__ inline double block_sum(double v, double* sh) {
 // 1 move, 2 move_periodic.  LOCATE: relocation walk with removal bookkeeping.
 // DEPOSIT: CG1 charge deposit q*lambda_i at the (new) cell.
 // ---------------------------------------------------------------------------
-template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
+template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT, int DMODE>
 __global__ void __launch_bounds__(256)
 k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
             const double4* __restrict__ E, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes,
-            unsigned long long* __restrict__ absorbed, const int dep_mode) {
+            unsigned long long* __restrict__ absorbed) {
     __shared__ double sh[8];
     const PopArrays A = S->cur;
     const int64_t n = S->n;
@@ -240,7 +250,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     chunk_range(n, s, e);
     const int32_t cmask = (1 << cell_bits) - 1;
     double ke = 0.0;
-    const bool use_lds = DEPOSIT && dep_mode == 2;
+    constexpr bool use_lds = DEPOSIT && DMODE == 2;
     if (use_lds) {
         lds_clear(lds_table());
         __syncthreads();
@@ -253,6 +263,9 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         int32_t dkey = -1;  // deposit cell of this lane (-1: nothing to deposit)
         double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
         int32_t leave_key = -1;  // particle left: species*MAX_BND + bnd, or ABS_LOST
+        bool dready = false;     // dl/ddof already hold the deposit weights and dofs
+        double dl[4] = {0.0, 0.0, 0.0, 0.0};
+        int32_t ddof[4] = {0, 0, 0, 0};
         if (active) {
         const int32_t cs = A.cs[i];
         const int sp = cs >> cell_bits;
@@ -324,7 +337,21 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         bool alive = true;
         if (LOCATE) {
-            const int32_t c2 = walk(M, c, x, y, z);
+            // Certainly-inside shortcut: if all barycentric coordinates of the
+            // new position in the old cell exceed the mesh's rounding margin,
+            // every facet projection is negative and the walk would return
+            // the old cell; the coordinates are then also the deposit weights.
+            bool stay = false;
+            if (M.inside_tol > 0.0) {
+                const CellRec& R = M.rec[c];
+                bary4(R.bary, x, y, z, dl);
+                const double tol = M.inside_tol;
+                stay = dl[0] > tol && dl[1] > tol && dl[2] > tol && dl[3] > tol;
+                if (stay)
+                    for (int k = 0; k < 4; ++k) ddof[k] = R.dof[k];
+            }
+            const int32_t c2 = stay ? c : walk(M, c, x, y, z);
+            dready = stay;
             if (c2 < 0) {
                 alive = false;
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
@@ -343,7 +370,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         }  // active
         if (LOCATE) record_leavers(S, holes, absorbed, leave_key, i);
-        if (DEPOSIT) deposit_cg1(M, rho_raw, dkey, dx_, dy_, dz_, dq, dep_mode);
+        if (DEPOSIT) deposit_cg1<DMODE>(M, rho_raw, dkey, dx_, dy_, dz_, dq, dready, dl, ddof);
         if (use_lds) {
             __syncthreads();
             if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
@@ -873,24 +900,31 @@ __global__ void k_next_step(DevState* __restrict__ S) {
 #define LAUNCH_CHECK() \
     do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return e_; } while (0)
 
-template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT>
+template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT, int DMODE>
 static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
-    const size_t shmem = (DEPOSIT && L.dep_mode == 2) ? LDS_TABLE_BYTES : 0;
-    hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT>), dim3(L.grid), dim3(L.block), shmem, st,
-                       L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
-                       L.rho_raw, L.ke_part, L.holes, L.absorbed, L.dep_mode);
+    const size_t shmem = (DEPOSIT && DMODE == 2) ? LDS_TABLE_BYTES : 0;
+    hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT, DMODE>), dim3(L.grid), dim3(L.block), shmem,
+                       st, L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
+                       L.rho_raw, L.ke_part, L.holes, L.absorbed);
     LAUNCH_CHECK();
     return hipSuccess;
 }
 
+template <int PUSH, int MOVE, bool LOCATE>
+static hipError_t launch_pml(const ParticleLaunch& L, hipStream_t st) {
+    if (!L.deposit) return launch_particles_t<PUSH, MOVE, LOCATE, false, 0>(L, st);
+    switch (L.dep_mode) {
+        case 0: return launch_particles_t<PUSH, MOVE, LOCATE, true, 0>(L, st);
+        case 1: return launch_particles_t<PUSH, MOVE, LOCATE, true, 1>(L, st);
+        case 2: return launch_particles_t<PUSH, MOVE, LOCATE, true, 2>(L, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int PUSH, int MOVE>
 static hipError_t launch_pm(const ParticleLaunch& L, hipStream_t st) {
-    if (L.locate) {
-        if (L.deposit) return launch_particles_t<PUSH, MOVE, true, true>(L, st);
-        return launch_particles_t<PUSH, MOVE, true, false>(L, st);
-    }
-    if (L.deposit) return launch_particles_t<PUSH, MOVE, false, true>(L, st);
-    return launch_particles_t<PUSH, MOVE, false, false>(L, st);
+    if (L.locate) return launch_pml<PUSH, MOVE, true>(L, st);
+    return launch_pml<PUSH, MOVE, false>(L, st);
 }
 
 hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st) {

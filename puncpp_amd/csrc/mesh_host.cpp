@@ -355,6 +355,65 @@ bool build_topology(const HostMesh& m, MeshTopo& t, std::string& err) {
         }
         t.volume += tet_volume(vc);
     }
+    // Margin for the "certainly inside" shortcut of the relocation walk.
+    // Notation for cell c: P_j its vertices, pi_i the affine function of the
+    // stored plane of facet i (opposite P_j for j = i), lh_i the affine
+    // function of stored barycentric row i, mu_j the exact barycentric
+    // coordinates.  Computed values differ from exact ones by at most
+    // ep_i = 8u(|c0| + |n|_1 X) and el_i = 8u(|A_i0| + |A_i|_1 X) (lambda_3:
+    // the sum of the others plus 16u), u = 2^-53, X >= every |coordinate|.
+    // Affine functions are fixed by their vertex values, so with
+    // alpha = max |lh_i(P_j) - delta_ij|, eta_i = max_{j != i} |pi_i(P_j)| and
+    // h_i = -pi_i(P_i) > 0:  if all computed lambda_i > tol then
+    // S = sum |mu_j| <= 1/(1 - 8 alpha), mu_i > tol - el_i - alpha S and the
+    // computed pi_i < -(tol - el_i - alpha S) h_i + S eta_i + ep_i, which is
+    // negative once tol > el_i + alpha S + (S eta_i + ep_i)/h_i.  The vertex
+    // values are themselves computed, so each enters with its own bound.
+    {
+        double X = 0.0;
+        for (double v : m.xyz) X = std::max(X, fabs(v));
+        X = 2.0 * X + 1.0;
+        const double u = 0x1.0p-53;
+        double worst = 0.0;
+        for (int64_t c = 0; c < nc && worst < 1e300; ++c) {
+            const double* A = t.rec[c].bary;
+            const double* P = &t.vcoords[c * 12];
+            double el[4], lv[4][4];  // lv[i][j]: lambda_i at vertex j
+            for (int r = 0; r < 3; ++r)
+                el[r] = 8 * u * (fabs(A[4 * r]) + (fabs(A[4 * r + 1]) + fabs(A[4 * r + 2]) + fabs(A[4 * r + 3])) * X);
+            el[3] = el[0] + el[1] + el[2] + 16 * u;
+            for (int j = 0; j < 4; ++j) {
+                const double* q = P + 3 * j;
+                for (int r = 0; r < 3; ++r)
+                    lv[r][j] = ((A[4 * r] + A[4 * r + 1] * q[0]) + A[4 * r + 2] * q[1]) + A[4 * r + 3] * q[2];
+                lv[3][j] = ((1 - lv[0][j]) - lv[1][j]) - lv[2][j];
+            }
+            double alpha = 0.0;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) alpha = std::max(alpha, fabs(lv[i][j] - (i == j ? 1.0 : 0.0)) + el[i]);
+            if (!(alpha < 0.01)) {
+                worst = 1e300;
+                break;
+            }
+            const double S = 1.0 / (1.0 - 8.0 * alpha);
+            for (int i = 0; i < 4; ++i) {
+                const double* p = t.planes[c].p + 4 * i;
+                const double ep = 8 * u * (fabs(p[0]) + (fabs(p[1]) + fabs(p[2]) + fabs(p[3])) * X);
+                double h = 0.0, eta = 0.0;
+                for (int j = 0; j < 4; ++j) {
+                    const double* q = P + 3 * j;
+                    const double v = ((p[0] + p[1] * q[0]) + p[2] * q[1]) + p[3] * q[2];
+                    if (j == i)
+                        h = -v - ep;
+                    else
+                        eta = std::max(eta, fabs(v) + ep);
+                }
+                worst = (h > 0) ? std::max(worst, el[i] + alpha * S + (S * eta + ep) / h) : 1e300;
+            }
+        }
+        t.inside_tol = 2 * worst;
+        if (!(t.inside_tol < 1e-6)) t.inside_tol = 0;  // degenerate mesh: always walk
+    }
     t.bnd_ids.clear();
     for (int b = 1; b < 64; ++b)
         if (seen_bnd[b]) t.bnd_ids.push_back(b);

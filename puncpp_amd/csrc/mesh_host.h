@@ -53,6 +53,11 @@ struct MeshTopo {
     std::vector<int32_t> bnd_ids;     // distinct boundary ids present (>= 1)
     double bbox_lo[3], bbox_hi[3];
     double coord_min = 0, coord_max = 0;  // min/max over all coordinates (UniformPosition domain)
+    // Barycentric margin above which a point is certainly inside a cell for
+    // the plane test: if all four computed lambda_i exceed it, every computed
+    // facet projection is negative (rounding bounds of both, see
+    // build_topology).  0 disables the shortcut.
+    double inside_tol = 0;
     double volume = 0;                // total mesh volume
     // Uniform bucket grid for point location.
     int grid_n[3] = {1, 1, 1};
