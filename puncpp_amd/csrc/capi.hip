@@ -86,6 +86,7 @@ struct punc_pop_s {
     int cell_bits = 0;
     int64_t capacity = 0;
     int grid = 0, block = 256;
+    int num_cu = 1;
     int dep_mode = 1;
 
     DevState* d_state = nullptr;
@@ -97,6 +98,9 @@ struct punc_pop_s {
     unsigned long long* d_cnt = nullptr;
     unsigned long long* d_errs = nullptr;  // [0] load failures, [1] injection failures
     double4* d_E = nullptr;
+    double* d_cellE = nullptr;  // per-cell copy of the field (16 doubles per cell)
+    int32_t* d_walk = nullptr;    // fused step: walk list
+    int32_t* d_wcount = nullptr;  // fused step: walk-list segment lengths (ke_np entries)
     bool field_set = false;
     double* d_rho_raw = nullptr;
     double* d_rho = nullptr;
@@ -373,6 +377,9 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_cnt);
     hipFree(P->d_errs);
     hipFree(P->d_E);
+    hipFree(P->d_cellE);
+    hipFree(P->d_walk);
+    hipFree(P->d_wcount);
     hipFree(P->d_rho_raw);
     hipFree(P->d_rho);
     hipFree(P->d_dv_inv);
@@ -441,6 +448,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     hipDeviceProp_t prop;
     PCHK(hipGetDeviceProperties(&prop, P->device));
     P->grid = prop.multiProcessorCount * 8;
+    P->num_cu = prop.multiProcessorCount;
     P->ke_np = 65536;
     for (int b = 0; b < 2; ++b) {
         for (int k = 0; k < 6; ++k) PCHK(dalloc((double**)&P->bufs[b][k], capacity));
@@ -448,6 +456,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     }
     PCHK(dalloc(&P->d_state, 1));
     PCHK(dalloc(&P->d_holes, capacity));
+    PCHK(dalloc(&P->d_walk, capacity));
     PCHK(dalloc(&P->d_H, capacity));
     PCHK(dalloc(&P->d_S, capacity));
     PCHK(dalloc(&P->d_absorbed, (size_t)MAX_SPECIES * MAX_BND));
@@ -459,8 +468,11 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_rho, ndof));
     PCHK(dalloc(&P->d_dv_inv, ndof));
     PCHK(dalloc(&P->d_ke_part, P->ke_np));
+    PCHK(dalloc(&P->d_wcount, P->ke_np));
     PCHK(dalloc(&P->d_ke, 4));
     const int64_t nc = m->topo.nc;
+    PCHK(dalloc(&P->d_cellE, nc * 16));
+    PCHK(hipMemset(P->d_cellE, 0, nc * 16 * sizeof(double)));
     PCHK(dalloc(&P->d_hist, nc));
     PCHK(dalloc(&P->d_off, nc));
     P->bsum_len = std::max<int64_t>(nc / 1024 + 2, 4096);
@@ -759,7 +771,9 @@ static ParticleLaunch base_launch(punc_pop_s* P) {
     memset(&L, 0, sizeof(L));
     L.mesh = P->mesh->dm;
     L.state = P->d_state;
-    L.E = P->d_E;
+    L.cellE = P->d_cellE;
+    L.walk = P->d_walk;
+    L.wcount = P->d_wcount;
     L.cell_bits = P->cell_bits;
     L.rho_raw = P->d_rho_raw;
     L.ke_part = P->d_ke_part;
@@ -771,8 +785,10 @@ static ParticleLaunch base_launch(punc_pop_s* P) {
     return L;
 }
 
-static hipError_t timed_launch(punc_pop_s* P, const ParticleLaunch& L) {
-    if (!P->timing) return launch_particles(L, P->stream);
+// passes: launch the fused step's two passes instead of one particle kernel.
+static hipError_t timed_launch(punc_pop_s* P, const ParticleLaunch& L, bool passes = false) {
+    auto go = [&]() { return passes ? launch_step_passes(L, P->stream) : launch_particles(L, P->stream); };
+    if (!P->timing) return go();
     if (P->ev_used == P->ev.size()) {
         std::pair<hipEvent_t, hipEvent_t> pr;
         hipError_t e = hipEventCreate(&pr.first);
@@ -784,7 +800,7 @@ static hipError_t timed_launch(punc_pop_s* P, const ParticleLaunch& L) {
     auto& pr = P->ev[P->ev_used++];
     hipError_t e = hipEventRecord(pr.first, P->stream);
     if (e != hipSuccess) return e;
-    e = launch_particles(L, P->stream);
+    e = go();
     if (e != hipSuccess) return e;
     return hipEventRecord(pr.second, P->stream);
 }
@@ -812,6 +828,7 @@ extern "C" int punc_gpu_set_field(punc_pop_t* p, const double* E) {
     const int64_t ndof = p->mesh->topo.ndof;
     HIPCHK(hipMemcpy2DAsync(p->d_E, sizeof(double4), E, 3 * sizeof(double), 3 * sizeof(double), ndof,
                             hipMemcpyHostToDevice, p->stream));
+    HIPCHK(launch_cell_field(p->mesh->dm, p->d_E, p->d_cellE, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     p->field_set = true;
     return 0;
@@ -1171,7 +1188,7 @@ static int step_prepare(punc_pop_s* P, const StepArgs& a, const SpCoef** coef) {
 // Kernel launches only: safe to capture into a graph.
 static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bool advance_step) {
     const int64_t ndof = P->mesh->topo.ndof;
-    HIPCHK(launch_fill_f64(P->d_rho_raw, ndof, 0.0, P->stream));
+    HIPCHK(launch_step_begin(P->d_rho_raw, ndof, P->stream));
     ParticleLaunch L = base_launch(P);
     L.coef = coef;
     L.push = a.pusher;
@@ -1180,7 +1197,7 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
     for (int i = 0; i < 3; ++i) L.Ld[i] = a.Ld[i];
     L.locate = true;
     L.deposit = true;
-    HIPCHK(timed_launch(P, L));
+    HIPCHK(timed_launch(P, L, true));
     HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
     HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_H, P->d_S, P->grid, P->stream));
     if (a.inject) {

@@ -10,7 +10,7 @@ struct ParticleLaunch {
     DevMesh mesh;
     DevState* state;
     const SpCoef* coef;
-    const double4* E;
+    const double* cellE;  // per-cell field copy (k_cell_field)
     double dt;
     double Ld[3];
     int cell_bits;
@@ -18,11 +18,13 @@ struct ParticleLaunch {
     double* ke_part;
     int32_t* holes;
     unsigned long long* absorbed;
+    int32_t* walk;        // fused step: walk list (capacity entries, one segment per block)
+    int32_t* wcount;      // fused step: walk-list segment length per block (grid entries)
     int push;  // 0 none, 1 accel, 2 boris
     int move;  // 0 none, 1 plain, 2 periodic
     bool locate;
     bool deposit;
-    int dep_mode;  // 0: per-particle atomics, 1: wavefront run reduction first
+    int dep_mode;  // 0: per-particle atomics, 1: wavefront run reduction, 2: + LDS table
     int grid;
     int block;
 };
@@ -51,6 +53,14 @@ struct InjectLaunch {
 };
 
 hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st);
+// Fused step in two passes (push + move + in-cell relocation + deposit, then
+// the relocation walk + deposit for the particles that left their cell).
+// Requires L.push != 0, L.move != 0, L.locate and L.deposit.
+hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st);
+// Zero rho_raw before a fused step.
+hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st);
+// Per-cell field copy: cellE[c*16 + 3k + j] = E[dof_k(c)].j.
+hipError_t launch_cell_field(const DevMesh& M, const double4* E, double* cellE, hipStream_t st);
 hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double* ke_part, int grid,
                           hipStream_t st);
 hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);

@@ -18,6 +18,35 @@ namespace punc_gpu {
 // ---------------------------------------------------------------------------
 // Helpers
 // ---------------------------------------------------------------------------
+// Particle array pointers are read from DevState, so the compiler cannot tell
+// their address space and would emit FLAT loads and stores.  FLAT operations
+// also count against the LDS/scalar wait counter: every wait for an LDS
+// result (deposit table, cross-lane shuffles) would then also wait for all
+// outstanding particle loads and stores.  Viewing them through global-address-
+// space pointers gives plain global_load/global_store.
+#define GLOBAL __attribute__((address_space(1)))
+struct GArrays {
+    GLOBAL double* x;
+    GLOBAL double* y;
+    GLOBAL double* z;
+    GLOBAL double* vx;
+    GLOBAL double* vy;
+    GLOBAL double* vz;
+    GLOBAL int32_t* cs;
+};
+
+__device__ inline GArrays gview(const PopArrays& a) {
+    GArrays g;
+    g.x = (GLOBAL double*)a.x;
+    g.y = (GLOBAL double*)a.y;
+    g.z = (GLOBAL double*)a.z;
+    g.vx = (GLOBAL double*)a.vx;
+    g.vy = (GLOBAL double*)a.vy;
+    g.vz = (GLOBAL double*)a.vz;
+    g.cs = (GLOBAL int32_t*)a.cs;
+    return g;
+}
+
 __device__ inline int64_t logical_block() {
     const int64_t G = gridDim.x, b = blockIdx.x;
     if (G % 8 == 0) return (b % 8) * (G / 8) + (b / 8);
@@ -231,6 +260,57 @@ __device__ inline double block_sum(double v, double* sh) {
     return r;
 }
 
+// Per-cell copy of the CG1 field: the 3 components at the cell's 4 dofs in
+// local vertex order (12 doubles, 128-byte stride).  Reading it instead of
+// E[dof] removes one dependent gather from the push.
+constexpr int CELL_FIELD_STRIDE = 16;
+
+__device__ inline void load_cell_field(const double* __restrict__ cellE, int32_t c, double ef[12]) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const GLOBAL v2d* p = (const GLOBAL v2d*)(cellE + (int64_t)c * CELL_FIELD_STRIDE);
+    for (int m = 0; m < 6; ++m) {
+        const v2d t = p[m];
+        ef[2 * m] = t.x;
+        ef[2 * m + 1] = t.y;
+    }
+}
+
+// One velocity update at barycentric coordinates l in a cell whose field
+// values are ef.  PUSH 1: accel_cg1, 2: boris_cg1 (coefficients from SpCoef).
+template <int PUSH>
+__device__ inline void push_velocity(const SpCoef& K, const double l[4], const double ef[12], double& vx,
+                                     double& vy, double& vz, double& ke) {
+    // Ei = sum_i coeffs[i]*values[j*4+i] starting from 0.0 (pusher.h:188-192, 324-328)
+    double Ex = 0.0, Ey = 0.0, Ez = 0.0;
+    Ex += l[0] * ef[0]; Ex += l[1] * ef[3]; Ex += l[2] * ef[6]; Ex += l[3] * ef[9];
+    Ey += l[0] * ef[1]; Ey += l[1] * ef[4]; Ey += l[2] * ef[7]; Ey += l[3] * ef[10];
+    Ez += l[0] * ef[2]; Ez += l[1] * ef[5]; Ez += l[2] * ef[8]; Ez += l[3] * ef[11];
+    if (PUSH == 1) {
+        // accel_cg1 (pusher.h:194-196): Ei *= dt*(q/m); KE += 0.5*m*v*(v+Ei); v += Ei
+        Ex = Ex * K.a; Ey = Ey * K.a; Ez = Ez * K.a;
+        ke += (K.half_m * vx) * (vx + Ex);
+        ke += (K.half_m * vy) * (vy + Ey);
+        ke += (K.half_m * vz) * (vz + Ez);
+        vx += Ex; vy += Ey; vz += Ez;
+    } else {
+        // boris_cg1 (pusher.h:343-368)
+        const double mx = vx + K.a * Ex, my = vy + K.a * Ey, mz = vz + K.a * Ez;
+        ke += (K.half_m * mx) * mx;
+        ke += (K.half_m * my) * my;
+        ke += (K.half_m * mz) * mz;
+        const double cx = my * K.t[2] - mz * K.t[1];
+        const double cy = -mx * K.t[2] + mz * K.t[0];
+        const double cz = mx * K.t[1] - my * K.t[0];
+        const double px = mx + cx, py = my + cy, pz = mz + cz;
+        const double dx = py * K.s[2] - pz * K.s[1];
+        const double dy = -px * K.s[2] + pz * K.s[0];
+        const double dz = px * K.s[1] - py * K.s[0];
+        vx = (mx + dx) + K.a * Ex;
+        vy = (my + dy) + K.a * Ey;
+        vz = (mz + dz) + K.a * Ez;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The particle kernel.  PUSH: 0 none, 1 accel_cg1, 2 boris_cg1.  MOVE: 0 none,
 // 1 move, 2 move_periodic.  LOCATE: relocation walk with removal bookkeeping.
@@ -239,12 +319,12 @@ __device__ inline double block_sum(double v, double* sh) {
 template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT, int DMODE>
 __global__ void __launch_bounds__(256)
 k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
-            const double4* __restrict__ E, const double dt, const double Lx, const double Ly,
+            const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes,
             unsigned long long* __restrict__ absorbed) {
     __shared__ double sh[8];
-    const PopArrays A = S->cur;
+    const GArrays A = gview(S->cur);
     const int64_t n = S->n;
     int64_t s, e;
     chunk_range(n, s, e);
@@ -284,39 +364,10 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         if (PUSH != 0) {
             const CellRec& R = M.rec[c];
-            double l[4];
+            double l[4], ef[12];
             bary4(R.bary, x, y, z, l);
-            const double4 e0 = E[R.dof[0]], e1 = E[R.dof[1]], e2 = E[R.dof[2]], e3 = E[R.dof[3]];
-            // Ei = sum_i coeffs[i]*values[j*4+i] starting from 0.0 (pusher.h:188-192, 324-328)
-            double Ex = 0.0, Ey = 0.0, Ez = 0.0;
-            Ex += l[0] * e0.x; Ex += l[1] * e1.x; Ex += l[2] * e2.x; Ex += l[3] * e3.x;
-            Ey += l[0] * e0.y; Ey += l[1] * e1.y; Ey += l[2] * e2.y; Ey += l[3] * e3.y;
-            Ez += l[0] * e0.z; Ez += l[1] * e1.z; Ez += l[2] * e2.z; Ez += l[3] * e3.z;
-            const SpCoef& K = coef[sp];
-            if (PUSH == 1) {
-                // accel_cg1 (pusher.h:194-196): Ei *= dt*(q/m); KE += 0.5*m*v*(v+Ei); v += Ei
-                Ex = Ex * K.a; Ey = Ey * K.a; Ez = Ez * K.a;
-                ke += (K.half_m * vx) * (vx + Ex);
-                ke += (K.half_m * vy) * (vy + Ey);
-                ke += (K.half_m * vz) * (vz + Ez);
-                vx += Ex; vy += Ey; vz += Ez;
-            } else {
-                // boris_cg1 (pusher.h:343-368)
-                const double mx = vx + K.a * Ex, my = vy + K.a * Ey, mz = vz + K.a * Ez;
-                ke += (K.half_m * mx) * mx;
-                ke += (K.half_m * my) * my;
-                ke += (K.half_m * mz) * mz;
-                const double cx = my * K.t[2] - mz * K.t[1];
-                const double cy = -mx * K.t[2] + mz * K.t[0];
-                const double cz = mx * K.t[1] - my * K.t[0];
-                const double px = mx + cx, py = my + cy, pz = mz + cz;
-                const double dx = py * K.s[2] - pz * K.s[1];
-                const double dy = -px * K.s[2] + pz * K.s[0];
-                const double dz = px * K.s[1] - py * K.s[0];
-                vx = (mx + dx) + K.a * Ex;
-                vy = (my + dy) + K.a * Ey;
-                vz = (mz + dz) + K.a * Ez;
-            }
+            load_cell_field(cellE, c, ef);
+            push_velocity<PUSH>(coef[sp], l, ef, vx, vy, vz, ke);
             A.vx[i] = vx;
             A.vy[i] = vy;
             A.vz[i] = vz;
@@ -383,12 +434,184 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused step, pass 1: push, move, and the in-cell test of the relocation.  A
+// particle whose barycentric coordinates in its old cell all exceed the mesh
+// margin stays there (the walk would return that cell) and deposits with
+// those coordinates; every other particle is appended to the walk list for
+// pass 2.  No loop in here depends on the particle, so whole wavefronts run
+// the same instruction stream.  Each block writes its walkers into its own
+// segment of the list (starting at its chunk start, so it cannot overflow)
+// through an LDS counter, and stores the segment length in wcount: no global
+// atomics, which would all hit one address.
+// ---------------------------------------------------------------------------
+template <int PUSH, int MOVE, int DMODE>
+__global__ void __launch_bounds__(256)
+k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
+            const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
+            const double Lz, const int cell_bits, double* __restrict__ rho_raw,
+            double* __restrict__ ke_part, int32_t* __restrict__ wlist, int32_t* __restrict__ wcount) {
+    __shared__ double sh[8];
+    __shared__ int32_t wcnt;
+    const GArrays A = gview(S->cur);
+    const int64_t n = S->n;
+    int64_t s, e;
+    chunk_range(n, s, e);
+    const int32_t cmask = (1 << cell_bits) - 1;
+    const double tol = M.inside_tol;
+    double ke = 0.0;
+    constexpr bool use_lds = DMODE == 2;
+    if (threadIdx.x == 0) wcnt = 0;
+    if (use_lds) lds_clear(lds_table());
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = s; base < e; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        const bool active = i < e;
+        int32_t dkey = -1;
+        double dq = 0.0;
+        double dl[4] = {0.0, 0.0, 0.0, 0.0};
+        int32_t ddof[4] = {0, 0, 0, 0};
+        bool walker = false;
+        if (active) {
+            const int32_t cs = A.cs[i];
+            const int sp = cs >> cell_bits;
+            const int32_t c = cs & cmask;
+            double x = A.x[i], y = A.y[i], z = A.z[i];
+            double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+            const CellRec& R = M.rec[c];
+            double l[4], ef[12];
+            bary4(R.bary, x, y, z, l);
+            load_cell_field(cellE, c, ef);
+            const SpCoef& K = coef[sp];
+            push_velocity<PUSH>(K, l, ef, vx, vy, vz, ke);
+            A.vx[i] = vx;
+            A.vy[i] = vy;
+            A.vz[i] = vz;
+            // move / move_periodic (pusher.h:658, 676-677)
+            x += dt * vx;
+            y += dt * vy;
+            z += dt * vz;
+            if (MOVE == 2) {
+                x -= Lx * floor(x / Lx);
+                y -= Ly * floor(y / Ly);
+                z -= Lz * floor(z / Lz);
+            }
+            A.x[i] = x;
+            A.y[i] = y;
+            A.z[i] = z;
+            bary4(R.bary, x, y, z, dl);
+            const bool stay = tol > 0.0 && dl[0] > tol && dl[1] > tol && dl[2] > tol && dl[3] > tol;
+            if (stay) {
+                dkey = c;
+                dq = K.q;
+                for (int k = 0; k < 4; ++k) ddof[k] = R.dof[k];
+            } else {
+                walker = true;
+            }
+        }
+        // Wavefront-aggregated append to this block's walk-list segment.
+        const unsigned long long wm = __ballot(walker);
+        if (wm) {
+            const int leader = __ffsll((long long)wm) - 1;
+            int wb = 0;
+            if (lane == leader) wb = atomicAdd(&wcnt, __popcll(wm));
+            wb = __shfl(wb, leader, 64);
+            if (walker) wlist[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int32_t)i;
+        }
+        deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
+        if (use_lds) {
+            __syncthreads();
+            if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
+        }
+    }
+    if (use_lds) lds_flush(lds_table(), rho_raw);
+    const double t = block_sum(ke, sh);  // synchronizes: wcnt is final
+    if (threadIdx.x == 0) {
+        ke_part[blockIdx.x] = t;
+        wcount[logical_block()] = wcnt;
+    }
+}
+
+// Fused step, pass 2: the relocation walk (reference relocate_fast) for the
+// particles on the walk list, removal bookkeeping, and their deposit.  Runs
+// with pass 1's grid and block: block b handles the segment pass 1's block b
+// wrote.
+template <int DMODE>
+__global__ void __launch_bounds__(256)
+k_step_walk(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, const int cell_bits,
+            double* __restrict__ rho_raw, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
+            const int32_t* __restrict__ wlist, const int32_t* __restrict__ wcount) {
+    const GArrays A = gview(S->cur);
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    e = s + wcount[logical_block()];
+    const int32_t cmask = (1 << cell_bits) - 1;
+    constexpr bool use_lds = DMODE == 2;
+    if (use_lds) {
+        lds_clear(lds_table());
+        __syncthreads();
+    }
+    for (int64_t base = s; base < e; base += blockDim.x) {
+        const int64_t j = base + threadIdx.x;
+        int32_t dkey = -1;
+        double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
+        int32_t leave_key = -1;
+        int64_t i = 0;
+        if (j < e) {
+            i = wlist[j];
+            const int32_t cs = A.cs[i];
+            const int sp = cs >> cell_bits;
+            const int32_t c = cs & cmask;
+            const double x = A.x[i], y = A.y[i], z = A.z[i];
+            const int32_t c2 = walk(M, c, x, y, z);
+            if (c2 < 0) {
+                leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
+                A.cs[i] = c2;
+            } else {
+                if (c2 != c) A.cs[i] = (sp << cell_bits) | c2;
+                dkey = c2;
+                dx_ = x;
+                dy_ = y;
+                dz_ = z;
+                dq = coef[sp].q;
+            }
+        }
+        record_leavers(S, holes, absorbed, leave_key, i);
+        deposit_cg1<DMODE>(M, rho_raw, dkey, dx_, dy_, dz_, dq, false, nullptr, nullptr);
+        if (use_lds) {
+            __syncthreads();
+            if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
+        }
+    }
+    if (use_lds) lds_flush(lds_table(), rho_raw);
+}
+
+__global__ void k_step_begin(double* __restrict__ rho_raw, int64_t ndof) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ndof; k += (int64_t)gridDim.x * blockDim.x)
+        rho_raw[k] = 0.0;
+}
+
+__global__ void k_cell_field(const DevMesh M, const double4* __restrict__ E, double* __restrict__ cellE) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < M.nc; c += (int64_t)gridDim.x * blockDim.x) {
+        const CellRec& R = M.rec[c];
+        double* o = cellE + c * CELL_FIELD_STRIDE;
+        for (int k = 0; k < 4; ++k) {
+            const double4 v = E[R.dof[k]];
+            o[3 * k] = v.x;
+            o[3 * k + 1] = v.y;
+            o[3 * k + 2] = v.z;
+        }
+        for (int k = 12; k < CELL_FIELD_STRIDE; ++k) o[k] = 0.0;
+    }
+}
+
 // Kinetic energy sum 0.5*m*v.v (reference diagnostics.h:248-265).
 __global__ void __launch_bounds__(256)
 k_kinetic(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
           double* __restrict__ ke_part) {
     __shared__ double sh[8];
-    const PopArrays A = S->cur;
+    const GArrays A = gview(S->cur);
     int64_t s, e;
     chunk_range(S->n, s, e);
     double ke = 0.0;
@@ -447,7 +670,7 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
 __global__ void k_compact_collect(DevState* __restrict__ S, const int32_t* __restrict__ holes,
                                   int32_t* __restrict__ H, int32_t* __restrict__ Sv) {
     const int64_t n = S->n, L = S->n_holes, nn = n - L;
-    const int32_t* cs = S->cur.cs;
+    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     for (int64_t k = t0; k < L; k += stride) {
@@ -460,7 +683,7 @@ __global__ void k_compact_collect(DevState* __restrict__ S, const int32_t* __res
 
 __global__ void k_compact_move(DevState* __restrict__ S, const int32_t* __restrict__ H,
                                const int32_t* __restrict__ Sv) {
-    const PopArrays A = S->cur;
+    const GArrays A = gview(S->cur);
     const int64_t m = S->nH;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -493,7 +716,7 @@ k_count_species(DevState* __restrict__ S, int cell_bits, int nsp, unsigned long 
     __shared__ unsigned int c[256];
     for (int k = threadIdx.x; k < nsp; k += blockDim.x) c[k] = 0;
     __syncthreads();
-    const int32_t* cs = S->cur.cs;
+    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
     const int64_t n = S->n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -593,7 +816,7 @@ __device__ inline RunInfo lane_run(int32_t key) {
 // take their slots with one atomic per run (the population is re-sorted
 // often, so runs are long).
 __global__ void k_sort_hist(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ hist) {
-    const int32_t* cs = S->cur.cs;
+    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
     const int64_t n = S->n;
     const int32_t cmask = (1 << cell_bits) - 1;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
@@ -605,7 +828,7 @@ __global__ void k_sort_hist(DevState* __restrict__ S, int cell_bits, int64_t* __
 }
 
 __global__ void k_sort_scatter(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ cursor) {
-    const PopArrays A = S->cur, B = S->alt;
+    const GArrays A = gview(S->cur), B = gview(S->alt);
     const int64_t n = S->n;
     const int32_t cmask = (1 << cell_bits) - 1;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
@@ -650,7 +873,7 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin, double cmax,
        double amp, double kw, int axis, double vth, double vd0, double vd1, double vd2,
        unsigned long long* __restrict__ errs) {
-    const PopArrays A = S->cur;
+    const GArrays A = gview(S->cur);
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N;
          k += (int64_t)gridDim.x * blockDim.x) {
         double p[3];
@@ -839,7 +1062,7 @@ k_inject_append(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restr
                 int64_t cand_cap, const int64_t* __restrict__ ntot, const double* __restrict__ cx,
                 const int32_t* __restrict__ ccs, const int64_t* __restrict__ pos, int cell_bits,
                 double* __restrict__ rho_raw) {
-    const PopArrays A = S->cur;
+    const GArrays A = gview(S->cur);
     const int64_t n0 = S->n, cap = S->capacity;
     const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
     const int32_t cmask = (1 << cell_bits) - 1;
@@ -904,7 +1127,7 @@ template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT, int DMODE>
 static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
     const size_t shmem = (DEPOSIT && DMODE == 2) ? LDS_TABLE_BYTES : 0;
     hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT, DMODE>), dim3(L.grid), dim3(L.block), shmem,
-                       st, L.mesh, L.state, L.coef, L.E, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
+                       st, L.mesh, L.state, L.coef, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
                        L.rho_raw, L.ke_part, L.holes, L.absorbed);
     LAUNCH_CHECK();
     return hipSuccess;
@@ -925,6 +1148,54 @@ template <int PUSH, int MOVE>
 static hipError_t launch_pm(const ParticleLaunch& L, hipStream_t st) {
     if (L.locate) return launch_pml<PUSH, MOVE, true>(L, st);
     return launch_pml<PUSH, MOVE, false>(L, st);
+}
+
+static int fill_grid(int64_t n);
+
+template <int PUSH, int MOVE, int DMODE>
+static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
+    const size_t shmem = DMODE == 2 ? LDS_TABLE_BYTES : 0;
+    hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
+                       L.coef, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw, L.ke_part,
+                       L.walk, L.wcount);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_step_walk<DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
+                       L.coef, L.cell_bits, L.rho_raw, L.holes, L.absorbed, L.walk, L.wcount);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int PUSH, int MOVE>
+static hipError_t launch_step_pm(const ParticleLaunch& L, hipStream_t st) {
+    switch (L.dep_mode) {
+        case 0: return launch_step_t<PUSH, MOVE, 0>(L, st);
+        case 1: return launch_step_t<PUSH, MOVE, 1>(L, st);
+        case 2: return launch_step_t<PUSH, MOVE, 2>(L, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st) {
+    if (!L.locate || !L.deposit || !L.walk || !L.wcount) return hipErrorInvalidValue;
+    switch (L.push * 3 + L.move) {
+        case 4: return launch_step_pm<1, 1>(L, st);
+        case 5: return launch_step_pm<1, 2>(L, st);
+        case 7: return launch_step_pm<2, 1>(L, st);
+        case 8: return launch_step_pm<2, 2>(L, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_begin, dim3(fill_grid(ndof)), dim3(256), 0, st, rho_raw, ndof);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_cell_field(const DevMesh& M, const double4* E, double* cellE, hipStream_t st) {
+    hipLaunchKernelGGL(k_cell_field, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, E, cellE);
+    LAUNCH_CHECK();
+    return hipSuccess;
 }
 
 hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st) {
