@@ -99,8 +99,7 @@ struct punc_pop_s {
     unsigned long long* d_errs = nullptr;  // [0] load failures, [1] injection failures
     double4* d_E = nullptr;
     double* d_cellE = nullptr;  // per-cell copy of the field (16 doubles per cell)
-    int32_t* d_walk = nullptr;    // fused step: walk list
-    int32_t* d_wcount = nullptr;  // fused step: walk-list segment lengths (ke_np entries)
+    double4* d_walk = nullptr;  // fused step: walk list (32-byte records)
     bool field_set = false;
     double* d_rho_raw = nullptr;
     double* d_rho = nullptr;
@@ -379,7 +378,6 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_E);
     hipFree(P->d_cellE);
     hipFree(P->d_walk);
-    hipFree(P->d_wcount);
     hipFree(P->d_rho_raw);
     hipFree(P->d_rho);
     hipFree(P->d_dv_inv);
@@ -468,7 +466,6 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_rho, ndof));
     PCHK(dalloc(&P->d_dv_inv, ndof));
     PCHK(dalloc(&P->d_ke_part, P->ke_np));
-    PCHK(dalloc(&P->d_wcount, P->ke_np));
     PCHK(dalloc(&P->d_ke, 4));
     const int64_t nc = m->topo.nc;
     PCHK(dalloc(&P->d_cellE, nc * 16));
@@ -773,7 +770,6 @@ static ParticleLaunch base_launch(punc_pop_s* P) {
     L.state = P->d_state;
     L.cellE = P->d_cellE;
     L.walk = P->d_walk;
-    L.wcount = P->d_wcount;
     L.cell_bits = P->cell_bits;
     L.rho_raw = P->d_rho_raw;
     L.ke_part = P->d_ke_part;

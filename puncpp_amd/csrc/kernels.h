@@ -18,8 +18,7 @@ struct ParticleLaunch {
     double* ke_part;
     int32_t* holes;
     unsigned long long* absorbed;
-    int32_t* walk;        // fused step: walk list (capacity entries, one segment per block)
-    int32_t* wcount;      // fused step: walk-list segment length per block (grid entries)
+    void* walk;           // fused step: walk list, capacity records of 32 bytes
     int push;  // 0 none, 1 accel, 2 boris
     int move;  // 0 none, 1 plain, 2 periodic
     bool locate;
@@ -53,8 +52,8 @@ struct InjectLaunch {
 };
 
 hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st);
-// Fused step in two passes (push + move + in-cell relocation + deposit, then
-// the relocation walk + deposit for the particles that left their cell).
+// Fused step (push + move + in-cell relocation + deposit, then the
+// relocation walk + deposit for the particles that left their cell).
 // Requires L.push != 0, L.move != 0, L.locate and L.deposit.
 hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st);
 // Zero rho_raw before a fused step.

@@ -435,22 +435,30 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Fused step, pass 1: push, move, and the in-cell test of the relocation.  A
-// particle whose barycentric coordinates in its old cell all exceed the mesh
-// margin stays there (the walk would return that cell) and deposits with
-// those coordinates; every other particle is appended to the walk list for
-// pass 2.  No loop in here depends on the particle, so whole wavefronts run
-// the same instruction stream.  Each block writes its walkers into its own
-// segment of the list (starting at its chunk start, so it cannot overflow)
-// through an LDS counter, and stores the segment length in wcount: no global
-// atomics, which would all hit one address.
+// Fused step.  Phase 1: push, move, and the in-cell test of the relocation.
+// A particle whose barycentric coordinates in its old cell all exceed the
+// mesh margin stays there (the walk would return that cell) and deposits with
+// those coordinates; every other particle is written, with its new position,
+// to this block's segment of the walk list (starting at the block's chunk
+// start, so it cannot overflow; an LDS counter, no global atomics).  No loop
+// in phase 1 depends on the particle, so whole wavefronts run the same
+// instruction stream.  Phase 2: the same block runs the relocation walk
+// (reference relocate_fast) for its own walk-list records, removal
+// bookkeeping, and their deposit into the same LDS table.
 // ---------------------------------------------------------------------------
+struct WalkRec {
+    double x, y, z;
+    int32_t i, cs;
+};
+static_assert(sizeof(WalkRec) == 32, "WalkRec layout");
+
 template <int PUSH, int MOVE, int DMODE>
 __global__ void __launch_bounds__(256)
 k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
             const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
-            double* __restrict__ ke_part, int32_t* __restrict__ wlist, int32_t* __restrict__ wcount) {
+            double* __restrict__ ke_part, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
+            WalkRec* __restrict__ wrec) {
     __shared__ double sh[8];
     __shared__ int32_t wcnt;
     const GArrays A = gview(S->cur);
@@ -473,6 +481,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         double dl[4] = {0.0, 0.0, 0.0, 0.0};
         int32_t ddof[4] = {0, 0, 0, 0};
         bool walker = false;
+        WalkRec wr;
         if (active) {
             const int32_t cs = A.cs[i];
             const int sp = cs >> cell_bits;
@@ -508,6 +517,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 for (int k = 0; k < 4; ++k) ddof[k] = R.dof[k];
             } else {
                 walker = true;
+                wr.x = x;
+                wr.y = y;
+                wr.z = z;
+                wr.i = (int32_t)i;
+                wr.cs = cs;
             }
         }
         // Wavefront-aggregated append to this block's walk-list segment.
@@ -517,7 +531,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             int wb = 0;
             if (lane == leader) wb = atomicAdd(&wcnt, __popcll(wm));
             wb = __shfl(wb, leader, 64);
-            if (walker) wlist[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int32_t)i;
+            if (walker) wrec[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = wr;
         }
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
         if (use_lds) {
@@ -525,55 +539,31 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
         }
     }
-    if (use_lds) lds_flush(lds_table(), rho_raw);
-    const double t = block_sum(ke, sh);  // synchronizes: wcnt is final
-    if (threadIdx.x == 0) {
-        ke_part[blockIdx.x] = t;
-        wcount[logical_block()] = wcnt;
-    }
-}
-
-// Fused step, pass 2: the relocation walk (reference relocate_fast) for the
-// particles on the walk list, removal bookkeeping, and their deposit.  Runs
-// with pass 1's grid and block: block b handles the segment pass 1's block b
-// wrote.
-template <int DMODE>
-__global__ void __launch_bounds__(256)
-k_step_walk(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, const int cell_bits,
-            double* __restrict__ rho_raw, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
-            const int32_t* __restrict__ wlist, const int32_t* __restrict__ wcount) {
-    const GArrays A = gview(S->cur);
-    int64_t s, e;
-    chunk_range(S->n, s, e);
-    e = s + wcount[logical_block()];
-    const int32_t cmask = (1 << cell_bits) - 1;
-    constexpr bool use_lds = DMODE == 2;
-    if (use_lds) {
-        lds_clear(lds_table());
-        __syncthreads();
-    }
-    for (int64_t base = s; base < e; base += blockDim.x) {
+    const double t = block_sum(ke, sh);  // synchronizes: wcnt and the segment are complete
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
+    // Phase 2.
+    const int64_t we = s + wcnt;
+    for (int64_t base = s; base < we; base += blockDim.x) {
         const int64_t j = base + threadIdx.x;
         int32_t dkey = -1;
         double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
         int32_t leave_key = -1;
         int64_t i = 0;
-        if (j < e) {
-            i = wlist[j];
-            const int32_t cs = A.cs[i];
-            const int sp = cs >> cell_bits;
-            const int32_t c = cs & cmask;
-            const double x = A.x[i], y = A.y[i], z = A.z[i];
-            const int32_t c2 = walk(M, c, x, y, z);
+        if (j < we) {
+            const WalkRec r = wrec[j];
+            i = r.i;
+            const int sp = r.cs >> cell_bits;
+            const int32_t c = r.cs & cmask;
+            const int32_t c2 = walk(M, c, r.x, r.y, r.z);
             if (c2 < 0) {
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
                 A.cs[i] = c2;
             } else {
                 if (c2 != c) A.cs[i] = (sp << cell_bits) | c2;
                 dkey = c2;
-                dx_ = x;
-                dy_ = y;
-                dz_ = z;
+                dx_ = r.x;
+                dy_ = r.y;
+                dz_ = r.z;
                 dq = coef[sp].q;
             }
         }
@@ -1157,10 +1147,7 @@ static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
     const size_t shmem = DMODE == 2 ? LDS_TABLE_BYTES : 0;
     hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
                        L.coef, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw, L.ke_part,
-                       L.walk, L.wcount);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_step_walk<DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
-                       L.coef, L.cell_bits, L.rho_raw, L.holes, L.absorbed, L.walk, L.wcount);
+                       L.holes, L.absorbed, (WalkRec*)L.walk);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1176,7 +1163,7 @@ static hipError_t launch_step_pm(const ParticleLaunch& L, hipStream_t st) {
 }
 
 hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st) {
-    if (!L.locate || !L.deposit || !L.walk || !L.wcount) return hipErrorInvalidValue;
+    if (!L.locate || !L.deposit || !L.walk) return hipErrorInvalidValue;
     switch (L.push * 3 + L.move) {
         case 4: return launch_step_pm<1, 1>(L, st);
         case 5: return launch_step_pm<1, 2>(L, st);
