@@ -40,6 +40,23 @@ CONFIGS = {
 }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_main_large_traffic.json")
+
+
+def measured_traffic(cfg, tune):
+    """HBM bytes per launch of the fused kernel from the committed counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel factor), used
+    only when the run has the workload and launch shape those runs measured."""
+    if cfg != "large" or not tune or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    shape = f"{tune['grid']},{tune['block']},{tune['deposit_mode']}"
+    if t.get("launch") != shape:
+        return None
+    return t["traffic_bytes"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -54,7 +71,7 @@ def synthetic_field(hm, seed=1):
     return np.ascontiguousarray(E)
 
 
-def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
+def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
     import puncpp_amd as P
     from puncpp_amd import _native as N
 
@@ -88,7 +105,10 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
 
     # Launch shape from the autotuner (runs on a snapshot; the state is kept).
     tune = None
-    if autotune:
+    if launch is not None:
+        N.check(lib.punc_gpu_set_launch(pop._h, *launch))
+        tune = {"grid": launch[0], "block": launch[1], "deposit_mode": launch[2], "step_ms": None}
+    elif autotune:
         g, b, md, ms = N.C.c_int32(), N.C.c_int32(), N.C.c_int32(), N.C.c_double()
         N.check(lib.punc_gpu_autotune(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, 3, N.C.byref(g),
                                       N.C.byref(b), N.C.byref(md), N.C.byref(ms)))
@@ -116,10 +136,12 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
     N.check(lib.punc_gpu_synchronize(pop._h))
     t2 = time.perf_counter()
     n_end = pop.size()
-    # Kernel timing pass: eager steps with HIP events around the fused particle
-    # kernel on the population's stream.
+    # Kernel timing pass: one full re-sort cycle of eager steps with HIP events
+    # around the fused particle kernel on the population's stream, so that the
+    # average covers freshly sorted and aged particle orders alike.
+    N.check(lib.punc_gpu_sort(pop._h))
     N.check(lib.punc_gpu_timing(pop._h, 1))
-    for k in range(10):
+    for k in range(max(sort_every, 1)):
         N.check(lib.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, k_done[0] + k))
     ms_tot = N.C.c_double()
     nl = N.C.c_int64()
@@ -129,6 +151,18 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
     ab = np.zeros((2, 64), dtype=np.int64)
     kin = N.C.c_int64()
     N.check(lib.punc_gpu_step_results(pop._h, N.C.byref(ke), N.i64ptr(ab), N.C.byref(kin)))
+    # Host hand-over per step of the drop-in time loop (not part of `value`):
+    # field dofs in (plus the per-cell field copy), charge dofs out.
+    rho = np.empty(mesh.ndof)
+    t_in = time.perf_counter()
+    for _ in range(5):
+        N.check(lib.punc_gpu_set_field(pop._h, N.dptr(E)))
+    t_mid = time.perf_counter()
+    for _ in range(5):
+        N.check(lib.punc_gpu_get_rho(pop._h, N.dptr(rho)))
+    t_out = time.perf_counter()
+    xfer = {"field_in_ms": 1e3 * (t_mid - t_in) / 5, "rho_out_ms": 1e3 * (t_out - t_mid) / 5,
+            "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
     elapsed = t2 - t1
     n_avg = 0.5 * (n_start + n_end)
     value = n_avg * steps / elapsed
@@ -140,7 +174,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True):
     return {
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
-        "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune,
+        "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
     }
 
 
@@ -182,9 +216,18 @@ def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1)
     """Bias sweep: n_inst independent sphere-probe instances, meshes n^3 x 6
     tets with n ~ U{16,20,24,28,32}, bias linspace(-5, 5) shaping each
     instance's (frozen, synthetic) probe field, density scale ~ U[0.5, 2].
-    Boris dt=0.02 with injection.  The plain loop steps each instance with
-    its own population and stream; the batched path steps all instances in
-    one population (one launch per phase, graph-replayed)."""
+    Boris dt=0.02 with injection.
+
+    Three ways to advance all instances by one step:
+      plain loop  - per instance, the drop-in call sequence of the reference's
+                    time loop: boris (returns KE), move, update (returns the
+                    absorbed counts), inject, distribute (returns rho);
+      fused loop  - per instance, one fused single-instance step call;
+      batched     - all instances in one population (instance-offset tables),
+                    one launch per phase, graph-replayed, autotuned.
+    The plain populations run `warmup + steps` drop-in steps and then `steps`
+    fused steps; the batched population runs the same total, and the
+    per-instance particle counts are compared at the end."""
     import puncpp_amd as P
     from puncpp_amd import _native as N
     from puncpp_amd.batch import BatchedPopulation
@@ -216,16 +259,18 @@ def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1)
         fields.append(E)
     total = sum(int(s.num) for sp in species for s in sp)
     log(f"[bench] batched: {n_inst} instances, {total} particles, meshes built ({time.time() - t0:.1f}s)")
-    # plain: one population per instance
-    plain = []
+    plain, dvs = [], []
     for i in range(n_inst):
-        pop = P.Population(meshes[i], [(-1.0, 1.0), (1.0, 1836.0)], capacity=int(sum(s.num for s in species[i]) * 1.1) + 100000)
+        pop = P.Population(meshes[i], [(-1.0, 1.0), (1.0, 1836.0)],
+                           capacity=int(sum(s.num for s in species[i]) * 1.1) + 100000)
         for s, S in enumerate(species[i]):
-            N.check(lib.punc_gpu_pop_load(pop._h, s, int(S.num), 1000 + i, 0.0, 0.0, 0, S.vdf.vth(), N.dptr(S.vdf.vd())))
+            N.check(lib.punc_gpu_pop_load(pop._h, s, int(S.num), 1000 + i, 0.0, 0.0, 0, S.vdf.vth(),
+                                          N.dptr(S.vdf.vd())))
         pop.sort()
         P.punc._register_injectors(pop, species[i])
         N.check(lib.punc_gpu_set_field(pop._h, N.dptr(fields[i])))
-        N.check(lib.punc_gpu_distribute(pop._h, N.dptr(P.element_volume(meshes[i])), None))
+        dvs.append(np.ascontiguousarray(P.element_volume(meshes[i])))
+        N.check(lib.punc_gpu_distribute(pop._h, N.dptr(dvs[i]), None))
         plain.append(pop)
     bp = BatchedPopulation(hms, species, [1000 + i for i in range(n_inst)], capacity=int(total * 1.1) + 1_000_000)
     bp.load()
@@ -233,42 +278,75 @@ def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1)
     bp.register_injectors()
     N.check(lib.punc_gpu_set_field(bp.pop._h, N.dptr(np.concatenate(fields))))
     N.check(lib.punc_gpu_distribute(bp.pop._h, N.dptr(P.element_volume(bp.mesh)), None))
-    log(f"[bench] batched: populations loaded ({time.time() - t0:.1f}s)")
+    g, b, md, ms = N.C.c_int32(), N.C.c_int32(), N.C.c_int32(), N.C.c_double()
+    N.check(lib.punc_gpu_autotune(bp.pop._h, 2, dt, N.dptr(B), dt, None, 1, 0, 3, N.C.byref(g), N.C.byref(b),
+                                  N.C.byref(md), N.C.byref(ms)))
+    tune = {"grid": g.value, "block": b.value, "deposit_mode": md.value, "step_ms": ms.value}
+    log(f"[bench] batched: populations loaded, batched autotune {tune} ({time.time() - t0:.1f}s)")
+    ke = N.C.c_double()
+    absorbed = np.zeros((2, 64), dtype=np.int64)
+    ninj = N.C.c_int64()
+    rhos = [np.empty(m.ndof) for m in meshes]
 
-    def plain_steps(k0, k):
+    def dropin_steps(k0, k):
+        for n in range(k0, k0 + k):
+            for i, pop in enumerate(plain):
+                h = pop._h
+                N.check(lib.punc_gpu_boris(h, N.dptr(B), dt, N.C.byref(ke)))
+                N.check(lib.punc_gpu_move(h, dt))
+                N.check(lib.punc_gpu_update(h, N.i64ptr(absorbed)))
+                N.check(lib.punc_gpu_inject(h, dt, 1000 + i, n, N.C.byref(ninj)))
+                N.check(lib.punc_gpu_distribute(h, N.dptr(dvs[i]), N.dptr(rhos[i])))
+
+    def fused_steps(k0, k):
         for n in range(k0, k0 + k):
             for i, pop in enumerate(plain):
                 N.check(lib.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 1, 1000 + i, n))
         for pop in plain:
             N.check(lib.punc_gpu_synchronize(pop._h))
 
-    plain_steps(0, warmup)
-    n0 = sum(p.size() for p in plain)
+    def count(pops):
+        return sum(p.size() for p in pops)
+
+    dropin_steps(0, warmup)
+    n0 = count(plain)
     t1 = time.perf_counter()
-    plain_steps(warmup, steps)
+    dropin_steps(warmup, steps)
     t2 = time.perf_counter()
-    n1 = sum(p.size() for p in plain)
-    plain_rate = 0.5 * (n0 + n1) * steps / (t2 - t1)
+    n1 = count(plain)
+    dropin_rate = 0.5 * (n0 + n1) * steps / (t2 - t1)
+    t3 = time.perf_counter()
+    fused_steps(warmup + steps, steps)
+    t4 = time.perf_counter()
+    n2 = count(plain)
+    fused_rate = 0.5 * (n1 + n2) * steps / (t4 - t3)
     N.check(lib.punc_gpu_step_graph(bp.pop._h, 2, dt, N.dptr(B), dt, None, 1, 0, 0))
     N.check(lib.punc_gpu_graph_launch(bp.pop._h, warmup))
     N.check(lib.punc_gpu_synchronize(bp.pop._h))
     m0 = bp.pop.size()
-    t3 = time.perf_counter()
+    t5 = time.perf_counter()
     N.check(lib.punc_gpu_graph_launch(bp.pop._h, steps))
     N.check(lib.punc_gpu_synchronize(bp.pop._h))
-    t4 = time.perf_counter()
+    t6 = time.perf_counter()
     m1 = bp.pop.size()
-    batched_rate = 0.5 * (m0 + m1) * steps / (t4 - t3)
+    batched_rate = 0.5 * (m0 + m1) * steps / (t6 - t5)
+    N.check(lib.punc_gpu_graph_launch(bp.pop._h, steps))
+    N.check(lib.punc_gpu_synchronize(bp.pop._h))
     # identical evolution: per-instance particle counts after the same steps
     counts_b = np.bincount(bp.pop.get_particles()["species"] // 2, minlength=n_inst)
     counts_p = np.array([p.size() for p in plain])
     same = bool(np.array_equal(counts_b, counts_p))
-    log(f"[bench] batched {batched_rate:.4g} vs plain {plain_rate:.4g} particle-steps/s "
-        f"({batched_rate / plain_rate:.2f}x), per-instance counts identical: {same}")
+    log(f"[bench] batched {batched_rate:.4g} vs drop-in loop {dropin_rate:.4g} ({batched_rate / dropin_rate:.2f}x) "
+        f"vs fused loop {fused_rate:.4g} ({batched_rate / fused_rate:.2f}x) particle-steps/s; "
+        f"per-instance counts identical: {same}")
     return {"instances": n_inst, "particles": int(0.5 * (m0 + m1)), "steps": steps,
-            "value": batched_rate, "plain_loop_value": plain_rate, "speedup": batched_rate / plain_rate,
-            "unit": "particle-steps/s", "per_instance_counts_identical": same,
-            "ms_per_step_batched": 1e3 * (t4 - t3) / steps, "ms_per_step_plain": 1e3 * (t2 - t1) / steps}
+            "value": batched_rate, "unit": "particle-steps/s",
+            "plain_loop_value": dropin_rate, "speedup": batched_rate / dropin_rate,
+            "plain_loop": "per instance: boris, move, update, inject, distribute (drop-in calls, host results)",
+            "fused_loop_value": fused_rate, "speedup_vs_fused_loop": batched_rate / fused_rate,
+            "per_instance_counts_identical": same, "launch": tune,
+            "ms_per_step_batched": 1e3 * (t6 - t5) / steps, "ms_per_step_plain": 1e3 * (t2 - t1) / steps,
+            "ms_per_step_fused_loop": 1e3 * (t4 - t3) / steps}
 
 
 def main():
@@ -281,6 +359,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default=None,
+                    help="GRID,BLOCK,MODE: fixed launch shape instead of the autotuner (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
@@ -298,7 +378,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               f"sharding of this path is a design note, not a build target.", file=sys.stderr)
         sys.exit(2)
-    res = run_gpu(args.config, args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune)
+    launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
+    res = run_gpu(args.config, args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune,
+                  launch=launch)
     batched = None if args.no_batched else run_batched(args.batched_instances, args.batched_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
     out = {
@@ -318,10 +400,12 @@ def main():
                    "particles": int(res["n_avg"]), "sort_every": args.sort_every, "launch": res["tune"],
                    "field": "frozen synthetic CG1 field on device"},
         "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK, "traffic": None,
-                     "kernel": "k_particles<boris,move,locate,deposit>",
+                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK,
+                     "traffic": measured_traffic(args.config, res["tune"]),
+                     "kernel": "k_step_main<boris,move> (push, move, relocation, deposit)",
                      "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]},
         "cpu_baseline": cpu,
+        "host_transfer": res["xfer"],
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -208,13 +208,16 @@ int punc_gpu_adjoint_forward(punc_adj_t* a, const double* x0, const int64_t* cel
 int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, double* dJdv0, double* dJdx0);
 
 /* ---- launch control, timing -------------------------------------------- */
-/* Launch shape of the particle kernels: persistent grid size, block size
- * (64, 128 or 256) and deposit mode (0: four FP64 atomics per particle;
- * 1: wavefront runs of equal cells are summed across lanes first). */
+/* Launch shape of the particle kernels: grid size (each block takes one
+ * contiguous chunk of particles), block size (64, 128 or 256) and deposit
+ * mode (0: four FP64 atomics per particle; 1: wavefront runs of equal cells
+ * are summed across lanes first; 2: run sums go into a per-block LDS table
+ * keyed by dof, flushed with one atomic per occupied slot).  Default:
+ * 32 x CUs, 256, mode 2. */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
 /* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every
- * launch shape (grid 2/4/8/16 x CUs, block 128/256, deposit mode 0/1) on a
+ * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0/1/2) on a
  * snapshot of the population, restores the snapshot (particles, counts,
  * absorbed counters, rho, KE), and keeps the fastest shape.  A captured step
  * graph is dropped.  No counterpart in the reference. */

@@ -87,7 +87,7 @@ struct punc_pop_s {
     int64_t capacity = 0;
     int grid = 0, block = 256;
     int num_cu = 1;
-    int dep_mode = 1;
+    int dep_mode = 2;
 
     DevState* d_state = nullptr;
     void* bufs[2][7] = {};
@@ -445,7 +445,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     }
     hipDeviceProp_t prop;
     PCHK(hipGetDeviceProperties(&prop, P->device));
-    P->grid = prop.multiProcessorCount * 8;
+    P->grid = prop.multiProcessorCount * 32;
     P->num_cu = prop.multiProcessorCount;
     P->ke_np = 65536;
     for (int b = 0; b < 2; ++b) {
@@ -1463,7 +1463,7 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    const int grids[] = {2, 4, 8, 16};
+    const int grids[] = {4, 8, 16, 32, 64, 128};
     const int blocks[] = {128, 256};
     const int modes[] = {0, 1, 2};
     int rc = 0;

@@ -328,6 +328,10 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     const int64_t n = S->n;
     int64_t s, e;
     chunk_range(n, s, e);
+    if (s >= e) {  // block-uniform: nothing in this block's chunk
+        if (PUSH != 0 && threadIdx.x == 0) ke_part[blockIdx.x] = 0.0;
+        return;
+    }
     const int32_t cmask = (1 << cell_bits) - 1;
     double ke = 0.0;
     constexpr bool use_lds = DEPOSIT && DMODE == 2;
@@ -465,6 +469,10 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     const int64_t n = S->n;
     int64_t s, e;
     chunk_range(n, s, e);
+    if (s >= e) {  // block-uniform: nothing in this block's chunk
+        if (threadIdx.x == 0) ke_part[blockIdx.x] = 0.0;
+        return;
+    }
     const int32_t cmask = (1 << cell_bits) - 1;
     const double tol = M.inside_tol;
     double ke = 0.0;
