@@ -277,6 +277,7 @@ static int mesh_ensure_device(punc_mesh_s* M) {
     D.nc = nc;
     D.ndof = M->host.ndof;
     D.inside_tol = T.inside_tol;
+    D.coord_bound = T.coord_bound;
     for (int d = 0; d < 3; ++d) {
         D.gn[d] = T.grid_n[d];
         D.glo[d] = T.grid_lo[d];

@@ -59,7 +59,8 @@ struct DevMesh {
     const int32_t* bcells;
     int64_t nc;
     int64_t ndof;
-    double inside_tol;  // 0: no "certainly inside" shortcut
+    double inside_tol;   // 0: no "certainly inside" shortcut
+    double coord_bound;  // margin valid for |x|, |y|, |z| <= coord_bound
     int gn[3];
     double glo[3];
     double gh[3];

@@ -89,6 +89,59 @@ __device__ inline int32_t walk(const DevMesh& M, int32_t c, double x, double y, 
     return WALK_FAILED;
 }
 
+// The same walk with the facet tests decided by the barycentric coordinates
+// where the mesh margin makes them certain.  With m = inside_tol *
+// max(1, sum |lambda_j|): lambda_i > m implies the computed proj_i < 0 and
+// lambda_i < -m implies the computed proj_i > 0 (the margin's rounding bounds
+// grow at most linearly with sum |lambda_j|, which bounds the exact
+// sum |mu_j|); only for |lambda_i| <= m is the stored plane evaluated, as in
+// walk().  So every decision, and the result, equals walk()'s.  The record
+// of each visited cell (barycentric rows, dofs, neighbours: one 128-byte
+// line) serves both the decisions and, at the final cell, the deposit: l and
+// dof return lambda and the dofs there.  Points outside the coordinate range
+// the margin was derived for take walk() itself.
+__device__ inline int32_t walk_bary(const DevMesh& M, int32_t c, double x, double y, double z, double l[4],
+                                    int32_t dof[4]) {
+    const double X = M.coord_bound;
+    if (!(M.inside_tol > 0.0 && fabs(x) <= X && fabs(y) <= X && fabs(z) <= X)) {
+        const int32_t c2 = walk(M, c, x, y, z);
+        if (c2 >= 0) {
+            const CellRec& R = M.rec[c2];
+            bary4(R.bary, x, y, z, l);
+            for (int k = 0; k < 4; ++k) dof[k] = R.dof[k];
+        }
+        return c2;
+    }
+    for (int h = 0; h < WALK_MAX_HOPS; ++h) {
+        const CellRec& R = M.rec[c];
+        bary4(R.bary, x, y, z, l);
+        const double S = (fabs(l[0]) + fabs(l[1])) + (fabs(l[2]) + fabs(l[3]));
+        const double m = M.inside_tol * (S > 1.0 ? S : 1.0);
+        int f = -1;
+        for (int i = 0; i < 4; ++i) {
+            bool pos;
+            if (l[i] > m)
+                pos = false;
+            else if (l[i] < -m)
+                pos = true;
+            else
+                pos = plane_proj(M.planes[c].p + 4 * i, x, y, z) > 0.0;
+            if (pos) {
+                f = i;
+                break;
+            }
+        }
+        if (f < 0) {
+            for (int k = 0; k < 4; ++k) dof[k] = R.dof[k];
+            return c;
+        }
+        const int32_t nb = R.nbr[f];
+        if (nb < 0) return nb;
+        c = nb;
+    }
+    return WALK_FAILED;
+}
+
 // CG1 deposit of one particle per lane (distribute_cg1, distributor.h:189-203:
 // accum_i += q*lambda_i, then rho[dof_i] += accum_i).  key = cell, or -1 for
 // lanes with nothing to deposit.  mode 0: four FP64 atomics per particle.
@@ -491,20 +544,26 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         bool walker = false;
         WalkRec wr;
         if (active) {
-            const int32_t cs = A.cs[i];
+            // Particle data is touched once per step: non-temporal accesses
+            // keep it from evicting cell records and field copies from L2.
+            const int32_t cs = __builtin_nontemporal_load(&A.cs[i]);
             const int sp = cs >> cell_bits;
             const int32_t c = cs & cmask;
-            double x = A.x[i], y = A.y[i], z = A.z[i];
-            double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+            double x = __builtin_nontemporal_load(&A.x[i]);
+            double y = __builtin_nontemporal_load(&A.y[i]);
+            double z = __builtin_nontemporal_load(&A.z[i]);
+            double vx = __builtin_nontemporal_load(&A.vx[i]);
+            double vy = __builtin_nontemporal_load(&A.vy[i]);
+            double vz = __builtin_nontemporal_load(&A.vz[i]);
             const CellRec& R = M.rec[c];
             double l[4], ef[12];
             bary4(R.bary, x, y, z, l);
             load_cell_field(cellE, c, ef);
             const SpCoef& K = coef[sp];
             push_velocity<PUSH>(K, l, ef, vx, vy, vz, ke);
-            A.vx[i] = vx;
-            A.vy[i] = vy;
-            A.vz[i] = vz;
+            __builtin_nontemporal_store(vx, &A.vx[i]);
+            __builtin_nontemporal_store(vy, &A.vy[i]);
+            __builtin_nontemporal_store(vz, &A.vz[i]);
             // move / move_periodic (pusher.h:658, 676-677)
             x += dt * vx;
             y += dt * vy;
@@ -514,9 +573,9 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 y -= Ly * floor(y / Ly);
                 z -= Lz * floor(z / Lz);
             }
-            A.x[i] = x;
-            A.y[i] = y;
-            A.z[i] = z;
+            __builtin_nontemporal_store(x, &A.x[i]);
+            __builtin_nontemporal_store(y, &A.y[i]);
+            __builtin_nontemporal_store(z, &A.z[i]);
             bary4(R.bary, x, y, z, dl);
             const bool stay = tol > 0.0 && dl[0] > tol && dl[1] > tol && dl[2] > tol && dl[3] > tol;
             if (stay) {
@@ -554,7 +613,9 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     for (int64_t base = s; base < we; base += blockDim.x) {
         const int64_t j = base + threadIdx.x;
         int32_t dkey = -1;
-        double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
+        double dq = 0.0;
+        double dl[4] = {0.0, 0.0, 0.0, 0.0};
+        int32_t ddof[4] = {0, 0, 0, 0};
         int32_t leave_key = -1;
         int64_t i = 0;
         if (j < we) {
@@ -562,21 +623,18 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             i = r.i;
             const int sp = r.cs >> cell_bits;
             const int32_t c = r.cs & cmask;
-            const int32_t c2 = walk(M, c, r.x, r.y, r.z);
+            const int32_t c2 = walk_bary(M, c, r.x, r.y, r.z, dl, ddof);
             if (c2 < 0) {
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
                 A.cs[i] = c2;
             } else {
                 if (c2 != c) A.cs[i] = (sp << cell_bits) | c2;
                 dkey = c2;
-                dx_ = r.x;
-                dy_ = r.y;
-                dz_ = r.z;
                 dq = coef[sp].q;
             }
         }
         record_leavers(S, holes, absorbed, leave_key, i);
-        deposit_cg1<DMODE>(M, rho_raw, dkey, dx_, dy_, dz_, dq, false, nullptr, nullptr);
+        deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
         if (use_lds) {
             __syncthreads();
             if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);

@@ -412,6 +412,7 @@ bool build_topology(const HostMesh& m, MeshTopo& t, std::string& err) {
             }
         }
         t.inside_tol = 2 * worst;
+        t.coord_bound = X;
         if (!(t.inside_tol < 1e-6)) t.inside_tol = 0;  // degenerate mesh: always walk
     }
     t.bnd_ids.clear();

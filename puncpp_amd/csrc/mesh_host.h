@@ -58,6 +58,7 @@ struct MeshTopo {
     // facet projection is negative (rounding bounds of both, see
     // build_topology).  0 disables the shortcut.
     double inside_tol = 0;
+    double coord_bound = 0;  // |coordinate| bound the margin was derived for
     double volume = 0;                // total mesh volume
     // Uniform bucket grid for point location.
     int grid_n[3] = {1, 1, 1};

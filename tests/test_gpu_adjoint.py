@@ -159,3 +159,47 @@ def test_adjoint_vs_central_fd(gpu, target):
     assert flipped <= 4
     assert worst <= 1e-6
     assert worst_plain <= 1e-4
+
+
+def test_adjoint_vs_oracle_fd(gpu):
+    """Central differences of the oracle (the serial CPU restatement of the
+    reference path: boris_cg1, move_periodic, update, distribute_cg1) against
+    the adjoint gradient of J = ||rho||^2, in directions of v0 and of E."""
+    print("[case] adjoint vs oracle FD", flush=True)
+    hm, mesh, st, E, phi, G, runner, Ld = _setup(npart=(2000, 2000), K=8, every=4)
+    dv = P.element_volume(mesh)
+    v0 = torch.tensor(st["v"], requires_grad=True)
+    Et = torch.tensor(E, requires_grad=True)
+    J = ((torch.from_numpy(dv) * push_deposit(v0, Et, runner)) ** 2).sum()
+    J.backward()
+    gv, gE = v0.grad.numpy(), Et.grad.numpy()
+
+    def J_oracle(v, e):
+        o = O.OraclePop(hm, [QE, QI], [ME, MI])
+        o.add_located(st["x"], v, st["species"], st["cell"])
+        for _ in range(8):
+            o.boris(e, [0, 0, 1.0], 0.05)
+            o.move_periodic(0.05, Ld)
+            o.update()
+        rho = o.distribute(dv)
+        return math.fsum(rho ** 2)
+
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    worst = 0.0
+    for k in range(8):
+        if k % 2 == 0:
+            d = rng.normal(size=st["v"].shape)
+            d /= np.linalg.norm(d)
+            fd = (J_oracle(st["v"] + h * d, E) - J_oracle(st["v"] - h * d, E)) / (2 * h)
+            ad, gn = float(np.sum(gv * d)), np.linalg.norm(gv)
+        else:
+            d = rng.normal(size=E.shape)
+            d /= np.linalg.norm(d)
+            fd = (J_oracle(st["v"], E + h * d) - J_oracle(st["v"], E - h * d)) / (2 * h)
+            ad, gn = float(np.sum(gE * d)), np.linalg.norm(gE)
+        rel = abs(fd - ad) / max(abs(ad), gn)
+        worst = max(worst, rel)
+        print(f"dir {k}: oracle fd {fd:.12e} ad {ad:.12e} rel {rel:.2e}", flush=True)
+    print(f"worst relative error vs oracle FD {worst:.3e}")
+    assert worst <= 1e-6

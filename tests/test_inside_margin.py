@@ -83,3 +83,36 @@ def test_margin_is_needed():
             sel = np.all(lam > 0.0, axis=1)
             bad += int(np.any(proj(loc["planes"][sel], x[sel]) > 0.0, axis=1).sum())
     assert bad > 0
+
+
+@pytest.mark.parametrize("make", [lambda: sphere_mesh(10), lambda: periodic_mesh(7)], ids=["sphere", "periodic"])
+def test_margin_decides_facets_outside(make):
+    """The walk decides facet i from lambda_i alone when |lambda_i| exceeds
+    m = inside_tol * max(1, sum |lambda_j|): lambda_i < -m must give a positive
+    plane projection, lambda_i > m a negative one.  Checked at points beyond
+    one facet by a few margins up to half a cell, where the sum exceeds 1."""
+    mesh = P.Mesh(make())
+    tol = inside_tol(mesh)
+    loc = mesh.localizer()
+    nc = mesh.num_cells
+    V = loc["vcoords"].reshape(nc, 4, 3)
+    rng = np.random.default_rng(11)
+    decided_neg = decided_pos = 0
+    for face in range(4):
+        for t in (1.0 * tol, 1.5 * tol, 4.0 * tol, 1e3 * tol, 1e-3, 0.5):
+            b = rng.random((nc, 4)) + 0.05
+            b[:, face] = 0.0
+            b /= b.sum(axis=1, keepdims=True)
+            b *= 1.0 + t
+            b[:, face] = -t
+            x = np.einsum("cj,cjd->cd", b, V)
+            lam = bary4(loc["bary"], x)
+            m = tol * np.maximum(1.0, np.abs(lam).sum(axis=1))
+            pr = proj(loc["planes"], x)
+            neg = lam < -m[:, None]
+            pos = lam > m[:, None]
+            assert np.all(pr[neg] > 0.0), (face, t)
+            assert np.all(pr[pos] < 0.0), (face, t)
+            decided_neg += int(neg.sum())
+            decided_pos += int(pos.sum())
+    assert decided_neg > 2 * nc and decided_pos > 8 * nc
