@@ -142,7 +142,8 @@ int punc_gpu_update(punc_pop_t* p, int64_t* absorbed);
 int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* rho);
 /* kinetic_energy(pop) (diagnostics.h:248-265). */
 int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE);
-/* Re-sort the particles by cell (counting sort, order within a cell arbitrary). */
+/* Re-sort the particles by cell (counting sort, order within a cell
+ * arbitrary).  Enqueued on the population's stream; returns without waiting. */
 int punc_gpu_sort(punc_pop_t* p);
 
 /* ---- injection --------------------------------------------------------- */
@@ -212,12 +213,13 @@ int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, doub
  * contiguous chunk of particles), block size (64, 128 or 256) and deposit
  * mode (0: four FP64 atomics per particle; 1: wavefront runs of equal cells
  * are summed across lanes first; 2: run sums go into a per-block LDS table
- * keyed by dof, flushed with one atomic per occupied slot).  Default:
+ * keyed by dof, flushed with one atomic per occupied slot; 3: every lane
+ * adds its four contributions to that LDS table directly).  Default:
  * 32 x CUs, 256, mode 2. */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
 /* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every
- * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0/1/2) on a
+ * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0..3) on a
  * snapshot of the population, restores the snapshot (particles, counts,
  * absorbed counters, rho, KE), and keeps the fastest shape.  A captured step
  * graph is dropped.  No counterpart in the reference. */

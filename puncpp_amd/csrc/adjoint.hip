@@ -423,8 +423,11 @@ extern "C" int punc_gpu_adjoint_forward(punc_adj_t* A, const double* x0, const i
         for (int i = 0; i < 3; ++i) c.s[i] = 2 * c.t[i] / (1 + t_mag2);
     }
     AHIP(hipMemcpy(A->d_coef, h.data(), A->nsp * sizeof(AdjCoef), hipMemcpyHostToDevice));
-    AHIP(hipMemcpy2D(A->d_E, sizeof(double4), E, 3 * sizeof(double), 3 * sizeof(double), A->ndof,
-                     hipMemcpyHostToDevice));
+    // Contiguous copy into the dJ/dE buffer (3 per dof, free until the reverse
+    // sweep), then expanded to one double4 per dof on the device.
+    AHIP(hipMemcpy(A->d_gE, E, A->ndof * 3 * sizeof(double), hipMemcpyHostToDevice));
+    AHIP(launch_expand_field(A->d_gE, A->d_E, A->ndof, A->stream));
+    AHIP(hipStreamSynchronize(A->stream));
     AHIP(hipMemset(A->d_lost, 0, 8));
     const int nck = K / every + 1;
     A->ckpt.resize(nck);

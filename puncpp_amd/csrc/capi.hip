@@ -97,7 +97,7 @@ struct punc_pop_s {
     unsigned long long* d_absorbed = nullptr;
     unsigned long long* d_cnt = nullptr;
     unsigned long long* d_errs = nullptr;  // [0] load failures, [1] injection failures
-    double4* d_E = nullptr;
+    double* d_E = nullptr;  // field dofs, 3 components per dof (packed as handed over)
     double* d_cellE = nullptr;  // per-cell copy of the field (16 doubles per cell)
     double4* d_walk = nullptr;  // fused step: walk list (32-byte records)
     bool field_set = false;
@@ -462,7 +462,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_cnt, MAX_SPECIES));
     PCHK(dalloc(&P->d_errs, 4));
     const int64_t ndof = m->topo.ndof;
-    PCHK(dalloc(&P->d_E, ndof));
+    PCHK(dalloc(&P->d_E, 3 * ndof));
     PCHK(dalloc(&P->d_rho_raw, ndof));
     PCHK(dalloc(&P->d_rho, ndof));
     PCHK(dalloc(&P->d_dv_inv, ndof));
@@ -492,7 +492,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipMemcpy(P->d_state, &s, sizeof(s), hipMemcpyHostToDevice));
     PCHK(hipMemset(P->d_absorbed, 0, (size_t)MAX_SPECIES * MAX_BND * 8));
     PCHK(hipMemset(P->d_errs, 0, 4 * 8));
-    PCHK(hipMemset(P->d_E, 0, ndof * sizeof(double4)));
+    PCHK(hipMemset(P->d_E, 0, 3 * ndof * sizeof(double)));
     PCHK(hipMemset(P->d_rho_raw, 0, ndof * 8));
     PCHK(hipMemset(P->d_rho, 0, ndof * 8));
     PCHK(hipMemset(P->d_ke, 0, 4 * 8));
@@ -769,6 +769,7 @@ static ParticleLaunch base_launch(punc_pop_s* P) {
     memset(&L, 0, sizeof(L));
     L.mesh = P->mesh->dm;
     L.state = P->d_state;
+    L.nsp = P->nsp;
     L.cellE = P->d_cellE;
     L.walk = P->d_walk;
     L.cell_bits = P->cell_bits;
@@ -823,8 +824,8 @@ extern "C" int punc_gpu_set_field(punc_pop_t* p, const double* E) {
     if (!p || !E) return fail("set_field: null argument");
     HIPCHK(hipSetDevice(p->device));
     const int64_t ndof = p->mesh->topo.ndof;
-    HIPCHK(hipMemcpy2DAsync(p->d_E, sizeof(double4), E, 3 * sizeof(double), 3 * sizeof(double), ndof,
-                            hipMemcpyHostToDevice, p->stream));
+    // One contiguous copy: a pitched 2D copy from pageable memory runs row by row.
+    HIPCHK(hipMemcpyAsync(p->d_E, E, 3 * ndof * sizeof(double), hipMemcpyHostToDevice, p->stream));
     HIPCHK(launch_cell_field(p->mesh->dm, p->d_E, p->d_cellE, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     p->field_set = true;
@@ -947,8 +948,7 @@ extern "C" int punc_gpu_sort(punc_pop_t* p) {
     if (!p) return fail("sort: null population");
     HIPCHK(hipSetDevice(p->device));
     HIPCHK(launch_sort(p->d_state, p->cell_bits, p->mesh->topo.nc, p->d_hist, p->d_off, p->d_bsum,
-                       p->d_total, p->grid, p->stream));
-    HIPCHK(hipStreamSynchronize(p->stream));
+                       p->d_total, p->capacity, p->stream));
     return 0;
 }
 
@@ -1319,7 +1319,7 @@ extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int depos
     if (!p) return fail("set_launch: null population");
     if (grid < 1 || grid > p->ke_np) return fail("set_launch: grid out of range");
     if (block != 64 && block != 128 && block != 256) return fail("set_launch: block must be 64, 128 or 256");
-    if (deposit_mode < 0 || deposit_mode > 2) return fail("set_launch: deposit_mode must be 0, 1 or 2");
+    if (deposit_mode < 0 || deposit_mode > 3) return fail("set_launch: deposit_mode must be 0, 1, 2 or 3");
     graph_free(p);  // a captured graph holds the old launch shape
     p->grid = grid;
     p->block = block;
@@ -1466,7 +1466,7 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     hipEventCreate(&e1);
     const int grids[] = {4, 8, 16, 32, 64, 128};
     const int blocks[] = {128, 256};
-    const int modes[] = {0, 1, 2};
+    const int modes[] = {0, 1, 2, 3};
     int rc = 0;
     for (int gm : grids) {
         for (int bl : blocks) {

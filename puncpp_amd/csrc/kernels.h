@@ -10,6 +10,7 @@ struct ParticleLaunch {
     DevMesh mesh;
     DevState* state;
     const SpCoef* coef;
+    int nsp;              // entries of coef
     const double* cellE;  // per-cell field copy (k_cell_field)
     double dt;
     double Ld[3];
@@ -59,7 +60,9 @@ hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st);
 // Zero rho_raw before a fused step.
 hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st);
 // Per-cell field copy: cellE[c*16 + 3k + j] = E[dof_k(c)].j.
-hipError_t launch_cell_field(const DevMesh& M, const double4* E, double* cellE, hipStream_t st);
+// E3: 3 FP64 components per dof, packed.
+hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st);
+hipError_t launch_expand_field(const double* E3, double4* E, int64_t ndof, hipStream_t st);
 hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double* ke_part, int grid,
                           hipStream_t st);
 hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);
@@ -74,8 +77,10 @@ hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned lo
                                 int grid, hipStream_t st);
 hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
                        hipStream_t st);
+// Counting sort by cell into the alternate buffers, then a buffer swap.
+// capacity: particle slots per buffer (sizes the one-particle-per-thread grid).
 hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
-                       int64_t* bsum, int64_t* total, int grid, hipStream_t st);
+                       int64_t* bsum, int64_t* total, int64_t capacity, hipStream_t st);
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
                        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
                        double cmax, double amp, double kw, int axis, double vth, const double vd[3],

@@ -10,6 +10,8 @@
 // velocities and cell decisions are bit-identical to the CPU path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -272,6 +274,12 @@ __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_ra
             }
         }
     }
+    if (mode == 3) {
+        if (key < 0) return;
+        const LdsTable t = lds_table();
+        for (int k = 0; k < 4; ++k) lds_add(t, rho_raw, dof[k], d[k]);
+        return;
+    }
     if (mode >= 1) {
         // A run is a maximal stretch of consecutive lanes with one key; the
         // same key may reappear in a later run, which is summed separately.
@@ -387,7 +395,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     }
     const int32_t cmask = (1 << cell_bits) - 1;
     double ke = 0.0;
-    constexpr bool use_lds = DEPOSIT && DMODE == 2;
+    constexpr bool use_lds = DEPOSIT && DMODE >= 2;
     if (use_lds) {
         lds_clear(lds_table());
         __syncthreads();
@@ -492,16 +500,26 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Fused step.  Phase 1: push, move, and the in-cell test of the relocation.
-// A particle whose barycentric coordinates in its old cell all exceed the
-// mesh margin stays there (the walk would return that cell) and deposits with
-// those coordinates; every other particle is written, with its new position,
-// to this block's segment of the walk list (starting at the block's chunk
-// start, so it cannot overflow; an LDS counter, no global atomics).  No loop
-// in phase 1 depends on the particle, so whole wavefronts run the same
-// instruction stream.  Phase 2: the same block runs the relocation walk
-// (reference relocate_fast) for its own walk-list records, removal
-// bookkeeping, and their deposit into the same LDS table.
+// Fused step: push, move, relocation with removal bookkeeping, and deposit in
+// one pass over the population.
+//
+// Phase 1, one lane per particle, software-pipelined: while a group of
+// particles is processed, the next group's positions, velocities and cell
+// fields are already in flight.  Vector-memory results arrive in issue order,
+// so the next group's loads are issued after the current group's cell-record
+// and field gathers (waiting for those does not wait for the prefetch), and
+// nothing the current group needs is loaded after them (species
+// coefficients are read from LDS).  A particle whose barycentric coordinates
+// at its new position in its old cell all exceed the mesh margin stays there
+// (the reference's walk, relocate_fast population.h:539-569, would return
+// that cell) and deposits with those coordinates; every other particle goes,
+// with its new position, to this block's segment of the walk list (starting
+// at the block's chunk start, so it cannot overflow; the counter lives in
+// LDS).  There is no block barrier in this loop: the deposit table in LDS is
+// only flushed at the end (a full table sends further contributions straight
+// to HBM).
+// Phase 2: the same block runs the walks of its list, the removal
+// bookkeeping of the particles that leave, and their deposit.
 // ---------------------------------------------------------------------------
 struct WalkRec {
     double x, y, z;
@@ -509,15 +527,54 @@ struct WalkRec {
 };
 static_assert(sizeof(WalkRec) == 32, "WalkRec layout");
 
+struct PartRegs {
+    double x, y, z, vx, vy, vz;
+    int32_t cs;
+};
+
+__device__ inline void load_particle(const GArrays& A, int64_t i, PartRegs& p) {
+    // Particle data is touched once per step: non-temporal accesses keep it
+    // from evicting cell records and field copies from L2.
+    p.cs = __builtin_nontemporal_load(&A.cs[i]);
+    p.x = __builtin_nontemporal_load(&A.x[i]);
+    p.y = __builtin_nontemporal_load(&A.y[i]);
+    p.z = __builtin_nontemporal_load(&A.z[i]);
+    p.vx = __builtin_nontemporal_load(&A.vx[i]);
+    p.vy = __builtin_nontemporal_load(&A.vy[i]);
+    p.vz = __builtin_nontemporal_load(&A.vz[i]);
+}
+
+// All four barycentric coordinates above the margin: the point is certainly
+// inside (every computed facet projection is negative).
+__device__ inline bool certainly_inside(const double l[4], double tol) {
+    return l[0] > tol && l[1] > tol && l[2] > tol && l[3] > tol;
+}
+
+// The fused step's dynamic LDS: the deposit table (mode 2), then the species
+// coefficients.
+template <int DMODE>
+__device__ inline SpCoef* step_coef_lds() {
+    extern __shared__ double dyn_smem[];
+    return (SpCoef*)(dyn_smem + (DMODE >= 2 ? LDS_TABLE_BYTES / 8 : 0));
+}
+
 template <int PUSH, int MOVE, int DMODE>
 __global__ void __launch_bounds__(256)
-k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
+k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ gcoef, const int nsp,
             const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
             WalkRec* __restrict__ wrec) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    typedef int32_t v4i __attribute__((ext_vector_type(4)));
     __shared__ double sh[8];
     __shared__ int32_t wcnt;
+    // Species coefficients in LDS: read per lane (species differ within a
+    // wavefront) without adding to the vector-memory queue that the
+    // prefetched particle loads sit in.
+    SpCoef* coef = step_coef_lds<DMODE>();
+    for (int k = threadIdx.x; k < nsp * (int)(sizeof(SpCoef) / 8); k += blockDim.x)
+        ((double*)coef)[k] = ((const double*)gcoef)[k];
     const GArrays A = gview(S->cur);
     const int64_t n = S->n;
     int64_t s, e;
@@ -528,67 +585,86 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     }
     const int32_t cmask = (1 << cell_bits) - 1;
     const double tol = M.inside_tol;
+    const double X = M.coord_bound;
     double ke = 0.0;
-    constexpr bool use_lds = DMODE == 2;
+    constexpr bool use_lds = DMODE >= 2;
     if (threadIdx.x == 0) wcnt = 0;
     if (use_lds) lds_clear(lds_table());
     __syncthreads();
     const int lane = threadIdx.x & 63;
+    const int64_t last = e - 1;
+    PartRegs nxt;
+    load_particle(A, s + threadIdx.x < e ? s + threadIdx.x : last, nxt);
     for (int64_t base = s; base < e; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
         const bool active = i < e;
-        int32_t dkey = -1;
-        double dq = 0.0;
-        double dl[4] = {0.0, 0.0, 0.0, 0.0};
-        int32_t ddof[4] = {0, 0, 0, 0};
-        bool walker = false;
-        WalkRec wr;
+        const PartRegs p = nxt;
+        const int sp = p.cs >> cell_bits;
+        const int32_t c = p.cs & cmask;
+        // Cell record and field copy of this particle's cell.
+        const GLOBAL v2d* rp = (const GLOBAL v2d*)(M.rec + c);
+        const GLOBAL v2d* ep = (const GLOBAL v2d*)(cellE + (int64_t)c * CELL_FIELD_STRIDE);
+        double rb[12], ef[12];
+        int32_t rd[8];  // dof[4], nbr[4]
+        for (int m = 0; m < 6; ++m) {
+            const v2d t = rp[m];
+            rb[2 * m] = t.x;
+            rb[2 * m + 1] = t.y;
+        }
+        for (int m = 0; m < 2; ++m) {
+            const v4i t = ((const GLOBAL v4i*)(rp + 6))[m];
+            rd[4 * m] = t.x;
+            rd[4 * m + 1] = t.y;
+            rd[4 * m + 2] = t.z;
+            rd[4 * m + 3] = t.w;
+        }
+        for (int m = 0; m < 6; ++m) {
+            const v2d t = ep[m];
+            ef[2 * m] = t.x;
+            ef[2 * m + 1] = t.y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // Prefetch the next group (clamped index: lanes past the end reload
+        // the chunk's last particle and discard it).
+        const int64_t inx = i + blockDim.x;
+        load_particle(A, inx < e ? inx : last, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+        double x = p.x, y = p.y, z = p.z, vx = p.vx, vy = p.vy, vz = p.vz;
+        double l[4];
+        bary4(rb, x, y, z, l);
+        const SpCoef& K = coef[sp];
+        double ke_l = 0.0;
+        push_velocity<PUSH>(K, l, ef, vx, vy, vz, ke_l);
+        // move / move_periodic (pusher.h:658, 676-677)
+        x += dt * vx;
+        y += dt * vy;
+        z += dt * vz;
+        if (MOVE == 2) {
+            x -= Lx * floor(x / Lx);
+            y -= Ly * floor(y / Ly);
+            z -= Lz * floor(z / Lz);
+        }
         if (active) {
-            // Particle data is touched once per step: non-temporal accesses
-            // keep it from evicting cell records and field copies from L2.
-            const int32_t cs = __builtin_nontemporal_load(&A.cs[i]);
-            const int sp = cs >> cell_bits;
-            const int32_t c = cs & cmask;
-            double x = __builtin_nontemporal_load(&A.x[i]);
-            double y = __builtin_nontemporal_load(&A.y[i]);
-            double z = __builtin_nontemporal_load(&A.z[i]);
-            double vx = __builtin_nontemporal_load(&A.vx[i]);
-            double vy = __builtin_nontemporal_load(&A.vy[i]);
-            double vz = __builtin_nontemporal_load(&A.vz[i]);
-            const CellRec& R = M.rec[c];
-            double l[4], ef[12];
-            bary4(R.bary, x, y, z, l);
-            load_cell_field(cellE, c, ef);
-            const SpCoef& K = coef[sp];
-            push_velocity<PUSH>(K, l, ef, vx, vy, vz, ke);
+            ke += ke_l;
             __builtin_nontemporal_store(vx, &A.vx[i]);
             __builtin_nontemporal_store(vy, &A.vy[i]);
             __builtin_nontemporal_store(vz, &A.vz[i]);
-            // move / move_periodic (pusher.h:658, 676-677)
-            x += dt * vx;
-            y += dt * vy;
-            z += dt * vz;
-            if (MOVE == 2) {
-                x -= Lx * floor(x / Lx);
-                y -= Ly * floor(y / Ly);
-                z -= Lz * floor(z / Lz);
-            }
             __builtin_nontemporal_store(x, &A.x[i]);
             __builtin_nontemporal_store(y, &A.y[i]);
             __builtin_nontemporal_store(z, &A.z[i]);
-            bary4(R.bary, x, y, z, dl);
-            const bool stay = tol > 0.0 && dl[0] > tol && dl[1] > tol && dl[2] > tol && dl[3] > tol;
-            if (stay) {
+        }
+        double dl[4];
+        bary4(rb, x, y, z, dl);
+        const bool in_range = tol > 0.0 && fabs(x) <= X && fabs(y) <= X && fabs(z) <= X;
+        int32_t dkey = -1;  // deposit cell (-1: nothing to deposit in this phase)
+        int32_t ddof[4];
+        bool walker = false;
+        if (active) {
+            if (in_range && certainly_inside(dl, tol)) {
                 dkey = c;
-                dq = K.q;
-                for (int k = 0; k < 4; ++k) ddof[k] = R.dof[k];
+                for (int k = 0; k < 4; ++k) ddof[k] = rd[k];
             } else {
                 walker = true;
-                wr.x = x;
-                wr.y = y;
-                wr.z = z;
-                wr.i = (int32_t)i;
-                wr.cs = cs;
             }
         }
         // Wavefront-aggregated append to this block's walk-list segment.
@@ -598,17 +674,22 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             int wb = 0;
             if (lane == leader) wb = atomicAdd(&wcnt, __popcll(wm));
             wb = __shfl(wb, leader, 64);
-            if (walker) wrec[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = wr;
+            if (walker) {
+                WalkRec wr;
+                wr.x = x;
+                wr.y = y;
+                wr.z = z;
+                wr.i = (int32_t)i;
+                wr.cs = p.cs;
+                wrec[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = wr;
+            }
         }
-        deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
-        if (use_lds) {
-            __syncthreads();
-            if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
-        }
+        deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, K.q, true, dl, ddof);
     }
     const double t = block_sum(ke, sh);  // synchronizes: wcnt and the segment are complete
     if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
-    // Phase 2.
+    // Phase 2: the walks of this block's list.  A.cs is rewritten only when
+    // the cell changes.
     const int64_t we = s + wcnt;
     for (int64_t base = s; base < we; base += blockDim.x) {
         const int64_t j = base + threadIdx.x;
@@ -635,10 +716,6 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         record_leavers(S, holes, absorbed, leave_key, i);
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
-        if (use_lds) {
-            __syncthreads();
-            if (*lds_table().count > LDS_SLOTS * 5 / 8) lds_flush(lds_table(), rho_raw);
-        }
     }
     if (use_lds) lds_flush(lds_table(), rho_raw);
 }
@@ -648,18 +725,25 @@ __global__ void k_step_begin(double* __restrict__ rho_raw, int64_t ndof) {
         rho_raw[k] = 0.0;
 }
 
-__global__ void k_cell_field(const DevMesh M, const double4* __restrict__ E, double* __restrict__ cellE) {
+// E3: the CG1 field as the caller hands it over, 3 components per dof.
+__global__ void k_cell_field(const DevMesh M, const double* __restrict__ E3, double* __restrict__ cellE) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < M.nc; c += (int64_t)gridDim.x * blockDim.x) {
         const CellRec& R = M.rec[c];
         double* o = cellE + c * CELL_FIELD_STRIDE;
         for (int k = 0; k < 4; ++k) {
-            const double4 v = E[R.dof[k]];
-            o[3 * k] = v.x;
-            o[3 * k + 1] = v.y;
-            o[3 * k + 2] = v.z;
+            const double* v = E3 + 3 * (int64_t)R.dof[k];
+            o[3 * k] = v[0];
+            o[3 * k + 1] = v[1];
+            o[3 * k + 2] = v[2];
         }
         for (int k = 12; k < CELL_FIELD_STRIDE; ++k) o[k] = 0.0;
     }
+}
+
+// Packed 3-component field -> one double4 per dof (w = 0).
+__global__ void k_expand_field(const double* __restrict__ E3, double4* __restrict__ E, int64_t ndof) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ndof; j += (int64_t)gridDim.x * blockDim.x)
+        E[j] = make_double4(E3[3 * j], E3[3 * j + 1], E3[3 * j + 2], 0.0);
 }
 
 // Kinetic energy sum 0.5*m*v.v (reference diagnostics.h:248-265).
@@ -868,44 +952,54 @@ __device__ inline RunInfo lane_run(int32_t key) {
     return r;
 }
 
-// Counting sort by cell.  Within a wavefront, runs of particles in one cell
-// take their slots with one atomic per run (the population is re-sorted
-// often, so runs are long).
-__global__ void k_sort_hist(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ hist) {
-    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
+// Counting sort by cell, one particle per thread over a grid that covers the
+// capacity (blocks past the live count return at once).  Within a wavefront,
+// runs of particles in one cell count and take their slots with one atomic
+// per run; the population is re-sorted often, so runs are long.  The scatter
+// issues the particle loads before the slot atomic, so the atomic's round
+// trip overlaps them.  Expects a compacted population (no negative cells).
+__global__ void __launch_bounds__(256) k_sort_hist(DevState* __restrict__ S, int cell_bits,
+                                                   int64_t* __restrict__ hist) {
     const int64_t n = S->n;
-    const int32_t cmask = (1 << cell_bits) - 1;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const int32_t key = i < n ? (cs[i] & cmask) : -1;
-        const RunInfo r = lane_run(key);
-        if (key >= 0 && r.rank == 0) atomicAdd((unsigned long long*)&hist[key], (unsigned long long)r.len);
-    }
+    const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+    if (b0 >= n) return;  // block-uniform
+    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
+    const int64_t i = b0 + threadIdx.x;
+    const int32_t key = i < n ? (__builtin_nontemporal_load(&cs[i]) & ((1 << cell_bits) - 1)) : -1;
+    const RunInfo r = lane_run(key);
+    if (key >= 0 && r.rank == 0) atomicAdd((unsigned long long*)&hist[key], (unsigned long long)r.len);
 }
 
-__global__ void k_sort_scatter(DevState* __restrict__ S, int cell_bits, int64_t* __restrict__ cursor) {
-    const GArrays A = gview(S->cur), B = gview(S->alt);
+__global__ void __launch_bounds__(256) k_sort_scatter(DevState* __restrict__ S, int cell_bits,
+                                                      int64_t* __restrict__ cursor) {
     const int64_t n = S->n;
-    const int32_t cmask = (1 << cell_bits) - 1;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        const bool active = i < n;
-        const int32_t cs = active ? A.cs[i] : 0;
-        const int32_t key = active ? (cs & cmask) : -1;
-        const RunInfo r = lane_run(key);
-        unsigned long long slot = 0;
-        if (key >= 0 && r.rank == 0) slot = atomicAdd((unsigned long long*)&cursor[key], (unsigned long long)r.len);
-        slot = __shfl(slot, r.head_lane, 64);
-        if (!active) continue;
-        const int64_t d = (int64_t)slot + r.rank;
-        B.x[d] = A.x[i];
-        B.y[d] = A.y[i];
-        B.z[d] = A.z[i];
-        B.vx[d] = A.vx[i];
-        B.vy[d] = A.vy[i];
-        B.vz[d] = A.vz[i];
-        B.cs[d] = cs;
-    }
+    const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+    if (b0 >= n) return;  // block-uniform
+    const GArrays A = gview(S->cur), B = gview(S->alt);
+    const int64_t i = b0 + threadIdx.x;
+    const bool active = i < n;
+    const int64_t j = active ? i : n - 1;
+    const int32_t cs = __builtin_nontemporal_load(&A.cs[j]);
+    const double x = __builtin_nontemporal_load(&A.x[j]);
+    const double y = __builtin_nontemporal_load(&A.y[j]);
+    const double z = __builtin_nontemporal_load(&A.z[j]);
+    const double vx = __builtin_nontemporal_load(&A.vx[j]);
+    const double vy = __builtin_nontemporal_load(&A.vy[j]);
+    const double vz = __builtin_nontemporal_load(&A.vz[j]);
+    const int32_t key = active ? (cs & ((1 << cell_bits) - 1)) : -1;
+    const RunInfo r = lane_run(key);
+    unsigned long long slot = 0;
+    if (key >= 0 && r.rank == 0) slot = atomicAdd((unsigned long long*)&cursor[key], (unsigned long long)r.len);
+    slot = __shfl(slot, r.head_lane, 64);
+    if (!active) return;
+    const int64_t d = (int64_t)slot + r.rank;
+    __builtin_nontemporal_store(x, &B.x[d]);
+    __builtin_nontemporal_store(y, &B.y[d]);
+    __builtin_nontemporal_store(z, &B.z[d]);
+    __builtin_nontemporal_store(vx, &B.vx[d]);
+    __builtin_nontemporal_store(vy, &B.vy[d]);
+    __builtin_nontemporal_store(vz, &B.vz[d]);
+    __builtin_nontemporal_store(cs, &B.cs[d]);
 }
 
 __global__ void k_swap_buffers(DevState* __restrict__ S) {
@@ -1181,7 +1275,7 @@ __global__ void k_next_step(DevState* __restrict__ S) {
 
 template <int PUSH, int MOVE, bool LOCATE, bool DEPOSIT, int DMODE>
 static hipError_t launch_particles_t(const ParticleLaunch& L, hipStream_t st) {
-    const size_t shmem = (DEPOSIT && DMODE == 2) ? LDS_TABLE_BYTES : 0;
+    const size_t shmem = (DEPOSIT && DMODE >= 2) ? LDS_TABLE_BYTES : 0;
     hipLaunchKernelGGL((k_particles<PUSH, MOVE, LOCATE, DEPOSIT, DMODE>), dim3(L.grid), dim3(L.block), shmem,
                        st, L.mesh, L.state, L.coef, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits,
                        L.rho_raw, L.ke_part, L.holes, L.absorbed);
@@ -1196,6 +1290,7 @@ static hipError_t launch_pml(const ParticleLaunch& L, hipStream_t st) {
         case 0: return launch_particles_t<PUSH, MOVE, LOCATE, true, 0>(L, st);
         case 1: return launch_particles_t<PUSH, MOVE, LOCATE, true, 1>(L, st);
         case 2: return launch_particles_t<PUSH, MOVE, LOCATE, true, 2>(L, st);
+        case 3: return launch_particles_t<PUSH, MOVE, LOCATE, true, 3>(L, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1210,9 +1305,10 @@ static int fill_grid(int64_t n);
 
 template <int PUSH, int MOVE, int DMODE>
 static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
-    const size_t shmem = DMODE == 2 ? LDS_TABLE_BYTES : 0;
+    if (L.nsp < 1 || L.nsp > MAX_SPECIES) return hipErrorInvalidValue;
+    const size_t shmem = (DMODE >= 2 ? LDS_TABLE_BYTES : 0) + (size_t)L.nsp * sizeof(SpCoef);
     hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
-                       L.coef, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw, L.ke_part,
+                       L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw, L.ke_part,
                        L.holes, L.absorbed, (WalkRec*)L.walk);
     LAUNCH_CHECK();
     return hipSuccess;
@@ -1224,6 +1320,7 @@ static hipError_t launch_step_pm(const ParticleLaunch& L, hipStream_t st) {
         case 0: return launch_step_t<PUSH, MOVE, 0>(L, st);
         case 1: return launch_step_t<PUSH, MOVE, 1>(L, st);
         case 2: return launch_step_t<PUSH, MOVE, 2>(L, st);
+        case 3: return launch_step_t<PUSH, MOVE, 3>(L, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1245,8 +1342,15 @@ hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st) {
     return hipSuccess;
 }
 
-hipError_t launch_cell_field(const DevMesh& M, const double4* E, double* cellE, hipStream_t st) {
-    hipLaunchKernelGGL(k_cell_field, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, E, cellE);
+hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st) {
+    hipLaunchKernelGGL(k_cell_field, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, E3, cellE);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_expand_field(const double* E3, double4* E, int64_t ndof, hipStream_t st) {
+    if (ndof <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_field, dim3(fill_grid(ndof)), dim3(256), 0, st, E3, E, ndof);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1347,14 +1451,15 @@ hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bs
 }
 
 hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
-                       int64_t* bsum, int64_t* total, int grid, hipStream_t st) {
+                       int64_t* bsum, int64_t* total, int64_t capacity, hipStream_t st) {
     hipError_t e = launch_fill_u64((unsigned long long*)hist, nc, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sort_hist, dim3(grid), dim3(256), 0, st, S, cell_bits, hist);
+    const int64_t grid = std::max<int64_t>((capacity + 255) / 256, 1);
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)grid), dim3(256), 0, st, S, cell_bits, hist);
     LAUNCH_CHECK();
     e = launch_scan(hist, off, nc, bsum, total, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, st, S, cell_bits, off);
+    hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)grid), dim3(256), 0, st, S, cell_bits, off);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_swap_buffers, dim3(1), dim3(64), 0, st, S);
     LAUNCH_CHECK();

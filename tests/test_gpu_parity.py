@@ -127,8 +127,8 @@ def test_distribute_cg1(gpu):
     assert tot == pytest.approx(5000 * QE + 5000 * QI, abs=1e-9)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_distribute_unsorted_both_modes(gpu, mode):
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_distribute_unsorted_modes(gpu, mode):
     """Particles in random order: equal cells recur in non-adjacent lanes."""
     _print_case("distribute_unsorted", mode=mode)
     hm = sphere_mesh(n=4, L=2.0, r=0.7)
@@ -214,13 +214,16 @@ def test_sort_preserves_set(gpu):
     assert np.all(np.diff(after["cell"]) >= 0)
 
 
-@pytest.mark.parametrize("pusher", [1, 2])
-def test_fused_step_matches_sequence(gpu, pusher):
+@pytest.mark.parametrize("pusher,mode,grid", [(1, 2, 0), (2, 2, 0), (2, 3, 0), (1, 3, 7), (2, 1, 3)])
+def test_fused_step_matches_sequence(gpu, pusher, mode, grid):
     """punc_gpu_step (push+move+update+inject+deposit) against the oracle's
-    reference-ordered sequence (interaction.cpp:384-422, then distribute)."""
-    _print_case("fused_step", pusher=pusher)
+    reference-ordered sequence (interaction.cpp:384-422, then distribute), for
+    deposit modes 1-3; grid 0 keeps the default grid, a small grid gives each
+    block a long chunk (several prefetched groups and a long walk list)."""
+    _print_case("fused_step", pusher=pusher, mode=mode, grid=grid)
     hm = sphere_mesh(n=6, L=3.0, r=1.0)
     o, mesh, pop = loaded_pair(hm, (4000, 4000))
+    pop.set_launch(grid or pop.get_launch()[0], 256, mode)
     E = synthetic_field(hm, scale=0.2)
     B = np.array([0.0, 0.0, 1.0])
     dv = P.element_volume(mesh)
@@ -257,6 +260,37 @@ def test_fused_step_matches_sequence(gpu, pusher):
         rho_o = o.distribute(dv)
         scale = rho_scale(o, hm, dv)
         assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_fused_step_table_overflow(gpu, mode):
+    """One block processes the whole population on a mesh with more dofs than
+    the LDS deposit table has slots, so the table fills up and further
+    contributions take the direct HBM path; results as the oracle's."""
+    _print_case("fused_step_overflow", mode=mode)
+    hm = sphere_mesh(n=14, L=4.0, r=1.0)
+    assert hm.ndof > 2048
+    o, mesh, pop = loaded_pair(hm, (9000, 9000))
+    pop.set_launch(1, 256, mode)
+    E = synthetic_field(hm, scale=0.2)
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    L = N
+    L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
+    L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
+    for n in range(2):
+        L.check(L.lib().punc_gpu_step(pop._h, 2, 0.05, L.dptr(B), 0.05, None, 0, 5, n))
+        ab = np.zeros((2, 64), dtype=np.int64)
+        L.check(L.lib().punc_gpu_step_results(pop._h, None, L.i64ptr(ab), None))
+        o.boris(E, B, 0.05)
+        o.move(0.05)
+        ab_o, _, lost = o.update()
+        np.testing.assert_array_equal(ab, ab_o)
+        assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what=f"overflow step {n}")
+        rho_g = np.empty(hm.ndof)
+        L.check(L.lib().punc_gpu_get_rho(pop._h, L.dptr(rho_g)))
+        rho_o = o.distribute(dv)
+        assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * rho_scale(o, hm, dv))
 
 
 def test_graph_replay_matches_eager(gpu):
