@@ -80,6 +80,12 @@ struct punc_pop_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // Side stream of the step: injection generation runs there beside the
+    // particle kernel (forked from and joined back to `stream` with events).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // A re-sort's cell order is pending on the device (see DevState::perm).
+    bool perm_pending = false;
     int nsp = 0;
     int max_sp = 0;
     std::vector<double> q, m;
@@ -110,8 +116,9 @@ struct punc_pop_s {
     int ke_np = 0;
     std::vector<CoefEntry> coef_cache;
     // scan / sort scratch
-    int64_t* d_hist = nullptr;
-    int64_t* d_off = nullptr;
+    int64_t* d_scount = nullptr;  // cell re-sort: per (digit, tile) counts and their scan
+    int64_t* d_soffset = nullptr;
+    int64_t* d_sbsum = nullptr;
     int64_t* d_bsum = nullptr;
     int64_t* d_total = nullptr;  // [0] scan total, [1] injection candidates, [2] kept
     int64_t bsum_len = 0;
@@ -385,8 +392,9 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_ke_part);
     hipFree(P->d_ke);
     for (auto& c : P->coef_cache) hipFree(c.d);
-    hipFree(P->d_hist);
-    hipFree(P->d_off);
+    hipFree(P->d_scount);
+    hipFree(P->d_soffset);
+    hipFree(P->d_sbsum);
     hipFree(P->d_bsum);
     hipFree(P->d_total);
     hipFree(P->d_spi);
@@ -402,6 +410,9 @@ static void pop_free(punc_pop_s* P) {
         hipEventDestroy(e.first);
         hipEventDestroy(e.second);
     }
+    if (P->ev_fork) hipEventDestroy(P->ev_fork);
+    if (P->ev_join) hipEventDestroy(P->ev_join);
+    if (P->side) hipStreamDestroy(P->side);
     if (P->own_stream && P->stream) hipStreamDestroy(P->stream);
 }
 
@@ -444,6 +455,9 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
         PCHK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         P->own_stream = true;
     }
+    PCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+    PCHK(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    PCHK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
     hipDeviceProp_t prop;
     PCHK(hipGetDeviceProperties(&prop, P->device));
     P->grid = prop.multiProcessorCount * 32;
@@ -471,8 +485,10 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     const int64_t nc = m->topo.nc;
     PCHK(dalloc(&P->d_cellE, nc * 16));
     PCHK(hipMemset(P->d_cellE, 0, nc * 16 * sizeof(double)));
-    PCHK(dalloc(&P->d_hist, nc));
-    PCHK(dalloc(&P->d_off, nc));
+    const int64_t nsc = sort_count_entries(capacity);
+    PCHK(dalloc(&P->d_scount, nsc));
+    PCHK(dalloc(&P->d_soffset, nsc));
+    PCHK(dalloc(&P->d_sbsum, nsc / 1024 + 2));
     P->bsum_len = std::max<int64_t>(nc / 1024 + 2, 4096);
     PCHK(dalloc(&P->d_bsum, P->bsum_len));
     PCHK(dalloc(&P->d_total, 8));
@@ -566,8 +582,19 @@ static int check_errors(punc_pop_s* P, const char* what) {
     return 0;
 }
 
+// Operations other than the fused step see the particles in the array
+// order, so a pending cell order is applied first.
+static int ensure_order(punc_pop_s* P) {
+    if (!P->perm_pending) return 0;
+    HIPCHK(launch_apply_order(P->d_state, P->capacity, P->stream));
+    P->perm_pending = false;
+    return 0;
+}
+
 static int append_host(punc_pop_s* P, int64_t n, const double* xs, const double* vs,
                        const int32_t* cs_packed) {
+    int r0 = ensure_order(P);
+    if (r0) return r0;
     DevState s;
     int r = read_state(P, s);
     if (r) return r;
@@ -637,6 +664,10 @@ extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t
     if (axis < 0 || axis > 2) return fail("pop_load: axis must be 0, 1 or 2");
     if (N < 0) return fail("pop_load: negative count");
     HIPCHK(hipSetDevice(p->device));
+    {
+        int r0 = ensure_order(p);
+        if (r0) return r0;
+    }
     DevState s;
     int r = read_state(p, s);
     if (r) return r;
@@ -680,6 +711,10 @@ extern "C" int punc_gpu_pop_size(punc_pop_t* p, int64_t* n) {
 extern "C" int punc_gpu_pop_counts(punc_pop_t* p, int64_t* per_species) {
     if (!p || !per_species) return fail("pop_counts: null argument");
     HIPCHK(hipSetDevice(p->device));
+    {
+        int r0 = ensure_order(p);
+        if (r0) return r0;
+    }
     HIPCHK(launch_fill_u64(p->d_cnt, p->nsp, p->stream));
     HIPCHK(launch_count_species(p->d_state, p->cell_bits, p->nsp, p->d_cnt, p->grid, p->stream));
     HIPCHK(hipMemcpyAsync(per_species, p->d_cnt, p->nsp * 8, hipMemcpyDeviceToHost, p->stream));
@@ -690,8 +725,10 @@ extern "C" int punc_gpu_pop_counts(punc_pop_t* p, int64_t* per_species) {
 extern "C" int punc_gpu_pop_get(punc_pop_t* p, double* xs, double* vs, int32_t* species, int64_t* cells) {
     if (!p) return fail("pop_get: null population");
     HIPCHK(hipSetDevice(p->device));
+    int r = ensure_order(p);
+    if (r) return r;
     DevState s;
-    int r = read_state(p, s);
+    r = read_state(p, s);
     if (r) return r;
     const int64_t n = s.n;
     if (n == 0) return 0;
@@ -805,8 +842,10 @@ static hipError_t timed_launch(punc_pop_s* P, const ParticleLaunch& L, bool pass
 
 static int push_common(punc_pop_s* P, int kind, double dt, const double* B, double* KE) {
     if (!P->field_set) return fail("push: no electric field set (punc_gpu_set_field)");
+    int r = ensure_order(P);
+    if (r) return r;
     const SpCoef* coef;
-    int r = get_coef(P, kind, dt, B, &coef);
+    r = get_coef(P, kind, dt, B, &coef);
     if (r) return r;
     ParticleLaunch L = base_launch(P);
     L.coef = coef;
@@ -845,8 +884,10 @@ extern "C" int punc_gpu_boris(punc_pop_t* p, const double B[3], double dt, doubl
 }
 
 static int move_common(punc_pop_s* P, double dt, const double* Ld) {
+    int r = ensure_order(P);
+    if (r) return r;
     const SpCoef* coef;
-    int r = get_coef(P, 0, 0.0, nullptr, &coef);
+    r = get_coef(P, 0, 0.0, nullptr, &coef);
     if (r) return r;
     ParticleLaunch L = base_launch(P);
     L.coef = coef;
@@ -883,8 +924,10 @@ static int read_absorbed(punc_pop_s* P, int64_t* absorbed) {
 extern "C" int punc_gpu_update(punc_pop_t* p, int64_t* absorbed) {
     if (!p) return fail("update: null population");
     HIPCHK(hipSetDevice(p->device));
+    int r = ensure_order(p);
+    if (r) return r;
     const SpCoef* coef;
-    int r = get_coef(p, 0, 0.0, nullptr, &coef);
+    r = get_coef(p, 0, 0.0, nullptr, &coef);
     if (r) return r;
     HIPCHK(launch_fill_u64(p->d_absorbed, (int64_t)p->nsp * MAX_BND, p->stream));
     ParticleLaunch L = base_launch(p);
@@ -914,7 +957,9 @@ static int upload_dv(punc_pop_s* P, const double* dv_inv) {
 extern "C" int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* rho) {
     if (!p) return fail("distribute: null population");
     HIPCHK(hipSetDevice(p->device));
-    int r = upload_dv(p, dv_inv);
+    int r = ensure_order(p);
+    if (r) return r;
+    r = upload_dv(p, dv_inv);
     if (r) return r;
     const SpCoef* coef;
     r = get_coef(p, 0, 0.0, nullptr, &coef);
@@ -934,8 +979,10 @@ extern "C" int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* 
 extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
     if (!p || !KE) return fail("kinetic_energy: null argument");
     HIPCHK(hipSetDevice(p->device));
+    int r = ensure_order(p);
+    if (r) return r;
     const SpCoef* coef;
-    int r = get_coef(p, 0, 0.0, nullptr, &coef);
+    r = get_coef(p, 0, 0.0, nullptr, &coef);
     if (r) return r;
     HIPCHK(launch_kinetic(p->d_state, coef, p->cell_bits, p->d_ke_part, p->grid, p->stream));
     HIPCHK(launch_sum_partials(p->d_ke_part, p->grid, p->d_ke + 1, p->stream));
@@ -947,8 +994,27 @@ extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
 extern "C" int punc_gpu_sort(punc_pop_t* p) {
     if (!p) return fail("sort: null population");
     HIPCHK(hipSetDevice(p->device));
-    HIPCHK(launch_sort(p->d_state, p->cell_bits, p->mesh->topo.nc, p->d_hist, p->d_off, p->d_bsum,
-                       p->d_total, p->capacity, p->stream));
+    SortLaunch L;
+    L.state = p->d_state;
+    L.cell_bits = p->cell_bits;
+    L.capacity = p->capacity;
+    // The (cell, index) pairs of the passes live in the walk-list buffer
+    // (32 bytes per slot), which is only used inside a step.  The resulting
+    // order is read by the next fused step, which writes the walk list: it
+    // goes to the compaction scratch, used only after that step's particle
+    // kernel has finished.
+    uint32_t* w = (uint32_t*)p->d_walk;
+    L.keys0 = w;
+    L.vals0 = w + p->capacity;
+    L.keys1 = w + 2 * p->capacity;
+    L.vals1 = w + 3 * p->capacity;
+    L.perm = (uint32_t*)p->d_S;
+    L.counts = p->d_scount;
+    L.offsets = p->d_soffset;
+    L.bsum = p->d_sbsum;
+    L.total = p->d_total;
+    HIPCHK(launch_sort(L, p->stream));
+    p->perm_pending = true;
     return 0;
 }
 
@@ -1081,12 +1147,10 @@ static int inject_prepare(punc_pop_s* P, double dt) {
 
 // Enqueues one injection (no host synchronization); the step index is read
 // from the device state.
-static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, double* rho_raw) {
-    if (P->npairs == 0) return 0;
+static int inject_launch_args(punc_pop_s* P, double dt, uint64_t seed, double* rho_raw, InjectLaunch& L) {
     const SpCoef* coef;
     int r = get_coef(P, 0, 0.0, nullptr, &coef);
     if (r) return r;
-    InjectLaunch L;
     memset(&L, 0, sizeof(L));
     L.mesh = P->mesh->dm;
     L.inj.fverts = P->mesh->d_fverts;
@@ -1115,6 +1179,15 @@ static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, double* rho_r
     L.pos = P->d_pos;
     L.rho_raw = rho_raw;
     L.errs = P->d_errs + 1;
+    return 0;
+}
+
+static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, double* rho_raw) {
+    if (P->npairs == 0) return 0;
+    InjectLaunch L;
+    memset(&L, 0, sizeof(L));
+    int r = inject_launch_args(P, dt, seed, rho_raw, L);
+    if (r) return r;
     HIPCHK(launch_inject(L, P->stream));
     return 0;
 }
@@ -1122,7 +1195,9 @@ static int inject_enqueue(punc_pop_s* P, double dt, uint64_t seed, double* rho_r
 extern "C" int punc_gpu_inject(punc_pop_t* p, double dt, uint64_t seed, uint64_t step, int64_t* n_injected) {
     if (!p) return fail("inject: null population");
     HIPCHK(hipSetDevice(p->device));
-    int r = inject_prepare(p, dt);
+    int r = ensure_order(p);
+    if (r) return r;
+    r = inject_prepare(p, dt);
     if (r) return r;
     HIPCHK(launch_fill_u64((unsigned long long*)(p->d_total + 1), 2, p->stream));
     HIPCHK(launch_set_step(p->d_state, step, p->stream));
@@ -1185,6 +1260,20 @@ static int step_prepare(punc_pop_s* P, const StepArgs& a, const SpCoef** coef) {
 // Kernel launches only: safe to capture into a graph.
 static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bool advance_step) {
     const int64_t ndof = P->mesh->topo.ndof;
+    // Injection generation depends only on the mesh, the injection tables and
+    // the step counter: it runs on the side stream beside the particle kernel
+    // and is joined back before its particles are appended.
+    InjectLaunch IL;
+    memset(&IL, 0, sizeof(IL));
+    const bool inj = a.inject && P->npairs > 0;
+    if (inj) {
+        int r = inject_launch_args(P, a.dt, a.seed, P->d_rho_raw, IL);
+        if (r) return r;
+        HIPCHK(hipEventRecord(P->ev_fork, P->stream));
+        HIPCHK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+        HIPCHK(launch_inject_gen(IL, P->side));
+        HIPCHK(hipEventRecord(P->ev_join, P->side));
+    }
     HIPCHK(launch_step_begin(P->d_rho_raw, ndof, P->stream));
     ParticleLaunch L = base_launch(P);
     L.coef = coef;
@@ -1195,11 +1284,12 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
     L.locate = true;
     L.deposit = true;
     HIPCHK(timed_launch(P, L, true));
+    HIPCHK(launch_step_end(P->d_state, P->stream));
     HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
     HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_H, P->d_S, P->grid, P->stream));
-    if (a.inject) {
-        int r = inject_enqueue(P, a.dt, a.seed, P->d_rho_raw);
-        if (r) return r;
+    if (inj) {
+        HIPCHK(hipStreamWaitEvent(P->stream, P->ev_join, 0));
+        HIPCHK(launch_inject_append(IL, P->stream));
     }
     HIPCHK(launch_scale_rho(P->d_rho_raw, P->d_dv_inv, P->d_rho, ndof, P->stream));
     if (advance_step) HIPCHK(launch_next_step(P->d_state, P->stream));
@@ -1234,7 +1324,10 @@ extern "C" int punc_gpu_step(punc_pop_t* p, int pusher, double push_dt, const do
     r = step_prepare(p, a, &coef);
     if (r) return r;
     HIPCHK(launch_set_step(p->d_state, step, p->stream));
-    return step_enqueue(p, a, coef, false);
+    r = step_enqueue(p, a, coef, false);
+    if (r) return r;
+    p->perm_pending = false;  // the fused step applied it
+    return 0;
 }
 
 static void graph_free(punc_pop_s* P) {
@@ -1250,6 +1343,8 @@ extern "C" int punc_gpu_step_graph(punc_pop_t* p, int pusher, double push_dt, co
     int r = step_check(p, pusher, B);
     if (r) return r;
     HIPCHK(hipSetDevice(p->device));
+    r = ensure_order(p);
+    if (r) return r;
     StepArgs a = make_args(pusher, push_dt, B, dt, Ld, inject, seed);
     const SpCoef* coef;
     r = step_prepare(p, a, &coef);
@@ -1283,7 +1378,10 @@ extern "C" int punc_gpu_graph_launch(punc_pop_t* p, int count) {
     if (!p) return fail("graph_launch: null population");
     if (!p->graph_exec) return fail("graph_launch: no captured step (punc_gpu_step_graph)");
     HIPCHK(hipSetDevice(p->device));
-    for (int k = 0; k < count; ++k) HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
+    for (int k = 0; k < count; ++k) {
+        HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
+        p->perm_pending = false;  // the captured fused step applied it
+    }
     return 0;
 }
 
@@ -1416,6 +1514,9 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     if (!p) return fail("autotune: null population");
     if (reps < 1) return fail("autotune: reps must be >= 1");
     int r = step_check(p, pusher, B);
+    if (r) return r;
+    HIPCHK(hipSetDevice(p->device));
+    r = ensure_order(p);
     if (r) return r;
     HIPCHK(hipSetDevice(p->device));
     StepArgs a = make_args(pusher, push_dt, B, dt, Ld, inject, seed);

@@ -37,7 +37,12 @@ struct DevState {
     int64_t inj_kept;  // injection: kept candidates this call
     int64_t overflow;  // particles dropped because capacity was full
     uint64_t step;     // injection stream step index
-    int64_t pad[6];
+    // Pending cell order from the last re-sort: cur[perm[j]] is the j-th
+    // particle in cell order.  The next fused step reads through it and writes
+    // the alternate buffers; other operations apply it first (gather + swap).
+    const uint32_t* perm;
+    int64_t perm_pending;
+    int64_t pad[4];
 };
 
 // Per-species coefficients for one call (depends on dt and B).

@@ -77,14 +77,42 @@ hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned lo
                                 int grid, hipStream_t st);
 hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
                        hipStream_t st);
-// Counting sort by cell into the alternate buffers, then a buffer swap.
-// capacity: particle slots per buffer (sizes the one-particle-per-thread grid).
-hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
-                       int64_t* bsum, int64_t* total, int64_t capacity, hipStream_t st);
+// Cell re-sort (LSD radix sort of (cell, index) pairs, then a gather of the
+// particle data into the alternate buffers and a buffer swap).
+struct SortLaunch {
+    DevState* state;
+    int cell_bits;
+    int64_t capacity;   // particle slots per buffer
+    uint32_t* keys0;    // capacity each: pass scratch, free once the sort has run
+    uint32_t* vals0;
+    uint32_t* keys1;
+    uint32_t* vals1;
+    uint32_t* perm;     // capacity: the resulting order; must stay untouched until
+                        // the next fused step has read it (that step writes its
+                        // walk list, so not in the walk-list buffer)
+    int64_t* counts;    // sort_count_entries(capacity) each
+    int64_t* offsets;
+    int64_t* bsum;      // scan scratch: sort_count_entries / 1024 + 2
+    int64_t* total;
+};
+// Computes the cell order and leaves it pending (DevState::perm): the next
+// fused step reads through it; launch_apply_order applies it otherwise.
+hipError_t launch_sort(const SortLaunch& L, hipStream_t st);
+// Gather + swap of a pending cell order (no-op on the device when none is).
+hipError_t launch_apply_order(DevState* S, int64_t capacity, hipStream_t st);
+// After a fused step: swap in the buffers a permuted step wrote.
+hipError_t launch_step_end(DevState* S, hipStream_t st);
+int64_t sort_count_entries(int64_t capacity);
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
                        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
                        double cmax, double amp, double kw, int axis, double vth, const double vd[3],
                        unsigned long long* errs, hipStream_t st);
+// Injection in two halves: generation (per-facet counts, candidates,
+// rejection-sampled velocities, location; reads only the mesh, the tables and
+// the step counter, so it may run beside a particle kernel) and the append of
+// the kept candidates at the end of the population.  launch_inject = both.
+hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st);
+hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st);
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
 hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
 hipError_t launch_next_step(DevState* S, hipStream_t st);

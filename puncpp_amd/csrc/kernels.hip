@@ -583,6 +583,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         if (threadIdx.x == 0) ke_part[blockIdx.x] = 0.0;
         return;
     }
+    // A pending cell order: particle j of this step is cur[perm[j]], and the
+    // results go to the alternate buffers at j (swapped in after the kernel).
+    const bool permuted = S->perm_pending != 0;
+    const GLOBAL uint32_t* perm = (const GLOBAL uint32_t*)S->perm;
+    const GArrays O = permuted ? gview(S->alt) : A;
     const int32_t cmask = (1 << cell_bits) - 1;
     const double tol = M.inside_tol;
     const double X = M.coord_bound;
@@ -593,8 +598,16 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t last = e - 1;
+    // Source index of a particle: through the pending cell order, if any.
+    // The order is loaded one group ahead of the particle data it addresses.
+    auto clampi = [&](int64_t j) { return j < e ? j : last; };
     PartRegs nxt;
-    load_particle(A, s + threadIdx.x < e ? s + threadIdx.x : last, nxt);
+    int64_t src_next = 0;
+    {
+        const int64_t j0 = clampi(s + threadIdx.x);
+        load_particle(A, permuted ? (int64_t)perm[j0] : j0, nxt);
+        if (permuted) src_next = perm[clampi(s + threadIdx.x + blockDim.x)];
+    }
     for (int64_t base = s; base < e; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
         const bool active = i < e;
@@ -625,9 +638,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         __builtin_amdgcn_sched_barrier(0);
         // Prefetch the next group (clamped index: lanes past the end reload
-        // the chunk's last particle and discard it).
-        const int64_t inx = i + blockDim.x;
-        load_particle(A, inx < e ? inx : last, nxt);
+        // the chunk's last particle and discard it), then the order entries
+        // of the group after it.
+        const int64_t inx = clampi(i + blockDim.x);
+        load_particle(A, permuted ? src_next : inx, nxt);
+        if (permuted) src_next = perm[clampi(i + 2 * (int64_t)blockDim.x)];
         __builtin_amdgcn_sched_barrier(0);
         double x = p.x, y = p.y, z = p.z, vx = p.vx, vy = p.vy, vz = p.vz;
         double l[4];
@@ -646,12 +661,13 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         if (active) {
             ke += ke_l;
-            __builtin_nontemporal_store(vx, &A.vx[i]);
-            __builtin_nontemporal_store(vy, &A.vy[i]);
-            __builtin_nontemporal_store(vz, &A.vz[i]);
-            __builtin_nontemporal_store(x, &A.x[i]);
-            __builtin_nontemporal_store(y, &A.y[i]);
-            __builtin_nontemporal_store(z, &A.z[i]);
+            __builtin_nontemporal_store(vx, &O.vx[i]);
+            __builtin_nontemporal_store(vy, &O.vy[i]);
+            __builtin_nontemporal_store(vz, &O.vz[i]);
+            __builtin_nontemporal_store(x, &O.x[i]);
+            __builtin_nontemporal_store(y, &O.y[i]);
+            __builtin_nontemporal_store(z, &O.z[i]);
+            if (permuted) __builtin_nontemporal_store(p.cs, &O.cs[i]);
         }
         double dl[4];
         bary4(rb, x, y, z, dl);
@@ -707,9 +723,9 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             const int32_t c2 = walk_bary(M, c, r.x, r.y, r.z, dl, ddof);
             if (c2 < 0) {
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
-                A.cs[i] = c2;
+                O.cs[i] = c2;
             } else {
-                if (c2 != c) A.cs[i] = (sp << cell_bits) | c2;
+                if (c2 != c) O.cs[i] = (sp << cell_bits) | c2;
                 dkey = c2;
                 dq = coef[sp].q;
             }
@@ -718,6 +734,24 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
     }
     if (use_lds) lds_flush(lds_table(), rho_raw);
+}
+
+// After the fused step: a step that read through a pending cell order wrote
+// the alternate buffers, which become current.
+__global__ void k_step_end(DevState* __restrict__ S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && S->perm_pending) {
+        const PopArrays t = S->cur;
+        S->cur = S->alt;
+        S->alt = t;
+        S->perm_pending = 0;
+    }
+}
+
+__global__ void k_set_perm(DevState* __restrict__ S, const uint32_t* __restrict__ perm) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->perm = perm;
+        S->perm_pending = 1;
+    }
 }
 
 __global__ void k_step_begin(double* __restrict__ rho_raw, int64_t ndof) {
@@ -767,12 +801,17 @@ k_kinetic(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bi
     if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
 }
 
-// Deterministic final sum of the per-block partials (one block).
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ part, int np,
-                                                      double* __restrict__ out) {
-    __shared__ double sh[8];
-    double v = 0.0;
-    for (int k = threadIdx.x; k < np; k += blockDim.x) v += part[k];
+// Deterministic final sum of the per-block partials (one block of 1024;
+// eight independent loads in flight per thread).
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ part, int np,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[16];
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int k = threadIdx.x;
+    for (; k + 7 * 1024 < np; k += 8 * 1024)
+        for (int u = 0; u < 8; ++u) a[u] += part[k + u * 1024];
+    for (; k < np; k += 1024) a[0] += part[k];
+    const double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     const double t = block_sum(v, sh);
     if (threadIdx.x == 0) *out = t;
 }
@@ -926,89 +965,115 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t len, const int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Cell re-sort: counting sort of the population by cell id into the alternate
-// buffers, then a pointer swap.  Order within a cell is arbitrary.
+// Cell re-sort: least-significant-digit radix sort of (cell, index) pairs,
+// RADIX_BITS per pass, then one gather of the particle data into the
+// alternate buffers and a pointer swap.  No global atomics: every wavefront
+// ranks one tile of RADIX_TILE consecutive keys, stably, with ballot-matched
+// lane groups and wavefront-private digit counters in LDS; the tile's digit
+// counts, scanned digit-major over all tiles, give each (digit, tile) its
+// output offset.  The result is a stable sort by cell (particles of one cell
+// keep their previous relative order).
 // ---------------------------------------------------------------------------
-// Runs of equal keys in consecutive lanes of a wavefront: the run's first
-// lane, this lane's offset in its run, and the run length.  Keys < 0 are
-// runs too (callers skip them).  Every lane of the wavefront must call this.
-struct RunInfo {
-    int head_lane;
-    int rank;
-    int len;
-};
-__device__ inline RunInfo lane_run(int32_t key) {
-    const int lane = threadIdx.x & 63;
-    const int32_t kprev = __shfl_up(key, 1, 64);
-    const unsigned long long heads = __ballot(lane == 0 || kprev != key);
-    const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-    const int head = 63 - __clzll((long long)upto);
-    const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-    const int next_head = later ? __ffsll((long long)later) - 1 : 64;
-    RunInfo r;
-    r.head_lane = head;
-    r.rank = lane - head;
-    r.len = next_head - head;
-    return r;
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int RADIX_TILE = 4096;  // keys per wavefront tile
+
+// Lanes of the wavefront whose digit equals this lane's.
+__device__ inline unsigned long long match_digit(uint32_t dg) {
+    unsigned long long m = ~0ull;
+    for (int b = 0; b < RADIX_BITS; ++b) {
+        const unsigned long long bb = __ballot((dg >> b) & 1u);
+        m &= ((dg >> b) & 1u) ? bb : ~bb;
+    }
+    return m;
 }
 
-// Counting sort by cell, one particle per thread over a grid that covers the
-// capacity (blocks past the live count return at once).  Within a wavefront,
-// runs of particles in one cell count and take their slots with one atomic
-// per run; the population is re-sorted often, so runs are long.  The scatter
-// issues the particle loads before the slot atomic, so the atomic's round
-// trip overlaps them.  Expects a compacted population (no negative cells).
-__global__ void __launch_bounds__(256) k_sort_hist(DevState* __restrict__ S, int cell_bits,
-                                                   int64_t* __restrict__ hist) {
+// Key of element i in pass `shift`: the first pass reads cells from the
+// particle array, later passes the previous pass's keys.
+__device__ inline uint32_t radix_key(const GLOBAL int32_t* cs, const uint32_t* keys, int32_t cmask, int64_t i) {
+    return cs ? (uint32_t)(cs[i] & cmask) : keys[i];
+}
+
+// Digit counts of each tile: counts[digit * ntiles + tile].
+__global__ void __launch_bounds__(256)
+k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __restrict__ keys, int shift,
+              int64_t ntiles, int64_t* __restrict__ counts) {
+    __shared__ uint32_t cnt[4][RADIX];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + w;
     const int64_t n = S->n;
-    const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
-    if (b0 >= n) return;  // block-uniform
-    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
-    const int64_t i = b0 + threadIdx.x;
-    const int32_t key = i < n ? (__builtin_nontemporal_load(&cs[i]) & ((1 << cell_bits) - 1)) : -1;
-    const RunInfo r = lane_run(key);
-    if (key >= 0 && r.rank == 0) atomicAdd((unsigned long long*)&hist[key], (unsigned long long)r.len);
-}
-
-__global__ void __launch_bounds__(256) k_sort_scatter(DevState* __restrict__ S, int cell_bits,
-                                                      int64_t* __restrict__ cursor) {
-    const int64_t n = S->n;
-    const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
-    if (b0 >= n) return;  // block-uniform
-    const GArrays A = gview(S->cur), B = gview(S->alt);
-    const int64_t i = b0 + threadIdx.x;
-    const bool active = i < n;
-    const int64_t j = active ? i : n - 1;
-    const int32_t cs = __builtin_nontemporal_load(&A.cs[j]);
-    const double x = __builtin_nontemporal_load(&A.x[j]);
-    const double y = __builtin_nontemporal_load(&A.y[j]);
-    const double z = __builtin_nontemporal_load(&A.z[j]);
-    const double vx = __builtin_nontemporal_load(&A.vx[j]);
-    const double vy = __builtin_nontemporal_load(&A.vy[j]);
-    const double vz = __builtin_nontemporal_load(&A.vz[j]);
-    const int32_t key = active ? (cs & ((1 << cell_bits) - 1)) : -1;
-    const RunInfo r = lane_run(key);
-    unsigned long long slot = 0;
-    if (key >= 0 && r.rank == 0) slot = atomicAdd((unsigned long long*)&cursor[key], (unsigned long long)r.len);
-    slot = __shfl(slot, r.head_lane, 64);
-    if (!active) return;
-    const int64_t d = (int64_t)slot + r.rank;
-    __builtin_nontemporal_store(x, &B.x[d]);
-    __builtin_nontemporal_store(y, &B.y[d]);
-    __builtin_nontemporal_store(z, &B.z[d]);
-    __builtin_nontemporal_store(vx, &B.vx[d]);
-    __builtin_nontemporal_store(vy, &B.vy[d]);
-    __builtin_nontemporal_store(vz, &B.vz[d]);
-    __builtin_nontemporal_store(cs, &B.cs[d]);
-}
-
-__global__ void k_swap_buffers(DevState* __restrict__ S) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const PopArrays t = S->cur;
-        S->cur = S->alt;
-        S->alt = t;
+    const GLOBAL int32_t* cs = keys ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int d = lane; d < RADIX; d += 64) cnt[w][d] = 0;
+    if (tile < ntiles) {
+        const int64_t t0 = tile * RADIX_TILE;
+        const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
+        for (int64_t base = t0; base < t1; base += 64) {  // wavefront-uniform
+            const int64_t i = base + lane;
+            const bool act = i < t1;
+            const uint32_t dg = act ? (radix_key(cs, keys, cmask, i) >> shift) & (RADIX - 1) : RADIX;
+            const unsigned long long m = match_digit(dg) & __ballot(act);
+            if (act && lane == __ffsll((long long)m) - 1) cnt[w][dg] += (uint32_t)__popcll(m);
+        }
+        for (int d = lane; d < RADIX; d += 64) counts[(int64_t)d * ntiles + tile] = cnt[w][d];
     }
 }
+
+// Stable scatter of each tile's (key, value) pairs to offsets[digit * ntiles +
+// tile] plus the rank among the tile's earlier keys of that digit.  vals ==
+// nullptr: values are the element indices (first pass).  keys_out == nullptr:
+// the last pass, only values are written.
+__global__ void __launch_bounds__(256)
+k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint32_t* __restrict__ keys,
+                const uint32_t* __restrict__ vals, int shift, int64_t ntiles, const int64_t* __restrict__ offsets,
+                uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t run[4][RADIX];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + w;
+    if (tile >= ntiles) return;  // wavefront-uniform, and no block barrier below
+    const int64_t n = S->n;
+    const GLOBAL int32_t* cs = keys ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int d = lane; d < RADIX; d += 64) run[w][d] = (uint32_t)offsets[(int64_t)d * ntiles + tile];
+    const int64_t t0 = tile * RADIX_TILE;
+    const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int64_t base = t0; base < t1; base += 64) {  // wavefront-uniform
+        const int64_t i = base + lane;
+        const bool act = i < t1;
+        const uint32_t key = act ? radix_key(cs, keys, cmask, i) : 0u;
+        const uint32_t val = act ? (vals ? vals[i] : (uint32_t)i) : 0u;
+        const uint32_t dg = act ? (key >> shift) & (RADIX - 1) : RADIX;
+        const unsigned long long m = match_digit(dg) & __ballot(act);
+        uint32_t dst = 0;
+        if (act) dst = run[w][dg] + (uint32_t)__popcll(m & below);
+        // Every lane has read its group's counter before the group's lowest
+        // lane advances it (one wavefront: LDS operations complete in order).
+        if (act && lane == __ffsll((long long)m) - 1) run[w][dg] += (uint32_t)__popcll(m);
+        if (act) {
+            if (keys_out) keys_out[dst] = key;
+            vals_out[dst] = val;
+        }
+    }
+}
+
+// Apply a pending cell order outside a fused step: alt[j] = cur[perm[j]]
+// for all particle arrays (k_step_end then swaps the buffers).
+__global__ void __launch_bounds__(256) k_sort_gather(DevState* __restrict__ S) {
+    const int64_t n = S->n;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !S->perm_pending) return;
+    const GArrays A = gview(S->cur), B = gview(S->alt);
+    const int64_t i = S->perm[j];
+    __builtin_nontemporal_store(A.x[i], &B.x[j]);
+    __builtin_nontemporal_store(A.y[i], &B.y[j]);
+    __builtin_nontemporal_store(A.z[i], &B.z[j]);
+    __builtin_nontemporal_store(A.vx[i], &B.vx[j]);
+    __builtin_nontemporal_store(A.vy[i], &B.vy[j]);
+    __builtin_nontemporal_store(A.vz[i], &B.vz[j]);
+    __builtin_nontemporal_store(A.cs[i], &B.cs[j]);
+}
+
 
 // ---------------------------------------------------------------------------
 // Initial loading (reference load_particles, injector.h:244-278): positions by
@@ -1088,16 +1153,20 @@ __global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S
     }
 }
 
-// Candidates are processed 64 per wavefront.  The velocity rejection sampler
-// (injector.cpp:51-76 against the flux pdf, distributions.h:93-102) runs
-// cooperatively: for one candidate at a time the 64 lanes evaluate attempts
-// 1+64r+lane of round r and the lowest accepting attempt wins - exactly the
-// draw the sequential loop accepts, without a lane idling while the slowest
-// candidate of its wavefront keeps rejecting (acceptance is about 1% for a
-// non-drifting Maxwellian in the vd +- 5 vth box).  Then every lane finishes
-// its own candidate: facet point (injector.cpp:78-102), push by U*dt*v
-// (injector.h:212-216), location by a walk from the facet's cell.
+// Candidates are processed INJECT_PER_WAVE per wavefront (lanes 0.. own one
+// each).  The velocity rejection sampler (injector.cpp:51-76 against the flux
+// pdf, distributions.h:93-102) runs cooperatively: for one candidate at a
+// time the 64 lanes evaluate attempts 1+64r+lane of round r and the lowest
+// accepting attempt wins - exactly the draw the sequential loop accepts,
+// without a lane idling while the slowest candidate of its wavefront keeps
+// rejecting (acceptance is about 1% for a non-drifting Maxwellian in the
+// vd +- 5 vth box).  Few candidates per wavefront keep the per-wavefront
+// chain of rounds short and spread a step's candidates over many
+// wavefronts.  Then every owning lane finishes its candidate: facet point
+// (injector.cpp:78-102), push by U*dt*v (injector.h:212-216), location by a
+// walk from the facet's cell.
 constexpr uint64_t INJECT_MAX_ATTEMPTS = 1000000ull;
+constexpr int INJECT_PER_WAVE = 8;
 
 __global__ void __launch_bounds__(256)
 k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
@@ -1110,9 +1179,10 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t g0 = wave * 64; g0 < cand_cap; g0 += nwaves * 64) {
+    for (int64_t g0 = wave * INJECT_PER_WAVE; g0 < cand_cap; g0 += nwaves * INJECT_PER_WAVE) {
         const int64_t k = g0 + lane;
-        const bool mine = k < total;
+        const bool owner = lane < INJECT_PER_WAVE;
+        const bool mine = owner && k < total;
         // This lane's candidate: its pair (last p with foff[p] <= k; count > 0)
         int64_t p = 0, s = 0, j = 0, slot = 0;
         if (mine) {
@@ -1134,7 +1204,7 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
         const double pmax = I.pmax[p];
         double v[3] = {0.0, 0.0, 0.0};
         bool ok = false;
-        const int64_t ng = total - g0 < 64 ? total - g0 : 64;  // wavefront-uniform
+        const int64_t ng = total - g0 < INJECT_PER_WAVE ? total - g0 : INJECT_PER_WAVE;  // wavefront-uniform
         for (int c = 0; c < ng; ++c) {
             // broadcast candidate c's parameters
             const uint64_t key_c = __shfl(key, c, 64), sj_c = __shfl(sj, c, 64);
@@ -1176,7 +1246,7 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
                 }
             }
         }
-        if (k >= cand_cap) continue;
+        if (!owner || k >= cand_cap) continue;
         if (!mine) {
             keep[k] = 0;
             continue;
@@ -1378,7 +1448,7 @@ hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double
 }
 
 hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, np, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, st, part, np, out);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1450,20 +1520,52 @@ hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bs
     return hipSuccess;
 }
 
-hipError_t launch_sort(DevState* S, int cell_bits, int64_t nc, int64_t* hist, int64_t* off,
-                       int64_t* bsum, int64_t* total, int64_t capacity, hipStream_t st) {
-    hipError_t e = launch_fill_u64((unsigned long long*)hist, nc, st);
-    if (e != hipSuccess) return e;
-    const int64_t grid = std::max<int64_t>((capacity + 255) / 256, 1);
-    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)grid), dim3(256), 0, st, S, cell_bits, hist);
-    LAUNCH_CHECK();
-    e = launch_scan(hist, off, nc, bsum, total, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)grid), dim3(256), 0, st, S, cell_bits, off);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_swap_buffers, dim3(1), dim3(64), 0, st, S);
+hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
+    const int64_t ntiles = std::max<int64_t>((L.capacity + RADIX_TILE - 1) / RADIX_TILE, 1);
+    const unsigned blocks = (unsigned)((ntiles + 3) / 4);
+    const int passes = (L.cell_bits + RADIX_BITS - 1) / RADIX_BITS;
+    const int64_t ncount = ntiles * RADIX;
+    uint32_t* kbuf[2] = {L.keys0, L.keys1};
+    uint32_t* vbuf[2] = {L.vals0, L.vals1};
+    const uint32_t* kin = nullptr;
+    const uint32_t* vin = nullptr;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int shift = ps * RADIX_BITS;
+        hipLaunchKernelGGL(k_radix_count, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
+                           L.counts);
+        LAUNCH_CHECK();
+        hipError_t e = launch_scan(L.counts, L.offsets, ncount, L.bsum, L.total, st);
+        if (e != hipSuccess) return e;
+        uint32_t* kout = ps + 1 < passes ? kbuf[ps & 1] : nullptr;
+        uint32_t* vout = ps + 1 < passes ? vbuf[ps & 1] : L.perm;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, vin, shift,
+                           ntiles, L.offsets, kout, vout);
+        LAUNCH_CHECK();
+        kin = kout;
+        vin = vout;
+    }
+    hipLaunchKernelGGL(k_set_perm, dim3(1), dim3(64), 0, st, L.state, vin);
     LAUNCH_CHECK();
     return hipSuccess;
+}
+
+hipError_t launch_apply_order(DevState* S, int64_t capacity, hipStream_t st) {
+    const int64_t gblocks = std::max<int64_t>((capacity + 255) / 256, 1);
+    hipLaunchKernelGGL(k_sort_gather, dim3((unsigned)gblocks), dim3(256), 0, st, S);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, S);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_step_end(DevState* S, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, S);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+int64_t sort_count_entries(int64_t capacity) {
+    return std::max<int64_t>((capacity + RADIX_TILE - 1) / RADIX_TILE, 1) * RADIX;
 }
 
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
@@ -1477,25 +1579,35 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
     return hipSuccess;
 }
 
-hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
+hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st) {
     const int64_t np = L.inj.npairs;
     hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(np)), dim3(256), 0, st, L.inj, L.state, L.spi,
                        L.seed, L.fcount);
     LAUNCH_CHECK();
     hipError_t e = launch_scan(L.fcount, L.foff, np, L.bsum, L.ntot, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_inject_gen, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.inj,
+    // INJECT_PER_WAVE candidates per wavefront, 4 wavefronts per block.
+    const int64_t gen_blocks = std::min<int64_t>((L.cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE), 16384);
+    hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st, L.mesh, L.inj,
                        L.state, L.spi, L.foff, L.ntot, L.seed, L.dt, L.cell_bits, L.cand_cap,
                        L.cx, L.ccs, L.keep, L.errs);
     LAUNCH_CHECK();
-    e = launch_scan(L.keep, L.pos, L.cand_cap, L.bsum, L.nkept, st);
-    if (e != hipSuccess) return e;
+    return launch_scan(L.keep, L.pos, L.cand_cap, L.bsum, L.nkept, st);
+}
+
+hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st) {
     hipLaunchKernelGGL(k_inject_append, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.state,
                        L.coef, L.cand_cap, L.ntot, L.cx, L.ccs, L.pos, L.cell_bits, L.rho_raw);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_inject_finish, dim3(1), dim3(64), 0, st, L.state, L.ntot, L.nkept, L.cand_cap);
     LAUNCH_CHECK();
     return hipSuccess;
+}
+
+hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
+    hipError_t e = launch_inject_gen(L, st);
+    if (e != hipSuccess) return e;
+    return launch_inject_append(L, st);
 }
 
 }  // namespace punc_gpu

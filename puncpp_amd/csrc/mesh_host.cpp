@@ -170,31 +170,47 @@ bool meshgen_box(int n, const double lo[3], const double hi[3], double sphere_r,
                 xyz[v * 3 + 2] = (k == n) ? hi[2] : lo[2] + k * h[2];
             }
     static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-    std::vector<int32_t> cells;
-    cells.reserve((size_t)n * n * n * 24);
-    const double r2 = sphere_r * sphere_r;
+    // Cubes in Morton (Z-curve) order of their lattice indices, the six tets
+    // of a cube consecutive: cells close in space get close ids, so a range
+    // of cell ids (and of cell-sorted particles) is a compact region and a
+    // particle crossing into a neighbouring cube usually stays in that range.
+    auto spread3 = [](uint64_t v) {
+        uint64_t r = 0;
+        for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (3 * b);
+        return r;
+    };
+    std::vector<std::pair<uint64_t, int64_t>> order;
+    order.reserve((size_t)n * n * n);
     for (int64_t k = 0; k < n; ++k)
         for (int64_t j = 0; j < n; ++j)
             for (int64_t i = 0; i < n; ++i)
-                for (int p = 0; p < 6; ++p) {
-                    int64_t ijk[3] = {i, j, k};
-                    int32_t tv[4];
-                    tv[0] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
-                    for (int s = 0; s < 2; ++s) {
-                        ijk[perms[p][s]] += 1;
-                        tv[s + 1] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
-                    }
-                    tv[3] = (int32_t)((i + 1) + n1 * ((j + 1) + n1 * (k + 1)));
-                    std::sort(tv, tv + 4);
-                    if (sphere_r > 0) {
-                        double c[3] = {0, 0, 0};
-                        for (int q = 0; q < 4; ++q)
-                            for (int d = 0; d < 3; ++d) c[d] += xyz[(size_t)tv[q] * 3 + d];
-                        for (int d = 0; d < 3; ++d) c[d] /= 4.0;
-                        if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] < r2) continue;
-                    }
-                    cells.insert(cells.end(), tv, tv + 4);
-                }
+                order.emplace_back(spread3(i) | (spread3(j) << 1) | (spread3(k) << 2), i + n * (j + n * k));
+    std::sort(order.begin(), order.end());
+    std::vector<int32_t> cells;
+    cells.reserve((size_t)n * n * n * 24);
+    const double r2 = sphere_r * sphere_r;
+    for (const auto& ok : order) {
+        const int64_t i = ok.second % n, j = (ok.second / n) % n, k = ok.second / ((int64_t)n * n);
+        for (int p = 0; p < 6; ++p) {
+            int64_t ijk[3] = {i, j, k};
+            int32_t tv[4];
+            tv[0] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
+            for (int s = 0; s < 2; ++s) {
+                ijk[perms[p][s]] += 1;
+                tv[s + 1] = (int32_t)(ijk[0] + n1 * (ijk[1] + n1 * ijk[2]));
+            }
+            tv[3] = (int32_t)((i + 1) + n1 * ((j + 1) + n1 * (k + 1)));
+            std::sort(tv, tv + 4);
+            if (sphere_r > 0) {
+                double c[3] = {0, 0, 0};
+                for (int q = 0; q < 4; ++q)
+                    for (int d = 0; d < 3; ++d) c[d] += xyz[(size_t)tv[q] * 3 + d];
+                for (int d = 0; d < 3; ++d) c[d] /= 4.0;
+                if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] < r2) continue;
+            }
+            cells.insert(cells.end(), tv, tv + 4);
+        }
+    }
     const int64_t nc = (int64_t)cells.size() / 4;
     // Compact away vertices that no cell uses (the sphere interior).
     std::vector<int32_t> remap((size_t)nv_full, -1);

@@ -331,3 +331,58 @@ def test_graph_replay_matches_eager(gpu):
         L.check(L.lib().punc_gpu_get_rho(popA._h, L.dptr(ra)))
         L.check(L.lib().punc_gpu_get_rho(popB._h, L.dptr(rb)))
         assert np.allclose(ra, rb, rtol=1e-11, atol=1e-11 * np.abs(ra).max())
+
+
+@pytest.mark.parametrize("mode,graph", [(2, False), (3, False), (2, True)])
+def test_fused_step_after_sort(gpu, mode, graph):
+    """A re-sort leaves its cell order pending and the next fused step reads
+    the particles through it (writing the alternate buffers); the result is
+    the oracle's sequence.  Particles are added in random order first, so the
+    order is far from sorted.  graph: the steps are replays of a captured
+    step with re-sorts between them."""
+    _print_case("fused_step_after_sort", mode=mode, graph=graph)
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    import oracle as O
+    o = O.OraclePop(hm, [QE, QI], [ME, MI])
+    o.load(0, 5000, 11, 0.0, 0.0, 0, 1.0, [0.0, 0.0, 0.5])
+    o.load(1, 5000, 11, 0.0, 0.0, 0, 0.05, [0.0, 0.0, 0.5])
+    st = o.get_particles()
+    perm = np.random.default_rng(4).permutation(st["x"].shape[0])
+    mesh = P.Mesh(hm)
+    pop = P.Population(mesh, [(QE, ME), (QI, MI)], capacity=40000)
+    pop.add_located(st["x"][perm], st["v"][perm], st["species"][perm], st["cell"][perm])
+    pop.set_launch(5, 256, mode)
+    E = synthetic_field(hm, scale=0.2)
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    L = N
+    L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
+    L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
+    dt = 0.05
+    if graph:
+        L.check(L.lib().punc_gpu_step_graph(pop._h, 2, dt, L.dptr(B), dt, None, 0, 3, 0))
+    for n in range(3):
+        L.check(L.lib().punc_gpu_sort(pop._h))
+        if graph:
+            L.check(L.lib().punc_gpu_graph_launch(pop._h, 1))
+        else:
+            L.check(L.lib().punc_gpu_step(pop._h, 2, dt, L.dptr(B), dt, None, 0, 3, n))
+        ab = np.zeros((2, 64), dtype=np.int64)
+        ke = L.C.c_double()
+        L.check(L.lib().punc_gpu_step_results(pop._h, L.C.byref(ke), L.i64ptr(ab), None))
+        ke_o = o.boris(E, B, dt)
+        o.move(dt)
+        ab_o, _, lost = o.update()
+        np.testing.assert_array_equal(ab, ab_o)
+        assert abs(ke.value - ke_o) <= 1e-11 * abs(ke_o)
+        got = pop.get_particles()
+        assert_same_sets(got, o.get_particles(), rel=0.0, what=f"sorted step {n}")
+        rho_g = np.empty(hm.ndof)
+        L.check(L.lib().punc_gpu_get_rho(pop._h, L.dptr(rho_g)))
+        rho_o = o.distribute(dv)
+        assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * rho_scale(o, hm, dv))
+    # a re-sort applied by a drop-in read (no fused step): cells in order
+    L.check(L.lib().punc_gpu_sort(pop._h))
+    got = pop.get_particles()
+    assert np.all(np.diff(got["cell"]) >= 0)
+    assert_same_sets(got, o.get_particles(), rel=0.0, what="sorted read")
