@@ -6,7 +6,7 @@ of 64^3 cubes x 6 tetrahedra with the r=2 sphere object removed (1,572,864
 tets before removal), 64M particles (32M electrons q=-1 m=1 vth=1, 32M ions
 q=+1 m=1836 vth=1/sqrt(1836), both drifting (0,0,0.5)), Boris push with
 B=(0,0,1), dt=0.02, Maxwellian flux injection at the exterior boundary, CG1
-deposit each step, cell re-sort every 20 steps.  The field is a frozen
+deposit each step, cell re-sort every 12 steps.  The field is a frozen
 synthetic CG1 field resident on the GPU (the host field solve is outside this
 path).  One step = push + move + relocation walk with absorption + removal +
 injection + deposit, all on the device.
@@ -355,7 +355,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="large", choices=sorted(CONFIGS))
-    ap.add_argument("--sort-every", type=int, default=20)
+    ap.add_argument("--sort-every", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")

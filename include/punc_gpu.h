@@ -215,7 +215,7 @@ int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, doub
  * are summed across lanes first; 2: run sums go into a per-block LDS table
  * keyed by dof, flushed with one atomic per occupied slot; 3: every lane
  * adds its four contributions to that LDS table directly).  Default:
- * 32 x CUs, 256, mode 2. */
+ * 128 x CUs, 256, mode 3. */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
 /* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every

@@ -93,7 +93,7 @@ struct punc_pop_s {
     int64_t capacity = 0;
     int grid = 0, block = 256;
     int num_cu = 1;
-    int dep_mode = 2;
+    int dep_mode = 3;
 
     DevState* d_state = nullptr;
     void* bufs[2][7] = {};
@@ -460,9 +460,11 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
     hipDeviceProp_t prop;
     PCHK(hipGetDeviceProperties(&prop, P->device));
-    P->grid = prop.multiProcessorCount * 32;
-    P->num_cu = prop.multiProcessorCount;
     P->ke_np = 65536;
+    // Default launch: 128 blocks per CU (short chunks, many blocks in
+    // flight), deposit mode 3; the autotuner can pick another shape.
+    P->grid = std::min(prop.multiProcessorCount * 128, P->ke_np);
+    P->num_cu = prop.multiProcessorCount;
     for (int b = 0; b < 2; ++b) {
         for (int k = 0; k < 6; ++k) PCHK(dalloc((double**)&P->bufs[b][k], capacity));
         PCHK(dalloc((int32_t**)&P->bufs[b][6], capacity));

@@ -602,7 +602,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     // The order is loaded one group ahead of the particle data it addresses.
     auto clampi = [&](int64_t j) { return j < e ? j : last; };
     PartRegs nxt;
-    int64_t src_next = 0;
+    uint32_t src_next = 0;
     {
         const int64_t j0 = clampi(s + threadIdx.x);
         load_particle(A, permuted ? (int64_t)perm[j0] : j0, nxt);
@@ -637,12 +637,14 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             ef[2 * m + 1] = t.y;
         }
         __builtin_amdgcn_sched_barrier(0);
-        // Prefetch the next group (clamped index: lanes past the end reload
-        // the chunk's last particle and discard it), then the order entries
-        // of the group after it.
+        // Prefetch: the order entries of the group after next, then the next
+        // group (clamped index: lanes past the end reload the chunk's last
+        // particle and discard it).  The entries go first so that taking
+        // them over waits for nothing issued later.
         const int64_t inx = clampi(i + blockDim.x);
-        load_particle(A, permuted ? src_next : inx, nxt);
-        if (permuted) src_next = perm[clampi(i + 2 * (int64_t)blockDim.x)];
+        const uint32_t src_after = permuted ? perm[clampi(i + 2 * (int64_t)blockDim.x)] : 0u;
+        load_particle(A, permuted ? (int64_t)src_next : inx, nxt);
+        src_next = src_after;
         __builtin_amdgcn_sched_barrier(0);
         double x = p.x, y = p.y, z = p.z, vx = p.vx, vy = p.vy, vz = p.vz;
         double l[4];
