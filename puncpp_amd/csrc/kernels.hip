@@ -252,6 +252,41 @@ __device__ inline void lds_flush(LdsTable t, double* __restrict__ rho_raw) {
 }
 
 // ready: lpre/dpre already hold lambda at (x, y, z) in cell `key` and its dofs.
+// Move of a double from another lane with DPP (both 32-bit halves with the
+// same control; lanes without a source read 0).
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Segmented inclusive sums over runs of lanes: afterwards lane L holds the
+// sum of v over lanes head..L of its run (head: the run's first lane).  The
+// steps of a wavefront scan on GFX9 DPP: shifts by 1, 2, 4, 8 inside each
+// 16-lane row, then the last lane of row 0 (2) broadcast into row 1 (3),
+// then lane 31 into rows 2 and 3; a step adds only a source lane of the
+// same run.  All lanes of the wavefront must be active.
+__device__ inline void run_scan4(double v[4], int lane, int head) {
+    const int r0 = lane & ~15;  // first lane of this lane's row
+#define PUNC_RUN_STEP(CTRL, ROWM, SRC_OK)                               \
+    {                                                                   \
+        const bool ok = (SRC_OK);                                       \
+        for (int k = 0; k < 4; ++k) {                                   \
+            const double t = dpp_f64<CTRL, ROWM>(v[k]);                 \
+            if (ok) v[k] += t;                                          \
+        }                                                               \
+    }
+    PUNC_RUN_STEP(0x111, 0xf, lane - 1 >= head && lane - 1 >= r0)
+    PUNC_RUN_STEP(0x112, 0xf, lane - 2 >= head && lane - 2 >= r0)
+    PUNC_RUN_STEP(0x114, 0xf, lane - 4 >= head && lane - 4 >= r0)
+    PUNC_RUN_STEP(0x118, 0xf, lane - 8 >= head && lane - 8 >= r0)
+    PUNC_RUN_STEP(0x142, 0xa, ((lane >> 4) & 1) && r0 - 1 >= head)
+    PUNC_RUN_STEP(0x143, 0xc, lane >= 32 && 31 >= head)
+#undef PUNC_RUN_STEP
+}
+
 template <int mode>
 __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_raw, int32_t key, double x,
                                    double y, double z, double q, bool ready, const double* lpre,
@@ -283,19 +318,16 @@ __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_ra
     if (mode >= 1) {
         // A run is a maximal stretch of consecutive lanes with one key; the
         // same key may reappear in a later run, which is summed separately.
+        // The run's last lane ends up with the run's sums and deposits them.
         const int lane = threadIdx.x & 63;
         const int32_t kprev = __shfl_up(key, 1, 64);
         const bool head = lane == 0 || kprev != key;
         const unsigned long long heads = __ballot(head);
-        const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-        const int run_end = later ? __ffsll((long long)later) - 1 : 64;
-        for (int o = 1; o < 64; o <<= 1) {
-            double t[4];
-            for (int k = 0; k < 4; ++k) t[k] = __shfl_down(d[k], o, 64);
-            if (lane + o < run_end)
-                for (int k = 0; k < 4; ++k) d[k] += t[k];
-        }
-        if (key < 0 || !head) return;
+        const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+        const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
+        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+        run_scan4(d, lane, hl);
+        if (key < 0 || !tail) return;
         if (mode == 2) {
             const LdsTable t = lds_table();
             for (int k = 0; k < 4; ++k) lds_add(t, rho_raw, dof[k], d[k]);
