@@ -163,6 +163,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
     t_out = time.perf_counter()
     xfer = {"field_in_ms": 1e3 * (t_mid - t_in) / 5, "rho_out_ms": 1e3 * (t_out - t_mid) / 5,
             "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
+    diag = run_diagnostics(pop, mesh, dv)
     elapsed = t2 - t1
     n_avg = 0.5 * (n_start + n_end)
     value = n_avg * steps / elapsed
@@ -175,7 +176,48 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
         "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
+        "diag": diag,
     }
+
+
+def run_diagnostics(pop, mesh, dv, reps=5):
+    """Wall time per call of the particle diagnostics over the whole
+    population (each call ends with a stream synchronize; results stay on the
+    device except the scalars).  Algorithmic HBM bytes per call: the particle
+    fields read (x, y, z, cs: 28 B; statistics: vx, vy, vz, cs), the cell
+    records (128 B per cell) and the dof/cell arrays read or written."""
+    from puncpp_amd import _native as N
+    lib = N.lib()
+    n = pop.size()
+    nc, nd = mesh.num_cells, mesh.ndof
+    phi = np.ones(nd)
+    out = N.C.c_double()
+    stats = np.zeros(4)
+
+    def timed(f):
+        f()
+        t = time.perf_counter()
+        for _ in range(reps):
+            f()
+        return 1e3 * (time.perf_counter() - t) / reps
+
+    res = {
+        "potential_energy": (timed(lambda: N.check(lib.punc_gpu_potential_energy(pop._h, N.dptr(phi),
+                                                                                 N.C.byref(out)))),
+                             28.0 * n + 128.0 * nc + 8.0 * nd),
+        "density_cg1": (timed(lambda: N.check(lib.punc_gpu_density_cg1(pop._h, N.dptr(dv), 1.0, 1.0, None,
+                                                                       None))),
+                        28.0 * n + 128.0 * nc + 2 * 8.0 * nd * 3),
+        "density_dg0": (timed(lambda: N.check(lib.punc_gpu_density_dg0(pop._h, 1.0, 1.0, None, None))),
+                        4.0 * n + 2 * 8.0 * nc * 3 + 8.0 * nc),
+        "statistics": (timed(lambda: N.check(lib.punc_gpu_statistics(pop._h, N.dptr(stats)))), 28.0 * n),
+    }
+    out_d = {"particles": n}
+    for k, (ms, b) in res.items():
+        out_d[k] = {"ms_per_call": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9}
+    log(f"[bench] diagnostics: " + ", ".join(f"{k} {v['ms_per_call']:.3f} ms ({v['GBps']:.0f} GB/s)"
+                                             for k, v in out_d.items() if k != "particles"))
+    return out_d
 
 
 def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
@@ -406,6 +448,7 @@ def main():
                      "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]},
         "cpu_baseline": cpu,
         "host_transfer": res["xfer"],
+        "diagnostics": res["diag"],
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -127,8 +127,11 @@ private:
 //! df::Function on the particle path; E has 3 components, rho has 1).
 class Function {
 public:
-    Function(const Mesh& mesh, int value_dim) : value_dim(value_dim), values(mesh.ndof * value_dim, 0.0) {}
+    //! dg0: one value per cell and component (cell-major) instead of per dof.
+    Function(const Mesh& mesh, int value_dim, bool dg0 = false)
+        : value_dim(value_dim), dg0(dg0), values((dg0 ? mesh.num_cells() : mesh.ndof) * value_dim, 0.0) {}
     int value_dim;
+    bool dg0;
     std::vector<double> values;  ///< dof-major: values[dof*value_dim + component]
 };
 

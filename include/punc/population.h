@@ -164,6 +164,10 @@ public:
     //! Re-sort particles by cell (keeps per-cell data reads coalesced).
     void sort() { check(punc_gpu_sort(handle())); }
 
+    //! statistics (population.h:746-805): mean and standard deviation of the
+    //! speed of the q < 0 (stats[0..1]) and q > 0 (stats[2..3]) particles.
+    void statistics(double* stats) { check(punc_gpu_statistics(handle(), stats)); }
+
     //! Seed of the injection stream (the reference's RNG is unseeded).
     void set_seed(uint64_t seed) { seed_ = seed; }
     uint64_t seed() const { return seed_; }

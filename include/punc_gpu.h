@@ -142,6 +142,35 @@ int punc_gpu_update(punc_pop_t* p, int64_t* absorbed);
 int punc_gpu_distribute(punc_pop_t* p, const double* dv_inv, double* rho);
 /* kinetic_energy(pop) (diagnostics.h:248-265). */
 int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE);
+/* particle_potential_energy_cg1(pop, phi) (diagnostics.h:379-412): sum over
+ * the particles of 0.5 q phi(x), phi given by its CG1 dof values (ndof). */
+int punc_gpu_potential_energy(punc_pop_t* p, const double* phi, double* PE);
+/* density_cg1(V, pop, species, ne, ni, dv_inv) (diagnostics.h:492-555):
+ * particles with q < 0 count as electrons, the others as ions; weight_e and
+ * weight_i are species[0].weight and species[1].weight.  dv_inv as for
+ * punc_gpu_distribute (NULL: the one already uploaded, else the Voronoi
+ * volumes).  ne, ni (ndof each) may be NULL to keep the result on the
+ * device for punc_gpu_density_ema. */
+int punc_gpu_density_cg1(punc_pop_t* p, const double* dv_inv, double weight_e, double weight_i, double* ne,
+                         double* ni);
+/* density_dg0(Q, pop, species, ne, ni) (diagnostics.h:432-465): q > 0 counts
+ * as ions, the others as electrons; one value per cell (nc each, indexed by
+ * cell id), weights summed and divided by the cell volume. */
+int punc_gpu_density_dg0(punc_pop_t* p, double weight_e, double weight_i, double* ne, double* ni);
+/* Population::statistics(stats) (population.h:746-805): stats[0], stats[1]
+ * mean and sample standard deviation of the speed of the q < 0 particles,
+ * stats[2], stats[3] of the q > 0 ones; a class without particles leaves its
+ * entries unchanged. */
+int punc_gpu_statistics(punc_pop_t* p, double* stats);
+/* ema(rho, rho_ema, dt, tau) (diagnostics.h:573-588) with rho the
+ * population's last deposited charge density and rho_ema kept on the device
+ * (zero when first used; reset != 0 zeroes it first).  g (ndof) receives
+ * rho_ema when non-NULL. */
+int punc_gpu_rho_ema(punc_pop_t* p, double dt, double tau, int reset, double* g);
+/* ema(ne, ne_ema, dt, tau) and ema(ni, ni_ema, dt, tau) over the last density
+ * computed by punc_gpu_density_cg1/dg0, the averages kept on the device
+ * (reset != 0, needed when the density's length changed, zeroes them). */
+int punc_gpu_density_ema(punc_pop_t* p, double dt, double tau, int reset, double* ne_g, double* ni_g);
 /* Re-sort the particles by cell (counting sort, order within a cell
  * arbitrary).  Enqueued on the population's stream; returns without waiting. */
 int punc_gpu_sort(punc_pop_t* p);

@@ -46,6 +46,10 @@ _SIGS = {
     "oracle_update": (C.c_int64, [vp, i64_p, d_p]),
     "oracle_distribute": (None, [vp, d_p, d_p]),
     "oracle_kinetic_energy": (C.c_double, [vp]),
+    "oracle_potential_energy": (C.c_double, [vp, d_p]),
+    "oracle_density_cg1": (None, [vp, d_p, C.c_double, C.c_double, d_p, d_p]),
+    "oracle_density_dg0": (None, [vp, C.c_double, C.c_double, d_p, d_p]),
+    "oracle_statistics": (None, [vp, d_p]),
     "oracle_flux": (None, [C.c_double, d_p, d_p, C.c_double, d_p, d_p]),
     "oracle_set_injector": (None, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
     "oracle_inject": (C.c_int64, [vp, C.c_double, C.c_uint64, C.c_uint64]),
@@ -212,6 +216,26 @@ class OraclePop:
 
     def kinetic_energy(self):
         return lib().oracle_kinetic_energy(self._h)
+
+    def potential_energy(self, phi):
+        return lib().oracle_potential_energy(self._h, _d(phi))
+
+    def density_cg1(self, dv_inv, weight_e=1.0, weight_i=1.0):
+        nd = self.ndof
+        ne, ni = np.zeros(nd), np.zeros(nd)
+        lib().oracle_density_cg1(self._h, _d(dv_inv), float(weight_e), float(weight_i), _d(ne), _d(ni))
+        return ne, ni
+
+    def density_dg0(self, weight_e=1.0, weight_i=1.0):
+        nc = self.nc
+        ne, ni = np.zeros(nc), np.zeros(nc)
+        lib().oracle_density_dg0(self._h, float(weight_e), float(weight_i), _d(ne), _d(ni))
+        return ne, ni
+
+    def statistics(self, stats=None):
+        st = np.full(4, np.nan) if stats is None else np.array(stats, dtype=np.float64)
+        lib().oracle_statistics(self._h, _d(st))
+        return st
 
     def set_injector(self, species, vth, vd, n, vrange=5.0):
         vd = np.ascontiguousarray(vd, dtype=np.float64)

@@ -22,6 +22,9 @@
 //   distribute_cg1                     punc/include/punc/distributor.h:168-210
 //   element_volume                     punc/src/distributor.cpp:35-69
 //   kinetic_energy                     punc/include/punc/diagnostics.h:248-265
+//   particle_potential_energy_cg1      diagnostics.h:379-412
+//   density_dg0, density_cg1           diagnostics.h:432-465, 492-555
+//   Population::statistics             population.h:746-805
 //   inject_particles                   punc/include/punc/injector.h:157-232
 //   rejection_sampler, random_facet_points, create_flux  punc/src/injector.cpp:25-171
 //   load_particles                     injector.h:244-278
@@ -873,6 +876,115 @@ int64_t oracle_load(void* h, int sp, int64_t N, uint64_t seed, double amp, doubl
         P.cells[c].particles.push_back(p);
     }
     return failed;
+}
+
+// particle_potential_energy_cg1 (diagnostics.h:379-412): PE = sum_p 0.5 q_p
+// phi(x_p), phi interpolated with the barycentric coordinates.
+double oracle_potential_energy(void* h, const double* phi) {
+    Pop& P = *(Pop*)h;
+    double phi_x, PE = 0.0;
+    double coeffs[4], values[4];
+    for (auto& cell : P.cells) {
+        for (int i = 0; i < 4; ++i) values[i] = phi[cell.dofs[i]];
+        for (auto& particle : cell.particles) {
+            cell.barycentric(particle.x, coeffs);
+            phi_x = 0.0;
+            for (int i = 0; i < 4; ++i) phi_x += coeffs[i] * values[i];
+            PE += 0.5 * particle.q * phi_x;
+        }
+    }
+    return PE;
+}
+
+// density_cg1 (diagnostics.h:492-555): q < 0 counts as electrons, anything
+// else as ions; n_j = dv_inv_j * w * sum_p psi_j(x_p).
+void oracle_density_cg1(void* h, const double* dv_inv, double weight_e, double weight_i, double* ne, double* ni) {
+    Pop& P = *(Pop*)h;
+    std::vector<double> ne0(P.ndof, 0.0), ni0(P.ndof, 0.0);
+    double cell_coords[4];
+    for (auto& cell : P.cells) {
+        const int32_t* dof_id = cell.dofs;
+        double accum_e[4] = {0, 0, 0, 0}, accum_i[4] = {0, 0, 0, 0};
+        for (auto& particle : cell.particles) {
+            cell.barycentric(particle.x, cell_coords);
+            if (particle.q < 0) {
+                for (int i = 0; i < 4; ++i) accum_e[i] += cell_coords[i];
+            } else {
+                for (int i = 0; i < 4; ++i) accum_i[i] += cell_coords[i];
+            }
+        }
+        for (int i = 0; i < 4; ++i) {
+            ne0[dof_id[i]] += accum_e[i];
+            ni0[dof_id[i]] += accum_i[i];
+        }
+    }
+    for (int64_t i = 0; i < P.ndof; ++i) {
+        ne0[i] *= dv_inv[i] * weight_e;
+        ni0[i] *= dv_inv[i] * weight_i;
+    }
+    memcpy(ne, ne0.data(), P.ndof * 8);
+    memcpy(ni, ni0.data(), P.ndof * 8);
+}
+
+// density_dg0 (diagnostics.h:432-465): q > 0 counts as ions, anything else as
+// electrons; n_k = (sum of weights in cell k) / volume of cell k.
+void oracle_density_dg0(void* h, double weight_e, double weight_i, double* ne, double* ni) {
+    Pop& P = *(Pop*)h;
+    for (auto& cell : P.cells) {
+        double accum_e = 0.0, accum_i = 0.0;
+        for (auto& particle : cell.particles) {
+            if (particle.q > 0)
+                accum_i += weight_i;
+            else
+                accum_e += weight_e;
+        }
+        const double vol = cell_volume(cell);
+        ne[cell.id] = accum_e / vol;
+        ni[cell.id] = accum_i / vol;
+    }
+}
+
+// Population::statistics (population.h:746-805): running (Welford) mean and
+// standard deviation of the speed of the particles with q < 0 (stats[0..1])
+// and q > 0 (stats[2..3]); entries of a class with no particles are left as
+// they were.
+void oracle_statistics(void* h, double* stats) {
+    Pop& P = *(Pop*)h;
+    std::size_t m = 0, n = 0;
+    double v, m_e_old = 0, m_i_old = 0;
+    for (auto& cell : P.cells) {
+        for (auto& particle : cell.particles) {
+            v = 0;
+            for (int i = 0; i < 3; ++i) v += particle.v[i] * particle.v[i];
+            v = sqrt(v);
+            if (particle.q < 0) {
+                m++;
+                if (m == 1) {
+                    m_e_old = v;
+                    stats[0] = v;
+                    stats[1] = 0.0;
+                } else {
+                    stats[0] = m_e_old + (v - m_e_old) / m;
+                    stats[1] += (v - m_e_old) * (v - stats[0]);
+                    m_e_old = stats[0];
+                }
+            }
+            if (particle.q > 0) {
+                n++;
+                if (n == 1) {
+                    m_i_old = v;
+                    stats[2] = v;
+                    stats[3] = 0.0;
+                } else {
+                    stats[2] = m_i_old + (v - m_i_old) / n;
+                    stats[3] += (v - m_i_old) * (v - stats[2]);
+                    m_i_old = stats[2];
+                }
+            }
+        }
+    }
+    if (m > 0) stats[1] = sqrt(stats[1] / (m - 1));
+    if (n > 0) stats[3] = sqrt(stats[3] / (n - 1));
 }
 
 }  // extern "C"

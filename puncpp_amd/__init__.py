@@ -6,5 +6,6 @@ operator API (puncpp_amd.punc) and the build recipe (puncpp_amd.build).
 """
 from .punc import (CosinePosition, ExteriorFacet, HostMesh, Maxwellian, Mesh, Object,  # noqa: F401
                    ParticleAmountType, Population, Species, UniformPosition, accel_cg1, boris_cg1,
-                   box_mesh, create_flux, distribute_cg1, element_volume, inject_particles,
-                   kinetic_energy, load_particles, move, move_periodic)
+                   box_mesh, create_flux, density_cg1, density_dg0, density_ema, distribute_cg1,
+                   element_volume, inject_particles, kinetic_energy, load_particles, move, move_periodic,
+                   particle_potential_energy_cg1, rho_ema)

@@ -54,6 +54,12 @@ _SIGS = {
     "punc_gpu_update": (C.c_int, [vp, i64_p]),
     "punc_gpu_distribute": (C.c_int, [vp, d_p, d_p]),
     "punc_gpu_kinetic_energy": (C.c_int, [vp, d_p]),
+    "punc_gpu_potential_energy": (C.c_int, [vp, d_p, d_p]),
+    "punc_gpu_density_cg1": (C.c_int, [vp, d_p, C.c_double, C.c_double, d_p, d_p]),
+    "punc_gpu_density_dg0": (C.c_int, [vp, C.c_double, C.c_double, d_p, d_p]),
+    "punc_gpu_statistics": (C.c_int, [vp, d_p]),
+    "punc_gpu_rho_ema": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, d_p]),
+    "punc_gpu_density_ema": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, d_p, d_p]),
     "punc_gpu_sort": (C.c_int, [vp]),
     "punc_gpu_injector_set": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
     "punc_gpu_species_instance": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int64,

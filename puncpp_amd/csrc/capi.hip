@@ -144,6 +144,16 @@ struct punc_pop_s {
     int32_t* d_ccs = nullptr;
     int64_t* d_keep = nullptr;
     int64_t* d_pos = nullptr;
+    // diagnostics (allocated on first use)
+    double* d_diag = nullptr;      // per-block partials (6 per block) + results
+    double* d_phi = nullptr;       // ndof
+    double* d_draw = nullptr;      // 2 * max(ndof, nc): raw density sums
+    double* d_dens = nullptr;      // 2 * max(ndof, nc): last density (ne, then ni)
+    double* d_dens_ema = nullptr;  // 2 * max(ndof, nc)
+    double* d_rho_ema = nullptr;   // ndof
+    double* d_cvol = nullptr;      // nc
+    int64_t dens_len = 0;          // entries per species class of the last density (0: none)
+    int64_t dens_ema_len = 0;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -391,6 +401,13 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_dv_inv);
     hipFree(P->d_ke_part);
     hipFree(P->d_ke);
+    hipFree(P->d_diag);
+    hipFree(P->d_phi);
+    hipFree(P->d_draw);
+    hipFree(P->d_dens);
+    hipFree(P->d_dens_ema);
+    hipFree(P->d_rho_ema);
+    hipFree(P->d_cvol);
     for (auto& c : P->coef_cache) hipFree(c.d);
     hipFree(P->d_scount);
     hipFree(P->d_soffset);
@@ -989,6 +1006,141 @@ extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
     HIPCHK(launch_kinetic(p->d_state, coef, p->cell_bits, p->d_ke_part, p->grid, p->stream));
     HIPCHK(launch_sum_partials(p->d_ke_part, p->grid, p->d_ke + 1, p->stream));
     HIPCHK(hipMemcpyAsync(KE, p->d_ke + 1, 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (reference diagnostics.h, Population::statistics)
+// ---------------------------------------------------------------------------
+static int diag_grid(const punc_pop_s* P) { return std::max(1, std::min(P->grid, P->num_cu * 16)); }
+
+static int diag_alloc(punc_pop_s* P) {
+    if (P->d_diag) return 0;
+    const int64_t ndof = P->mesh->topo.ndof, nc = P->mesh->topo.nc;
+    const int64_t len = std::max(ndof, nc);
+    HIPCHK(dalloc(&P->d_diag, 6 * (size_t)P->ke_np + 8));
+    HIPCHK(dalloc(&P->d_phi, ndof));
+    HIPCHK(dalloc(&P->d_draw, 2 * len));
+    HIPCHK(dalloc(&P->d_dens, 2 * len));
+    HIPCHK(dalloc(&P->d_dens_ema, 2 * len));
+    HIPCHK(dalloc(&P->d_rho_ema, ndof));
+    std::vector<double> vol;
+    cell_volumes(P->mesh->topo, vol);
+    HIPCHK(dalloc(&P->d_cvol, nc));
+    HIPCHK(hipMemcpy(P->d_cvol, vol.data(), nc * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(P->d_rho_ema, 0, ndof * 8));
+    HIPCHK(hipMemset(P->d_dens_ema, 0, 2 * len * 8));
+    return 0;
+}
+
+static int diag_begin(punc_pop_s* P, const SpCoef** coef) {
+    HIPCHK(hipSetDevice(P->device));
+    int r = ensure_order(P);
+    if (r) return r;
+    r = diag_alloc(P);
+    if (r) return r;
+    return get_coef(P, 0, 0.0, nullptr, coef);
+}
+
+extern "C" int punc_gpu_potential_energy(punc_pop_t* p, const double* phi, double* PE) {
+    if (!p || !phi || !PE) return fail("potential_energy: null argument");
+    const SpCoef* coef;
+    int r = diag_begin(p, &coef);
+    if (r) return r;
+    const int g = diag_grid(p);
+    HIPCHK(hipMemcpyAsync(p->d_phi, phi, p->mesh->topo.ndof * 8, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(launch_potential_energy(p->mesh->dm, p->d_state, coef, p->cell_bits, p->d_phi, p->d_diag, g, p->stream));
+    HIPCHK(launch_sum_partials(p->d_diag, g, p->d_diag + 6 * (size_t)p->ke_np, p->stream));
+    HIPCHK(hipMemcpyAsync(PE, p->d_diag + 6 * (size_t)p->ke_np, 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+static int density_common(punc_pop_s* P, bool dg0, const double* dv_inv, double we, double wi, double* ne,
+                          double* ni) {
+    const SpCoef* coef;
+    int r = diag_begin(P, &coef);
+    if (r) return r;
+    const int64_t len = dg0 ? P->mesh->topo.nc : P->mesh->topo.ndof;
+    const double* scale = P->d_cvol;
+    if (!dg0) {
+        r = upload_dv(P, dv_inv);
+        if (r) return r;
+        scale = P->d_dv_inv;
+    }
+    HIPCHK(launch_fill_f64(P->d_draw, 2 * len, 0.0, P->stream));
+    HIPCHK(launch_density(P->mesh->dm, P->d_state, coef, P->cell_bits, dg0, P->d_draw, diag_grid(P), P->stream));
+    HIPCHK(launch_density_scale(dg0, P->d_draw, scale, we, wi, P->d_dens, P->d_dens + len, len, P->stream));
+    P->dens_len = len;
+    if (ne) HIPCHK(hipMemcpyAsync(ne, P->d_dens, len * 8, hipMemcpyDeviceToHost, P->stream));
+    if (ni) HIPCHK(hipMemcpyAsync(ni, P->d_dens + len, len * 8, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_density_cg1(punc_pop_t* p, const double* dv_inv, double weight_e, double weight_i,
+                                    double* ne, double* ni) {
+    if (!p) return fail("density_cg1: null population");
+    return density_common(p, false, dv_inv, weight_e, weight_i, ne, ni);
+}
+
+extern "C" int punc_gpu_density_dg0(punc_pop_t* p, double weight_e, double weight_i, double* ne, double* ni) {
+    if (!p) return fail("density_dg0: null population");
+    return density_common(p, true, nullptr, weight_e, weight_i, ne, ni);
+}
+
+extern "C" int punc_gpu_statistics(punc_pop_t* p, double* stats) {
+    if (!p || !stats) return fail("statistics: null argument");
+    const SpCoef* coef;
+    int r = diag_begin(p, &coef);
+    if (r) return r;
+    const int g = diag_grid(p);
+    double* out = p->d_diag + 6 * (size_t)p->ke_np;
+    HIPCHK(launch_statistics(p->d_state, coef, p->cell_bits, p->d_diag, out, g, p->stream));
+    double h[6];
+    HIPCHK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    // population.h:802-803: sample standard deviation from M2; a class with
+    // no particles leaves its two entries as they were.
+    for (int k = 0; k < 2; ++k) {
+        if (h[3 * k] > 0) {
+            stats[2 * k] = h[3 * k + 1];
+            stats[2 * k + 1] = sqrt(h[3 * k + 2] / (h[3 * k] - 1));
+        }
+    }
+    return 0;
+}
+
+// ema (diagnostics.h:573-588) on device-resident state: w = 1 - exp(-dt/tau).
+extern "C" int punc_gpu_rho_ema(punc_pop_t* p, double dt, double tau, int reset, double* g) {
+    if (!p) return fail("rho_ema: null population");
+    HIPCHK(hipSetDevice(p->device));
+    int r = diag_alloc(p);
+    if (r) return r;
+    const int64_t ndof = p->mesh->topo.ndof;
+    const double w = 1.0 - exp(-dt / tau);
+    if (reset) HIPCHK(launch_fill_f64(p->d_rho_ema, ndof, 0.0, p->stream));
+    HIPCHK(launch_ema(p->d_rho, p->d_rho_ema, w, ndof, p->stream));
+    if (g) HIPCHK(hipMemcpyAsync(g, p->d_rho_ema, ndof * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+extern "C" int punc_gpu_density_ema(punc_pop_t* p, double dt, double tau, int reset, double* ne_g,
+                                    double* ni_g) {
+    if (!p) return fail("density_ema: null population");
+    if (p->dens_len == 0) return fail("density_ema: no density computed yet (punc_gpu_density_cg1/dg0)");
+    HIPCHK(hipSetDevice(p->device));
+    const int64_t len = p->dens_len;
+    if (!reset && p->dens_ema_len != len)
+        return fail("density_ema: the averaged density has another length (pass reset = 1)");
+    const double w = 1.0 - exp(-dt / tau);
+    if (reset) HIPCHK(launch_fill_f64(p->d_dens_ema, 2 * len, 0.0, p->stream));
+    p->dens_ema_len = len;
+    HIPCHK(launch_ema(p->d_dens, p->d_dens_ema, w, 2 * len, p->stream));
+    if (ne_g) HIPCHK(hipMemcpyAsync(ne_g, p->d_dens_ema, len * 8, hipMemcpyDeviceToHost, p->stream));
+    if (ni_g) HIPCHK(hipMemcpyAsync(ni_g, p->d_dens_ema + len, len * 8, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }

@@ -1372,6 +1372,188 @@ __global__ void k_next_step(DevState* __restrict__ S) {
 }
 
 // ---------------------------------------------------------------------------
+// Diagnostics (reference diagnostics.h, Population::statistics)
+// ---------------------------------------------------------------------------
+// particle_potential_energy_cg1 (diagnostics.h:379-412): per block the sum of
+// 0.5 q phi(x) over its chunk, phi interpolated with the barycentric
+// coordinates in the reference's order.
+__global__ void __launch_bounds__(256)
+k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
+                   const double* __restrict__ phi, double* __restrict__ part) {
+    __shared__ double sh[8];
+    const GArrays A = gview(S->cur);
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    const int32_t cmask = (1 << cell_bits) - 1;
+    double pe = 0.0;
+    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+        const int32_t cs = A.cs[i];
+        const CellRec& R = M.rec[cs & cmask];
+        double l[4];
+        bary4(R.bary, A.x[i], A.y[i], A.z[i], l);
+        double phi_x = 0.0;
+        for (int k = 0; k < 4; ++k) phi_x += l[k] * phi[R.dof[k]];
+        pe += 0.5 * coef[cs >> cell_bits].q * phi_x;
+    }
+    const double t = block_sum(pe, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Number densities (density_cg1, diagnostics.h:492-555; density_dg0,
+// diagnostics.h:432-465) accumulated in the block's LDS table keyed by
+// 2 * index + class and flushed into raw (2 entries per dof or cell).
+// CG1: class 0 for q < 0, else 1; adds the barycentric coordinates to the
+// cell's four dofs.  DG0: class 1 for q > 0, else 0; adds 1 to the cell.
+template <bool DG0>
+__global__ void __launch_bounds__(256)
+k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
+          double* __restrict__ raw) {
+    const GArrays A = gview(S->cur);
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    if (s >= e) return;  // block-uniform
+    const int32_t cmask = (1 << cell_bits) - 1;
+    const LdsTable t = lds_table();
+    lds_clear(t);
+    __syncthreads();
+    for (int64_t b = s; b < e; b += blockDim.x) {  // block-uniform trip count
+        const int64_t i = b + threadIdx.x;
+        if (i < e) {
+            const int32_t cs = A.cs[i];
+            const int32_t c = cs & cmask;
+            const double q = coef[cs >> cell_bits].q;
+            if (DG0) {
+                lds_add(t, raw, 2 * c + (q > 0 ? 1 : 0), 1.0);
+            } else {
+                const CellRec& R = M.rec[c];
+                double l[4];
+                bary4(R.bary, A.x[i], A.y[i], A.z[i], l);
+                const int cls = q < 0 ? 0 : 1;
+                for (int k = 0; k < 4; ++k) lds_add(t, raw, 2 * R.dof[k] + cls, l[k]);
+            }
+        }
+        // one decision for the whole block (the count read is a heuristic)
+        if (__syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8)) lds_flush(t, raw);
+    }
+    lds_flush(t, raw);
+}
+
+// CG1: n_j = raw_j * (dv_inv_j * w) (diagnostics.h:549-553).  DG0: n_k =
+// (w + w + ... count_k times) / volume_k, the weights summed one particle at a
+// time as diagnostics.h:452-459 does (count_k * w can round differently).
+__device__ inline double repeat_sum(double count, double w) {
+    double a = 0.0;
+    for (int64_t k = (int64_t)count; k > 0; --k) a += w;
+    return a;
+}
+
+template <bool DG0>
+__global__ void k_density_scale(const double* __restrict__ raw, const double* __restrict__ scale, double we,
+                                double wi, double* __restrict__ ne, double* __restrict__ ni, int64_t n) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        if (DG0) {
+            ne[j] = repeat_sum(raw[2 * j], we) / scale[j];
+            ni[j] = repeat_sum(raw[2 * j + 1], wi) / scale[j];
+        } else {
+            ne[j] = raw[2 * j] * (scale[j] * we);
+            ni[j] = raw[2 * j + 1] * (scale[j] * wi);
+        }
+    }
+}
+
+// Population::statistics (population.h:746-805) in parallel: every thread
+// runs the reference's running update (Welford) over its particles, then
+// the partial (count, mean, M2) triples are merged pairwise (Chan, Golub and
+// LeVeque: mean += delta*nb/n, M2 += M2b + delta^2*na*nb/n).  Class 0:
+// q < 0, class 1: q > 0.  part: per block 6 doubles (n, mean, M2 per class).
+struct Welford {
+    double n, mean, m2;
+};
+__device__ inline void welford_merge(Welford& a, const Welford& b) {
+    if (b.n == 0.0) return;
+    if (a.n == 0.0) {
+        a = b;
+        return;
+    }
+    const double n = a.n + b.n;
+    const double delta = b.mean - a.mean;
+    a.mean = a.mean + delta * (b.n / n);
+    a.m2 = (a.m2 + b.m2) + delta * delta * (a.n * b.n / n);
+    a.n = n;
+}
+
+__global__ void __launch_bounds__(256)
+k_statistics(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits, double* __restrict__ part) {
+    __shared__ Welford sh[2][256];
+    const GArrays A = gview(S->cur);
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    Welford w[2] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+        const double q = coef[A.cs[i] >> cell_bits].q;
+        if (q == 0.0) continue;
+        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+        double v = 0;
+        v += vx * vx;
+        v += vy * vy;
+        v += vz * vz;
+        v = sqrt(v);
+        Welford& W = w[q < 0 ? 0 : 1];
+        W.n += 1.0;
+        const double old = W.mean;
+        W.mean = old + (v - old) / W.n;
+        W.m2 += (v - old) * (v - W.mean);
+    }
+    for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] = w[k];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 2; ++k) welford_merge(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) {
+        const Welford& r = sh[threadIdx.x][0];
+        part[6 * blockIdx.x + 3 * threadIdx.x] = r.n;
+        part[6 * blockIdx.x + 3 * threadIdx.x + 1] = r.mean;
+        part[6 * blockIdx.x + 3 * threadIdx.x + 2] = r.m2;
+    }
+}
+
+// Merge of the per-block triples (one block of 256): each thread merges a
+// contiguous range of blocks, then a tree over the threads.
+__global__ void __launch_bounds__(256) k_statistics_merge(const double* __restrict__ part, int np,
+                                                          double* __restrict__ out) {
+    __shared__ Welford sh[2][256];
+    const int per = (np + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per, b1 = min(np, b0 + per);
+    for (int k = 0; k < 2; ++k) {
+        Welford a = {0.0, 0.0, 0.0};
+        for (int b = b0; b < b1; ++b) {
+            const Welford w = {part[6 * b + 3 * k], part[6 * b + 3 * k + 1], part[6 * b + 3 * k + 2]};
+            welford_merge(a, w);
+        }
+        sh[k][threadIdx.x] = a;
+    }
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 2; ++k) welford_merge(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) {
+        out[3 * threadIdx.x] = sh[threadIdx.x][0].n;
+        out[3 * threadIdx.x + 1] = sh[threadIdx.x][0].mean;
+        out[3 * threadIdx.x + 2] = sh[threadIdx.x][0].m2;
+    }
+}
+
+// ema (diagnostics.h:573-588): g = w f + (1 - w) g.
+__global__ void k_ema(const double* __restrict__ f, double* __restrict__ g, double w, int64_t n) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        g[j] = w * f[j] + (1.0 - w) * g[j];
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 #define LAUNCH_CHECK() \
@@ -1670,6 +1852,52 @@ __global__ void __launch_bounds__(256) k_copy16(const double2* __restrict__ a, d
 }
 hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (const double2*)a, (double2*)b, n16);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+}  // namespace punc_gpu
+
+namespace punc_gpu {
+hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
+                                   const double* phi, double* part, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_potential_energy, dim3(grid), dim3(256), 0, st, M, S, coef, cell_bits, phi, part);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits, bool dg0,
+                          double* raw, int grid, hipStream_t st) {
+    if (dg0)
+        hipLaunchKernelGGL(k_density<true>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, cell_bits, raw);
+    else
+        hipLaunchKernelGGL(k_density<false>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, cell_bits, raw);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale, double we, double wi,
+                                double* ne, double* ni, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (dg0)
+        hipLaunchKernelGGL(k_density_scale<true>, dim3(fill_grid(n)), dim3(256), 0, st, raw, scale, we, wi, ne, ni, n);
+    else
+        hipLaunchKernelGGL(k_density_scale<false>, dim3(fill_grid(n)), dim3(256), 0, st, raw, scale, we, wi, ne, ni, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, double* part, double* out, int grid,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_statistics, dim3(grid), dim3(256), 0, st, S, coef, cell_bits, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_statistics_merge, dim3(1), dim3(256), 0, st, part, grid, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ema, dim3(fill_grid(n)), dim3(256), 0, st, f, g, w, n);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -527,6 +527,11 @@ void element_volume(const HostMesh& m, const MeshTopo& t, bool voronoi, std::vec
     for (int64_t i = 0; i < m.ndof; ++i) dv[i] = voronoi ? (g_dim + 1.0) / dv[i] : 1.0 / dv[i];
 }
 
+void cell_volumes(const MeshTopo& t, std::vector<double>& vol) {
+    vol.resize((size_t)t.nc);
+    for (int64_t c = 0; c < t.nc; ++c) vol[c] = tet_volume(&t.vcoords[c * 12]);
+}
+
 int64_t locate_host(const MeshTopo& t, const double* x) {
     return locate_in_grid(t.planes.data(), t.bucket_start.data(), t.bucket_cells.data(), t.grid_n,
                           t.grid_lo, t.grid_h, x[0], x[1], x[2]);

@@ -77,6 +77,10 @@ void element_volume(const HostMesh& m, const MeshTopo& t, bool voronoi, std::vec
 // (reference population.cpp:24-39); the two can differ only for points on faces.
 int64_t locate_host(const MeshTopo& t, const double* x);
 
+// Volume of every cell, |det(P2-P1, P3-P1, P4-P1)| / 6 (DOLFIN Cell::volume as
+// used by reference diagnostics.h:461-462).
+void cell_volumes(const MeshTopo& t, std::vector<double>& vol);
+
 // Maxwellian flux helpers (reference distributions.cpp:52-72).
 double maxwellian_flux_num(double vth, const double vd[3], const double n[3], double S);
 double maxwellian_flux_max(double vth, const double vd[3], const double n[3]);

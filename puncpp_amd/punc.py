@@ -294,6 +294,7 @@ class Population:
         self._h = h
         self._inj_set = set()
         self.dv_inv_uploaded = False
+        self._dens_len = None  # length of the last density computed on the device
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -371,6 +372,14 @@ class Population:
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
 
+    def statistics(self, stats=None):
+        """Population::statistics (population.h:746-805): [mean, std] of the
+        speed of the q < 0 particles, then of the q > 0 ones.  Entries of a
+        class without particles keep the values passed in stats."""
+        out = np.zeros(4) if stats is None else np.array(stats, dtype=np.float64)
+        check(N.lib().punc_gpu_statistics(self._h, dptr(out)))
+        return out
+
     def set_launch(self, grid, block=256, deposit_mode=1):
         check(N.lib().punc_gpu_set_launch(self._h, int(grid), int(block), int(deposit_mode)))
 
@@ -442,6 +451,65 @@ def kinetic_energy(pop):
     ke = N.C.c_double()
     check(N.lib().punc_gpu_kinetic_energy(pop._h, N.C.byref(ke)))
     return ke.value
+
+
+def particle_potential_energy_cg1(pop, phi):
+    """diagnostics.h:379-412: sum over the particles of 0.5 q phi(x_p), phi a
+    CG1 function given by its ndof dof values."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    if phi.shape != (pop.mesh.ndof,):
+        raise ValueError(f"phi must hold {pop.mesh.ndof} dof values")
+    pe = N.C.c_double()
+    check(N.lib().punc_gpu_potential_energy(pop._h, dptr(phi), N.C.byref(pe)))
+    return pe.value
+
+
+def density_cg1(V, pop, species, dv_inv=None):
+    """diagnostics.h:492-555: CG1 number densities (ne, ni), each ndof long.
+    V (the function space) is not needed: the dofs are the mesh's.  Weights
+    are species[0].weight and species[1].weight; q < 0 counts as electron."""
+    nd = pop.mesh.ndof
+    ne, ni = np.empty(nd), np.empty(nd)
+    dv = None
+    if dv_inv is not None:
+        dv = np.ascontiguousarray(dv_inv, dtype=np.float64)
+        if dv.shape != (nd,):
+            raise ValueError(f"dv_inv must hold {nd} values")
+    check(N.lib().punc_gpu_density_cg1(pop._h, dptr(dv) if dv is not None else None,
+                                       float(species[0].weight), float(species[1].weight),
+                                       dptr(ne), dptr(ni)))
+    pop._dens_len = nd
+    return ne, ni
+
+
+def density_dg0(Q, pop, species):
+    """diagnostics.h:432-465: DG0 number densities (ne, ni), one value per cell
+    (indexed by cell id).  q > 0 counts as ion."""
+    nc = pop.mesh.num_cells
+    ne, ni = np.empty(nc), np.empty(nc)
+    check(N.lib().punc_gpu_density_dg0(pop._h, float(species[0].weight), float(species[1].weight),
+                                       dptr(ne), dptr(ni)))
+    pop._dens_len = nc
+    return ne, ni
+
+
+def rho_ema(pop, dt, tau, reset=False):
+    """ema(rho, rho_ema, dt, tau) (diagnostics.h:573-588) over the population's
+    last deposited charge density, the average kept on the device; returns it."""
+    g = np.empty(pop.mesh.ndof)
+    check(N.lib().punc_gpu_rho_ema(pop._h, float(dt), float(tau), int(bool(reset)), dptr(g)))
+    return g
+
+
+def density_ema(pop, dt, tau, reset=False):
+    """ema(ne, ne_ema, dt, tau), ema(ni, ni_ema, dt, tau) over the last density
+    computed (density_cg1 or density_dg0); returns (ne_ema, ni_ema)."""
+    n = pop._dens_len
+    if n is None:
+        raise RuntimeError("density_ema: no density computed yet")
+    ne, ni = np.empty(n), np.empty(n)
+    check(N.lib().punc_gpu_density_ema(pop._h, float(dt), float(tau), int(bool(reset)), dptr(ne), dptr(ni)))
+    return ne, ni
 
 
 def _register_injectors(pop, species):

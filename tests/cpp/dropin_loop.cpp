@@ -2,9 +2,11 @@
 // against the drop-in C++ API (include/punc.h): distribute, [host field solve
 // replaced by a fixed field], count, push (half step at n == 0), move,
 // update, inject.  Prints one line per step and checks particle-number
-// bookkeeping: N(n+1) = N(n) - absorbed + injected.
+// bookkeeping: N(n+1) = N(n) - absorbed + injected, and the diagnostics'
+// conservation properties.
 #include <punc.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,7 +28,9 @@ int main(int argc, char** argv) {
     pop.set_seed(42);
     load_particles(pop, species);
     auto dv_inv = element_volume(mesh);
-    Function rho(mesh, 1), E(mesh, 3);
+    Function rho(mesh, 1), E(mesh, 3), phi_one(mesh, 1), ne(mesh, 1), ni(mesh, 1), rho_avg(mesh, 1);
+    Function ne0(mesh, 1, true), ni0(mesh, 1, true);
+    for (double& v : phi_one.values) v = 1.0;
     for (int64_t d = 0; d < mesh.ndof; ++d) E.values[3 * d + 2] = 0.01;
     ObjectVector objects{std::make_shared<Object>(2)};
     std::vector<double> B{0.0, 0.0, 1.0};
@@ -38,6 +42,28 @@ int main(int argc, char** argv) {
         double qsum = 0.0;
         for (int64_t d = 0; d < mesh.ndof; ++d) qsum += rho.values[d] / dv_inv[d];
         size_t num_e = pop.num_of_negatives(), num_i = pop.num_of_positives();
+        // diagnostics: phi = 1 gives PE = sum q / 2; CG1 densities keep
+        // sum_j n_j / dv_inv_j = weight * count; rho_ema after a reset is w rho
+        double PE = particle_potential_energy_cg1(pop, phi_one);
+        density_cg1(mesh, pop, species, ne, ni, dv_inv);
+        density_dg0(mesh, pop, species, ne0, ni0);
+        double ne_sum = 0.0, ni_sum = 0.0, ne0_max = 0.0;
+        for (int64_t d = 0; d < mesh.ndof; ++d) {
+            ne_sum += ne.values[d] / dv_inv[d];
+            ni_sum += ni.values[d] / dv_inv[d];
+        }
+        for (double v : ne0.values) ne0_max = std::max(ne0_max, v);
+        double stats[4] = {0, 0, 0, 0};
+        pop.statistics(stats);
+        rho_ema(pop, rho_avg, dt, 10 * dt, n == 0);
+        const double w = 1.0 - std::exp(-dt / (10 * dt));
+        if (n == 0)
+            for (int64_t d = 0; d < mesh.ndof; ++d)
+                if (std::fabs(rho_avg.values[d] - w * rho.values[d]) > 1e-12 * (1 + std::fabs(rho.values[d]))) bad++;
+        if (std::fabs(PE - 0.5 * ((double)num_i - (double)num_e)) > 1e-8 * (num_e + num_i)) bad++;
+        if (std::fabs(ne_sum - species[0].weight * num_e) > 1e-8 * num_e) bad++;
+        if (std::fabs(ni_sum - species[1].weight * num_i) > 1e-8 * num_i) bad++;
+        if (!(ne0_max > 0) || !(stats[0] > 0) || !(stats[1] > 0) || !(stats[2] > 0) || !(stats[3] > 0)) bad++;
         double KE = boris_cg1(pop, E, B, (1.0 - 0.5 * (n == 0)) * dt);
         if (n == 0) KE = kinetic_energy(pop);
         move(pop, dt);

@@ -335,6 +335,92 @@ def test_deposit_particle_at_vertex():
 
 
 # ---------------------------------------------------------------------------
+# Diagnostics
+# ---------------------------------------------------------------------------
+
+
+def _three_species(hm, n=400):
+    """Electrons, ions and a neutral species (q = 0), which the CG1 density
+    counts as ion and the DG0 density as electron."""
+    o = O.OraclePop(hm, [QE, QI, 0.0], [ME, MI, 1.0])
+    for s, vth in enumerate((1.0, 0.2, 0.5)):
+        o.load(s, n, 3, 0.0, 0.0, 0, vth, [0.0, 0.0, 0.5])
+    return o
+
+
+def test_potential_energy_linear_phi(small):
+    hm, _ = small
+    o = _two_species(hm, n=600)
+    st = o.get_particles()
+    q = np.where(st["species"] == 0, QE, QI)
+    # CG1 interpolation reproduces a linear phi exactly (a wrong dof order or
+    # barycentric row fails this); a constant phi gives c * sum q / 2
+    a, b = np.array([0.7, -1.3, 0.4]), 0.25
+    phi = hm.xyz @ a + b
+    assert o.potential_energy(phi) == pytest.approx(0.5 * np.sum(q * (st["x"] @ a + b)), abs=1e-10)
+    assert o.potential_energy(np.full(hm.ndof, 2.0)) == pytest.approx(q.sum(), abs=1e-10)
+
+
+def test_density_cg1_moments(small):
+    hm, _ = small
+    o = _three_species(hm)
+    st = o.get_particles()
+    dv = o.element_volume()
+    we, wi = 3.0, 0.5
+    ne, ni = o.density_cg1(dv, we, wi)
+    e, i = st["species"] == 0, st["species"] != 0  # q < 0, else (neutral with the ions)
+    # zeroth and first moments per class: sum_j lambda_j = 1, sum_j lambda_j x_j = x_p
+    assert np.sum(ne / dv) == pytest.approx(we * e.sum(), rel=1e-12)
+    assert np.sum(ni / dv) == pytest.approx(wi * i.sum(), rel=1e-12)
+    assert np.allclose((ne / dv) @ hm.xyz, we * st["x"][e].sum(axis=0), atol=1e-9)
+    assert np.allclose((ni / dv) @ hm.xyz, wi * st["x"][i].sum(axis=0), atol=1e-9)
+
+
+def test_density_dg0_brute_force(small):
+    hm, _ = small
+    o = _three_species(hm)
+    st = o.get_particles()
+    we, wi = 3.0, 0.1
+    ne, ni = o.density_dg0(we, wi)
+    vc = o.localizer()["vcoords"].reshape(-1, 4, 3)
+    vol = np.abs(np.linalg.det(vc[:, 1:] - vc[:, :1])) / 6.0
+    ions = st["species"] == 1  # q > 0 only; the neutral species counts as electron
+    cnt_i = np.bincount(st["cell"][ions], minlength=hm.nc)
+    cnt_e = np.bincount(st["cell"][~ions], minlength=hm.nc)
+    np.testing.assert_allclose(ne, we * cnt_e / vol, rtol=1e-13)
+    np.testing.assert_allclose(ni, wi * cnt_i / vol, rtol=1e-13)
+    # the weights are summed particle by particle (0.1 added k times can
+    # differ from k * 0.1 in the last bit): exact against that sum over the
+    # |det| / 6 volume evaluated in plain double arithmetic
+    for c in np.flatnonzero(cnt_i)[:200]:
+        acc = 0.0
+        for _ in range(int(cnt_i[c])):
+            acc += wi
+        v = vc[c]
+        a, b, e = v[1] - v[0], v[2] - v[0], v[3] - v[0]
+        det = (float(a[0]) * (float(b[1]) * float(e[2]) - float(b[2]) * float(e[1]))
+               + float(a[1]) * (float(b[2]) * float(e[0]) - float(b[0]) * float(e[2]))
+               + float(a[2]) * (float(b[0]) * float(e[1]) - float(b[1]) * float(e[0])))
+        assert ni[c] == acc / (abs(det) / 6.0)
+
+
+def test_statistics_against_numpy(small):
+    hm, _ = small
+    o = _three_species(hm)
+    st = o.get_particles()
+    sp = np.linalg.norm(st["v"], axis=1)
+    stats = o.statistics(np.full(4, -7.0))
+    e, i = sp[st["species"] == 0], sp[st["species"] == 1]  # the neutral species is in neither
+    np.testing.assert_allclose(stats, [e.mean(), e.std(ddof=1), i.mean(), i.std(ddof=1)], rtol=1e-12)
+    # a class with no particles leaves its entries untouched; one particle
+    # gives 0/0 for the standard deviation, as the reference's formula does
+    o1 = O.OraclePop(hm, [QE, QI], [ME, MI])
+    o1.add_particles(0, np.array([[0.1, 1.5, -1.2]]), np.array([[3.0, 4.0, 0.0]]))
+    s1 = o1.statistics(np.full(4, -7.0))
+    assert s1[0] == 5.0 and np.isnan(s1[1]) and s1[2] == -7.0 and s1[3] == -7.0
+
+
+# ---------------------------------------------------------------------------
 # Flux tables and injection
 # ---------------------------------------------------------------------------
 
