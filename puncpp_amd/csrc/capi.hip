@@ -152,6 +152,7 @@ struct punc_pop_s {
     double* d_dens_ema = nullptr;  // 2 * max(ndof, nc)
     double* d_rho_ema = nullptr;   // ndof
     double* d_cvol = nullptr;      // nc
+    double4* d_cellphi = nullptr;  // nc: phi at each cell's four dofs
     int64_t dens_len = 0;          // entries per species class of the last density (0: none)
     int64_t dens_ema_len = 0;
     // timing
@@ -408,6 +409,7 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_dens_ema);
     hipFree(P->d_rho_ema);
     hipFree(P->d_cvol);
+    hipFree(P->d_cellphi);
     for (auto& c : P->coef_cache) hipFree(c.d);
     hipFree(P->d_scount);
     hipFree(P->d_soffset);
@@ -1028,6 +1030,7 @@ static int diag_alloc(punc_pop_s* P) {
     std::vector<double> vol;
     cell_volumes(P->mesh->topo, vol);
     HIPCHK(dalloc(&P->d_cvol, nc));
+    HIPCHK(dalloc(&P->d_cellphi, nc));
     HIPCHK(hipMemcpy(P->d_cvol, vol.data(), nc * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(P->d_rho_ema, 0, ndof * 8));
     HIPCHK(hipMemset(P->d_dens_ema, 0, 2 * len * 8));
@@ -1050,7 +1053,8 @@ extern "C" int punc_gpu_potential_energy(punc_pop_t* p, const double* phi, doubl
     if (r) return r;
     const int g = diag_grid(p);
     HIPCHK(hipMemcpyAsync(p->d_phi, phi, p->mesh->topo.ndof * 8, hipMemcpyHostToDevice, p->stream));
-    HIPCHK(launch_potential_energy(p->mesh->dm, p->d_state, coef, p->cell_bits, p->d_phi, p->d_diag, g, p->stream));
+    HIPCHK(launch_potential_energy(p->mesh->dm, p->d_state, coef, p->cell_bits, p->d_phi, p->d_cellphi, p->d_diag, g,
+                                   p->stream));
     HIPCHK(launch_sum_partials(p->d_diag, g, p->d_diag + 6 * (size_t)p->ke_np, p->stream));
     HIPCHK(hipMemcpyAsync(PE, p->d_diag + 6 * (size_t)p->ke_np, 8, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));

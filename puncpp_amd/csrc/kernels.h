@@ -118,13 +118,13 @@ hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
 hipError_t launch_next_step(DevState* S, hipStream_t st);
 hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStream_t st);
 
-// Diagnostics.  potential energy: per-block partial sums into part (grid
-// entries); density: raw sums (2 per dof for CG1, 2 per cell for DG0) into
+// Diagnostics.  potential energy: per-cell copy of phi into cellPhi (nc),
+// then per-block partial sums into part (grid entries); density: raw sums (2 per dof for CG1, 2 per cell for DG0) into
 // raw, which must be zeroed first; density_scale: CG1 n = raw * (dv_inv * w),
 // DG0 n = (raw * w) / volume; statistics: per-block (n, mean, M2) per class
 // into part (6 * grid), merged into out (6).
 hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
-                                   const double* phi, double* part, int grid, hipStream_t st);
+                                   const double* phi, double4* cellPhi, double* part, int grid, hipStream_t st);
 hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits, bool dg0,
                           double* raw, int grid, hipStream_t st);
 hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale, double we, double wi,

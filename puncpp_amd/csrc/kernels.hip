@@ -1374,36 +1374,71 @@ __global__ void k_next_step(DevState* __restrict__ S) {
 // ---------------------------------------------------------------------------
 // Diagnostics (reference diagnostics.h, Population::statistics)
 // ---------------------------------------------------------------------------
+// Per-cell copy of a scalar CG1 function: cellPhi[c] = phi at the cell's
+// four dofs in local vertex order (one 32-byte load per particle instead of a
+// dependent gather through the cell's dof list).
+__global__ void k_cell_scalar(const DevMesh M, const double* __restrict__ phi, double4* __restrict__ cellPhi) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < M.nc; c += (int64_t)gridDim.x * blockDim.x) {
+        const CellRec& R = M.rec[c];
+        cellPhi[c] = make_double4(phi[R.dof[0]], phi[R.dof[1]], phi[R.dof[2]], phi[R.dof[3]]);
+    }
+}
+
 // particle_potential_energy_cg1 (diagnostics.h:379-412): per block the sum of
-// 0.5 q phi(x) over its chunk, phi interpolated with the barycentric
-// coordinates in the reference's order.
+// 0.5 q phi(x) over its chunk, phi_x = sum_k lambda_k phi_k in the
+// reference's order.  Each thread takes DIAG_UNROLL particles per iteration
+// (independent loads in flight; the cell gathers are the latency).
+constexpr int DIAG_UNROLL = 4;
+
 __global__ void __launch_bounds__(256)
 k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
-                   const double* __restrict__ phi, double* __restrict__ part) {
+                   const double4* __restrict__ cellPhi, double* __restrict__ part) {
     __shared__ double sh[8];
     const GArrays A = gview(S->cur);
     int64_t s, e;
     chunk_range(S->n, s, e);
     const int32_t cmask = (1 << cell_bits) - 1;
+    const int64_t bd = blockDim.x;
     double pe = 0.0;
-    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-        const int32_t cs = A.cs[i];
-        const CellRec& R = M.rec[cs & cmask];
-        double l[4];
-        bary4(R.bary, A.x[i], A.y[i], A.z[i], l);
-        double phi_x = 0.0;
-        for (int k = 0; k < 4; ++k) phi_x += l[k] * phi[R.dof[k]];
-        pe += 0.5 * coef[cs >> cell_bits].q * phi_x;
+    for (int64_t b = s + threadIdx.x; b < e; b += DIAG_UNROLL * bd) {
+        int32_t cs[DIAG_UNROLL];
+        double x[DIAG_UNROLL], y[DIAG_UNROLL], z[DIAG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < DIAG_UNROLL; ++u) {
+            const int64_t i = b + u * bd;
+            const bool ok = i < e;
+            cs[u] = ok ? A.cs[i] : -1;
+            x[u] = ok ? A.x[i] : 0.0;
+            y[u] = ok ? A.y[i] : 0.0;
+            z[u] = ok ? A.z[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < DIAG_UNROLL; ++u) {
+            if (cs[u] < 0) continue;
+            const int32_t c = cs[u] & cmask;
+            double l[4];
+            bary4(M.rec[c].bary, x[u], y[u], z[u], l);
+            const double4 f = cellPhi[c];
+            double phi_x = 0.0;
+            phi_x += l[0] * f.x;
+            phi_x += l[1] * f.y;
+            phi_x += l[2] * f.z;
+            phi_x += l[3] * f.w;
+            pe += 0.5 * coef[cs[u] >> cell_bits].q * phi_x;
+        }
     }
     const double t = block_sum(pe, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // Number densities (density_cg1, diagnostics.h:492-555; density_dg0,
-// diagnostics.h:432-465) accumulated in the block's LDS table keyed by
-// 2 * index + class and flushed into raw (2 entries per dof or cell).
-// CG1: class 0 for q < 0, else 1; adds the barycentric coordinates to the
-// cell's four dofs.  DG0: class 1 for q > 0, else 0; adds 1 to the cell.
+// diagnostics.h:432-465) summed into raw with 2 entries per dof or cell:
+// raw[2 * index + class].  CG1: class 0 for q < 0, else 1; the barycentric
+// coordinates go to the cell's four dofs.  DG0: class 1 for q > 0, else 0; one
+// count per particle to its cell.  Per wavefront, runs of consecutive lanes
+// with the same (cell, class) are summed first (cell-sorted particles give
+// long runs); the run's last lane adds the sums into the block's LDS table,
+// flushed to raw with one global atomic per occupied slot.
 template <bool DG0>
 __global__ void __launch_bounds__(256)
 k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
@@ -1416,21 +1451,38 @@ k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ 
     const LdsTable t = lds_table();
     lds_clear(t);
     __syncthreads();
+    const int lane = threadIdx.x & 63;
     for (int64_t b = s; b < e; b += blockDim.x) {  // block-uniform trip count
         const int64_t i = b + threadIdx.x;
+        int32_t key = -1;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        int32_t dof[4] = {0, 0, 0, 0};
         if (i < e) {
             const int32_t cs = A.cs[i];
             const int32_t c = cs & cmask;
             const double q = coef[cs >> cell_bits].q;
             if (DG0) {
-                lds_add(t, raw, 2 * c + (q > 0 ? 1 : 0), 1.0);
+                key = 2 * c + (q > 0 ? 1 : 0);
             } else {
-                const CellRec& R = M.rec[c];
-                double l[4];
-                bary4(R.bary, A.x[i], A.y[i], A.z[i], l);
                 const int cls = q < 0 ? 0 : 1;
-                for (int k = 0; k < 4; ++k) lds_add(t, raw, 2 * R.dof[k] + cls, l[k]);
+                key = 2 * c + cls;
+                const CellRec& R = M.rec[c];
+                bary4(R.bary, A.x[i], A.y[i], A.z[i], d);
+                for (int k = 0; k < 4; ++k) dof[k] = 2 * R.dof[k] + cls;
             }
+        }
+        const int32_t kprev = __shfl_up(key, 1, 64);
+        const bool head = lane == 0 || kprev != key;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+        const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
+        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+        if (DG0) {
+            if (key >= 0 && tail) lds_add(t, raw, key, (double)(lane - hl + 1));
+        } else {
+            run_scan4(d, lane, hl);
+            if (key >= 0 && tail)
+                for (int k = 0; k < 4; ++k) lds_add(t, raw, dof[k], d[k]);
         }
         // one decision for the whole block (the count read is a heuristic)
         if (__syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8)) lds_flush(t, raw);
@@ -1859,8 +1911,10 @@ hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStrea
 
 namespace punc_gpu {
 hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
-                                   const double* phi, double* part, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_potential_energy, dim3(grid), dim3(256), 0, st, M, S, coef, cell_bits, phi, part);
+                                   const double* phi, double4* cellPhi, double* part, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_cell_scalar, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, phi, cellPhi);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_potential_energy, dim3(grid), dim3(256), 0, st, M, S, coef, cell_bits, cellPhi, part);
     LAUNCH_CHECK();
     return hipSuccess;
 }
