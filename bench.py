@@ -164,6 +164,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
     xfer = {"field_in_ms": 1e3 * (t_mid - t_in) / 5, "rho_out_ms": 1e3 * (t_out - t_mid) / 5,
             "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
     diag = run_diagnostics(pop, mesh, dv)
+    flux_mc = run_flux_mc(mesh)
     elapsed = t2 - t1
     n_avg = 0.5 * (n_start + n_end)
     value = n_avg * steps / elapsed
@@ -176,8 +177,43 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
         "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
-        "diag": diag,
+        "diag": diag, "flux_mc": flux_mc,
     }
+
+
+# create_flux workload for the non-Maxwellian distributions: KappaCairns
+# (kind 3, vth 1, drift (0.2, 0, 0.5), kappa 4, alpha 0.3, range 15) has no
+# closed form for either table, so both come from the Monte Carlo.
+KC_FLUX = (3, 1.0, [0.2, 0.0, 0.5], 4.0, 0.3, 15.0)
+
+
+def P_box_mesh(n):
+    import puncpp_amd as P
+    return P.box_mesh(n, [0.0] * 3, [1.0] * 3)
+
+
+def run_flux_mc(mesh, n_iter=500000):
+    """Device create_flux Monte Carlo over every exterior facet of the mesh,
+    the reference's n_iter samples per facet and table (2 tables)."""
+    from puncpp_amd import _native as N
+    lib = N.lib()
+    ne = mesh.num_exterior
+    num, pmax = np.empty(ne), np.empty(ne)
+    kind, vth, vd, k, a, rr = KC_FLUX
+
+    def go(n):
+        N.check(lib.punc_gpu_vdf_flux(mesh._h, kind, vth, N.dptr(np.array(vd)), k, a, rr, 0, 0, n, N.dptr(num),
+                                      N.dptr(pmax)))
+
+    go(1000)
+    t = time.perf_counter()
+    go(n_iter)
+    ms = 1e3 * (time.perf_counter() - t)
+    samples = 2.0 * ne * n_iter
+    log(f"[bench] flux Monte Carlo: {ne} facets x 2 x {n_iter} samples in {ms:.1f} ms "
+        f"({samples / ms / 1e6:.3g} G samples/s)")
+    return {"facets": ne, "samples": samples, "ms": ms, "samples_per_s": samples / (ms * 1e-3),
+            "distribution": "KappaCairns kappa=4 alpha=0.3 drift (0.2,0,0.5) range 15"}
 
 
 def run_diagnostics(pop, mesh, dv, reps=5):
@@ -248,7 +284,17 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     t2 = time.perf_counter()
     v = ps / (t2 - t1)
     log(f"[bench] oracle {ps} particle-steps in {t2 - t1:.2f}s -> {v:.4g}/s")
+    # create_flux Monte Carlo (KappaCairns: both tables sampled) on a sample
+    # of facets: a 2^3-cube box has 48 exterior facets
+    ob = O.OraclePop(P_box_mesh(2), [-1.0], [1.0])
+    nf = 48
+    t3 = time.perf_counter()
+    ob.vdf_flux(*KC_FLUX[:6], seed=0, rng_sp=0, n_iter=100000)
+    t4 = time.perf_counter()
+    flux_rate = nf * 2 * 100000 / (t4 - t3)
+    log(f"[bench] oracle flux Monte Carlo {flux_rate:.4g} samples/s")
     return {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "flux_mc_samples_per_s": flux_rate,
             "sample": f"oracle (serial C++ restatement, g++ -O2) on the same mesh with "
                       f"{2 * sample_per_species} particles, {steps} steps of boris+move+update+inject+distribute; "
                       f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cores visible"}
@@ -449,6 +495,7 @@ def main():
         "cpu_baseline": cpu,
         "host_transfer": res["xfer"],
         "diagnostics": res["diag"],
+        "flux_monte_carlo": res["flux_mc"],
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -6,31 +6,48 @@
 
 #include <punc/population.h>
 
+#include <stdexcept>
 #include <vector>
 
 namespace punc {
 
-//! create_flux (injector.cpp:104-171) for Maxwellian species.
-inline void create_flux(const Mesh& mesh, std::vector<Species>& species) {
+//! The device description of a species' velocity distribution (any Pdf
+//! other than the library's velocity pdfs is rejected).
+inline const VelocityPdf& velocity_pdf(const Species& s) {
+    auto* v = dynamic_cast<const VelocityPdf*>(s.vdf.get());
+    if (!v) throw std::runtime_error("the GPU samplers take Maxwellian, Kappa, Cairns or KappaCairns velocity pdfs");
+    return *v;
+}
+
+//! create_flux (injector.cpp:104-171): closed forms where the distribution
+//! has them, the device Monte Carlo (500000 samples per facet, Philox stream
+//! keyed by `seed`; species i uses substream i) otherwise.
+inline void create_flux(const Mesh& mesh, std::vector<Species>& species, uint64_t seed = 0,
+                        int64_t n_iter = 500000) {
     const size_t nf = mesh.exterior_facets.size();
-    for (auto& s : species) {
+    for (size_t i = 0; i < species.size(); ++i) {
+        auto& s = species[i];
+        const VelocityPdf& v = velocity_pdf(s);
         s.vdf->num_particles.assign(nf, 0.0);
         s.vdf->pdf_max.assign(nf, 0.0);
         std::vector<double> vd = s.vdf->vd();
-        check(punc_gpu_maxwellian_flux(mesh.handle(), s.vdf->vth(), vd.data(), s.vdf->num_particles.data(),
-                                       s.vdf->pdf_max.data()));
+        check(punc_gpu_vdf_flux(mesh.handle(), v.kind, s.vdf->vth(), vd.data(), v.kappa, v.alpha, v.vdf_range, seed,
+                                (int)i, n_iter, s.vdf->num_particles.data(), s.vdf->pdf_max.data()));
     }
 }
 
-//! inject_particles (injector.h:157-232).
+//! inject_particles (injector.h:157-232) with the create_flux tables.
 template <typename PopulationType>
 void inject_particles(PopulationType& pop, std::vector<Species>& species,
                       std::vector<ExteriorFacet>& facets, double dt) {
-    (void)facets;  // the population's mesh holds the exterior facets
     for (auto& s : species) {
         int idx = pop.species_index(s.q, s.m);
+        const VelocityPdf& v = velocity_pdf(s);
         std::vector<double> vd = s.vdf->vd();
-        check(punc_gpu_injector_set(pop.handle(), idx, s.vdf->vth(), vd.data(), s.n, s.vdf->vdf_range));
+        const bool tables = s.vdf->num_particles.size() == facets.size() && s.vdf->pdf_max.size() == facets.size();
+        check(punc_gpu_injector_set_vdf(pop.handle(), idx, v.kind, s.vdf->vth(), vd.data(), v.kappa, v.alpha, s.n,
+                                        v.vdf_range, tables ? s.vdf->num_particles.data() : nullptr,
+                                        tables ? s.vdf->pdf_max.data() : nullptr));
     }
     int64_t k = 0;
     check(punc_gpu_inject(pop.handle(), dt, pop.seed(), pop.next_injection_step(), &k));
@@ -41,9 +58,10 @@ template <typename PopulationType>
 void load_particles(PopulationType& pop, std::vector<Species>& species) {
     for (auto& s : species) {
         int idx = pop.species_index(s.q, s.m);
+        const VelocityPdf& v = velocity_pdf(s);
         std::vector<double> vd = s.vdf->vd();
-        check(punc_gpu_pop_load(pop.handle(), idx, (int64_t)s.num, pop.seed(), s.pdf->amp, s.pdf->kw,
-                                s.pdf->axis, s.vdf->vth(), vd.data()));
+        check(punc_gpu_pop_load_vdf(pop.handle(), idx, (int64_t)s.num, pop.seed(), s.pdf->amp, s.pdf->kw, s.pdf->axis,
+                                    v.kind, s.vdf->vth(), vd.data(), v.kappa, v.alpha, v.vdf_range));
     }
 }
 

@@ -87,6 +87,27 @@ int punc_gpu_mesh_locate(const punc_mesh_t* m, int64_t n, const double* xs, int6
 int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const double vd[3], double* num,
                              double* pdf_max);
 
+/* Velocity distributions (distributions.h:233-360, distributions.cpp:23-245).
+ * kind: 0 Maxwellian, 1 Kappa (kappa > 3/2), 2 Cairns (alpha), 3 KappaCairns
+ * (kappa, alpha); vdf_range: half width of the sampling box in units of vth
+ * (the reference's defaults: 5, 7, 7, 15). */
+#define PUNC_VDF_MAXWELLIAN 0
+#define PUNC_VDF_KAPPA 1
+#define PUNC_VDF_CAIRNS 2
+#define PUNC_VDF_KAPPA_CAIRNS 3
+/* The constructor's normalisation factor, Pdf::max(), and the create_flux
+ * flags has_flux_number / has_flux_max (any output may be NULL). */
+int punc_gpu_vdf_info(int kind, double vth, const double vd[3], double kappa, double alpha, double vdf_range,
+                      double* factor, double* pdf_max, int* has_flux_num, int* has_flux_max);
+/* create_flux (injector.cpp:104-171) for one species over the exterior
+ * facets: closed forms where the distribution has them, otherwise n_iter
+ * (the reference: 500000) Monte Carlo samples per facet on the device from a
+ * Philox stream keyed by seed, with rng_species naming the species' stream.
+ * num[j], pdf_max[j]: per exterior facet. */
+int punc_gpu_vdf_flux(punc_mesh_t* m, int kind, double vth, const double vd[3], double kappa, double alpha,
+                      double vdf_range, uint64_t seed, int rng_species, int64_t n_iter, double* num,
+                      double* pdf_max);
+
 /* ---- population -------------------------------------------------------- */
 /* A population of nspecies species with weighted charge q[s] and mass m[s]
  * (Species::q, Species::m, population.cpp:61-64), room for capacity particles.
@@ -113,6 +134,11 @@ int punc_gpu_pop_add_located(punc_pop_t* p, int64_t n, const double* xs, const d
  * (amp = 0: UniformPosition) and Maxwellian velocities by inverse CDF. */
 int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp, double kw,
                       int axis, double vth, const double vd[3]);
+/* load_particles with any velocity distribution: Maxwellian velocities by
+ * inverse CDF (as punc_gpu_pop_load), the others by rejection against
+ * Pdf::max() in the box vd +- vdf_range*vth (injector.cpp:25-49). */
+int punc_gpu_pop_load_vdf(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp, double kw, int axis,
+                          int kind, double vth, const double vd[3], double kappa, double alpha, double vdf_range);
 int punc_gpu_pop_size(punc_pop_t* p, int64_t* n);
 /* Particles per species (num_of_particles/positives/negatives derive from it,
  * population.h:701-744). */
@@ -180,6 +206,14 @@ int punc_gpu_sort(punc_pop_t* p);
  * (Species::n) and rejection box half width in units of vth (Pdf::vdf_range). */
 int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
                           double density, double vrange);
+/* Flux source with any velocity distribution (kind, kappa, alpha as for
+ * punc_gpu_vdf_info): num / pdf_max are the species' create_flux tables (one
+ * entry per exterior facet, e.g. from punc_gpu_vdf_flux), or both NULL to
+ * compute them with seed 0 and n_iter 500000.  Registering the same species
+ * with unchanged parameters and tables keeps the built injection tables. */
+int punc_gpu_injector_set_vdf(punc_pop_t* p, int species, int kind, double vth, const double vd[3], double kappa,
+                              double alpha, double density, double vdf_range, const double* num,
+                              const double* pdf_max);
 /* Batched instances.  Binds species s to one instance of a concatenated mesh:
  * its particles live in cells [cell_begin, cell_end) (loading and host
  * location stay there), it injects through exterior facets [facet_begin,

@@ -52,6 +52,14 @@ _SIGS = {
     "oracle_statistics": (None, [vp, d_p]),
     "oracle_flux": (None, [C.c_double, d_p, d_p, C.c_double, d_p, d_p]),
     "oracle_set_injector": (None, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
+    "oracle_set_injector_vdf": (None, [vp, C.c_int, C.c_int, C.c_double, d_p, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, d_p, d_p]),
+    "oracle_vdf_flux": (None, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                               C.c_int, C.c_int64, d_p, d_p]),
+    "oracle_vdf_value": (C.c_double, [C.c_int, C.c_double, d_p, C.c_double, C.c_double, d_p]),
+    "oracle_vdf_max": (C.c_double, [C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.POINTER(C.c_int)]),
+    "oracle_load_vdf": (C.c_int64, [vp, C.c_int, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    C.c_double, d_p, C.c_double, C.c_double, C.c_double]),
     "oracle_inject": (C.c_int64, [vp, C.c_double, C.c_uint64, C.c_uint64]),
     "oracle_load": (C.c_int64, [vp, C.c_int, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_int,
                                 C.c_double, d_p]),
@@ -90,6 +98,22 @@ def unif(x):
 
 def erfc_inv(y):
     return lib().oracle_erfc_inv(float(y))
+
+
+def vdf_value(kind, vth, vd, v, k=0.0, alpha=0.0):
+    """Pdf::operator()(v) of a Maxwellian (0), Kappa (1), Cairns (2) or KappaCairns (3)."""
+    vd = np.ascontiguousarray(vd, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().oracle_vdf_value(int(kind), float(vth), vd.ctypes.data_as(d_p), float(k), float(alpha),
+                                  v.ctypes.data_as(d_p))
+
+
+def vdf_max(kind, vth, vd, k=0.0, alpha=0.0):
+    """(Pdf::max(), has_flux_number, has_flux_max)."""
+    vd = np.ascontiguousarray(vd, dtype=np.float64)
+    flags = (C.c_int * 2)()
+    m = lib().oracle_vdf_max(int(kind), float(vth), vd.ctypes.data_as(d_p), float(k), float(alpha), flags)
+    return m, bool(flags[0]), bool(flags[1])
 
 
 def flux(vth, vd, normal, area):
@@ -241,6 +265,29 @@ class OraclePop:
         vd = np.ascontiguousarray(vd, dtype=np.float64)
         lib().oracle_set_injector(self._h, int(species), float(vth), vd.ctypes.data_as(d_p), float(n),
                                   float(vrange))
+
+    def set_injector_vdf(self, species, kind, vth, vd, k, alpha, n, vrange, num, pmax):
+        vd = np.ascontiguousarray(vd, dtype=np.float64)
+        num = np.ascontiguousarray(num, dtype=np.float64)
+        pmax = np.ascontiguousarray(pmax, dtype=np.float64)
+        lib().oracle_set_injector_vdf(self._h, int(species), int(kind), float(vth), vd.ctypes.data_as(d_p),
+                                      float(k), float(alpha), float(n), float(vrange), num.ctypes.data_as(d_p),
+                                      pmax.ctypes.data_as(d_p))
+
+    def vdf_flux(self, kind, vth, vd, k=0.0, alpha=0.0, vrange=5.0, seed=0, rng_sp=0, n_iter=500000):
+        """create_flux tables (num, pdf_max) per exterior facet."""
+        vd = np.ascontiguousarray(vd, dtype=np.float64)
+        nf = lib().oracle_num_exterior(self._h)
+        num, pmax = np.empty(nf), np.empty(nf)
+        lib().oracle_vdf_flux(self._h, int(kind), float(vth), vd.ctypes.data_as(d_p), float(k), float(alpha),
+                              float(vrange), int(seed), int(rng_sp), int(n_iter), _d(num), _d(pmax))
+        return num, pmax
+
+    def load_vdf(self, species, N, seed, amp, kw, axis, kind, vth, vd, k=0.0, alpha=0.0, vrange=5.0):
+        vd = np.ascontiguousarray(vd, dtype=np.float64)
+        return int(lib().oracle_load_vdf(self._h, int(species), int(N), int(seed), float(amp), float(kw), int(axis),
+                                         int(kind), float(vth), vd.ctypes.data_as(d_p), float(k), float(alpha),
+                                         float(vrange)))
 
     def inject(self, dt, seed, step):
         return int(lib().oracle_inject(self._h, float(dt), int(seed), int(step)))

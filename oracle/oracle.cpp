@@ -29,6 +29,8 @@
 //   rejection_sampler, random_facet_points, create_flux  punc/src/injector.cpp:25-171
 //   load_particles                     injector.h:244-278
 //   Maxwellian pdf/icdf/flux_num_particles/flux_max  punc/src/distributions.cpp:23-77
+//   Kappa, Cairns, KappaCairns (pdf, max, flux number/max, flags)  distributions.cpp:79-245
+//   create_flux Monte Carlo branches                  injector.cpp:127-168
 //   Pdf flux operator, update_domain   punc/include/punc/distributions.h:93-102, 139-146
 //   Mesh::exterior_boundaries          punc/src/mesh.cpp:166-236
 //
@@ -98,6 +100,9 @@ struct Species {
     double q, m;
     bool inject = false;
     double vth = 0, vd[3] = {0, 0, 0}, n = 0, vrange = 5.0;
+    // velocity distribution: 0 Maxwellian, 1 Kappa, 2 Cairns, 3 KappaCairns
+    int kind = 0;
+    double k = 0, alpha = 0;
     std::vector<double> num_particles, pdf_max;  // per exterior facet
 };
 
@@ -144,6 +149,7 @@ double unif(uint64_t x) { return (double)((x >> 12) * 2 + 1) / 9007199254740992.
 constexpr uint64_t TAG_INJECT_COUNT = 0x494e4a43ull;
 constexpr uint64_t TAG_INJECT = 0x494e4a50ull;
 constexpr uint64_t TAG_LOAD = 0x4c4f4144ull;
+constexpr uint64_t TAG_FLUX = 0x464c5558ull;
 
 // erfc^-1 by Newton steps inside a shrinking bracket.  Near y = 1 the
 // equation is solved as erf(x) = 1 - y (the subtraction is exact there), so
@@ -725,19 +731,94 @@ double oracle_kinetic_energy(void* h) {
     return KE;
 }
 
-// Maxwellian flux tables (distributions.cpp:52-72, distributions.h:93-102).
-static double maxwellian(const Species& s, const double* v) {
-    double vth2 = s.vth * s.vth;
-    double factor = (1.0 / (pow(sqrt(2. * M_PI * vth2), 3)));
-    double v_sqrt = 0.0;
-    for (int i = 0; i < 3; ++i) v_sqrt += (v[i] - s.vd[i]) * (v[i] - s.vd[i]);
-    return factor * exp(-0.5 * v_sqrt / vth2);
+// Velocity distributions (distributions.cpp:23-245): each constructor's
+// normalisation, recomputed per evaluation as a fresh object would hold it.
+static double vdf_factor(const Species& s) {
+    const int dim = 3;
+    const double vth2 = s.vth * s.vth, k = s.k, alpha = s.alpha;
+    switch (s.kind) {
+        case 1:  // Kappa::Kappa, distributions.cpp:96-97
+            return (1.0 / pow(sqrt(M_PI * (2 * k - dim) * vth2), dim)) *
+                   (tgamma(k + 1.0) / tgamma(k + ((2.0 - dim) / 2.0)));
+        case 2:  // Cairns::Cairns, distributions.cpp:142
+            return (1.0 / (pow(sqrt(2 * M_PI * vth2), dim) * (1 + dim * (dim + 2) * alpha)));
+        case 3:  // KappaCairns::KappaCairns, distributions.cpp:196-198
+            return (1.0 / pow(sqrt(M_PI * (2. * k - dim) * vth2), dim)) *
+                   (1.0 / (1.0 + dim * (dim + 2.0) * alpha * ((k - dim / 2.0) / (k - (dim + 2) / 2.0)))) *
+                   (tgamma(k + 1.0) / tgamma(k + ((2.0 - dim) / 2.0)));
+        default:  // Maxwellian::Maxwellian, distributions.cpp:28
+            return (1.0 / (pow(sqrt(2. * M_PI * vth2), 3)));
+    }
 }
+
+// Pdf::operator()(v) of each distribution.
+static double vdf_value(const Species& s, const double* v) {
+    const int dim = 3;
+    const double vth2 = s.vth * s.vth, vth4 = vth2 * vth2, k = s.k, alpha = s.alpha;
+    const double factor = vdf_factor(s);
+    double v2 = 0.0;
+    for (int i = 0; i < 3; ++i) v2 += (v[i] - s.vd[i]) * (v[i] - s.vd[i]);
+    switch (s.kind) {
+        case 1:  // distributions.cpp:99-107
+            return factor * pow(1.0 + v2 / ((2 * k - dim) * vth2), -(k + 1.0));
+        case 2: {  // distributions.cpp:145-154
+            double v4 = v2 * v2;
+            return factor * (1 + alpha * v4 / vth4) * exp(-0.5 * v2 / vth2);
+        }
+        case 3: {  // distributions.cpp:201-211
+            double v4 = v2 * v2;
+            return factor * (1.0 + alpha * v4 / vth4) * pow(1.0 + v2 / ((2 * k - dim) * vth2), -(k + 1.0));
+        }
+        default:  // distributions.cpp:31-39
+            return factor * exp(-0.5 * v2 / vth2);
+    }
+}
+
+// Flux pdf (v.n)^+ f(v) (distributions.h:93-102).
 static double flux_pdf(const Species& s, const double* x, const double* n) {
     double vn = 0.0;
     for (int i = 0; i < 3; ++i) vn += x[i] * n[i];
-    return (vn > 0.0) * vn * maxwellian(s, x);
+    return (vn > 0.0) * vn * vdf_value(s, x);
 }
+
+// Pdf::max() (distributions.h:244, 278; distributions.cpp:156-173, 213-230).
+static double vdf_pdf_max(const Species& s) {
+    const int dim = 3;
+    const double k = s.k, alpha = s.alpha, factor = vdf_factor(s);
+    double v_max[3] = {s.vd[0], s.vd[1], s.vd[2]};
+    if (s.kind == 2) {
+        if (alpha < 0.25) return factor;
+        v_max[0] += s.vth * sqrt(2.0 + sqrt(4.0 - 1.0 / alpha));
+        double max = vdf_value(s, v_max);
+        max = factor > max ? factor : max;
+        return max;
+    }
+    if (s.kind == 3) {
+        double max;
+        if (alpha >= (k - 1.) * (k + 1.) / ((2. * k - dim) * (2. * k - dim))) {
+            v_max[0] += s.vth * sqrt(((2.0 * k - dim) / (k - 1.0)) +
+                                     sqrt(alpha * alpha * (2 * k - dim) * (2 * k - dim) - alpha * (k - 1.0) * (k + 1.0)) /
+                                         (alpha * (k - 1.0)));
+            max = vdf_value(s, v_max);
+        } else {
+            max = factor;
+        }
+        return max;
+    }
+    return factor;
+}
+
+// has_flux_number / has_flux_max as the constructors set them
+// (distributions.cpp:23-26, 80-94, 137-140, 190-194).
+static bool vdf_flux_number_flag(const Species& s) {
+    if (s.kind == 1) {
+        int sum_vd = 0;  // std::accumulate(_vd.begin(), _vd.end(), 0): an int accumulator
+        for (int i = 0; i < 3; ++i) sum_vd = sum_vd + s.vd[i];
+        return sum_vd == 0;
+    }
+    return s.kind != 3;
+}
+static bool vdf_flux_max_flag(const Species& s) { return s.kind == 0 || s.kind == 1; }
 
 void oracle_flux(double vth, const double* vd, const double* n, double S, double* num, double* pmax) {
     Species s;
@@ -753,11 +834,134 @@ void oracle_flux(double vth, const double* vd, const double* n, double S, double
     *pmax = flux_pdf(s, tmp, n);
 }
 
+// Closed-form flux number and flux-pdf maximum of the non-Maxwellian
+// distributions (Kappa: distributions.cpp:109-128; Cairns: 175-185).
+static double vdf_flux_num_closed(const Species& s, const double* n, double S) {
+    const int dim = 3;
+    const double _vth = s.vth, k = s.k, alpha = s.alpha;
+    if (s.kind == 1)
+        return S * ((_vth / (sqrt(2 * M_PI))) * (1.0 / sqrt(k - dim / 2.0)) *
+                    (tgamma(k - ((dim - 1.0) / 2.0)) / tgamma(k - dim / 2.0)));
+    double vdn = 0.0;
+    for (int i = 0; i < 3; ++i) vdn = vdn + n[i] * s.vd[i];
+    if (s.kind == 2)
+        return S * ((_vth / (sqrt(2 * M_PI))) * exp(-0.5 * (vdn / _vth) * (vdn / _vth)) *
+                        (1. + (dim + 1.) * (dim + 3.) * alpha + (vdn / _vth) * (vdn / _vth) * alpha) /
+                        (1. + dim * (dim + 2.) * alpha) +
+                    0.5 * vdn * (1. + erf(vdn / (sqrt(2) * _vth))));
+    double num, pmax;
+    oracle_flux(s.vth, s.vd, n, S, &num, &pmax);
+    return num;
+}
+static double vdf_flux_max_closed(const Species& s, const double* n) {
+    if (s.kind == 1) {
+        const int dim = 3;
+        double tmp[3];
+        for (int i = 0; i < 3; ++i) tmp[i] = sqrt((2. * s.k - dim) / (2. * s.k + 1.0)) * n[i] * s.vth;
+        return flux_pdf(s, tmp, n);
+    }
+    double num, pmax;
+    oracle_flux(s.vth, s.vd, n, 1.0, &num, &pmax);
+    return pmax;
+}
+
+// create_flux (injector.cpp:104-171) for one distribution over the exterior
+// facets, every branch: closed forms where the flags say so, else n_iter
+// uniform samples in the domain box per facet (Philox (seed, FLUX), counters
+// (rng_sp, facet, sample, 0) for the flux number, (..., 1) for the maximum).
+void oracle_vdf_flux(void* h, int kind, double vth, const double* vd, double k, double alpha, double vrange,
+                     uint64_t seed, int rng_sp, int64_t n_iter, double* num_out, double* max_out) {
+    Pop& P = *(Pop*)h;
+    Species s;
+    s.kind = kind;
+    s.vth = vth;
+    for (int i = 0; i < 3; ++i) s.vd[i] = vd[i];
+    s.k = k;
+    s.alpha = alpha;
+    const int dim = 3;
+    double domain[6];
+    for (int i = 0; i < dim; ++i) {
+        domain[i] = vd[i] - vrange * vth;
+        domain[i + dim] = vd[i] + vrange * vth;
+    }
+    double volume = 1.0;
+    for (int kk = 0; kk < dim; kk++) volume *= domain[kk + dim] - domain[kk];
+    const bool cn = vdf_flux_number_flag(s), cm = vdf_flux_max_flag(s);
+    for (size_t j = 0; j < P.ext.size(); ++j) {
+        const ExtFacet& f = P.ext[j];
+        if (cn) {
+            num_out[j] = vdf_flux_num_closed(s, f.normal, f.area);
+        } else {
+            double sum = 0;
+            double x[3];
+            for (int64_t n = 0; n < n_iter; ++n) {
+                auto r = philox({(uint64_t)rng_sp, (uint64_t)j, (uint64_t)n, 0}, seed, TAG_FLUX);
+                for (int kk = 0; kk < dim; ++kk) x[kk] = domain[kk] + unif(r[kk]) * (domain[kk + dim] - domain[kk]);
+                sum += flux_pdf(s, x, f.normal);
+            }
+            num_out[j] = f.area * sum * volume / n_iter;
+        }
+        if (cm) {
+            max_out[j] = vdf_flux_max_closed(s, f.normal);
+        } else {
+            double max = 0, pdf_x;
+            double x[3];
+            for (int64_t n = 0; n < n_iter; ++n) {
+                auto r = philox({(uint64_t)rng_sp, (uint64_t)j, (uint64_t)n, 1}, seed, TAG_FLUX);
+                for (int kk = 0; kk < dim; ++kk) x[kk] = domain[kk] + unif(r[kk]) * (domain[kk + dim] - domain[kk]);
+                pdf_x = flux_pdf(s, x, f.normal);
+                max = max > pdf_x ? max : pdf_x;
+            }
+            max_out[j] = max * 1.01;
+        }
+    }
+}
+
+// Pdf value, Pdf::max() and the flags, for the pins.
+double oracle_vdf_value(int kind, double vth, const double* vd, double k, double alpha, const double* v) {
+    Species s;
+    s.kind = kind;
+    s.vth = vth;
+    for (int i = 0; i < 3; ++i) s.vd[i] = vd[i];
+    s.k = k;
+    s.alpha = alpha;
+    return vdf_value(s, v);
+}
+double oracle_vdf_max(int kind, double vth, const double* vd, double k, double alpha, int* flags) {
+    Species s;
+    s.kind = kind;
+    s.vth = vth;
+    for (int i = 0; i < 3; ++i) s.vd[i] = vd[i];
+    s.k = k;
+    s.alpha = alpha;
+    flags[0] = vdf_flux_number_flag(s) ? 1 : 0;
+    flags[1] = vdf_flux_max_flag(s) ? 1 : 0;
+    return vdf_pdf_max(s);
+}
+
+// Species injection source with given create_flux tables.
+void oracle_set_injector_vdf(void* h, int sp, int kind, double vth, const double* vd, double k, double alpha,
+                             double n, double vrange, const double* num, const double* pmax) {
+    Pop& P = *(Pop*)h;
+    Species& s = P.species[sp];
+    s.inject = true;
+    s.kind = kind;
+    s.vth = vth;
+    for (int i = 0; i < 3; ++i) s.vd[i] = vd[i];
+    s.k = k;
+    s.alpha = alpha;
+    s.n = n;
+    s.vrange = vrange;
+    s.num_particles.assign(num, num + P.ext.size());
+    s.pdf_max.assign(pmax, pmax + P.ext.size());
+}
+
 // Species injection source (create_flux, injector.cpp:104-171, Maxwellian branch).
 void oracle_set_injector(void* h, int sp, double vth, const double* vd, double n, double vrange) {
     Pop& P = *(Pop*)h;
     Species& s = P.species[sp];
     s.inject = true;
+    s.kind = 0;
     s.vth = vth;
     for (int i = 0; i < 3; ++i) s.vd[i] = vd[i];
     s.n = n;
@@ -839,8 +1043,31 @@ int64_t oracle_inject(void* h, double dt, uint64_t seed, uint64_t step) {
 // velocities vd - sqrt(2)*vth*erfc_inv(2u).  Slot k of species sp uses
 // counters (k, attempt, sp, 0) and (k, 2^32-1, sp, 0), key (seed, LOAD).
 // Returns the number of particles that could not be placed.
+int64_t oracle_load_vdf(void* h, int sp, int64_t N, uint64_t seed, double amp, double kw, int axis, int kind,
+                        double vth, const double* vd, double k, double alpha, double vrange);
+
 int64_t oracle_load(void* h, int sp, int64_t N, uint64_t seed, double amp, double kw, int axis,
                     double vth, const double* vd) {
+    return oracle_load_vdf(h, sp, N, seed, amp, kw, axis, 0, vth, vd, 0.0, 0.0, 5.0);
+}
+
+// load_particles with any velocity distribution: Maxwellian velocities by the
+// inverse CDF, the others by the rejection sampler against Pdf::max() in the
+// box vd +- vrange*vth (injector.h:259-272), attempts (k, a, sp, 1).
+int64_t oracle_load_vdf(void* h, int sp, int64_t N, uint64_t seed, double amp, double kw, int axis, int kind,
+                        double vth, const double* vd, double k, double alpha, double vrange) {
+    Species vs_pdf;
+    vs_pdf.kind = kind;
+    vs_pdf.vth = vth;
+    for (int i = 0; i < 3; ++i) vs_pdf.vd[i] = vd[i];
+    vs_pdf.k = k;
+    vs_pdf.alpha = alpha;
+    const double vmax = vdf_pdf_max(vs_pdf);
+    double vdomain[6];
+    for (int i = 0; i < 3; ++i) {
+        vdomain[i] = vd[i] - vrange * vth;
+        vdomain[i + 3] = vd[i] + vrange * vth;
+    }
     Pop& P = *(Pop*)h;
     int64_t failed = 0;
     double domain[6];
@@ -849,12 +1076,12 @@ int64_t oracle_load(void* h, int sp, int64_t N, uint64_t seed, double amp, doubl
         domain[i + 3] = P.cmax;
     }
     const double pdf_max = 1.0 + amp;
-    for (int64_t k = 0; k < N; ++k) {
+    for (int64_t kk = 0; kk < N; ++kk) {
         double tmp[3];
         int64_t c = -1;
         bool ok = false;
         for (uint64_t a = 0; a < 1000000ull && !ok; ++a) {
-            auto r = philox({(uint64_t)k, a, (uint64_t)sp, 0}, seed, TAG_LOAD);
+            auto r = philox({(uint64_t)kk, a, (uint64_t)sp, 0}, seed, TAG_LOAD);
             for (int i = 0; i < 3; ++i) tmp[i] = domain[i] + (domain[i + 3] - domain[i]) * unif(r[i]);
             c = locate(P, tmp);
             double pdf = (c >= 0) * (1.0 + amp * cos(kw * tmp[axis]));
@@ -864,11 +1091,26 @@ int64_t oracle_load(void* h, int sp, int64_t N, uint64_t seed, double amp, doubl
             failed++;
             continue;
         }
-        auto r = philox({(uint64_t)k, 0xFFFFFFFFull, (uint64_t)sp, 0}, seed, TAG_LOAD);
         Particle p;
-        for (int j = 0; j < 3; ++j) {
-            p.x[j] = tmp[j];
-            p.v[j] = vd[j] - sqrt(2.0) * vth * erfc_inv_newton(2 * unif(r[j]));
+        for (int j = 0; j < 3; ++j) p.x[j] = tmp[j];
+        if (kind == 0) {
+            auto r = philox({(uint64_t)kk, 0xFFFFFFFFull, (uint64_t)sp, 0}, seed, TAG_LOAD);
+            for (int j = 0; j < 3; ++j) p.v[j] = vd[j] - sqrt(2.0) * vth * erfc_inv_newton(2 * unif(r[j]));
+        } else {
+            bool vok = false;
+            for (uint64_t a = 0; a < 1000000ull && !vok; ++a) {
+                auto r = philox({(uint64_t)kk, a, (uint64_t)sp, 1}, seed, TAG_LOAD);
+                double w[3];
+                for (int j = 0; j < 3; ++j) w[j] = vdomain[j] + (vdomain[j + 3] - vdomain[j]) * unif(r[j]);
+                if (unif(r[3]) * vmax < vdf_value(vs_pdf, w)) {
+                    for (int j = 0; j < 3; ++j) p.v[j] = w[j];
+                    vok = true;
+                }
+            }
+            if (!vok) {
+                failed++;
+                continue;
+            }
         }
         p.q = P.species[sp].q;
         p.m = P.species[sp].m;

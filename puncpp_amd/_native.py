@@ -36,6 +36,14 @@ _SIGS = {
     "punc_gpu_mesh_locate": (C.c_int, [vp, C.c_int64, d_p, i64_p]),
     "punc_internal_inside_tol": (C.c_int, [vp, d_p]),
     "punc_gpu_maxwellian_flux": (C.c_int, [vp, C.c_double, d_p, d_p, d_p]),
+    "punc_gpu_vdf_info": (C.c_int, [C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.c_double, d_p, d_p,
+                                    i32_p, i32_p]),
+    "punc_gpu_vdf_flux": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                    C.c_int, C.c_int64, d_p, d_p]),
+    "punc_gpu_pop_load_vdf": (C.c_int, [vp, C.c_int, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_int,
+                                        C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.c_double]),
+    "punc_gpu_injector_set_vdf": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, d_p, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, d_p, d_p]),
     "punc_gpu_pop_create": (C.c_int, [vp, C.c_int, d_p, d_p, C.c_int64, vp, C.POINTER(vp)]),
     "punc_gpu_pop_destroy": (None, [vp]),
     "punc_gpu_pop_add_species": (C.c_int, [vp, C.c_double, C.c_double, i32_p]),

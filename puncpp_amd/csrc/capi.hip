@@ -376,6 +376,68 @@ extern "C" int punc_gpu_maxwellian_flux(const punc_mesh_t* m, double vth, const 
     return 0;
 }
 
+extern "C" int punc_gpu_vdf_info(int kind, double vth, const double vd[3], double kappa, double alpha,
+                                 double vdf_range, double* factor, double* pdf_max, int* has_flux_num,
+                                 int* has_flux_max) {
+    if (!vd) return fail("vdf_info: null argument");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("vdf_info: " + err);
+    if (factor) *factor = F.factor;
+    if (pdf_max) *pdf_max = vdf_max(F);
+    if (has_flux_num) *has_flux_num = vdf_has_flux_num(F) ? 1 : 0;
+    if (has_flux_max) *has_flux_max = vdf_has_flux_max(F) ? 1 : 0;
+    return 0;
+}
+
+// create_flux for one species: closed forms where the distribution has them,
+// the device Monte Carlo otherwise.
+static int vdf_flux_tables(punc_mesh_s* M, const VdfParams& F, uint64_t seed, int rng_sp, int64_t n_iter,
+                           double* num, double* pdf_max) {
+    const auto& ext = M->topo.ext;
+    const int64_t nf = (int64_t)ext.size();
+    const bool cn = vdf_has_flux_num(F), cm = vdf_has_flux_max(F);
+    for (int64_t j = 0; j < nf; ++j) {
+        if (cn) num[j] = vdf_flux_num(F, ext[j].normal, ext[j].area);
+        if (cm) pdf_max[j] = vdf_flux_max(F, ext[j].normal);
+    }
+    if ((cn && cm) || nf == 0) return 0;
+    if (n_iter < 1) return fail("vdf_flux: n_iter must be positive");
+    HIPCHK(hipSetDevice(M->device));
+    int r = mesh_ensure_device(M);
+    if (r) return r;
+    std::vector<double> area((size_t)nf);
+    for (int64_t j = 0; j < nf; ++j) area[j] = ext[j].area;
+    double *d_area = nullptr, *d_num = nullptr, *d_max = nullptr;
+    auto cleanup = [&]() {
+        hipFree(d_area);
+        hipFree(d_num);
+        hipFree(d_max);
+    };
+    hipError_t e = dalloc(&d_area, nf);
+    if (e == hipSuccess) e = dalloc(&d_num, nf);
+    if (e == hipSuccess) e = dalloc(&d_max, nf);
+    if (e == hipSuccess) e = hipMemcpy(d_area, area.data(), nf * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_flux_mc(M->d_fnormal, d_area, nf, F, seed, rng_sp, n_iter, !cn, !cm, d_num, d_max, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && !cn) e = hipMemcpy(num, d_num, nf * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !cm) e = hipMemcpy(pdf_max, d_max, nf * 8, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return hipfail(e, "vdf_flux");
+    return 0;
+}
+
+extern "C" int punc_gpu_vdf_flux(punc_mesh_t* m, int kind, double vth, const double vd[3], double kappa,
+                                 double alpha, double vdf_range, uint64_t seed, int rng_species, int64_t n_iter,
+                                 double* num, double* pdf_max) {
+    if (!m || !vd || !num || !pdf_max) return fail("vdf_flux: null argument");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("vdf_flux: " + err);
+    return vdf_flux_tables(m, F, seed, rng_species, n_iter, num, pdf_max);
+}
+
 // ---------------------------------------------------------------------------
 // Population
 // ---------------------------------------------------------------------------
@@ -678,9 +740,8 @@ extern "C" int punc_gpu_pop_add_located(punc_pop_t* p, int64_t n, const double* 
     return append_host(p, n, xs, vs, cs.data());
 }
 
-extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp,
-                                 double kw, int axis, double vth, const double vd[3]) {
-    if (!p || !vd) return fail("pop_load: null argument");
+static int pop_load_common(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp, double kw, int axis,
+                           const VdfParams& F) {
     if (species < 0 || species >= p->nsp) return fail("pop_load: species out of range");
     if (axis < 0 || axis > 2) return fail("pop_load: axis must be 0, 1 or 2");
     if (N < 0) return fail("pop_load: negative count");
@@ -712,11 +773,30 @@ extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t
         seed = I.seed;
     }
     HIPCHK(launch_load(p->mesh->dm, p->d_state, s.n, N, species, rng_sp, c0, c1, p->cell_bits, seed, cmin,
-                       cmax, amp, kw, axis, vth, vd, p->d_errs, p->stream));
+                       cmax, amp, kw, axis, F, vdf_max(F), p->d_errs, p->stream));
     int64_t nn = s.n + N;
     HIPCHK(hipMemcpyAsync((char*)p->d_state + offsetof(DevState, n), &nn, 8, hipMemcpyHostToDevice, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return check_errors(p, "pop_load");
+}
+
+extern "C" int punc_gpu_pop_load(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp,
+                                 double kw, int axis, double vth, const double vd[3]) {
+    if (!p || !vd) return fail("pop_load: null argument");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(VDF_MAXWELLIAN, vth, vd, 0.0, 0.0, 5.0, F, err)) return fail("pop_load: " + err);
+    return pop_load_common(p, species, N, seed, amp, kw, axis, F);
+}
+
+extern "C" int punc_gpu_pop_load_vdf(punc_pop_t* p, int species, int64_t N, uint64_t seed, double amp, double kw,
+                                     int axis, int kind, double vth, const double vd[3], double kappa, double alpha,
+                                     double vdf_range) {
+    if (!p || !vd) return fail("pop_load_vdf: null argument");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("pop_load_vdf: " + err);
+    return pop_load_common(p, species, N, seed, amp, kw, axis, F);
 }
 
 extern "C" int punc_gpu_pop_size(punc_pop_t* p, int64_t* n) {
@@ -1181,27 +1261,54 @@ extern "C" int punc_gpu_sort(punc_pop_t* p) {
 // ---------------------------------------------------------------------------
 extern "C" int punc_gpu_injector_set(punc_pop_t* p, int species, double vth, const double vd[3],
                                      double density, double vrange) {
-    if (!p || !vd) return fail("injector_set: null argument");
-    if (species < 0 || species >= p->nsp) return fail("injector_set: species out of range");
-    if (!(vth > 0)) return fail("injector_set: vth must be positive");
-    SpInject& S = p->spi[species];
-    memset(&S, 0, sizeof(S));
-    S.vth = vth;
-    for (int i = 0; i < 3; ++i) S.vd[i] = vd[i];
-    S.vth2 = vth * vth;
-    S.factor = (1.0 / (pow(sqrt(2. * M_PI * S.vth2), 3)));  // distributions.cpp:28
-    S.vrange = vrange;
-    S.q = p->q[species];
-    S.m = p->m[species];
-    S.density = density;
-    p->inj_on[species] = 1;
-    const auto& ext = p->mesh->topo.ext;
-    const size_t nf = ext.size();
-    for (size_t j = 0; j < nf; ++j) {
-        // create_flux (injector.cpp:127-152); density and dt are applied per call
-        p->flux_num[species * nf + j] = maxwellian_flux_num(vth, vd, ext[j].normal, ext[j].area);
-        p->flux_max[species * nf + j] = maxwellian_flux_max(vth, vd, ext[j].normal);
+    // Maxwellian: closed-form flux number and maximum (distributions.cpp:52-72)
+    return punc_gpu_injector_set_vdf(p, species, VDF_MAXWELLIAN, vth, vd, 0.0, 0.0, density, vrange, nullptr,
+                                     nullptr);
+}
+
+extern "C" int punc_gpu_injector_set_vdf(punc_pop_t* p, int species, int kind, double vth, const double vd[3],
+                                         double kappa, double alpha, double density, double vdf_range,
+                                         const double* num, const double* pdf_max) {
+    if (!p || !vd) return fail("injector_set_vdf: null argument");
+    if (species < 0 || species >= p->nsp) return fail("injector_set_vdf: species out of range");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("injector_set_vdf: " + err);
+    const size_t nf = p->mesh->topo.ext.size();
+    std::vector<double> tn(nf), tm(nf);
+    if (num && pdf_max) {
+        memcpy(tn.data(), num, nf * 8);
+        memcpy(tm.data(), pdf_max, nf * 8);
+    } else {
+        // create_flux with the default stream: seed 0, n_iter 500000
+        int r = vdf_flux_tables(p->mesh, F, 0, species, 500000, tn.data(), tm.data());
+        if (r) return r;
     }
+    SpInject& S = p->spi[species];
+    SpInject T;
+    memset(&T, 0, sizeof(T));
+    T.vth = vth;
+    for (int i = 0; i < 3; ++i) T.vd[i] = vd[i];
+    T.vth2 = F.vth2;
+    T.factor = F.factor;
+    T.vrange = vdf_range;
+    T.q = p->q[species];
+    T.m = p->m[species];
+    T.density = density;
+    T.kind = kind;
+    T.k = F.k;
+    T.alpha = F.alpha;
+    // Unchanged parameters and tables keep the built pair table (a time loop
+    // that re-registers its species every step pays nothing).
+    const bool same = p->inj_on[species] && memcmp(&S, &T, offsetof(SpInject, seed)) == 0 && S.kind == T.kind &&
+                      S.k == T.k && S.alpha == T.alpha &&
+                      memcmp(&p->flux_num[species * nf], tn.data(), nf * 8) == 0 &&
+                      memcmp(&p->flux_max[species * nf], tm.data(), nf * 8) == 0;
+    if (same) return 0;
+    S = T;
+    p->inj_on[species] = 1;
+    memcpy(&p->flux_num[species * nf], tn.data(), nf * 8);
+    memcpy(&p->flux_max[species * nf], tm.data(), nf * 8);
     p->inj_dt = -1.0;  // rebuild the pair table
     return 0;
 }

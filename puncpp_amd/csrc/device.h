@@ -86,7 +86,7 @@ struct DevInject {
 struct SpInject {
     double vth;
     double vd[3];
-    double factor;  // 1/(sqrt(2*pi*vth^2))^3
+    double factor;  // the distribution's normalisation (VdfParams::factor)
     double vth2;
     double vrange;  // rejection box half width in units of vth
     double q, m;
@@ -98,7 +98,10 @@ struct SpInject {
     int32_t use_seed;  // 0: the seed given to the call
     int32_t rng_sp;
     int64_t facet_base;
-    double pad[3];
+    int32_t kind;  // VdfKind of the injected velocity distribution
+    int32_t pad0;
+    double k;      // kappa (Kappa, KappaCairns)
+    double alpha;  // alpha (Cairns, KappaCairns)
 };
 static_assert(sizeof(SpInject) == 128, "SpInject layout");
 

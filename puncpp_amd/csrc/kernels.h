@@ -105,8 +105,13 @@ hipError_t launch_step_end(DevState* S, hipStream_t st);
 int64_t sort_count_entries(int64_t capacity);
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
                        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
-                       double cmax, double amp, double kw, int axis, double vth, const double vd[3],
+                       double cmax, double amp, double kw, int axis, const VdfParams& F, double vmax,
                        unsigned long long* errs, hipStream_t st);
+// Flux tables by Monte Carlo (create_flux) over nf facets: num[j] when
+// do_num, pmax[j] when do_max (fnormal: 3 per facet, farea: 1 per facet).
+hipError_t launch_flux_mc(const double* fnormal, const double* farea, int64_t nf, const VdfParams& F, uint64_t seed,
+                          int rng_sp, int64_t n_iter, bool do_num, bool do_max, double* num, double* pmax,
+                          hipStream_t st);
 // Injection in two halves: generation (per-facet counts, candidates,
 // rejection-sampled velocities, location; reads only the mesh, the tables and
 // the step counter, so it may run beside a particle kernel) and the append of

@@ -1120,8 +1120,7 @@ __global__ void __launch_bounds__(256) k_sort_gather(DevState* __restrict__ S) {
 __global__ void __launch_bounds__(256)
 k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int sp, int rng_sp,
        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin, double cmax,
-       double amp, double kw, int axis, double vth, double vd0, double vd1, double vd2,
-       unsigned long long* __restrict__ errs) {
+       double amp, double kw, int axis, VdfParams F, double vmax, unsigned long long* __restrict__ errs) {
     const GArrays A = gview(S->cur);
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -1144,11 +1143,29 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
             atomicAdd(errs, 1ull);
             c = 0;
         }
-        U64x4 ctr = {{(uint64_t)k, 0xFFFFFFFFull, (uint64_t)rng_sp, 0ull}};
-        const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
-        const double vd[3] = {vd0, vd1, vd2};
         double v[3];
-        for (int d = 0; d < 3; ++d) v[d] = vd[d] - sqrt(2.0) * vth * erfc_inv_approx(2 * u01(r.v[d]));
+        if (F.kind == VDF_MAXWELLIAN) {
+            // inverse CDF per component (distributions.cpp:41-50)
+            U64x4 ctr = {{(uint64_t)k, 0xFFFFFFFFull, (uint64_t)rng_sp, 0ull}};
+            const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
+            for (int d = 0; d < 3; ++d) v[d] = F.vd[d] - sqrt(2.0) * F.vth * erfc_inv_approx(2 * u01(r.v[d]));
+        } else {
+            // rejection against the pdf maximum in the box vd +- range*vth
+            // (injector.cpp:25-49), attempts (k, a, species, 1)
+            bool vok = false;
+            double lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = F.vd[d] - F.range * F.vth;
+                hi[d] = F.vd[d] + F.range * F.vth;
+            }
+            for (uint64_t a = 0; a < 1000000ull && !vok; ++a) {
+                U64x4 ctr = {{(uint64_t)k, a, (uint64_t)rng_sp, 1ull}};
+                const U64x4 r = philox4x64_10(ctr, seed, TAG_LOAD);
+                for (int d = 0; d < 3; ++d) v[d] = lo[d] + (hi[d] - lo[d]) * u01(r.v[d]);
+                if (u01(r.v[3]) * vmax < vdf_eval(F, v[0], v[1], v[2])) vok = true;
+            }
+            if (!vok) atomicAdd(errs, 1ull);
+        }
         const int64_t i = base + k;
         A.x[i] = p[0];
         A.y[i] = p[1];
@@ -1245,7 +1262,16 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
             const uint64_t slot_c = (uint64_t)__shfl(slot, c, 64);
             const double vth = __shfl(P.vth, c, 64), vrange = __shfl(P.vrange, c, 64);
             const double vd0 = __shfl(P.vd[0], c, 64), vd1 = __shfl(P.vd[1], c, 64), vd2 = __shfl(P.vd[2], c, 64);
-            const double fac = __shfl(P.factor, c, 64), vth2 = __shfl(P.vth2, c, 64);
+            VdfParams F;
+            F.kind = __shfl(P.kind, c, 64);
+            F.factor = __shfl(P.factor, c, 64);
+            F.vth2 = __shfl(P.vth2, c, 64);
+            F.vth4 = F.vth2 * F.vth2;
+            F.k = __shfl(P.k, c, 64);
+            F.alpha = __shfl(P.alpha, c, 64);
+            F.vd[0] = vd0;
+            F.vd[1] = vd1;
+            F.vd[2] = vd2;
             const double m0 = __shfl(n0, c, 64), m1 = __shfl(n1, c, 64), m2 = __shfl(n2, c, 64);
             const double pm = __shfl(pmax, c, 64);
             const double lo0 = vd0 - vrange * vth, hi0 = vd0 + vrange * vth;
@@ -1257,15 +1283,7 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
                 const double w0 = lo0 + (hi0 - lo0) * u01(ra.v[0]);
                 const double w1 = lo1 + (hi1 - lo1) * u01(ra.v[1]);
                 const double w2 = lo2 + (hi2 - lo2) * u01(ra.v[2]);
-                double vn = 0.0;
-                vn += w0 * m0;
-                vn += w1 * m1;
-                vn += w2 * m2;
-                double vs = 0.0;
-                vs += (w0 - vd0) * (w0 - vd0);
-                vs += (w1 - vd1) * (w1 - vd1);
-                vs += (w2 - vd2) * (w2 - vd2);
-                const double f = (vn > 0.0) * vn * (fac * exp(-0.5 * vs / vth2));
+                const double f = vdf_flux(F, w0, w1, w2, m0, m1, m2);
                 const unsigned long long acc = __ballot(u01(ra.v[3]) * pm < f);
                 if (acc) {
                     const int win = __ffsll((long long)acc) - 1;
@@ -1606,6 +1624,65 @@ __global__ void k_ema(const double* __restrict__ f, double* __restrict__ g, doub
 }
 
 // ---------------------------------------------------------------------------
+// Flux tables by Monte Carlo (create_flux, injector.cpp:127-168) for the
+// distributions without a closed form: per exterior facet, n_iter uniform
+// velocities in the box vd +- range*vth give the flux integral
+// area * sum * volume / n_iter, and n_iter more give the flux-pdf maximum
+// (times 1.01).  One block per facet; sample s of facet j uses Philox
+// counters (species, j, s, 0) for the integral and (species, j, s, 1) for the
+// maximum, key (seed, FLUX).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_flux_mc(const double* __restrict__ fnormal, const double* __restrict__ farea, int64_t nf, VdfParams F,
+          uint64_t seed, int rng_sp, int64_t n_iter, int do_num, int do_max, double* __restrict__ num,
+          double* __restrict__ pmax) {
+    __shared__ double sh[8];
+    __shared__ double shm[256];
+    const int dim = 3;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = F.vd[d] - F.range * F.vth;  // Pdf::update_domain (distributions.h:139-146)
+        hi[d] = F.vd[d] + F.range * F.vth;
+    }
+    double volume = 1.0;
+    for (int d = 0; d < dim; ++d) volume *= hi[d] - lo[d];
+    for (int64_t j = blockIdx.x; j < nf; j += gridDim.x) {  // block-uniform
+        const double n0 = fnormal[3 * j], n1 = fnormal[3 * j + 1], n2 = fnormal[3 * j + 2];
+        double sum = 0.0, mx = 0.0;
+        for (int64_t s = threadIdx.x; s < n_iter; s += blockDim.x) {
+            if (do_num) {
+                U64x4 c = {{(uint64_t)rng_sp, (uint64_t)j, (uint64_t)s, 0ull}};
+                const U64x4 r = philox4x64_10(c, seed, TAG_FLUX);
+                double x[3];
+                for (int d = 0; d < 3; ++d) x[d] = lo[d] + u01(r.v[d]) * (hi[d] - lo[d]);
+                sum += vdf_flux(F, x[0], x[1], x[2], n0, n1, n2);
+            }
+            if (do_max) {
+                U64x4 c = {{(uint64_t)rng_sp, (uint64_t)j, (uint64_t)s, 1ull}};
+                const U64x4 r = philox4x64_10(c, seed, TAG_FLUX);
+                double x[3];
+                for (int d = 0; d < 3; ++d) x[d] = lo[d] + u01(r.v[d]) * (hi[d] - lo[d]);
+                const double p = vdf_flux(F, x[0], x[1], x[2], n0, n1, n2);
+                mx = mx > p ? mx : p;
+            }
+        }
+        const double tot = block_sum(sum, sh);
+        shm[threadIdx.x] = mx;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) shm[threadIdx.x] = shm[threadIdx.x] > shm[threadIdx.x + o] ? shm[threadIdx.x]
+                                                                                                 : shm[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (do_num) num[j] = farea[j] * tot * volume / n_iter;
+            if (do_max) pmax[j] = shm[0] * 1.01;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 #define LAUNCH_CHECK() \
@@ -1838,11 +1915,11 @@ int64_t sort_count_entries(int64_t capacity) {
 
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
                        int32_t cell_lo, int32_t cell_hi, int cell_bits, uint64_t seed, double cmin,
-                       double cmax, double amp, double kw, int axis, double vth, const double vd[3],
+                       double cmax, double amp, double kw, int axis, const VdfParams& F, double vmax,
                        unsigned long long* errs, hipStream_t st) {
     if (N <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_load, dim3(fill_grid(N)), dim3(256), 0, st, M, S, base, N, sp, rng_sp, cell_lo,
-                       cell_hi, cell_bits, seed, cmin, cmax, amp, kw, axis, vth, vd[0], vd[1], vd[2], errs);
+                       cell_hi, cell_bits, seed, cmin, cmax, amp, kw, axis, F, vmax, errs);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1952,6 +2029,19 @@ hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, dou
 hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_ema, dim3(fill_grid(n)), dim3(256), 0, st, f, g, w, n);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+}  // namespace punc_gpu
+
+namespace punc_gpu {
+hipError_t launch_flux_mc(const double* fnormal, const double* farea, int64_t nf, const VdfParams& F, uint64_t seed,
+                          int rng_sp, int64_t n_iter, bool do_num, bool do_max, double* num, double* pmax,
+                          hipStream_t st) {
+    if (nf <= 0 || (!do_num && !do_max)) return hipSuccess;
+    const int grid = (int)std::min<int64_t>(nf, 65536);
+    hipLaunchKernelGGL(k_flux_mc, dim3(grid), dim3(256), 0, st, fnormal, farea, nf, F, seed, rng_sp, n_iter,
+                       do_num ? 1 : 0, do_max ? 1 : 0, num, pmax);
     LAUNCH_CHECK();
     return hipSuccess;
 }

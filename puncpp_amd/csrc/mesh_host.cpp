@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 
 namespace punc_gpu {
@@ -562,6 +563,138 @@ double maxwellian_flux_max(double vth, const double vd[3], const double n[3]) {
     double vn = 0.0;
     for (int i = 0; i < 3; ++i) vn += tmp[i] * n[i];
     return (vn > 0.0) * vn * maxwellian_pdf(vth, vd, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity distributions: constructors, closed-form flux tables and maxima
+// (reference distributions.cpp:23-245).
+// ---------------------------------------------------------------------------
+static double inner3(const double* n, const double* vd) {
+    double vdn = 0.0;  // std::inner_product(n, n + 3, vd, 0.0)
+    for (int i = 0; i < 3; ++i) vdn = vdn + n[i] * vd[i];
+    return vdn;
+}
+
+bool vdf_setup(int kind, double vth, const double vd[3], double k, double alpha, double range, VdfParams& f,
+               std::string& err) {
+    const int dim = 3;
+    memset(&f, 0, sizeof(f));
+    if (kind < VDF_MAXWELLIAN || kind > VDF_KAPPA_CAIRNS) {
+        err = "unknown velocity distribution kind";
+        return false;
+    }
+    if (!(vth > 0)) {
+        err = "vth must be positive";
+        return false;
+    }
+    if ((kind == VDF_KAPPA || kind == VDF_KAPPA_CAIRNS) && !(k > 1.5)) {
+        err = "kappa must be bigger than 3/2";  // distributions.cpp:84
+        return false;
+    }
+    f.kind = kind;
+    f.vth = vth;
+    for (int i = 0; i < 3; ++i) f.vd[i] = vd[i];
+    f.k = k;
+    f.alpha = alpha;
+    f.range = range;
+    f.vth2 = vth * vth;
+    f.vth4 = f.vth2 * f.vth2;
+    switch (kind) {
+        case VDF_KAPPA:  // distributions.cpp:96-97
+            f.factor = (1.0 / pow(sqrt(M_PI * (2 * k - dim) * f.vth2), dim)) *
+                       (tgamma(k + 1.0) / tgamma(k + ((2.0 - dim) / 2.0)));
+            break;
+        case VDF_CAIRNS:  // distributions.cpp:142
+            f.factor = (1.0 / (pow(sqrt(2 * M_PI * f.vth2), dim) * (1 + dim * (dim + 2) * alpha)));
+            break;
+        case VDF_KAPPA_CAIRNS:  // distributions.cpp:196-198
+            f.factor = (1.0 / pow(sqrt(M_PI * (2. * k - dim) * f.vth2), dim)) *
+                       (1.0 / (1.0 + dim * (dim + 2.0) * alpha * ((k - dim / 2.0) / (k - (dim + 2) / 2.0)))) *
+                       (tgamma(k + 1.0) / tgamma(k + ((2.0 - dim) / 2.0)));
+            break;
+        default:  // distributions.cpp:28
+            f.factor = (1.0 / (pow(sqrt(2. * M_PI * f.vth2), dim)));
+    }
+    if (!std::isfinite(f.factor) || !(f.factor > 0)) {
+        err = "velocity distribution normalisation is not a positive finite number for these parameters";
+        return false;
+    }
+    return true;
+}
+
+bool vdf_has_flux_num(const VdfParams& f) {
+    switch (f.kind) {
+        case VDF_KAPPA: {
+            // distributions.cpp:86-94: std::accumulate(vd, 0) sums into an
+            // int (the initial value's type), truncating at every step, so
+            // drifts that truncate to zero still take the no-drift formula.
+            int sum_vd = 0;
+            for (int i = 0; i < 3; ++i) sum_vd = (int)(sum_vd + f.vd[i]);
+            return sum_vd == 0;
+        }
+        case VDF_CAIRNS:
+            return true;
+        case VDF_KAPPA_CAIRNS:
+            return false;
+        default:
+            return true;
+    }
+}
+
+bool vdf_has_flux_max(const VdfParams& f) { return f.kind == VDF_MAXWELLIAN || f.kind == VDF_KAPPA; }
+
+double vdf_flux_num(const VdfParams& f, const double n[3], double S) {
+    const int dim = 3;
+    const double vth = f.vth, k = f.k, alpha = f.alpha;
+    switch (f.kind) {
+        case VDF_KAPPA:  // distributions.cpp:110-118 (no drift)
+            return S * ((vth / (sqrt(2 * M_PI))) * (1.0 / sqrt(k - dim / 2.0)) *
+                        (tgamma(k - ((dim - 1.0) / 2.0)) / tgamma(k - dim / 2.0)));
+        case VDF_CAIRNS: {  // distributions.cpp:175-185
+            const double vdn = inner3(n, f.vd);
+            return S * ((vth / (sqrt(2 * M_PI))) * exp(-0.5 * (vdn / vth) * (vdn / vth)) *
+                            (1. + (dim + 1.) * (dim + 3.) * alpha + (vdn / vth) * (vdn / vth) * alpha) /
+                            (1. + dim * (dim + 2.) * alpha) +
+                        0.5 * vdn * (1. + erf(vdn / (sqrt(2) * vth))));
+        }
+        default:
+            return maxwellian_flux_num(vth, f.vd, n, S);
+    }
+}
+
+double vdf_flux_max(const VdfParams& f, const double n[3]) {
+    if (f.kind == VDF_KAPPA) {  // distributions.cpp:120-128 (evaluated without the drift shift)
+        const int dim = 3;
+        double tmp[3];
+        for (int i = 0; i < 3; ++i) tmp[i] = sqrt((2. * f.k - dim) / (2. * f.k + 1.0)) * n[i] * f.vth;
+        return vdf_flux(f, tmp[0], tmp[1], tmp[2], n[0], n[1], n[2]);
+    }
+    return maxwellian_flux_max(f.vth, f.vd, n);
+}
+
+double vdf_max(const VdfParams& f) {
+    const int dim = 3;
+    const double k = f.k, alpha = f.alpha;
+    double v_max[3] = {f.vd[0], f.vd[1], f.vd[2]};
+    switch (f.kind) {
+        case VDF_CAIRNS: {  // distributions.cpp:156-173
+            if (alpha < 0.25) return f.factor;
+            v_max[0] += f.vth * sqrt(2.0 + sqrt(4.0 - 1.0 / alpha));
+            const double m = vdf_eval(f, v_max[0], v_max[1], v_max[2]);
+            return f.factor > m ? f.factor : m;
+        }
+        case VDF_KAPPA_CAIRNS:  // distributions.cpp:213-230
+            if (alpha >= (k - 1.) * (k + 1.) / ((2. * k - dim) * (2. * k - dim))) {
+                v_max[0] += f.vth * sqrt(((2.0 * k - dim) / (k - 1.0)) +
+                                         sqrt(alpha * alpha * (2 * k - dim) * (2 * k - dim) -
+                                              alpha * (k - 1.0) * (k + 1.0)) /
+                                             (alpha * (k - 1.0)));
+                return vdf_eval(f, v_max[0], v_max[1], v_max[2]);
+            }
+            return f.factor;
+        default:  // Maxwellian, Kappa: the value at the drift (distributions.h:244, 278)
+            return f.factor;
+    }
 }
 
 }  // namespace punc_gpu

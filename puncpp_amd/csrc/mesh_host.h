@@ -86,4 +86,17 @@ double maxwellian_flux_num(double vth, const double vd[3], const double n[3], do
 double maxwellian_flux_max(double vth, const double vd[3], const double n[3]);
 double maxwellian_pdf(double vth, const double vd[3], const double v[3]);
 
+// Velocity distributions (reference distributions.cpp:23-245): parameters
+// with the constructor's normalisation factor, the create_flux flags
+// (has_flux_number, has_flux_max), the closed-form flux number and flux-pdf
+// maximum for a facet of unit normal n and area S, and the pdf maximum used
+// by the loading sampler.
+bool vdf_setup(int kind, double vth, const double vd[3], double k, double alpha, double range, VdfParams& f,
+               std::string& err);
+bool vdf_has_flux_num(const VdfParams& f);
+bool vdf_has_flux_max(const VdfParams& f);
+double vdf_flux_num(const VdfParams& f, const double n[3], double S);
+double vdf_flux_max(const VdfParams& f, const double n[3]);
+double vdf_max(const VdfParams& f);
+
 }  // namespace punc_gpu

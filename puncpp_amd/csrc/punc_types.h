@@ -100,6 +100,55 @@ PUNC_HD double u01(uint64_t x) {
 constexpr uint64_t TAG_INJECT_COUNT = 0x494e4a43ull;  // per (species, facet) count rounding
 constexpr uint64_t TAG_INJECT = 0x494e4a50ull;        // per injected candidate
 constexpr uint64_t TAG_LOAD = 0x4c4f4144ull;          // initial loading
+constexpr uint64_t TAG_FLUX = 0x464c5558ull;          // flux-table Monte Carlo (create_flux)
+
+// ----------------------------------------------------------------------------
+// Velocity distributions (reference distributions.cpp:23-245): the pdf value
+// at v, each expression in the reference's evaluation order.  factor, vth2
+// and vth4 come from vdf_setup (host, once per species).
+// ----------------------------------------------------------------------------
+enum VdfKind : int32_t { VDF_MAXWELLIAN = 0, VDF_KAPPA = 1, VDF_CAIRNS = 2, VDF_KAPPA_CAIRNS = 3 };
+
+struct VdfParams {
+    int32_t kind;
+    int32_t pad;
+    double vth, vth2, vth4, factor;
+    double k;      // spectral index kappa (Kappa, KappaCairns)
+    double alpha;  // Cairns alpha (Cairns, KappaCairns)
+    double vd[3];
+    double range;  // sampling box vd +- range * vth
+};
+
+PUNC_HD double vdf_eval(const VdfParams& f, double w0, double w1, double w2) {
+    const int dim = 3;
+    double v2 = 0.0;
+    v2 += (w0 - f.vd[0]) * (w0 - f.vd[0]);
+    v2 += (w1 - f.vd[1]) * (w1 - f.vd[1]);
+    v2 += (w2 - f.vd[2]) * (w2 - f.vd[2]);
+    switch (f.kind) {
+        case VDF_KAPPA:  // distributions.cpp:99-107
+            return f.factor * pow(1.0 + v2 / ((2 * f.k - dim) * f.vth2), -(f.k + 1.0));
+        case VDF_CAIRNS: {  // distributions.cpp:145-154
+            const double v4 = v2 * v2;
+            return f.factor * (1 + f.alpha * v4 / f.vth4) * exp(-0.5 * v2 / f.vth2);
+        }
+        case VDF_KAPPA_CAIRNS: {  // distributions.cpp:201-211
+            const double v4 = v2 * v2;
+            return f.factor * (1.0 + f.alpha * v4 / f.vth4) * pow(1.0 + v2 / ((2 * f.k - dim) * f.vth2), -(f.k + 1.0));
+        }
+        default:  // Maxwellian, distributions.cpp:31-39
+            return f.factor * exp(-0.5 * v2 / f.vth2);
+    }
+}
+
+// Flux pdf (v.n)^+ f(v) (distributions.h:93-102).
+PUNC_HD double vdf_flux(const VdfParams& f, double w0, double w1, double w2, double n0, double n1, double n2) {
+    double vn = 0.0;
+    vn += w0 * n0;
+    vn += w1 * n1;
+    vn += w2 * n2;
+    return (vn > 0.0) * vn * vdf_eval(f, w0, w1, w2);
+}
 
 // ----------------------------------------------------------------------------
 // Inverse complementary error function.  Initial guess from Abramowitz &

@@ -190,22 +190,102 @@ class CosinePosition(UniformPosition):
         return 1.0 + self.amp
 
 
-class Maxwellian:
-    """Reference distributions.h:233-249, distributions.cpp:23-77."""
+class _Vdf:
+    """Velocity distribution (reference Pdf, distributions.h:41-180): thermal
+    speed, drift, the half width of the sampling box in units of vth, and the
+    create_flux tables (num_particles, pdf_max per exterior facet) once
+    create_flux has run.  Normalisation, Pdf::max() and the create_flux flags
+    come from the library (punc_gpu_vdf_info)."""
 
-    def __init__(self, vth, vd, vdf_range=5.0):
+    kind = 0
+
+    def __init__(self, vth, vd, vdf_range, k=0.0, alpha=0.0):
         self._vth = float(vth)
         self._vd = vec3(vd)
         self.vdf_range = float(vdf_range)
+        self.k = float(k)
+        self.alpha = float(alpha)
         self.dim = 3
         self.num_particles = None
         self.pdf_max = None
+        fac, mx = N.C.c_double(), N.C.c_double()
+        fn, fm = N.C.c_int32(), N.C.c_int32()
+        check(N.lib().punc_gpu_vdf_info(self.kind, self._vth, dptr(self._vd), self.k, self.alpha, self.vdf_range,
+                                        N.C.byref(fac), N.C.byref(mx), N.C.byref(fn), N.C.byref(fm)))
+        self.factor = fac.value
+        self._max = mx.value
+        self.has_flux_number = bool(fn.value)
+        self.has_flux_max = bool(fm.value)
+        self.has_icdf = self.kind == 0
 
     def vth(self):
         return self._vth
 
     def vd(self):
         return self._vd.copy()
+
+    def max(self):
+        return self._max
+
+    def domain(self):
+        """Pdf::update_domain (distributions.h:139-146): [lo x3, hi x3]."""
+        return np.concatenate([self._vd - self.vdf_range * self._vth, self._vd + self.vdf_range * self._vth])
+
+    def _debye_factor(self):
+        return 1.0
+
+    def debye(self, m, q, n, eps0):
+        return math.sqrt(eps0 * m / (n * q * q * self._debye_factor())) * self._vth
+
+
+class Maxwellian(_Vdf):
+    """Reference distributions.h:233-249, distributions.cpp:23-77."""
+
+    kind = 0
+
+    def __init__(self, vth, vd, vdf_range=5.0):
+        super().__init__(vth, vd, vdf_range)
+
+    def debye(self, m, q, n, eps0):
+        return math.sqrt(eps0 * m / (n * q * q)) * self._vth  # distributions.cpp:74-77
+
+
+class Kappa(_Vdf):
+    """Reference distributions.h:265-297, distributions.cpp:79-134 (kappa > 3/2).
+    has_flux_number follows the reference's integer sum of the drift."""
+
+    kind = 1
+
+    def __init__(self, vth, vd, k, vdf_range=7.0):
+        super().__init__(vth, vd, vdf_range, k=k)
+
+    def _debye_factor(self):
+        return (self.k - 0.5) / (self.k - 1.5)  # distributions.cpp:130-134
+
+
+class Cairns(_Vdf):
+    """Reference distributions.h:299-333, distributions.cpp:136-191."""
+
+    kind = 2
+
+    def __init__(self, vth, vd, alpha, vdf_range=7.0):
+        super().__init__(vth, vd, vdf_range, alpha=alpha)
+
+    def _debye_factor(self):
+        return (1. + 3. * self.alpha) / (1. + 15 * self.alpha)  # distributions.cpp:187-191
+
+
+class KappaCairns(_Vdf):
+    """Reference distributions.h:335-360, distributions.cpp:193-243."""
+
+    kind = 3
+
+    def __init__(self, vth, vd, k, alpha, vdf_range=15.0):
+        super().__init__(vth, vd, vdf_range, k=k, alpha=alpha)
+
+    def _debye_factor(self):
+        k, a = self.k, self.alpha  # distributions.cpp:238-243
+        return ((k - 0.5) / (k - 1.5)) * ((1. + 3. * a * ((k - 1.5) / (k - 0.5))) / (1. + 15 * a * ((k - 1.5) / (k - 2.5))))
 
 
 class ParticleAmountType(enum.Enum):
@@ -235,19 +315,23 @@ class Species:
         self.num = int(self.n * mesh.volume())
         self.pdf = pdf
         self.vdf = vdf
-        self.debye = math.sqrt(eps0 * self.m / (self.n * self.q * self.q)) * vdf.vth()
+        self.debye = vdf.debye(self.m, self.q, self.n, eps0)  # population.cpp:68
 
 
-def create_flux(species, mesh):
-    """Reference injector.cpp:104-171 (Maxwellian: closed-form flux number and
-    flux-pdf maximum per exterior facet)."""
-    for s in species:
+def create_flux(species, mesh, seed=0, n_iter=500000):
+    """Reference injector.cpp:104-171: per species and exterior facet the
+    expected number of particles per unit density and time and the flux-pdf
+    maximum.  Closed forms where the distribution has them; otherwise n_iter
+    Monte Carlo samples per facet on the device, from the Philox stream keyed
+    by `seed` (the species' position in `species` names its substream)."""
+    for i, s in enumerate(species):
         ne = mesh.num_exterior
         num, pmax = np.empty(ne), np.empty(ne)
-        vd = s.vdf.vd()
-        check(N.lib().punc_gpu_maxwellian_flux(mesh._h, s.vdf.vth(), dptr(vd), dptr(num), dptr(pmax)))
-        s.vdf.num_particles = num
-        s.vdf.pdf_max = pmax
+        v = s.vdf
+        check(N.lib().punc_gpu_vdf_flux(mesh._h, v.kind, v.vth(), dptr(v.vd()), v.k, v.alpha, v.vdf_range,
+                                        int(seed), i, int(n_iter), dptr(num), dptr(pmax)))
+        v.num_particles = num
+        v.pdf_max = pmax
 
 
 class Object:
@@ -517,8 +601,13 @@ def _register_injectors(pop, species):
         idx = pop.species_index(s.q, s.m)
         if idx in pop._inj_set:
             continue
-        check(N.lib().punc_gpu_injector_set(pop._h, idx, s.vdf.vth(), dptr(s.vdf.vd()), float(s.n),
-                                            float(s.vdf.vdf_range)))
+        v = s.vdf
+        tables = v.num_particles is not None and v.pdf_max is not None
+        num = np.ascontiguousarray(v.num_particles, dtype=np.float64) if tables else None
+        pmax = np.ascontiguousarray(v.pdf_max, dtype=np.float64) if tables else None
+        check(N.lib().punc_gpu_injector_set_vdf(pop._h, idx, v.kind, v.vth(), dptr(v.vd()), v.k, v.alpha,
+                                                float(s.n), float(v.vdf_range),
+                                                dptr(num) if tables else None, dptr(pmax) if tables else None))
         pop._inj_set.add(idx)
 
 
@@ -536,5 +625,7 @@ def load_particles(pop, species, seed=0):
     for s in species:
         idx = pop.species_index(s.q, s.m)
         pdf = s.pdf
-        check(N.lib().punc_gpu_pop_load(pop._h, idx, int(s.num), int(seed), float(pdf.amp),
-                                        float(pdf.kw), int(pdf.axis), s.vdf.vth(), dptr(s.vdf.vd())))
+        v = s.vdf
+        check(N.lib().punc_gpu_pop_load_vdf(pop._h, idx, int(s.num), int(seed), float(pdf.amp), float(pdf.kw),
+                                            int(pdf.axis), v.kind, v.vth(), dptr(v.vd()), v.k, v.alpha,
+                                            float(v.vdf_range)))

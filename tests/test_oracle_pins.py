@@ -11,6 +11,7 @@ import math
 import numpy as np
 import pytest
 import scipy.special as sps
+from scipy import integrate, optimize
 
 import oracle as O
 import puncpp_amd as P
@@ -418,6 +419,153 @@ def test_statistics_against_numpy(small):
     o1.add_particles(0, np.array([[0.1, 1.5, -1.2]]), np.array([[3.0, 4.0, 0.0]]))
     s1 = o1.statistics(np.full(4, -7.0))
     assert s1[0] == 5.0 and np.isnan(s1[1]) and s1[2] == -7.0 and s1[3] == -7.0
+
+
+# ---------------------------------------------------------------------------
+# Non-Maxwellian velocity distributions
+# ---------------------------------------------------------------------------
+VDF_CASES = [(1, 2.0, 0.0), (1, 4.0, 0.0), (1, 10.0, 0.0), (2, 0.0, 0.1), (2, 0.0, 0.5),
+             (3, 4.0, 0.3), (3, 6.0, 1.0), (3, 3.0, 0.2)]
+
+
+def _radial(kind, r, k, a, vth=1.0):
+    return O.vdf_value(kind, vth, [0.0, 0.0, 0.0], [r, 0.0, 0.0], k, a)
+
+
+@pytest.mark.parametrize("kind,k,a", VDF_CASES)
+def test_vdf_normalised(kind, k, a):
+    # 4 pi int f(r) r^2 dr = 1 (a wrong power, gamma ratio or (1 + 15 alpha)
+    # factor fails), and the value is radial about the drift
+    vth = 1.3
+    tot = 4 * math.pi * integrate.quad(lambda r: _radial(kind, r, k, a, vth) * r * r, 0, np.inf, limit=200)[0]
+    assert tot == pytest.approx(1.0, abs=1e-10)
+    vd = np.array([0.2, -0.1, 0.4])
+    u = np.array([0.3, 0.5, -0.2])
+    assert O.vdf_value(kind, vth, vd, vd + u, k, a) == pytest.approx(
+        O.vdf_value(kind, vth, [0, 0, 0], [np.linalg.norm(u), 0, 0], k, a), rel=1e-14)
+
+
+def test_vdf_limits():
+    v = np.array([0.7, -0.4, 1.1])
+    vd = np.array([0.1, 0.0, 0.5])
+    mx = O.vdf_value(0, 1.2, vd, v)
+    # Kappa -> Maxwellian as kappa grows, with an O(1/kappa) error (tgamma
+    # overflows past kappa ~ 170, as in the reference); Cairns with alpha = 0
+    # is the Maxwellian
+    e40 = abs(O.vdf_value(1, 1.2, vd, v, 40.0) / mx - 1)
+    e150 = abs(O.vdf_value(1, 1.2, vd, v, 150.0) / mx - 1)
+    assert e150 < 1e-2 and e150 < e40 / 3
+    assert O.vdf_value(2, 1.2, vd, v, 0.0, 0.0) == mx
+    assert abs(O.vdf_value(3, 1.2, vd, v, 150.0, 0.0) / mx - 1) == pytest.approx(e150, rel=1e-9)
+
+
+@pytest.mark.parametrize("kind,k,a", VDF_CASES)
+def test_vdf_max_brute_force(kind, k, a):
+    m, _, _ = O.vdf_max(kind, 1.0, [0.0, 0.0, 0.0], k, a)
+    r = np.linspace(0, 6, 60001)
+    brute = max(_radial(kind, x, k, a) for x in r[::10])
+    r0 = r[::10][np.argmax([_radial(kind, x, k, a) for x in r[::10]])]
+    fine = optimize.minimize_scalar(lambda x: -_radial(kind, x, k, a), bounds=(max(r0 - 0.01, 0), r0 + 0.01),
+                                    method="bounded", options={"xatol": 1e-12})
+    brute = max(brute, -fine.fun)
+    assert m == pytest.approx(brute, rel=1e-9)
+
+
+def test_vdf_flux_flags():
+    # Kappa's has_flux_number comes from an integer-accumulated drift sum
+    assert O.vdf_max(1, 1.0, [0.0, 0.0, 0.5], 4.0)[1:] == (True, True)
+    assert O.vdf_max(1, 1.0, [0.6, 0.6, 0.0], 4.0)[1:] == (True, True)
+    assert O.vdf_max(1, 1.0, [0.0, 0.0, 1.5], 4.0)[1:] == (False, True)
+    assert O.vdf_max(2, 1.0, [0.0, 0.0, 0.5], 0.0, 0.3)[1:] == (True, False)
+    assert O.vdf_max(3, 1.0, [0.0, 0.0, 0.0], 4.0, 0.3)[1:] == (False, False)
+
+
+def _flux_quadrature(kind, k, a, u, vth=1.0):
+    """int (v.n)^+ f(v - u n) d^3v for a radial f: the marginal along n,
+    g(w) = 2 pi int_|w|^inf f(r) r dr, then int_0^inf v g(v - u) dv."""
+    g = lambda w: 2 * math.pi * integrate.quad(lambda r: _radial(kind, r, k, a, vth) * r, abs(w), np.inf)[0]
+    return integrate.quad(lambda v: v * g(v - u), 0, np.inf, limit=200)[0]
+
+
+@pytest.mark.parametrize("kind,k,a,u", [(1, 4.0, 0.0, 0.0), (1, 2.5, 0.0, 0.0), (2, 0.0, 0.3, 0.0),
+                                        (2, 0.0, 0.3, 0.7), (2, 0.0, 1.0, -0.4)])
+def test_vdf_flux_number_closed_forms(kind, k, a, u):
+    hm = P.box_mesh(1, [0.0] * 3, [1.0] * 3)
+    o = O.OraclePop(hm, [1.0], [1.0])
+    f = o.exterior_facets()
+    num, _ = o.vdf_flux(kind, 1.0, [0.0, 0.0, u], k, a, 7.0, n_iter=1)
+    j = int(np.argmax(f["normal"][:, 2]))  # a facet with normal +z
+    assert np.allclose(f["normal"][j], [0, 0, 1])
+    assert num[j] / f["area"][j] == pytest.approx(_flux_quadrature(kind, k, a, u), rel=1e-7)
+
+
+def test_vdf_kappa_flux_max_closed_form():
+    hm = P.box_mesh(1, [0.0] * 3, [1.0] * 3)
+    o = O.OraclePop(hm, [1.0], [1.0])
+    f = o.exterior_facets()
+    j = int(np.argmax(f["normal"][:, 2]))
+    _, pmax = o.vdf_flux(1, 1.0, [0.0, 0.0, 0.0], 4.0, 0.0, 7.0, n_iter=1)
+    # the flux pdf v_z f(v) peaks on the normal axis
+    best = optimize.minimize_scalar(lambda z: -z * _radial(1, z, 4.0, 0.0), bounds=(0, 5), method="bounded",
+                                    options={"xatol": 1e-12})
+    assert pmax[j] == pytest.approx(-best.fun, rel=1e-12)
+
+
+def test_vdf_flux_monte_carlo():
+    # KappaCairns: both tables by Monte Carlo; 24 facets per normal direction
+    # give independent estimates of the same flux integral
+    hm = P.box_mesh(2, [0.0] * 3, [1.0] * 3)
+    o = O.OraclePop(hm, [1.0], [1.0])
+    f = o.exterior_facets()
+    k, a, u = 4.0, 0.3, 0.4
+    num, pmax = o.vdf_flux(3, 1.0, [0.0, 0.0, u], k, a, 15.0, seed=3, rng_sp=0, n_iter=40000)
+    up = np.flatnonzero(np.isclose(f["normal"][:, 2], 1.0))
+    est = num[up] / f["area"][up]
+    assert len(up) == 8
+    # the estimator integrates over the sampling box vd +- 15 vth only (about
+    # 5% of this heavy-tailed flux lies outside): reference value from 8M
+    # numpy samples of the same box, normalised by the pinned pdf
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-15.0, 15.0, size=(8_000_000, 3))
+    x[:, 2] += u
+    r = np.linalg.norm(x - [0.0, 0.0, u], axis=1)
+    c0 = _radial(3, 0.0, k, a)  # the normalisation factor
+    fx = c0 * (1 + a * r ** 4) * (1 + r * r / (2 * k - 3)) ** (-(k + 1))
+    box = np.mean(np.where(x[:, 2] > 0, x[:, 2] * fx, 0.0)) * 30.0 ** 3
+    assert box < 0.97 * _flux_quadrature(3, k, a, u)
+    assert abs(est.mean() - box) < 4 * est.std(ddof=1) / math.sqrt(len(up)) + 0.005 * box
+    # the maximum is a sampled maximum times 1.01: close to the true one
+    # (grid over the (v_perp, v_z) half plane, then a local refinement)
+    fl = lambda w: -max(w[1], 0) * O.vdf_value(3, 1.0, [0, 0, u], [w[0], 0.0, w[1]], k, a)
+    vp, vz = np.meshgrid(np.linspace(0, 6, 301), np.linspace(0, 6, 301))
+    rr = np.sqrt(vp ** 2 + (vz - u) ** 2)
+    grid = vz * c0 * (1 + a * rr ** 4) * (1 + rr * rr / (2 * k - 3)) ** (-(k + 1))
+    i0 = np.unravel_index(np.argmax(grid), grid.shape)
+    best = optimize.minimize(fl, x0=[vp[i0], vz[i0]], method="Nelder-Mead",
+                             options={"xatol": 1e-10, "fatol": 1e-16})
+    true_max = max(-best.fun, grid.max())
+    assert np.all(pmax[up] <= 1.01 * true_max * (1 + 1e-9)) and np.all(pmax[up] >= 0.9 * true_max)
+    # another seed gives other samples, the same estimate
+    num2, _ = o.vdf_flux(3, 1.0, [0.0, 0.0, u], k, a, 15.0, seed=4, rng_sp=0, n_iter=40000)
+    assert not np.array_equal(num, num2)
+
+
+@pytest.mark.parametrize("kind,k,a", [(1, 3.0, 0.0), (2, 0.0, 0.5), (3, 4.0, 0.3)])
+def test_vdf_load_radial_distribution(kind, k, a):
+    # velocities of the loading sampler: |v - vd| inside the ball of radius
+    # range * vth follows f restricted to that ball
+    hm = P.box_mesh(2, [0.0] * 3, [1.0] * 3)
+    o = O.OraclePop(hm, [1.0], [1.0])
+    vd = np.array([0.0, 0.3, 0.0])
+    rng_r = 5.0
+    assert o.load_vdf(0, 6000, 9, 0.0, 0.0, 0, kind, 1.0, vd, k, a, rng_r) == 0
+    r = np.linalg.norm(o.get_particles()["v"] - vd, axis=1)
+    r = r[r < rng_r]
+    w = lambda x: 4 * math.pi * _radial(kind, x, k, a) * x * x
+    Z = integrate.quad(w, 0, rng_r)[0]
+    cdf = lambda x: integrate.quad(w, 0, min(x, rng_r))[0] / Z
+    from scipy import stats
+    assert stats.kstest(r, np.vectorize(cdf)).pvalue > 1e-3
 
 
 # ---------------------------------------------------------------------------
