@@ -6,7 +6,12 @@
 
 #include <punc/population.h>
 
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <limits>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace punc {
@@ -60,6 +65,98 @@ template <typename PopulationType>
 void density_ema(PopulationType& pop, Function& ne_g, Function& ni_g, double dt, double tau, bool reset = false) {
     check(punc_gpu_density_ema(pop.handle(), dt, tau, reset ? 1 : 0, ne_g.values.data(), ni_g.values.data()));
 }
+
+//! Run state for restarts (diagnostics.h:39-71, diagnostics.cpp:35-68).
+class State {
+public:
+    std::string fname;
+    explicit State(std::string fname) : fname(std::move(fname)) {}
+
+    //! First line: n, t, then per object charge and current (objects after
+    //! the first read 0, their values being on later lines).
+    void load(std::size_t& n, double& t, ObjectVector objects) {
+        std::ifstream ifile(fname);
+        std::string line;
+        std::getline(ifile, line);
+        char* s = (char*)line.c_str();
+        n = strtol(s, &s, 10);
+        t = strtod(s, &s);
+        for (auto& o : objects) {
+            o->charge = strtod(s, &s);
+            o->current = strtod(s, &s);
+        }
+    }
+
+    //! "n+1\tt\t", then "charge\tcurrent\n" per object (default stream format).
+    void save(std::size_t n, double t, const ObjectVector& objects) {
+        std::ofstream ofile(fname, std::ofstream::out);
+        ofile << n + 1 << "\t" << t << "\t";
+        for (auto& o : objects) {
+            ofile << o->charge << "\t";
+            ofile << o->current << "\n";
+        }
+    }
+};
+
+//! Time-series file (diagnostics.h:73-170, diagnostics.cpp:70-126).
+class History {
+public:
+    std::ofstream ofile;
+    std::size_t dim;
+    bool stats;
+    bool hex_output;  ///< accepted; decimal output either way, as in the reference
+
+    History(const std::string& fname, const ObjectVector& objects, std::size_t dim = 3, bool stats = false,
+            bool continue_simulation = false, bool hex_output = false)
+        : dim(dim), stats(stats), hex_output(hex_output) {
+        if (continue_simulation) {
+            ofile.open(fname, std::ofstream::out | std::ofstream::app);
+            return;
+        }
+        ofile.open(fname, std::ofstream::out);
+        ofile << "#:xaxis\tt\n";
+        ofile << "#:name\tn\tt\tne\tni\tKE\tPE";
+        for (std::size_t i = 0; i < objects.size(); ++i) ofile << "\tV[" << i << "]\tI[" << i << "]\tQ[" << i << "]";
+        if (stats) ofile << "\tmean_e\tstdev_e\tmean_i\tstdev_i";
+        ofile << "\n";
+        ofile << "#:long\ttimestep\ttime\t\"number of electrons\"\t";
+        ofile << "\"number of ions\"\t\"kinetic energy\"\t";
+        ofile << "\"potential energy\"";
+        for (std::size_t i = 0; i < objects.size(); ++i) ofile << "\tvoltage\tcurrent\tcharge";
+        if (stats)
+            ofile << "\tmean velocity of electrons\tstandard deviation of electron velocities"
+                  << "\tmean velocity of ions\tstandard deviation of ion velocities";
+        ofile << "\n";
+        ofile << "#:units\t1\ts\tm**(-3)\tm**(-3)\tJ\tJ";
+        for (std::size_t i = 0; i < objects.size(); ++i) {
+            ofile << "\tV";
+            if (dim == 1) ofile << "\tA/m**2";
+            else if (dim == 2) ofile << "\tA/m";
+            else if (dim == 3) ofile << "\tA";
+            ofile << "\tC";
+        }
+        if (stats) ofile << "\tm/s\tm/s\tm/s\tm/s";
+        ofile << "\n";
+    }
+
+    //! One line: n, then t, ne, ni, KE, PE, per object V, I, Q, and with
+    //! stats the population's speed statistics, all "%.16e".
+    template <typename PopulationType>
+    void save(std::size_t n, double t, double num_e, double num_i, double KE, double PE, const ObjectVector& objects,
+              PopulationType& pop) {
+        ofile << n << "\t";
+        ofile << std::setprecision(std::numeric_limits<double>::digits10 + 1);
+        ofile << std::scientific;
+        ofile << t << "\t" << num_e << "\t" << num_i << "\t" << KE << "\t" << PE;
+        for (auto& o : objects) ofile << "\t" << o->get_potential() << "\t" << o->current << "\t" << o->charge;
+        if (stats) {
+            double statistics[4] = {0, 0, 0, 0};
+            pop.statistics(statistics);
+            for (int i = 0; i < 4; ++i) ofile << "\t" << statistics[i];
+        }
+        ofile << std::endl;
+    }
+};
 
 }  // namespace punc
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 namespace punc {
@@ -163,6 +164,24 @@ public:
 
     //! Re-sort particles by cell (keeps per-cell data reads coalesced).
     void sort() { check(punc_gpu_sort(handle())); }
+
+    //! save_file (population.h:807-839): 64-byte records or "%.17g" text lines,
+    //! in cell order (the particles are re-sorted by cell first).
+    void save_file(const std::string& fname, bool binary) {
+        check(punc_gpu_pop_save_file(handle(), fname.c_str(), binary ? 1 : 0));
+    }
+
+    //! load_file (population.h:841-880): appends the file's particles; new
+    //! (q, m) pairs become species.
+    void load_file(const std::string& fname, bool binary) {
+        int64_t added = 0;
+        check(punc_gpu_pop_load_file(handle(), fname.c_str(), binary ? 1 : 0, &added));
+        int n = 0;
+        check(punc_gpu_pop_species(handle(), &n, nullptr, nullptr));
+        q_.assign((size_t)n, 0.0);
+        m_.assign((size_t)n, 0.0);
+        if (n > 0) check(punc_gpu_pop_species(handle(), &n, q_.data(), m_.data()));
+    }
 
     //! statistics (population.h:746-805): mean and standard deviation of the
     //! speed of the q < 0 (stats[0..1]) and q > 0 (stats[2..3]) particles.

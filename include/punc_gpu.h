@@ -120,6 +120,9 @@ void punc_gpu_pop_destroy(punc_pop_t* p);
  * already registered).  Population<3> registers species on first use, as the
  * reference's particles carry q and m (population.h:74-94). */
 int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, int* index);
+/* The population's species: their number, and (when q / mass are non-NULL,
+ * nspecies entries each) charge and mass in index order. */
+int punc_gpu_pop_species(punc_pop_t* p, int* nspecies, double* q, double* mass);
 /* add_particles(xs, vs, q, m) (population.h:411-435): locates each point and
  * appends it as species s; points outside the mesh are skipped (the reference
  * indexes out of bounds there) and *n_added tells how many were kept. */
@@ -145,6 +148,17 @@ int punc_gpu_pop_size(punc_pop_t* p, int64_t* n);
 int punc_gpu_pop_counts(punc_pop_t* p, int64_t* per_species);
 /* Download all particles: xs, vs n*3, species n, cells n (any may be NULL). */
 int punc_gpu_pop_get(punc_pop_t* p, double* xs, double* vs, int32_t* species, int64_t* cells);
+/* Population::save_file (population.h:807-839): every particle as the
+ * reference's 64-byte record x[3], v[3], q, m (binary != 0) or as the text line
+ * "%.17g\t" x 6, "%.17g\t %.17g\t", "\n"; in cell order (the population is
+ * re-sorted by cell first, as the reference writes cell by cell). */
+int punc_gpu_pop_save_file(punc_pop_t* p, const char* fname, int binary);
+/* Population::load_file (population.h:841-880): appends the records of a
+ * file in that format (text: the numbers of each line fill x, v, q, m in
+ * turn; a line that gives fewer keeps the previous line's values), locating
+ * each point and skipping those outside the mesh.  Every new (q, m) pair
+ * becomes a species (punc_gpu_pop_add_species).  n_added may be NULL. */
+int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int binary, int64_t* n_added);
 
 /* ---- the particle operators -------------------------------------------- */
 /* Upload the CG1 electric field, ndof*3 doubles (E_x,E_y,E_z per dof). */

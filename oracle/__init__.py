@@ -52,6 +52,8 @@ _SIGS = {
     "oracle_statistics": (None, [vp, d_p]),
     "oracle_flux": (None, [C.c_double, d_p, d_p, C.c_double, d_p, d_p]),
     "oracle_set_injector": (None, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
+    "oracle_save_file": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "oracle_load_file": (C.c_int64, [vp, C.c_char_p, C.c_int]),
     "oracle_set_injector_vdf": (None, [vp, C.c_int, C.c_int, C.c_double, d_p, C.c_double, C.c_double,
                                        C.c_double, C.c_double, d_p, d_p]),
     "oracle_vdf_flux": (None, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double, C.c_double, C.c_uint64,
@@ -288,6 +290,14 @@ class OraclePop:
         return int(lib().oracle_load_vdf(self._h, int(species), int(N), int(seed), float(amp), float(kw), int(axis),
                                          int(kind), float(vth), vd.ctypes.data_as(d_p), float(k), float(alpha),
                                          float(vrange)))
+
+    def save_file(self, fname, binary=True):
+        assert lib().oracle_save_file(self._h, str(fname).encode(), int(bool(binary))) == 0
+
+    def load_file(self, fname, binary=True):
+        n = int(lib().oracle_load_file(self._h, str(fname).encode(), int(bool(binary))))
+        assert n >= 0, f"cannot open {fname}"
+        return n
 
     def inject(self, dt, seed, step):
         return int(lib().oracle_inject(self._h, float(dt), int(seed), int(step)))

@@ -25,6 +25,7 @@
 //   particle_potential_energy_cg1      diagnostics.h:379-412
 //   density_dg0, density_cg1           diagnostics.h:432-465, 492-555
 //   Population::statistics             population.h:746-805
+//   Population::save_file / load_file  population.h:807-880
 //   inject_particles                   punc/include/punc/injector.h:157-232
 //   rejection_sampler, random_facet_points, create_flux  punc/src/injector.cpp:25-171
 //   load_particles                     injector.h:244-278
@@ -48,9 +49,13 @@
 //   * erfc_inv (Boost) is computed by safeguarded Newton iteration on erfc.
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <fstream>
+#include <sstream>
+#include <string>
 #include <array>
 #include <map>
 #include <vector>
@@ -1227,6 +1232,97 @@ void oracle_statistics(void* h, double* stats) {
     }
     if (m > 0) stats[1] = sqrt(stats[1] / (m - 1));
     if (n > 0) stats[3] = sqrt(stats[3] / (n - 1));
+}
+
+// Population::save_file (population.h:807-839): cell by cell, each particle
+// as x[3], v[3], q, m - 64 raw bytes, or one text line.
+int oracle_save_file(void* h, const char* fname, int binary) {
+    Pop& P = *(Pop*)h;
+    FILE* fout = fopen(fname, binary ? "wb" : "w");
+    if (!fout) return -1;
+    for (auto& cell : P.cells) {
+        for (auto& particle : cell.particles) {
+            if (binary) {
+                double rec[8];
+                for (int i = 0; i < 3; ++i) {
+                    rec[i] = particle.x[i];
+                    rec[3 + i] = particle.v[i];
+                }
+                rec[6] = particle.q;
+                rec[7] = particle.m;
+                fwrite(rec, sizeof(rec), 1, fout);
+            } else {
+                for (int i = 0; i < 3; ++i) fprintf(fout, "%.17g\t", particle.x[i]);
+                for (int i = 0; i < 3; ++i) fprintf(fout, "%.17g\t", particle.v[i]);
+                fprintf(fout, "%.17g\t %.17g\t", particle.q, particle.m);
+                fprintf(fout, "\n");
+            }
+        }
+    }
+    fclose(fout);
+    return 0;
+}
+
+// Species of a (q, m) pair: the first with both equal, else a new one.
+static int species_of(Pop& P, double q, double m) {
+    for (size_t s = 0; s < P.species.size(); ++s)
+        if (P.species[s].q == q && P.species[s].m == m) return (int)s;
+    Species sp;
+    sp.q = q;
+    sp.m = m;
+    P.species.push_back(sp);
+    return (int)P.species.size() - 1;
+}
+
+static bool add_one(Pop& P, const double* x, const double* v, double q, double m) {
+    int64_t cell_id = locate(P, x);
+    if (cell_id < 0) return false;  // outside the mesh: skipped
+    Particle p;
+    for (int j = 0; j < 3; ++j) {
+        p.x[j] = x[j];
+        p.v[j] = v[j];
+    }
+    p.q = q;
+    p.m = m;
+    p.species = species_of(P, q, m);
+    P.cells[cell_id].particles.push_back(p);
+    return true;
+}
+
+// Population::load_file (population.h:841-880): whole 64-byte records, or
+// text lines whose numbers fill x, v, q, m in turn (fields a line lacks keep
+// the previous line's values).  Returns the number of particles added, -1 if
+// the file cannot be opened.
+int64_t oracle_load_file(void* h, const char* fname, int binary) {
+    Pop& P = *(Pop*)h;
+    int64_t added = 0;
+    if (binary) {
+        FILE* fin = fopen(fname, "rb");
+        if (!fin) return -1;
+        double rec[8];
+        while (fread(rec, sizeof(rec), 1, fin)) added += add_one(P, rec, rec + 3, rec[6], rec[7]);
+        fclose(fin);
+        return added;
+    }
+    std::ifstream in(fname);
+    if (!in) return -1;
+    std::string line;
+    double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    double q = 0, m = 0;
+    while (std::getline(in, line)) {
+        double value;
+        std::stringstream ss(line);
+        size_t i = 0;
+        while (ss >> value) {
+            if (i < 3) x[i] = value;
+            else if (i < 6) v[i % 3] = value;
+            else if (i == 6) q = value;
+            else if (i == 7) m = value;
+            ++i;
+        }
+        added += add_one(P, x, v, q, m);
+    }
+    return added;
 }
 
 }  // extern "C"

@@ -54,6 +54,9 @@ _SIGS = {
     "punc_gpu_pop_size": (C.c_int, [vp, i64_p]),
     "punc_gpu_pop_counts": (C.c_int, [vp, i64_p]),
     "punc_gpu_pop_get": (C.c_int, [vp, d_p, d_p, i32_p, i64_p]),
+    "punc_gpu_pop_save_file": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "punc_gpu_pop_species": (C.c_int, [vp, i32_p, d_p, d_p]),
+    "punc_gpu_pop_load_file": (C.c_int, [vp, C.c_char_p, C.c_int, i64_p]),
     "punc_gpu_set_field": (C.c_int, [vp, d_p]),
     "punc_gpu_accel": (C.c_int, [vp, C.c_double, d_p]),
     "punc_gpu_boris": (C.c_int, [vp, d_p, C.c_double, d_p]),

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -642,6 +644,16 @@ extern "C" int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, in
     return 0;
 }
 
+extern "C" int punc_gpu_pop_species(punc_pop_t* p, int* nspecies, double* q, double* mass) {
+    if (!p || !nspecies) return fail("pop_species: null argument");
+    *nspecies = p->nsp;
+    for (int s = 0; s < p->nsp; ++s) {
+        if (q) q[s] = p->q[s];
+        if (mass) mass[s] = p->m[s];
+    }
+    return 0;
+}
+
 static int read_state(punc_pop_s* P, DevState& s) {
     HIPCHK(hipMemcpyAsync(&s, P->d_state, sizeof(DevState), hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
@@ -662,6 +674,7 @@ static int check_errors(punc_pop_s* P, const char* what) {
                     std::to_string(s.overflow) + " particle(s) dropped)");
     if (errs[0] || errs[1])
         return fail(std::string(what) + ": rejection sampler did not converge");
+    if (errs[2]) return fail(std::string(what) + ": file records whose (q, m) matched no species");
     return 0;
 }
 
@@ -1090,6 +1103,197 @@ extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
     HIPCHK(hipMemcpyAsync(KE, p->d_ke + 1, 8, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Population files (Population::save_file / load_file, population.h:807-880)
+// ---------------------------------------------------------------------------
+constexpr int64_t FILE_CHUNK = int64_t(1) << 22;  // records per device round trip (256 MB)
+
+// One text line per record: "%.17g\t" for x and v, "%.17g\t %.17g\t" for
+// q and m, then "\n" (population.h:822-835).  Formatted by up to 16 host
+// threads, written in record order.
+static bool write_text_records(FILE* f, const double* rec, int64_t cnt) {
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                                (cnt + 4095) / 4096}));
+    std::vector<std::string> out((size_t)nt);
+    auto work = [&](int t) {
+        const int64_t a = cnt * t / nt, b = cnt * (t + 1) / nt;
+        std::string& o = out[t];
+        o.reserve((size_t)(b - a) * 200);
+        char buf[64];
+        for (int64_t k = a; k < b; ++k) {
+            const double* r = rec + 8 * k;
+            for (int i = 0; i < 6; ++i) {
+                const int len = snprintf(buf, sizeof(buf), "%.17g\t", r[i]);
+                o.append(buf, (size_t)len);
+            }
+            const int len = snprintf(buf, sizeof(buf), "%.17g\t %.17g\t", r[6], r[7]);
+            o.append(buf, (size_t)len);
+            o.push_back('\n');
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    for (auto& o : out)
+        if (fwrite(o.data(), 1, o.size(), f) != o.size()) return false;
+    return true;
+}
+
+extern "C" int punc_gpu_pop_save_file(punc_pop_t* p, const char* fname, int binary) {
+    if (!p || !fname) return fail("pop_save_file: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    // cell order, as the reference writes cell by cell (a re-sort, applied now)
+    int r = punc_gpu_sort(p);
+    if (r) return r;
+    r = ensure_order(p);
+    if (r) return r;
+    const SpCoef* coef;
+    r = get_coef(p, 0, 0.0, nullptr, &coef);
+    if (r) return r;
+    DevState s;
+    r = read_state(p, s);
+    if (r) return r;
+    FILE* f = fopen(fname, binary ? "wb" : "w");
+    if (!f) return fail(std::string("pop_save_file: cannot open ") + fname);
+    const int64_t chunk = std::min<int64_t>(FILE_CHUNK, std::max<int64_t>(s.n, 1));
+    double* d_rec = nullptr;
+    hipError_t e = dalloc(&d_rec, (size_t)chunk * 8);
+    std::vector<double> h;
+    bool io_ok = true;
+    if (e == hipSuccess) h.resize((size_t)chunk * 8);
+    for (int64_t first = 0; e == hipSuccess && io_ok && first < s.n; first += chunk) {
+        const int64_t cnt = std::min(chunk, s.n - first);
+        e = launch_pack_records(p->d_state, coef, p->cell_bits, first, cnt, d_rec, p->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_rec, cnt * 64, hipMemcpyDeviceToHost, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) break;
+        io_ok = binary ? fwrite(h.data(), 64, (size_t)cnt, f) == (size_t)cnt : write_text_records(f, h.data(), cnt);
+    }
+    hipFree(d_rec);
+    io_ok = (fclose(f) == 0) && io_ok;
+    if (e != hipSuccess) return hipfail(e, "pop_save_file");
+    if (!io_ok) return fail(std::string("pop_save_file: write error on ") + fname);
+    return 0;
+}
+
+// Text records as the reference reads them (population.h:855-878): the
+// numbers of each line fill x, v, q, m in turn; fields a line does not give
+// keep the previous line's values (zeros before the first), numbers after
+// the eighth are ignored, and every line - an empty one too - is a particle.
+static void parse_text_line(const char* s, const char* end, double cur[8]) {
+    int i = 0;
+    while (s < end) {
+        while (s < end && (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\v' || *s == '\f')) ++s;
+        if (s >= end) break;
+        char* stop = nullptr;
+        const double v = strtod(s, &stop);
+        if (stop == s) break;  // extraction fails: the rest of the line is ignored
+        if (i < 8) cur[i] = v;
+        ++i;
+        s = stop;
+    }
+}
+
+extern "C" int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int binary, int64_t* n_added) {
+    if (!p || !fname) return fail("pop_load_file: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    int r = ensure_order(p);
+    if (r) return r;
+    FILE* f = fopen(fname, binary ? "rb" : "r");
+    if (!f) return fail(std::string("pop_load_file: cannot open ") + fname);
+    const int64_t chunk = FILE_CHUNK;
+    double *d_rec = nullptr, *d_cx = nullptr, *d_sp = nullptr;
+    int32_t* d_ccs = nullptr;
+    int64_t *d_keep = nullptr, *d_pos = nullptr, *d_bsum = nullptr, *d_tot = nullptr;
+    char* lbuf = nullptr;  // getline's line buffer
+    size_t lcap = 0;
+    auto cleanup = [&]() {
+        fclose(f);
+        free(lbuf);
+        hipFree(d_rec);
+        hipFree(d_cx);
+        hipFree(d_sp);
+        hipFree(d_ccs);
+        hipFree(d_keep);
+        hipFree(d_pos);
+        hipFree(d_bsum);
+        hipFree(d_tot);
+    };
+    hipError_t e = dalloc(&d_rec, (size_t)chunk * 8);
+    if (e == hipSuccess) e = dalloc(&d_cx, (size_t)chunk * 6);
+    if (e == hipSuccess) e = dalloc(&d_sp, 2 * (size_t)p->max_sp);
+    if (e == hipSuccess) e = dalloc(&d_ccs, (size_t)chunk);
+    if (e == hipSuccess) e = dalloc(&d_keep, (size_t)chunk);
+    if (e == hipSuccess) e = dalloc(&d_pos, (size_t)chunk);
+    if (e == hipSuccess) e = dalloc(&d_bsum, (size_t)(chunk / 1024 + 2));
+    if (e == hipSuccess) e = dalloc(&d_tot, 1);
+    if (e != hipSuccess) {
+        cleanup();
+        return hipfail(e, "pop_load_file");
+    }
+    std::vector<double> h((size_t)chunk * 8);
+    double cur[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t added = 0;
+    bool eof = false;
+    while (!eof) {
+        int64_t cnt = 0;
+        if (binary) {
+            cnt = (int64_t)fread(h.data(), 64, (size_t)chunk, f);  // whole records only
+            eof = cnt < chunk;
+        } else {
+            while (cnt < chunk) {
+                const ssize_t len = getline(&lbuf, &lcap, f);
+                if (len < 0) {  // std::getline: no line left
+                    eof = true;
+                    break;
+                }
+                const char* end = lbuf + len;
+                if (len > 0 && end[-1] == '\n') --end;
+                parse_text_line(lbuf, end, cur);
+                memcpy(&h[8 * cnt], cur, 64);
+                ++cnt;
+            }
+        }
+        if (cnt == 0) break;
+        // register every (q, m) pair of the chunk as a species (first seen, first numbered)
+        for (int64_t k = 0; k < cnt; ++k) {
+            const double q = h[8 * k + 6], m = h[8 * k + 7];
+            bool known = false;
+            for (int sp = 0; sp < p->nsp && !known; ++sp) known = p->q[sp] == q && p->m[sp] == m;
+            if (!known) {
+                int idx;
+                r = punc_gpu_pop_add_species(p, q, m, &idx);
+                if (r) {
+                    cleanup();
+                    return r;
+                }
+            }
+        }
+        std::vector<double> sp((size_t)2 * p->nsp);
+        for (int j = 0; j < p->nsp; ++j) {
+            sp[j] = p->q[j];
+            sp[p->nsp + j] = p->m[j];
+        }
+        e = hipMemcpyAsync(d_sp, sp.data(), sp.size() * 8, hipMemcpyHostToDevice, p->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_rec, h.data(), cnt * 64, hipMemcpyHostToDevice, p->stream);
+        if (e == hipSuccess)
+            e = launch_unpack_append(p->mesh->dm, p->d_state, d_rec, cnt, d_sp, d_sp + p->nsp, p->nsp, p->cell_bits,
+                                     d_cx, d_ccs, d_keep, d_pos, d_bsum, d_tot, p->d_errs + 2, p->stream);
+        int64_t kept = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&kept, d_tot, 8, hipMemcpyDeviceToHost, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) {
+            cleanup();
+            return hipfail(e, "pop_load_file");
+        }
+        added += kept;
+    }
+    cleanup();
+    if (n_added) *n_added = added;
+    return check_errors(p, "pop_load_file");
 }
 
 // ---------------------------------------------------------------------------

@@ -138,4 +138,16 @@ hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, dou
                              hipStream_t st);
 hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream_t st);
 
+// Population files: pack particles [first, first+count) as 8-double records
+// (x, y, z, vx, vy, vz, q, m); unpack `count` records, locate them and append
+// the ones inside the mesh in record order (cx: 6*count, ccs, keep, pos:
+// count each, bsum: count/1024 + 2, total: 1; errs counts records whose
+// (q, m) matches no species).
+hipError_t launch_pack_records(const DevState* S, const SpCoef* coef, int cell_bits, int64_t first, int64_t count,
+                               double* rec, hipStream_t st);
+hipError_t launch_unpack_append(const DevMesh& M, DevState* S, const double* rec, int64_t count, const double* spq,
+                                const double* spm, int nsp, int cell_bits, double* cx, int32_t* ccs, int64_t* keep,
+                                int64_t* pos, int64_t* bsum, int64_t* total, unsigned long long* errs,
+                                hipStream_t st);
+
 }  // namespace punc_gpu

@@ -1683,6 +1683,81 @@ k_flux_mc(const double* __restrict__ fnormal, const double* __restrict__ farea, 
 }
 
 // ---------------------------------------------------------------------------
+// Population files (Population::save_file / load_file, population.h:807-880):
+// one record per particle, the reference's Particle<3> layout of 8 doubles
+// x[3], v[3], q, m (64 bytes, no padding).
+// ---------------------------------------------------------------------------
+__global__ void k_pack_records(const DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
+                               int64_t first, int64_t count, double* __restrict__ rec) {
+    const GArrays A = gview(S->cur);
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = first + k;
+        const SpCoef& C = coef[A.cs[i] >> cell_bits];
+        double2* r = (double2*)(rec + 8 * k);
+        r[0] = make_double2(A.x[i], A.y[i]);
+        r[1] = make_double2(A.z[i], A.vx[i]);
+        r[2] = make_double2(A.vy[i], A.vz[i]);
+        r[3] = make_double2(C.q, C.m);
+    }
+}
+
+// Records -> located candidates: species by exact (q, m) match (the host has
+// registered every pair first), cell by the grid point location; points
+// outside the mesh are dropped (keep 0).
+__global__ void k_unpack_locate(const DevMesh M, const double* __restrict__ rec, int64_t count,
+                                const double* __restrict__ spq, const double* __restrict__ spm, int nsp,
+                                int cell_bits, int64_t cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
+                                int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x) {
+        const double* r = rec + 8 * k;
+        int s = -1;
+        for (int j = 0; j < nsp && s < 0; ++j)
+            if (spq[j] == r[6] && spm[j] == r[7]) s = j;
+        int32_t c = -1;
+        if (s >= 0)
+            c = locate_in_grid(M.planes, M.bstart, M.bcells, M.gn, M.glo, M.gh, r[0], r[1], r[2], 0, (int32_t)M.nc);
+        else
+            atomicAdd(errs, 1ull);
+        for (int d = 0; d < 6; ++d) cx[d * cap + k] = r[d];
+        ccs[k] = c >= 0 ? ((s << cell_bits) | c) : -1;
+        keep[k] = c >= 0 ? 1 : 0;
+    }
+}
+
+// Appends the kept records at n + pos[k] in record order, then n += kept.
+__global__ void k_file_append(DevState* __restrict__ S, int64_t count, int64_t cap, const double* __restrict__ cx,
+                              const int32_t* __restrict__ ccs, const int64_t* __restrict__ pos) {
+    const GArrays A = gview(S->cur);
+    const int64_t n0 = S->n, pcap = S->capacity;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t cs = ccs[k];
+        if (cs < 0) continue;
+        const int64_t d = n0 + pos[k];
+        if (d >= pcap) continue;  // counted by k_file_finish
+        A.x[d] = cx[k];
+        A.y[d] = cx[cap + k];
+        A.z[d] = cx[2 * cap + k];
+        A.vx[d] = cx[3 * cap + k];
+        A.vy[d] = cx[4 * cap + k];
+        A.vz[d] = cx[5 * cap + k];
+        A.cs[d] = cs;
+    }
+}
+
+__global__ void k_file_finish(DevState* __restrict__ S, const int64_t* __restrict__ kept) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t room = S->capacity - S->n;
+        const int64_t k = *kept;
+        if (k > room) {
+            S->overflow += k - room;
+            S->n = S->capacity;
+        } else {
+            S->n += k;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 #define LAUNCH_CHECK() \
@@ -2042,6 +2117,34 @@ hipError_t launch_flux_mc(const double* fnormal, const double* farea, int64_t nf
     const int grid = (int)std::min<int64_t>(nf, 65536);
     hipLaunchKernelGGL(k_flux_mc, dim3(grid), dim3(256), 0, st, fnormal, farea, nf, F, seed, rng_sp, n_iter,
                        do_num ? 1 : 0, do_max ? 1 : 0, num, pmax);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+}  // namespace punc_gpu
+
+namespace punc_gpu {
+hipError_t launch_pack_records(const DevState* S, const SpCoef* coef, int cell_bits, int64_t first, int64_t count,
+                               double* rec, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_records, dim3(fill_grid(count)), dim3(256), 0, st, S, coef, cell_bits, first, count,
+                       rec);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_unpack_append(const DevMesh& M, DevState* S, const double* rec, int64_t count, const double* spq,
+                                const double* spm, int nsp, int cell_bits, double* cx, int32_t* ccs, int64_t* keep,
+                                int64_t* pos, int64_t* bsum, int64_t* total, unsigned long long* errs,
+                                hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack_locate, dim3(fill_grid(count)), dim3(256), 0, st, M, rec, count, spq, spm, nsp,
+                       cell_bits, count, cx, ccs, keep, errs);
+    LAUNCH_CHECK();
+    hipError_t e = launch_scan(keep, pos, count, bsum, total, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_file_append, dim3(fill_grid(count)), dim3(256), 0, st, S, count, count, cx, ccs, pos);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_file_finish, dim3(1), dim3(64), 0, st, S, total);
     LAUNCH_CHECK();
     return hipSuccess;
 }

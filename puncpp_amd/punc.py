@@ -386,6 +386,13 @@ class Population:
             N.lib().punc_gpu_pop_destroy(h)
             self._h = None
 
+    def _sync_species(self):
+        n = N.C.c_int32()
+        check(N.lib().punc_gpu_pop_species(self._h, N.C.byref(n), None, None))
+        q, m = np.empty(n.value), np.empty(n.value)
+        check(N.lib().punc_gpu_pop_species(self._h, N.C.byref(n), dptr(q), dptr(m)))
+        self.q, self.m, self.nspecies = q, m, n.value
+
     def species_index(self, q, m):
         for s in range(self.nspecies):
             if self.q[s] == q and self.m[s] == m:
@@ -455,6 +462,21 @@ class Population:
 
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
+
+    def save_file(self, fname, binary=True):
+        """Population::save_file (population.h:807-839): every particle as
+        x[3], v[3], q, m - 64-byte binary records or "%.17g" text lines - in
+        cell order (the population is re-sorted by cell first)."""
+        check(N.lib().punc_gpu_pop_save_file(self._h, str(fname).encode(), int(bool(binary))))
+
+    def load_file(self, fname, binary=True):
+        """Population::load_file (population.h:841-880): appends the file's
+        particles (those inside the mesh), registering a species for every
+        new (q, m).  Returns the number added."""
+        k = N.C.c_int64()
+        check(N.lib().punc_gpu_pop_load_file(self._h, str(fname).encode(), int(bool(binary)), N.C.byref(k)))
+        self._sync_species()
+        return k.value
 
     def statistics(self, stats=None):
         """Population::statistics (population.h:746-805): [mean, std] of the
@@ -528,6 +550,133 @@ def distribute_cg1(pop, dv_inv, rho=None):
     dv = np.ascontiguousarray(dv_inv, dtype=np.float64)
     check(N.lib().punc_gpu_distribute(pop._h, dptr(dv), dptr(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Run state and history files (diagnostics.h:39-117, diagnostics.cpp:35-126)
+# ---------------------------------------------------------------------------
+def _c_strtol(s, i):
+    """C strtol(s + i, &end, 10): (value, end index); no number -> (0, i)."""
+    j = i
+    while j < len(s) and s[j] in " \t\n\v\f\r":
+        j += 1
+    k = j
+    if k < len(s) and s[k] in "+-":
+        k += 1
+    d = k
+    while d < len(s) and s[d].isdigit():
+        d += 1
+    if d == k:
+        return 0, i
+    return int(s[j:d]), d
+
+
+def _c_strtod(s, i):
+    """C strtod(s + i, &end) for decimal numbers, inf and nan: (value, end)."""
+    import re
+    m = re.compile(r"[ \t\n\v\f\r]*([+-]?(?:inf(?:inity)?|nan|(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?))",
+                   re.IGNORECASE).match(s, i)
+    if not m:
+        return 0.0, i
+    return float(m.group(1)), m.end()
+
+
+def _c_nan(x):
+    # glibc prints a NaN with its sign ("-nan" for the x86 default NaN)
+    return "-nan" if math.copysign(1.0, x) < 0 else "nan"
+
+
+def _fmt_default(x):
+    """A double through an ostream with default flags (precision 6, %g)."""
+    return _c_nan(x) if math.isnan(x) else "%g" % x
+
+
+def _fmt_sci16(x):
+    """A double through an ostream set to std::scientific, precision 16."""
+    return _c_nan(x) if math.isnan(x) else "%.16e" % x
+
+
+class State:
+    """Run state for restarts (reference State, diagnostics.h:39-71,
+    diagnostics.cpp:35-68): one line "n+1<TAB>t<TAB>" followed, per object,
+    by "charge<TAB>current<NEWLINE>".  load reads the first line only, so
+    objects after the first read 0 (as the reference does)."""
+
+    def __init__(self, fname):
+        self.fname = str(fname)
+
+    def load(self, objects=()):
+        with open(self.fname) as f:
+            line = f.readline()
+        line = line[:-1] if line.endswith("\n") else line
+        n, i = _c_strtol(line, 0)
+        t, i = _c_strtod(line, i)
+        for o in objects:
+            o.charge, i = _c_strtod(line, i)
+            o.current, i = _c_strtod(line, i)
+        return n, t
+
+    def save(self, n, t, objects=()):
+        with open(self.fname, "w") as f:
+            f.write(f"{int(n) + 1}\t{_fmt_default(t)}\t")
+            for o in objects:
+                f.write(f"{_fmt_default(o.charge)}\t")
+                f.write(f"{_fmt_default(o.current)}\n")
+
+
+class History:
+    """Time-series file (reference History, diagnostics.h:73-170,
+    diagnostics.cpp:70-126): a four-line header (unless continuing), then per
+    save one line: n, then t, ne, ni, KE, PE, per object V, I, Q, and with
+    stats the speed statistics, each as "%.16e"."""
+
+    def __init__(self, fname, objects=(), dim=3, stats=False, continue_simulation=False, hex_output=False):
+        self.dim = dim
+        self.stats = stats
+        self.hex_output = hex_output  # accepted; the reference writes decimal either way
+        self._f = open(str(fname), "a" if continue_simulation else "w")
+        if not continue_simulation:
+            f = self._f
+            nobj = len(objects)
+            f.write("#:xaxis\tt\n")
+            f.write("#:name\tn\tt\tne\tni\tKE\tPE")
+            for i in range(nobj):
+                f.write(f"\tV[{i}]\tI[{i}]\tQ[{i}]")
+            if stats:
+                f.write("\tmean_e\tstdev_e\tmean_i\tstdev_i")
+            f.write("\n")
+            f.write("#:long\ttimestep\ttime\t\"number of electrons\"\t")
+            f.write("\"number of ions\"\t\"kinetic energy\"\t")
+            f.write("\"potential energy\"")
+            for _ in range(nobj):
+                f.write("\tvoltage\tcurrent\tcharge")
+            if stats:
+                f.write("\tmean velocity of electrons\tstandard deviation of electron velocities"
+                        "\tmean velocity of ions\tstandard deviation of ion velocities")
+            f.write("\n")
+            f.write("#:units\t1\ts\tm**(-3)\tm**(-3)\tJ\tJ")
+            for _ in range(nobj):
+                f.write("\tV")
+                f.write({1: "\tA/m**2", 2: "\tA/m", 3: "\tA"}.get(dim, ""))
+                f.write("\tC")
+            if stats:
+                f.write("\tm/s" * 4)
+            f.write("\n")
+            f.flush()
+
+    def save(self, n, t, num_e, num_i, KE, PE, objects=(), pop=None):
+        e = _fmt_sci16
+        parts = [f"{int(n)}\t" + "\t".join(e(v) for v in (t, num_e, num_i, KE, PE))]
+        for o in objects:
+            parts.append(f"\t{e(o.get_potential())}\t{e(o.current)}\t{e(o.charge)}")
+        if self.stats:
+            st = pop.statistics(np.zeros(4))
+            parts.append("".join(f"\t{e(v)}" for v in st))
+        self._f.write("".join(parts) + "\n")
+        self._f.flush()
+
+    def close(self):
+        self._f.close()
 
 
 def kinetic_energy(pop):
