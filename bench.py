@@ -165,6 +165,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
             "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
     diag = run_diagnostics(pop, mesh, dv)
     flux_mc = run_flux_mc(mesh)
+    field = run_field_solve(pop, mesh)
     elapsed = t2 - t1
     n_avg = 0.5 * (n_start + n_end)
     value = n_avg * steps / elapsed
@@ -177,7 +178,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
         "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
-        "diag": diag, "flux_mc": flux_mc,
+        "diag": diag, "flux_mc": flux_mc, "field": field,
     }
 
 
@@ -214,6 +215,24 @@ def run_flux_mc(mesh, n_iter=500000):
         f"({samples / ms / 1e6:.3g} G samples/s)")
     return {"facets": ne, "samples": samples, "ms": ms, "samples_per_s": samples / (ms * 1e-3),
             "distribution": "KappaCairns kappa=4 alpha=0.3 drift (0.2,0,0.5) range 15"}
+
+
+def run_field_solve(pop, mesh, reps=3):
+    """Device field solve from the population's deposit: Poisson (grounded
+    exterior, the sphere at 0 V) then the E projection, the result left in
+    the population's field.  Zero initial guess every call, as the
+    reference's Krylov solver starts."""
+    import puncpp_amd as P
+    f = P.FieldSolver(mesh, eps0=1.0, exterior=True, n_objects=1)
+    f.solve_population(pop, object_potentials=[0.0], copy_out=False)
+    t = time.perf_counter()
+    for _ in range(reps):
+        f.solve_population(pop, object_potentials=[0.0], copy_out=False)
+    ms = 1e3 * (time.perf_counter() - t) / reps
+    log(f"[bench] field solve {ms:.2f} ms (Poisson {f.iterations[0]} + E {f.iterations[1]} CG iterations, "
+        f"{mesh.ndof} dofs)")
+    return {"ms": ms, "dofs": mesh.ndof, "poisson_iterations": f.iterations[0],
+            "efield_iterations": f.iterations[1], "preconditioner": "Jacobi", "rtol": 1e-12}
 
 
 def run_diagnostics(pop, mesh, dv, reps=5):
@@ -284,6 +303,19 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     t2 = time.perf_counter()
     v = ps / (t2 - t1)
     log(f"[bench] oracle {ps} particle-steps in {t2 - t1:.2f}s -> {v:.4g}/s")
+    # field solve: the oracle's assembly + sparse direct solves (Poisson and
+    # the E projection) on a 16^3-cube sphere mesh (direct factorisations of
+    # the larger meshes take minutes)
+    from oracle.field import FieldOracle
+    import puncpp_amd as P
+    hmf = P.box_mesh(16, [-10.0] * 3, [10.0] * 3, sphere_r=1.0)
+    of = O.OraclePop(hmf, [1.0], [1.0])
+    fo = FieldOracle(hmf, 1.0, of.localizer()["adj"])
+    rho_f = np.random.default_rng(0).normal(size=hmf.ndof)
+    t5 = time.perf_counter()
+    fo.efield(fo.poisson(rho_f, exterior=True, object_potentials=[0.0]))
+    field_s = time.perf_counter() - t5
+    log(f"[bench] oracle field solve {field_s:.2f} s ({hmf.ndof} dofs)")
     # create_flux Monte Carlo (KappaCairns: both tables sampled) on a sample
     # of facets: a 2^3-cube box has 48 exterior facets
     ob = O.OraclePop(P_box_mesh(2), [-1.0], [1.0])
@@ -295,6 +327,7 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     log(f"[bench] oracle flux Monte Carlo {flux_rate:.4g} samples/s")
     return {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "flux_mc_samples_per_s": flux_rate,
+            "field_solve_s_16cube_mesh": field_s, "field_solve_dofs": hmf.ndof,
             "sample": f"oracle (serial C++ restatement, g++ -O2) on the same mesh with "
                       f"{2 * sample_per_species} particles, {steps} steps of boris+move+update+inject+distribute; "
                       f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cores visible"}
@@ -496,6 +529,7 @@ def main():
         "host_transfer": res["xfer"],
         "diagnostics": res["diag"],
         "flux_monte_carlo": res["flux_mc"],
+        "field_solve": res["field"],
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -6,9 +6,11 @@
 #include <punc/diagnostics.h>
 #include <punc/distributions.h>
 #include <punc/distributor.h>
+#include <punc/efield.h>
 #include <punc/injector.h>
 #include <punc/mesh.h>
 #include <punc/object.h>
+#include <punc/poisson.h>
 #include <punc/population.h>
 #include <punc/pusher.h>
 

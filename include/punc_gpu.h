@@ -215,6 +215,35 @@ int punc_gpu_density_ema(punc_pop_t* p, double dt, double tau, int reset, double
  * arbitrary).  Enqueued on the population's stream; returns without waiting. */
 int punc_gpu_sort(punc_pop_t* p);
 
+/* ---- field solve -------------------------------------------------------- */
+/* PoissonSolver (poisson.cpp:260-376, Potential3D.ufl: eps0 grad(phi).grad(v)
+ * dx = rho v dx) and ESolver (efield.cpp:41-75, EField3D.ufl: the vector-CG1
+ * L2 projection of -grad(phi)) on the device.  Dirichlet conditions: the
+ * exterior boundary (boundary id 1) when dirichlet_exterior, and objects
+ * 0..n_objects-1 (boundary ids 2..) at given potentials; remove_null_space
+ * for problems without Dirichlet conditions (periodic boxes): the
+ * right-hand side is made orthogonal to the constant vector and the
+ * potential has zero mean.  Jacobi-preconditioned conjugate gradients to the
+ * reference's tolerances (relative 1e-12, absolute 1e-14). */
+typedef struct punc_field_s punc_field_t;
+int punc_gpu_field_create(punc_mesh_t* m, double eps0, int dirichlet_exterior, int n_objects, int remove_null_space,
+                          punc_field_t** out);
+void punc_gpu_field_destroy(punc_field_t* f);
+/* Iteration limit per solve (default 20000; failing to converge is an error). */
+int punc_gpu_field_set_max_iterations(punc_field_t* f, int maxit);
+/* rho (ndof) -> phi (ndof).  exterior_values: one value per dof, read at the
+ * exterior Dirichlet dofs (NULL: 0); object_potentials: n_objects values
+ * (NULL: 0).  iterations may be NULL. */
+int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* exterior_values,
+                           const double* object_potentials, double* phi, int* iterations);
+/* phi (ndof) -> E (3*ndof, dof-major x, y, z). */
+int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double* E, int* iterations);
+/* The population's last deposited rho -> phi -> E, written straight into the
+ * population's field for the next push (no host round trip); phi and E are
+ * copied out when non-NULL; iterations[0..1] (may be NULL): Poisson and E. */
+int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
+                             const double* object_potentials, double* phi, double* E, int* iterations);
+
 /* ---- injection --------------------------------------------------------- */
 /* Per species Maxwellian flux source: thermal speed, drift, density n
  * (Species::n) and rejection box half width in units of vth (Pdf::vdf_range). */

@@ -72,6 +72,12 @@ _SIGS = {
     "punc_gpu_rho_ema": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, d_p]),
     "punc_gpu_density_ema": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, d_p, d_p]),
     "punc_gpu_sort": (C.c_int, [vp]),
+    "punc_gpu_field_create": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    "punc_gpu_field_destroy": (None, [vp]),
+    "punc_gpu_field_set_max_iterations": (C.c_int, [vp, C.c_int]),
+    "punc_gpu_poisson_solve": (C.c_int, [vp, d_p, d_p, d_p, d_p, i32_p]),
+    "punc_gpu_efield_solve": (C.c_int, [vp, d_p, d_p, i32_p]),
+    "punc_gpu_field_solve_pop": (C.c_int, [vp, vp, d_p, d_p, d_p, d_p, i32_p]),
     "punc_gpu_injector_set": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
     "punc_gpu_species_instance": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int64,
                                             C.c_int64, C.c_int64]),

@@ -13,6 +13,7 @@
 
 #include "../../include/punc_gpu.h"
 #include "device.h"
+#include "field.h"
 #include "kernels.h"
 #include "mesh_host.h"
 
@@ -1102,6 +1103,98 @@ extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
     HIPCHK(launch_sum_partials(p->d_ke_part, p->grid, p->d_ke + 1, p->stream));
     HIPCHK(hipMemcpyAsync(KE, p->d_ke + 1, 8, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Field solve (PoissonSolver, poisson.cpp:260-376; ESolver, efield.cpp:41-75)
+// ---------------------------------------------------------------------------
+struct punc_field_s {
+    punc_mesh_s* mesh = nullptr;
+    FieldSolver* F = nullptr;
+    double* d_in = nullptr;  // host-given rho or phi staged on the device
+};
+
+extern "C" int punc_gpu_field_create(punc_mesh_t* m, double eps0, int dirichlet_exterior, int n_objects,
+                                     int remove_null_space, punc_field_t** out) {
+    if (!m || !out) return fail("field_create: null argument");
+    HIPCHK(hipSetDevice(m->device));
+    std::string err;
+    FieldSolver* F = field_create(m->host, m->topo, eps0, dirichlet_exterior != 0, n_objects, remove_null_space != 0,
+                                  m->device, nullptr, err);
+    if (!F) return fail("field_create: " + err);
+    punc_field_s* h = new punc_field_s();
+    h->mesh = m;
+    h->F = F;
+    hipError_t e = dalloc(&h->d_in, (size_t)m->topo.ndof);
+    if (e != hipSuccess) {
+        field_destroy(F);
+        delete h;
+        return hipfail(e, "field_create");
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" void punc_gpu_field_destroy(punc_field_t* f) {
+    if (!f) return;
+    field_destroy(f->F);
+    hipFree(f->d_in);
+    delete f;
+}
+
+extern "C" int punc_gpu_field_set_max_iterations(punc_field_t* f, int maxit) {
+    if (!f || maxit < 1) return fail("field_set_max_iterations: bad argument");
+    field_set_max_iterations(f->F, maxit);
+    return 0;
+}
+
+extern "C" int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* exterior_values,
+                                      const double* object_potentials, double* phi, int* iterations) {
+    if (!f || !rho || !phi) return fail("poisson_solve: null argument");
+    const int64_t n = f->mesh->topo.ndof;
+    HIPCHK(hipMemcpy(f->d_in, rho, n * 8, hipMemcpyHostToDevice));
+    std::string err;
+    if (!field_poisson(f->F, f->d_in, exterior_values, object_potentials, iterations, err))
+        return fail("poisson_solve: " + err);
+    HIPCHK(hipMemcpy(phi, field_phi(f->F), n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double* E, int* iterations) {
+    if (!f || !phi || !E) return fail("efield_solve: null argument");
+    const int64_t n = f->mesh->topo.ndof;
+    HIPCHK(hipMemcpy(f->d_in, phi, n * 8, hipMemcpyHostToDevice));
+    std::string err;
+    if (!field_efield(f->F, f->d_in, iterations, err)) return fail("efield_solve: " + err);
+    HIPCHK(hipMemcpy(E, field_E(f->F), n * 24, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
+                                        const double* object_potentials, double* phi, double* E, int* iterations) {
+    if (!f || !p) return fail("field_solve_pop: null argument");
+    if (p->mesh != f->mesh) return fail("field_solve_pop: the population lives on another mesh");
+    HIPCHK(hipSetDevice(p->device));
+    const int64_t n = f->mesh->topo.ndof;
+    // the population's deposited charge density (its stream's work first)
+    HIPCHK(hipStreamSynchronize(p->stream));
+    std::string err;
+    int it_phi = 0, it_e = 0;
+    if (!field_poisson(f->F, p->d_rho, exterior_values, object_potentials, &it_phi, err))
+        return fail("field_solve_pop: " + err);
+    if (!field_efield(f->F, field_phi(f->F), &it_e, err)) return fail("field_solve_pop: " + err);
+    // the field the next push reads: packed dofs and the per-cell copy
+    HIPCHK(hipMemcpy(p->d_E, field_E(f->F), n * 24, hipMemcpyDeviceToDevice));
+    HIPCHK(launch_cell_field(p->mesh->dm, p->d_E, p->d_cellE, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->field_set = true;
+    if (phi) HIPCHK(hipMemcpy(phi, field_phi(f->F), n * 8, hipMemcpyDeviceToHost));
+    if (E) HIPCHK(hipMemcpy(E, field_E(f->F), n * 24, hipMemcpyDeviceToHost));
+    if (iterations) {
+        iterations[0] = it_phi;
+        iterations[1] = it_e;
+    }
     return 0;
 }
 

@@ -1,0 +1,41 @@
+// Field solve on the device: the CG1 Poisson problem of PoissonSolver
+// (reference poisson.cpp:260-376, Potential3D.ufl) and the vector-CG1 L2
+// projection E = -grad(phi) of ESolver (efield.cpp:41-75, EField3D.ufl).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mesh_host.h"
+
+namespace punc_gpu {
+
+struct FieldSolver;
+
+// Dirichlet conditions: the exterior boundary (boundary id 1) when
+// dirichlet_exterior, and the boundaries of objects 0..n_objects-1 (ids 2..).
+// remove_null_space: the constant vector is the null space (no Dirichlet
+// condition at all, e.g. a periodic box); the right-hand side is
+// orthogonalised against it and the solution has zero mean.
+FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, bool dirichlet_exterior, int n_objects,
+                          bool remove_null_space, int device, hipStream_t stream, std::string& err);
+void field_destroy(FieldSolver* F);
+
+int64_t field_ndof(const FieldSolver* F);
+int field_num_objects(const FieldSolver* F);
+double* field_phi(FieldSolver* F);  // device, ndof
+double* field_E(FieldSolver* F);    // device, 3*ndof, dof-major (x, y, z)
+
+// Poisson: d_rho (device, ndof) -> field_phi.  ext_values: host, one value
+// per dof, read at the exterior Dirichlet dofs (NULL: 0); object_potentials:
+// host, n_objects values (NULL: 0).  Solves to |r| <= max(rtol |b|, atol)
+// with the reference's tolerances (rtol 1e-12, atol 1e-14).
+bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values, const double* object_potentials,
+                   int* iterations, std::string& err);
+// E projection of d_phi (device, ndof) into field_E.
+bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::string& err);
+// Iteration limit of every solve (default 20000).
+void field_set_max_iterations(FieldSolver* F, int maxit);
+
+}  // namespace punc_gpu

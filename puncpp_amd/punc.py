@@ -553,6 +553,77 @@ def distribute_cg1(pop, dv_inv, rho=None):
 
 
 # ---------------------------------------------------------------------------
+# Field solve (PoissonSolver, poisson.cpp:260-376; ESolver, efield.cpp:41-75)
+# ---------------------------------------------------------------------------
+class FieldSolver:
+    """CG1 Poisson solve eps0 grad(phi).grad(v) dx = rho v dx and the vector
+    CG1 projection E = -grad(phi), on the device.  exterior: Dirichlet on the
+    exterior boundary (id 1); n_objects: objects (ids 2..) held at given
+    potentials; remove_null_space: no Dirichlet condition (periodic box), the
+    potential has zero mean.  Tolerances as the reference's Krylov solvers
+    (relative 1e-12, absolute 1e-14)."""
+
+    def __init__(self, mesh, eps0=1.0, exterior=True, n_objects=0, remove_null_space=False):
+        self.mesh = mesh
+        self.n_objects = int(n_objects)
+        h = N.vp()
+        check(N.lib().punc_gpu_field_create(mesh._h, float(eps0), int(bool(exterior)), self.n_objects,
+                                            int(bool(remove_null_space)), N.C.byref(h)))
+        self._h = h
+        self.iterations = (0, 0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().punc_gpu_field_destroy(h)
+            self._h = None
+
+    def _bc(self, exterior_values, object_potentials):
+        ev = None if exterior_values is None else np.ascontiguousarray(exterior_values, dtype=np.float64)
+        if ev is not None and ev.shape != (self.mesh.ndof,):
+            raise ValueError("exterior_values must hold one value per dof")
+        op = None if object_potentials is None else np.ascontiguousarray(object_potentials, dtype=np.float64)
+        if op is not None and op.shape != (self.n_objects,):
+            raise ValueError(f"object_potentials must hold {self.n_objects} values")
+        return ev, op
+
+    def solve(self, rho, exterior_values=None, object_potentials=None):
+        """PoissonSolver::solve: rho (ndof) -> phi (ndof)."""
+        ev, op = self._bc(exterior_values, object_potentials)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        phi = np.empty(self.mesh.ndof)
+        it = N.C.c_int32()
+        check(N.lib().punc_gpu_poisson_solve(self._h, dptr(rho), dptr(ev) if ev is not None else None,
+                                             dptr(op) if op is not None else None, dptr(phi), N.C.byref(it)))
+        self.iterations = (it.value, 0)
+        return phi
+
+    def efield(self, phi):
+        """ESolver::solve: phi (ndof) -> E (ndof x 3)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        E = np.empty((self.mesh.ndof, 3))
+        it = N.C.c_int32()
+        check(N.lib().punc_gpu_efield_solve(self._h, dptr(phi), dptr(E), N.C.byref(it)))
+        self.iterations = (0, it.value)
+        return E
+
+    def solve_population(self, pop, exterior_values=None, object_potentials=None, copy_out=True):
+        """The population's deposited rho -> phi -> E, left in the
+        population's field for its next push.  Returns (phi, E) when
+        copy_out."""
+        ev, op = self._bc(exterior_values, object_potentials)
+        phi = np.empty(self.mesh.ndof) if copy_out else None
+        E = np.empty((self.mesh.ndof, 3)) if copy_out else None
+        its = (N.C.c_int32 * 2)()
+        check(N.lib().punc_gpu_field_solve_pop(self._h, pop._h, dptr(ev) if ev is not None else None,
+                                               dptr(op) if op is not None else None,
+                                               dptr(phi) if copy_out else None, dptr(E) if copy_out else None,
+                                               its))
+        self.iterations = (its[0], its[1])
+        return (phi, E) if copy_out else None
+
+
+# ---------------------------------------------------------------------------
 # Run state and history files (diagnostics.h:39-117, diagnostics.cpp:35-126)
 # ---------------------------------------------------------------------------
 def _c_strtol(s, i):

@@ -1,7 +1,8 @@
 // The reference time loop (interaction/interaction.cpp:342-423) written
-// against the drop-in C++ API (include/punc.h): distribute, [host field solve
-// replaced by a fixed field], count, push (half step at n == 0), move,
-// update, inject.  Prints one line per step and checks particle-number
+// against the drop-in C++ API (include/punc.h): distribute, field solve on
+// the device (Poisson with a grounded exterior and the object at 0 V, then
+// the E projection), count, push (half step at n == 0), move, update,
+// inject.  Prints one line per step and checks particle-number
 // bookkeeping: N(n+1) = N(n) - absorbed + injected, and the diagnostics'
 // conservation properties.
 #include <punc.h>
@@ -28,17 +29,27 @@ int main(int argc, char** argv) {
     pop.set_seed(42);
     load_particles(pop, species);
     auto dv_inv = element_volume(mesh);
-    Function rho(mesh, 1), E(mesh, 3), phi_one(mesh, 1), ne(mesh, 1), ni(mesh, 1), rho_avg(mesh, 1);
+    Function rho(mesh, 1), E(mesh, 3), phi(mesh, 1), phi_one(mesh, 1), ne(mesh, 1), ni(mesh, 1), rho_avg(mesh, 1);
     Function ne0(mesh, 1, true), ni0(mesh, 1, true);
     for (double& v : phi_one.values) v = 1.0;
-    for (int64_t d = 0; d < mesh.ndof; ++d) E.values[3 * d + 2] = 0.01;
     ObjectVector objects{std::make_shared<Object>(2)};
+    PoissonSolver poisson(mesh, objects, true, 1.0);
+    ESolver esolver(poisson);
     std::vector<double> B{0.0, 0.0, 1.0};
     const double dt = 0.05;
     size_t n_prev = pop.num_of_particles();
     int bad = 0;
     for (int n = 0; n <= steps; ++n) {
         distribute_cg1(pop, rho, dv_inv);
+        poisson.solve(phi, rho, objects);
+        esolver.solve(E, phi);
+        if (n == 0) {
+            // the device-resident path gives the same field
+            Function phi2(mesh, 1), E2(mesh, 3);
+            poisson.solve_population(pop, objects, &phi2, &E2);
+            for (int64_t d = 0; d < 3 * mesh.ndof; ++d)
+                if (std::fabs(E2.values[d] - E.values[d]) > 1e-9 * (1.0 + std::fabs(E.values[d]))) bad++;
+        }
         double qsum = 0.0;
         for (int64_t d = 0; d < mesh.ndof; ++d) qsum += rho.values[d] / dv_inv[d];
         size_t num_e = pop.num_of_negatives(), num_i = pop.num_of_positives();
