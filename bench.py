@@ -229,10 +229,26 @@ def run_field_solve(pop, mesh, reps=3):
     for _ in range(reps):
         f.solve_population(pop, object_potentials=[0.0], copy_out=False)
     ms = 1e3 * (time.perf_counter() - t) / reps
-    log(f"[bench] field solve {ms:.2f} ms (Poisson {f.iterations[0]} + E {f.iterations[1]} CG iterations, "
-        f"{mesh.ndof} dofs)")
-    return {"ms": ms, "dofs": mesh.ndof, "poisson_iterations": f.iterations[0],
-            "efield_iterations": f.iterations[1], "preconditioner": "Jacobi", "rtol": 1e-12}
+    its = f.iterations
+    log(f"[bench] field solve {ms:.2f} ms (Poisson {its[0]} + E {its[1]} CG iterations, {mesh.ndof} dofs)")
+    # warm start from the previous solution after one step of the particles:
+    # the time loop's case (the reference starts from zero)
+    f.set_warm_start(True)
+    from puncpp_amd import _native as N
+    lib = N.lib()
+    B = np.array([0.0, 0.0, 1.0])
+    wms = []
+    for k in range(reps):
+        N.check(lib.punc_gpu_step(pop._h, 2, 0.02, N.dptr(B), 0.02, None, 1, 0, 10000 + k))
+        N.check(lib.punc_gpu_synchronize(pop._h))
+        t = time.perf_counter()
+        f.solve_population(pop, object_potentials=[0.0], copy_out=False)
+        wms.append(1e3 * (time.perf_counter() - t))
+    log(f"[bench] field solve, warm start after a step: {np.mean(wms):.2f} ms (Poisson {f.iterations[0]} + "
+        f"E {f.iterations[1]} CG iterations)")
+    return {"ms": ms, "dofs": mesh.ndof, "poisson_iterations": its[0], "efield_iterations": its[1],
+            "warm_start_ms": float(np.mean(wms)), "warm_poisson_iterations": f.iterations[0],
+            "warm_efield_iterations": f.iterations[1], "preconditioner": "Jacobi", "rtol": 1e-12}
 
 
 def run_diagnostics(pop, mesh, dv, reps=5):

@@ -231,6 +231,9 @@ int punc_gpu_field_create(punc_mesh_t* m, double eps0, int dirichlet_exterior, i
 void punc_gpu_field_destroy(punc_field_t* f);
 /* Iteration limit per solve (default 20000; failing to converge is an error). */
 int punc_gpu_field_set_max_iterations(punc_field_t* f, int maxit);
+/* warm != 0: each solve starts from the previous solution (the reference's
+ * solvers start from zero; the result agrees to the solver tolerance). */
+int punc_gpu_field_set_warm_start(punc_field_t* f, int warm);
 /* rho (ndof) -> phi (ndof).  exterior_values: one value per dof, read at the
  * exterior Dirichlet dofs (NULL: 0); object_potentials: n_objects values
  * (NULL: 0).  iterations may be NULL. */

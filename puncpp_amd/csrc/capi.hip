@@ -1149,6 +1149,12 @@ extern "C" int punc_gpu_field_set_max_iterations(punc_field_t* f, int maxit) {
     return 0;
 }
 
+extern "C" int punc_gpu_field_set_warm_start(punc_field_t* f, int warm) {
+    if (!f) return fail("field_set_warm_start: null field");
+    field_set_warm_start(f->F, warm != 0);
+    return 0;
+}
+
 extern "C" int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* exterior_values,
                                       const double* object_potentials, double* phi, int* iterations) {
     if (!f || !rho || !phi) return fail("poisson_solve: null argument");

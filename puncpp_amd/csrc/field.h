@@ -37,5 +37,9 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
 bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::string& err);
 // Iteration limit of every solve (default 20000).
 void field_set_max_iterations(FieldSolver* F, int maxit);
+// Start each solve from the previous solution instead of zero (default off,
+// as the reference's solvers start from zero); the answer is the same to the
+// solver tolerance, reached in fewer iterations when the field changes little.
+void field_set_warm_start(FieldSolver* F, bool warm);
 
 }  // namespace punc_gpu

@@ -12,10 +12,11 @@
 // side is b - K phi_D on free dofs and phi_D on Dirichlet dofs.  This has
 // the solution of the reference's row-replaced system (DirichletBC::apply)
 // and keeps the operator symmetric positive definite, so the solves are
-// Jacobi-preconditioned conjugate gradients.  Every reduction is a per-block
-// partial plus a last-block sum in block order (deterministic), and the CG
-// scalars stay on the device: iterations run in batches without host
-// synchronisation.
+// Jacobi-preconditioned conjugate gradients.  Every reduction is a set of
+// per-block partials that every block of the next kernel adds in block order
+// (deterministic; no atomics or fences - the kernel boundary orders the
+// memory), and the CG scalars stay on the device: iterations run in batches
+// without host synchronisation.
 #include "field.h"
 
 #include <math.h>
@@ -30,15 +31,16 @@ namespace punc_gpu {
 namespace {
 
 constexpr int FB = 256;         // threads per block
-constexpr int MAX_BLOCKS = 2048;
+constexpr int MAX_BLOCKS = 1024;  // also the partials every block of the next kernel adds
 constexpr int BATCH = 32;       // CG iterations between convergence checks
 
 struct CgState {
-    double rz, pq, alpha, beta, rr, tol2;
-    double red[4];  // results of the last generic reduction
-    int64_t it;
+    double rz[2];  // r.z entering iteration k at [k & 1]
+    double tol2;   // stop when |r|^2 <= tol2
+    double mean;   // last k_finish_mean result
+    int64_t it;    // iterations when converged
     int32_t done;
-    uint32_t counter;
+    int32_t pad;
 };
 
 __device__ inline double block_reduce(double v, double* sh) {
@@ -53,157 +55,202 @@ __device__ inline double block_reduce(double v, double* sh) {
     return r;  // valid in thread 0
 }
 
-// Every block publishes NV partial sums; the last block to finish adds them
-// in block order and returns true (in all its threads) with the totals in tot.
-template <int NV>
-__device__ bool last_block_sum(const double (&v)[NV], double* part, uint32_t* counter, double (&tot)[NV]) {
-    __shared__ double sh[16];
-    __shared__ bool last;
-    for (int k = 0; k < NV; ++k) {
-        const double b = block_reduce(v[k], sh);
-        if (threadIdx.x == 0)
-            __hip_atomic_store(&part[(int64_t)NV * blockIdx.x + k], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+// Sum of cnt partials part[k * stride + off] in a fixed order; every block
+// computes the same value (the partials were written by an earlier kernel,
+// so no atomics or fences are needed).  Returned to every thread.
+__device__ inline double sum_partials(const double* __restrict__ part, int cnt, int stride, int off, double* sh) {
+    __shared__ double bc;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) s += part[(int64_t)k * stride + off];
+    const double r = block_reduce(s, sh);
+    if (threadIdx.x == 0) bc = r;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t t = atomicAdd(counter, 1u);
-        last = t == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s += __hip_atomic_load(&part[(int64_t)NV * b + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // threads hold strided partial sums; combine them in thread order
-        const double r = block_reduce(s, sh);
-        if (threadIdx.x == 0) tot[k] = r;
-    }
-    if (threadIdx.x == 0) *counter = 0;
-    return true;
+    return bc;
 }
 
-__device__ inline double row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                 const double* __restrict__ av, const double* __restrict__ x, int64_t i) {
+// Sparse rows are processed by RL = 16 consecutive lanes each (a row of a
+// tetrahedral CG1 matrix has about 15 entries): the lanes read the row's
+// entries contiguously and add their products with a butterfly over the 16
+// lanes, so every lane of the group ends with the row's value.
+constexpr int RL = 16;
+
+__device__ inline double row_dot16(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                   const double* __restrict__ av, const double* __restrict__ x, int64_t i,
+                                   int sub) {
     double s = 0.0;
-    for (int32_t j = rp[i]; j < rp[i + 1]; ++j) s += av[j] * x[ci[j]];
+    if (i >= 0)
+        for (int32_t j = rp[i] + sub; j < rp[i + 1]; j += RL) s += av[j] * x[ci[j]];
+    for (int o = RL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, RL);
     return s;
 }
+
+__device__ inline int64_t group_row(int64_t base, int64_t n) { return base < n ? base : -1; }
+
+#define ROW_LOOP_BEGIN                                                                   \
+    const int sub = threadIdx.x & (RL - 1);                                              \
+    const int64_t g0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / RL;            \
+    const int64_t gs = (int64_t)gridDim.x * blockDim.x / RL;                             \
+    for (int64_t bb = 0; bb < n; bb += gs) { /* uniform trip count: the butterfly */     \
+        const int64_t i = group_row(bb + g0, n);
 
 __global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                              const double* __restrict__ av, const double* __restrict__ x,
                                              double* __restrict__ y, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        y[i] = row_dot(rp, ci, av, x, i);
-}
-
-// Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
-// dofs, phi_D on Dirichlet dofs; the initial iterate x0 = phi_D.
-__global__ void k_rhs_dirichlet(const double* __restrict__ b, const double* __restrict__ kphid,
-                                const double* __restrict__ phid, const uint8_t* __restrict__ isd,
-                                double* __restrict__ rhs, double* __restrict__ x, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        rhs[i] = isd[i] ? phid[i] : b[i] - kphid[i];
-        x[i] = isd[i] ? phid[i] : 0.0;
+    ROW_LOOP_BEGIN
+        const double v = row_dot16(rp, ci, av, x, i, sub);
+        if (i >= 0 && sub == 0) y[i] = v;
     }
 }
 
-// Sum of v (red[0]) by the last block.
-__global__ void __launch_bounds__(FB) k_sum(const double* __restrict__ v, int64_t n, double* part, CgState* S) {
-    double a[1] = {0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        a[0] += v[i];
-    double t[1];
-    if (last_block_sum<1>(a, part, &S->counter, t) && threadIdx.x == 0) S->red[0] = t[0];
+// Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
+// dofs, phi_D on Dirichlet dofs; the initial iterate x0 = phi_D there.
+__global__ void k_rhs_dirichlet(const double* __restrict__ b, const double* __restrict__ kphid,
+                                const double* __restrict__ phid, const uint8_t* __restrict__ isd,
+                                const double* __restrict__ prev, double* __restrict__ rhs, double* __restrict__ x,
+                                int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        rhs[i] = isd[i] ? phid[i] : b[i] - kphid[i];
+        x[i] = isd[i] ? phid[i] : (prev ? prev[i] : 0.0);  // prev: warm start from the last solution
+    }
 }
 
-// v -= red[0] / n (orthogonalisation against the normalised constant vector).
+// Ec[i] = E3[3i + c] (warm start of a component solve), or 0.
+__global__ void k_unpack3(const double* __restrict__ E3, int c, double* __restrict__ e, int64_t n, int warm) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        e[i] = warm ? E3[3 * i + c] : 0.0;
+}
+
+// Per-block sums of v, then the mean (one block).
+__global__ void __launch_bounds__(FB) k_partial_sum(const double* __restrict__ v, int64_t n, double* part) {
+    __shared__ double sh[8];
+    double a = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a += v[i];
+    const double t = block_reduce(a, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(FB) k_finish_mean(const double* __restrict__ part, int cnt, int64_t n, CgState* S) {
+    __shared__ double sh[8];
+    const double t = sum_partials(part, cnt, 1, 0, sh);
+    if (threadIdx.x == 0) S->mean = t / (double)n;
+}
+
+// v -= mean (orthogonalisation against the normalised constant vector).
 __global__ void k_remove_mean(double* __restrict__ v, int64_t n, const CgState* __restrict__ S) {
-    const double m = S->red[0] / (double)n;
+    const double m = S->mean;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         v[i] -= m;
 }
 
-// CG start: r = b - A x (x given), z = D^-1 r, p = z; rz, rr, |b|^2 -> tol2.
+// CG start: r = b - A x (x given), z = D^-1 r, p = z; per-block r.z, r.r, b.b.
 __global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                 const double* __restrict__ av, const double* __restrict__ dinv,
                                                 const double* __restrict__ b, const double* __restrict__ x,
                                                 double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
-                                                int64_t n, double rtol, double atol, double* part, CgState* S) {
-    double a[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double ri = b[i] - row_dot(rp, ci, av, x, i);
-        const double zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        a[0] += ri * zi;
-        a[1] += ri * ri;
-        a[2] += b[i] * b[i];
+                                                int64_t n, double* part) {
+    __shared__ double sh[8];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    ROW_LOOP_BEGIN
+        const double ax = row_dot16(rp, ci, av, x, i, sub);
+        if (i >= 0 && sub == 0) {
+            const double ri = b[i] - ax;
+            const double zi = dinv[i] * ri;
+            r[i] = ri;
+            z[i] = zi;
+            p[i] = zi;
+            a0 += ri * zi;
+            a1 += ri * ri;
+            a2 += b[i] * b[i];
+        }
     }
-    double t[3];
-    if (last_block_sum<3>(a, part, &S->counter, t) && threadIdx.x == 0) {
-        S->rz = t[0];
-        S->rr = t[1];
-        const double tol = fmax(rtol * sqrt(t[2]), atol);
+    const double t0 = block_reduce(a0, sh), t1 = block_reduce(a1, sh), t2 = block_reduce(a2, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = t0;
+        part[3 * blockIdx.x + 1] = t1;
+        part[3 * blockIdx.x + 2] = t2;
+    }
+}
+
+__global__ void __launch_bounds__(FB) k_cg_init_finish(const double* __restrict__ part, int cnt, double rtol,
+                                                       double atol, CgState* S) {
+    __shared__ double sh[8];
+    const double rz = sum_partials(part, cnt, 3, 0, sh);
+    const double rr = sum_partials(part, cnt, 3, 1, sh);
+    const double bb = sum_partials(part, cnt, 3, 2, sh);
+    if (threadIdx.x == 0) {
+        S->rz[0] = rz;
+        const double tol = fmax(rtol * sqrt(bb), atol);
         S->tol2 = tol * tol;
         S->it = 0;
-        S->done = t[1] <= S->tol2 ? 1 : 0;
+        S->done = rr <= S->tol2 ? 1 : 0;
     }
 }
 
-// q = A p, alpha = rz / (p.q).
+// Iteration k, step 1: q = A p; per-block p.q.
 __global__ void __launch_bounds__(FB) k_cg_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                 const double* __restrict__ av, const double* __restrict__ p,
-                                                double* __restrict__ q, int64_t n, double* part, CgState* S) {
+                                                double* __restrict__ q, int64_t n, double* part_pq,
+                                                const CgState* __restrict__ S) {
+    __shared__ double sh[8];
     if (S->done) return;  // uniform: written by an earlier kernel
-    double a[1] = {0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double qi = row_dot(rp, ci, av, p, i);
-        q[i] = qi;
-        a[0] += p[i] * qi;
+    double a = 0.0;
+    ROW_LOOP_BEGIN
+        const double qi = row_dot16(rp, ci, av, p, i, sub);
+        if (i >= 0 && sub == 0) {
+            q[i] = qi;
+            a += p[i] * qi;
+        }
     }
-    double t[1];
-    if (last_block_sum<1>(a, part, &S->counter, t) && threadIdx.x == 0) {
-        S->pq = t[0];
-        S->alpha = S->rz / t[0];
-    }
+    const double t = block_reduce(a, sh);
+    if (threadIdx.x == 0) part_pq[blockIdx.x] = t;
 }
 
-// x += alpha p, r -= alpha q, z = D^-1 r; beta = rz' / rz; convergence.
-__global__ void __launch_bounds__(FB) k_cg_update(double* __restrict__ x, double* __restrict__ r,
+// Step 2: alpha = rz_k / p.q (every block sums the partials); x += alpha p,
+// r -= alpha q, z = D^-1 r; per-block r.z and r.r.
+__global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict__ x, double* __restrict__ r,
                                                   double* __restrict__ z, const double* __restrict__ p,
                                                   const double* __restrict__ q, const double* __restrict__ dinv,
-                                                  int64_t n, double* part, CgState* S) {
+                                                  int64_t n, const double* __restrict__ part_pq, int cnt_pq,
+                                                  double* part2, const CgState* __restrict__ S) {
+    __shared__ double sh[8];
     if (S->done) return;
-    const double alpha = S->alpha;
-    double a[2] = {0.0, 0.0};
+    const double alpha = S->rz[k & 1] / sum_partials(part_pq, cnt_pq, 1, 0, sh);
+    double a0 = 0.0, a1 = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
         const double zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
-        a[0] += ri * zi;
-        a[1] += ri * ri;
+        a0 += ri * zi;
+        a1 += ri * ri;
     }
-    double t[2];
-    if (last_block_sum<2>(a, part, &S->counter, t) && threadIdx.x == 0) {
-        S->beta = t[0] / S->rz;
-        S->rz = t[0];
-        S->rr = t[1];
-        S->it += 1;
-        if (t[1] <= S->tol2) S->done = 1;
+    const double t0 = block_reduce(a0, sh), t1 = block_reduce(a1, sh);
+    if (threadIdx.x == 0) {
+        part2[2 * blockIdx.x] = t0;
+        part2[2 * blockIdx.x + 1] = t1;
     }
 }
 
-// p = z + beta p.
-__global__ void k_cg_direction(double* __restrict__ p, const double* __restrict__ z, int64_t n,
-                               const CgState* __restrict__ S) {
+// Step 3: convergence on |r|^2, else beta = rz_{k+1} / rz_k and p = z + beta p.
+// Block 0 records rz_{k+1} in the other slot and, at convergence, the flag.
+__global__ void __launch_bounds__(FB) k_cg_direction(int64_t k, double* __restrict__ p, const double* __restrict__ z,
+                                                     int64_t n, const double* __restrict__ part2, int cnt2,
+                                                     CgState* S) {
+    __shared__ double sh[8];
     if (S->done) return;
-    const double beta = S->beta;
+    const double rz = sum_partials(part2, cnt2, 2, 0, sh);
+    const double rr = sum_partials(part2, cnt2, 2, 1, sh);
+    if (rr <= S->tol2) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->done = 1;
+            S->it = k + 1;
+        }
+        return;
+    }
+    const double beta = rz / S->rz[k & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->rz[(k + 1) & 1] = rz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = z[i] + beta * p[i];
 }
@@ -228,7 +275,9 @@ struct FieldSolver {
     int nobj = 0;
     bool ext = false, nullspace = false;
     int maxit = 20000;
-    int grid = 1;
+    bool warm = false;  // start from the previous solution (the reference starts from zero)
+    int grid = 1;       // vector kernels: one element per thread
+    int grid_rows = 1;  // sparse-row kernels: RL threads per row
     std::vector<int32_t> dir_kind;  // per dof: -1 free, 0 exterior, 1+j object j
     // device
     int32_t *rp = nullptr, *ci = nullptr;
@@ -380,6 +429,7 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     F->nullspace = remove_null_space;
     F->dir_kind = kind;
     F->grid = (int)std::min<int64_t>(MAX_BLOCKS, std::max<int64_t>(1, (n + FB - 1) / FB));
+    F->grid_rows = (int)std::min<int64_t>(MAX_BLOCKS, std::max<int64_t>(1, (n * RL + FB - 1) / FB));
     hipError_t e = hipSetDevice(device);
     auto up = [&](auto** d, const auto& h) {
         if (e != hipSuccess) return;
@@ -400,7 +450,7 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     for (double** v : vecs)
         if (e == hipSuccess) e = dmalloc(v, (size_t)n);
     if (e == hipSuccess) e = dmalloc(&F->E, (size_t)n * 3);
-    if (e == hipSuccess) e = dmalloc(&F->part, (size_t)F->grid * 4);
+    if (e == hipSuccess) e = dmalloc(&F->part, (size_t)std::max(F->grid, F->grid_rows) * 5);
     if (e == hipSuccess) e = dmalloc(&F->S, 1);
     if (e == hipSuccess) e = hipMemset(F->S, 0, sizeof(CgState));
     if (e == hipSuccess) e = hipMemset(F->phi, 0, n * 8);
@@ -427,6 +477,7 @@ int field_num_objects(const FieldSolver* F) { return F->nobj; }
 double* field_phi(FieldSolver* F) { return F->phi; }
 double* field_E(FieldSolver* F) { return F->E; }
 void field_set_max_iterations(FieldSolver* F, int maxit) { F->maxit = std::max(1, maxit); }
+void field_set_warm_start(FieldSolver* F, bool warm) { F->warm = warm; }
 
 #define FCHK(x)                                                      \
     do {                                                             \
@@ -441,30 +492,29 @@ void field_set_max_iterations(FieldSolver* F, int maxit) { F->maxit = std::max(1
 static bool run_cg(FieldSolver* F, const double* av, const double* dinv, const double* b, int* iterations,
                    std::string& err) {
     const int64_t n = F->n;
-    const dim3 g(F->grid), blk(FB);
-    hipLaunchKernelGGL(k_cg_init, g, blk, 0, F->st, F->rp, F->ci, av, dinv, b, F->x, F->r, F->z, F->p, n, 1e-12,
-                       1e-14, F->part, F->S);
+    const dim3 g(F->grid), gr(F->grid_rows), blk(FB);
+    double* part_pq = F->part;                       // grid_rows entries
+    double* part2 = F->part + 3 * (size_t)std::max(F->grid, F->grid_rows);  // 2 * grid entries
+    hipLaunchKernelGGL(k_cg_init, gr, blk, 0, F->st, F->rp, F->ci, av, dinv, b, F->x, F->r, F->z, F->p, n, F->part);
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), blk, 0, F->st, F->part, F->grid_rows, 1e-12, 1e-14, F->S);
     FCHK(hipGetLastError());
     CgState h;
+    int64_t k = 0;
     for (;;) {
-        for (int k = 0; k < BATCH; ++k) {
-            hipLaunchKernelGGL(k_cg_spmv, g, blk, 0, F->st, F->rp, F->ci, av, F->p, F->q, n, F->part, F->S);
-            hipLaunchKernelGGL(k_cg_update, g, blk, 0, F->st, F->x, F->r, F->z, F->p, F->q, dinv, n, F->part, F->S);
-            hipLaunchKernelGGL(k_cg_direction, g, blk, 0, F->st, F->p, F->z, n, F->S);
+        for (int j = 0; j < BATCH; ++j, ++k) {
+            hipLaunchKernelGGL(k_cg_spmv, gr, blk, 0, F->st, F->rp, F->ci, av, F->p, F->q, n, part_pq, F->S);
+            hipLaunchKernelGGL(k_cg_update, g, blk, 0, F->st, k, F->x, F->r, F->z, F->p, F->q, dinv, n, part_pq,
+                               F->grid_rows, part2, F->S);
+            hipLaunchKernelGGL(k_cg_direction, g, blk, 0, F->st, k, F->p, F->z, n, part2, F->grid, F->S);
         }
         FCHK(hipGetLastError());
         FCHK(hipMemcpyAsync(&h, F->S, sizeof(h), hipMemcpyDeviceToHost, F->st));
         FCHK(hipStreamSynchronize(F->st));
-        if (h.done || h.it >= F->maxit) break;
+        if (h.done || k >= F->maxit) break;
     }
-    if (iterations) *iterations = (int)h.it;
+    if (iterations) *iterations = (int)(h.done ? h.it : k);
     if (!h.done) {
-        err = "conjugate gradients did not converge in " + std::to_string(h.it) + " iterations (|r|^2 = " +
-              std::to_string(h.rr) + ")";
-        return false;
-    }
-    if (!std::isfinite(h.rr)) {
-        err = "conjugate gradients broke down";
+        err = "conjugate gradients did not converge in " + std::to_string(k) + " iterations";
         return false;
     }
     return true;
@@ -483,18 +533,22 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
         if (k >= 1) pd[i] = object_potentials ? object_potentials[k - 1] : 0.0;
     }
     FCHK(hipMemcpyAsync(F->phid, pd.data(), n * 8, hipMemcpyHostToDevice, F->st));
-    hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->rp, F->ci, F->mass, d_rho, F->b, n);       // b = M rho
-    hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->rp, F->ci, F->kfull, F->phid, F->t, n);    // K phi_D
-    hipLaunchKernelGGL(k_rhs_dirichlet, g, blk, 0, F->st, F->b, F->t, F->phid, F->isd, F->rhs, F->x, n);
+    const dim3 gr(F->grid_rows);
+    hipLaunchKernelGGL(k_spmv, gr, blk, 0, F->st, F->rp, F->ci, F->mass, d_rho, F->b, n);       // b = M rho
+    hipLaunchKernelGGL(k_spmv, gr, blk, 0, F->st, F->rp, F->ci, F->kfull, F->phid, F->t, n);    // K phi_D
+    hipLaunchKernelGGL(k_rhs_dirichlet, g, blk, 0, F->st, F->b, F->t, F->phid, F->isd,
+                       F->warm ? (const double*)F->phi : nullptr, F->rhs, F->x, n);
     FCHK(hipGetLastError());
     if (F->nullspace) {
-        hipLaunchKernelGGL(k_sum, g, blk, 0, F->st, F->rhs, n, F->part, F->S);
+        hipLaunchKernelGGL(k_partial_sum, g, blk, 0, F->st, F->rhs, n, F->part);
+        hipLaunchKernelGGL(k_finish_mean, dim3(1), blk, 0, F->st, F->part, F->grid, n, F->S);
         hipLaunchKernelGGL(k_remove_mean, g, blk, 0, F->st, F->rhs, n, F->S);
         FCHK(hipGetLastError());
     }
     if (!run_cg(F, F->kred, F->dinv_k, F->rhs, iterations, err)) return false;
     if (F->nullspace) {
-        hipLaunchKernelGGL(k_sum, g, blk, 0, F->st, F->x, n, F->part, F->S);
+        hipLaunchKernelGGL(k_partial_sum, g, blk, 0, F->st, F->x, n, F->part);
+        hipLaunchKernelGGL(k_finish_mean, dim3(1), blk, 0, F->st, F->part, F->grid, n, F->S);
         hipLaunchKernelGGL(k_remove_mean, g, blk, 0, F->st, F->x, n, F->S);
         FCHK(hipGetLastError());
     }
@@ -509,8 +563,8 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
     const dim3 g(F->grid), blk(FB);
     int total = 0;
     for (int c = 0; c < 3; ++c) {
-        hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->rp, F->ci, F->grad[c], d_phi, F->rhs, n);
-        FCHK(hipMemsetAsync(F->x, 0, n * 8, F->st));
+        hipLaunchKernelGGL(k_spmv, dim3(F->grid_rows), blk, 0, F->st, F->rp, F->ci, F->grad[c], d_phi, F->rhs, n);
+        hipLaunchKernelGGL(k_unpack3, g, blk, 0, F->st, F->E, c, F->x, n, F->warm ? 1 : 0);
         int it = 0;
         if (!run_cg(F, F->mass, F->dinv_m, F->rhs, &it, err)) return false;
         total += it;

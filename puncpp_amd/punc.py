@@ -563,7 +563,7 @@ class FieldSolver:
     potential has zero mean.  Tolerances as the reference's Krylov solvers
     (relative 1e-12, absolute 1e-14)."""
 
-    def __init__(self, mesh, eps0=1.0, exterior=True, n_objects=0, remove_null_space=False):
+    def __init__(self, mesh, eps0=1.0, exterior=True, n_objects=0, remove_null_space=False, warm_start=False):
         self.mesh = mesh
         self.n_objects = int(n_objects)
         h = N.vp()
@@ -571,6 +571,12 @@ class FieldSolver:
                                             int(bool(remove_null_space)), N.C.byref(h)))
         self._h = h
         self.iterations = (0, 0)
+        self.set_warm_start(warm_start)
+
+    def set_warm_start(self, warm):
+        """Start each solve from the previous solution (the reference starts
+        from zero; the result agrees to the solver tolerance)."""
+        check(N.lib().punc_gpu_field_set_warm_start(self._h, int(bool(warm))))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -108,3 +108,22 @@ def test_field_large_mesh(gpu):
     assert np.linalg.norm(res) <= 1e-10 * np.linalg.norm((fo.M @ rho)[free] - (fo.K @ np.where(free, 0, phi))[free])
     E = f.efield(phi)
     assert np.all(np.isfinite(E))
+
+
+def test_warm_start_same_answer(gpu):
+    _print_case("warm_start")
+    hm = sphere_mesh(n=10, L=4.0, r=1.5)
+    mesh = P.Mesh(hm)
+    rng = np.random.default_rng(7)
+    rho1 = rng.normal(size=hm.ndof)
+    rho2 = rho1 + 0.01 * rng.normal(size=hm.ndof)
+    cold = P.FieldSolver(mesh, exterior=True, n_objects=1)
+    warm = P.FieldSolver(mesh, exterior=True, n_objects=1, warm_start=True)
+    warm.solve(rho1, object_potentials=[0.3])
+    warm.efield(warm.solve(rho1, object_potentials=[0.3]))
+    phi_w = warm.solve(rho2, object_potentials=[0.3])
+    it_w = warm.iterations[0]
+    phi_c = cold.solve(rho2, object_potentials=[0.3])
+    assert it_w < cold.iterations[0]
+    _close(phi_w, phi_c, 1e-9)
+    _close(warm.efield(phi_w), cold.efield(phi_c), 1e-8)
