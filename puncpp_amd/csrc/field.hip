@@ -6,7 +6,8 @@
 //   stiffness  K_ij = eps0 V g_i.g_j                 (a = eps0 grad(phi).grad(v) dx)
 //   mass       M_ij = V (1 + delta_ij) / 20          (L = rho v dx, and the E projection)
 //   gradient   B_c,ij = -(V / 4) (g_j)_c             (L = -grad(phi).v dx, component c)
-// summed into CSR matrices with one sparsity (the dof graph of the cells).
+// summed into matrices with one sparsity (the dof graph of the cells), kept
+// on the device in ELL layout.
 // Dirichlet dofs are eliminated symmetrically: the operator keeps the free
 // rows and columns and has unit rows at the Dirichlet dofs; the right-hand
 // side is b - K phi_D on free dofs and phi_D on Dirichlet dofs.  This has
@@ -39,6 +40,7 @@ struct CgState {
     double tol2;   // stop when |r|^2 <= tol2
     double mean;   // last k_finish_mean result
     int64_t it;    // iterations when converged
+    int64_t base;  // iterations before the current batch
     int32_t done;
     int32_t pad;
 };
@@ -68,38 +70,21 @@ __device__ inline double sum_partials(const double* __restrict__ part, int cnt, 
     return bc;
 }
 
-// Sparse rows are processed by RL = 16 consecutive lanes each (a row of a
-// tetrahedral CG1 matrix has about 15 entries): the lanes read the row's
-// entries contiguously and add their products with a butterfly over the 16
-// lanes, so every lane of the group ends with the row's value.
-constexpr int RL = 16;
-
-__device__ inline double row_dot16(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                   const double* __restrict__ av, const double* __restrict__ x, int64_t i,
-                                   int sub) {
+// Sparse matrices are stored as ELL: every row padded to the widest row's w
+// entries (padding: the row's own column with value 0), entry k of row i at
+// k * n + i.  One thread per row: a wavefront's k-th entries are contiguous,
+// and the w products of a row are independent loads.
+__device__ inline double row_dot(const int32_t* __restrict__ col, const double* __restrict__ av, int w, int64_t n,
+                                 const double* __restrict__ x, int64_t i) {
     double s = 0.0;
-    if (i >= 0)
-        for (int32_t j = rp[i] + sub; j < rp[i + 1]; j += RL) s += av[j] * x[ci[j]];
-    for (int o = RL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, RL);
+    for (int k = 0; k < w; ++k) s += av[k * n + i] * x[col[k * n + i]];
     return s;
 }
 
-__device__ inline int64_t group_row(int64_t base, int64_t n) { return base < n ? base : -1; }
-
-#define ROW_LOOP_BEGIN                                                                   \
-    const int sub = threadIdx.x & (RL - 1);                                              \
-    const int64_t g0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / RL;            \
-    const int64_t gs = (int64_t)gridDim.x * blockDim.x / RL;                             \
-    for (int64_t bb = 0; bb < n; bb += gs) { /* uniform trip count: the butterfly */     \
-        const int64_t i = group_row(bb + g0, n);
-
-__global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                             const double* __restrict__ av, const double* __restrict__ x,
-                                             double* __restrict__ y, int64_t n) {
-    ROW_LOOP_BEGIN
-        const double v = row_dot16(rp, ci, av, x, i, sub);
-        if (i >= 0 && sub == 0) y[i] = v;
-    }
+__global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ col, int w, const double* __restrict__ av,
+                                             const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = row_dot(col, av, w, n, x, i);
 }
 
 // Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
@@ -144,25 +129,22 @@ __global__ void k_remove_mean(double* __restrict__ v, int64_t n, const CgState* 
 }
 
 // CG start: r = b - A x (x given), z = D^-1 r, p = z; per-block r.z, r.r, b.b.
-__global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                                const double* __restrict__ av, const double* __restrict__ dinv,
-                                                const double* __restrict__ b, const double* __restrict__ x,
-                                                double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
-                                                int64_t n, double* part) {
+__global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ col, int w, const double* __restrict__ av,
+                                                const double* __restrict__ dinv, const double* __restrict__ b,
+                                                const double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ z, double* __restrict__ p, int64_t n,
+                                                double* part) {
     __shared__ double sh[8];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    ROW_LOOP_BEGIN
-        const double ax = row_dot16(rp, ci, av, x, i, sub);
-        if (i >= 0 && sub == 0) {
-            const double ri = b[i] - ax;
-            const double zi = dinv[i] * ri;
-            r[i] = ri;
-            z[i] = zi;
-            p[i] = zi;
-            a0 += ri * zi;
-            a1 += ri * ri;
-            a2 += b[i] * b[i];
-        }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = b[i] - row_dot(col, av, w, n, x, i);
+        const double zi = dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        a0 += ri * zi;
+        a1 += ri * ri;
+        a2 += b[i] * b[i];
     }
     const double t0 = block_reduce(a0, sh), t1 = block_reduce(a1, sh), t2 = block_reduce(a2, sh);
     if (threadIdx.x == 0) {
@@ -183,30 +165,56 @@ __global__ void __launch_bounds__(FB) k_cg_init_finish(const double* __restrict_
         const double tol = fmax(rtol * sqrt(bb), atol);
         S->tol2 = tol * tol;
         S->it = 0;
+        S->base = 0;
         S->done = rr <= S->tol2 ? 1 : 0;
     }
 }
 
-// Iteration k, step 1: q = A p; per-block p.q.
-__global__ void __launch_bounds__(FB) k_cg_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                                const double* __restrict__ av, const double* __restrict__ p,
-                                                double* __restrict__ q, int64_t n, double* part_pq,
-                                                const CgState* __restrict__ S) {
+// Iteration t = base + k (batches hold an even number of iterations, so
+// t & 1 = k & 1).  rz[t & 1] holds r.z entering iteration t.
+//
+// Step 1 (direction and product in one pass): beta = rz_t / rz_{t-1} from the
+// previous update's partials (every block adds them), convergence on |r|^2;
+// then p = z + beta p and q = A p computed as A z + beta q (the product of
+// the updated p needs neighbours' p; A z only needs z, complete since the
+// last kernel).  Per-block p.q.  Block 0 records rz_t and, at convergence,
+// the flag and the iteration count.
+__global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const int32_t* __restrict__ col, int w,
+                                                    const double* __restrict__ av, const double* __restrict__ z,
+                                                    double* __restrict__ p, double* __restrict__ q, int64_t n,
+                                                    const double* __restrict__ part2, int cnt2, double* part_pq,
+                                                    CgState* S) {
     __shared__ double sh[8];
     if (S->done) return;  // uniform: written by an earlier kernel
-    double a = 0.0;
-    ROW_LOOP_BEGIN
-        const double qi = row_dot16(rp, ci, av, p, i, sub);
-        if (i >= 0 && sub == 0) {
-            q[i] = qi;
-            a += p[i] * qi;
+    const bool first = S->base == 0 && k == 0;
+    double beta = 0.0;
+    if (!first) {
+        const double rz = sum_partials(part2, cnt2, 2, 0, sh);
+        const double rr = sum_partials(part2, cnt2, 2, 1, sh);
+        if (rr <= S->tol2) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                S->done = 1;
+                S->it = S->base + k;
+            }
+            return;
         }
+        beta = rz / S->rz[(k + 1) & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->rz[k & 1] = rz;
+    }
+    double a = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double az = row_dot(col, av, w, n, z, i);
+        const double pi = first ? z[i] : z[i] + beta * p[i];
+        const double qi = first ? az : az + beta * q[i];
+        p[i] = pi;
+        q[i] = qi;
+        a += pi * qi;
     }
     const double t = block_reduce(a, sh);
     if (threadIdx.x == 0) part_pq[blockIdx.x] = t;
 }
 
-// Step 2: alpha = rz_k / p.q (every block sums the partials); x += alpha p,
+// Step 2: alpha = rz_t / p.q (every block adds the partials); x += alpha p,
 // r -= alpha q, z = D^-1 r; per-block r.z and r.r.
 __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict__ x, double* __restrict__ r,
                                                   double* __restrict__ z, const double* __restrict__ p,
@@ -233,26 +241,9 @@ __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict_
     }
 }
 
-// Step 3: convergence on |r|^2, else beta = rz_{k+1} / rz_k and p = z + beta p.
-// Block 0 records rz_{k+1} in the other slot and, at convergence, the flag.
-__global__ void __launch_bounds__(FB) k_cg_direction(int64_t k, double* __restrict__ p, const double* __restrict__ z,
-                                                     int64_t n, const double* __restrict__ part2, int cnt2,
-                                                     CgState* S) {
-    __shared__ double sh[8];
-    if (S->done) return;
-    const double rz = sum_partials(part2, cnt2, 2, 0, sh);
-    const double rr = sum_partials(part2, cnt2, 2, 1, sh);
-    if (rr <= S->tol2) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            S->done = 1;
-            S->it = k + 1;
-        }
-        return;
-    }
-    const double beta = rz / S->rz[k & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) S->rz[(k + 1) & 1] = rz;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = z[i] + beta * p[i];
+// End of a batch of BATCH iterations.
+__global__ void k_cg_batch_end(CgState* S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !S->done) S->base += BATCH;
 }
 
 // E3[3i + c] = Ec[i].
@@ -276,11 +267,11 @@ struct FieldSolver {
     bool ext = false, nullspace = false;
     int maxit = 20000;
     bool warm = false;  // start from the previous solution (the reference starts from zero)
-    int grid = 1;       // vector kernels: one element per thread
-    int grid_rows = 1;  // sparse-row kernels: RL threads per row
+    int grid = 1;  // one row / element per thread, grid-stride
+    int w = 0;     // ELL width
     std::vector<int32_t> dir_kind;  // per dof: -1 free, 0 exterior, 1+j object j
     // device
-    int32_t *rp = nullptr, *ci = nullptr;
+    int32_t* col = nullptr;  // ELL columns (w * n)
     double *kred = nullptr, *kfull = nullptr, *mass = nullptr, *grad[3] = {nullptr, nullptr, nullptr};
     double *dinv_k = nullptr, *dinv_m = nullptr;
     uint8_t* isd = nullptr;
@@ -288,11 +279,23 @@ struct FieldSolver {
     double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
     double *phi = nullptr, *E = nullptr, *part = nullptr;
     CgState* S = nullptr;
+    bool own_stream = false;
+    // one captured batch of iterations per operator (stiffness, mass)
+    struct Batch {
+        const double* av;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Batch> batches;
 };
 
 static void free_all(FieldSolver* F) {
-    hipFree(F->rp);
-    hipFree(F->ci);
+    for (auto& b : F->batches) {
+        hipGraphExecDestroy(b.exec);
+        hipGraphDestroy(b.graph);
+    }
+    F->batches.clear();
+    hipFree(F->col);
     hipFree(F->kred);
     hipFree(F->kfull);
     hipFree(F->mass);
@@ -313,6 +316,7 @@ static void free_all(FieldSolver* F) {
     hipFree(F->E);
     hipFree(F->part);
     hipFree(F->S);
+    if (F->own_stream && F->st) hipStreamDestroy(F->st);
 }
 
 // Barycentric gradients and volume of a tetrahedron: with J = [P1-P0, P2-P0,
@@ -419,7 +423,23 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
             }
         }
     }
+    // ELL layout of the CSR rows
+    int w = 0;
+    for (int64_t i = 0; i < n; ++i) w = std::max(w, rp[i + 1] - rp[i]);
+    std::vector<int32_t> ecol((size_t)w * n);
+    auto ell = [&](const std::vector<double>& v) {
+        std::vector<double> o((size_t)w * n, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t j = rp[i]; j < rp[i + 1]; ++j) o[(size_t)(j - rp[i]) * n + i] = v[j];
+        return o;
+    };
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < w; ++k) {
+            const int32_t j = rp[i] + k;
+            ecol[(size_t)k * n + i] = j < rp[i + 1] ? ci[j] : (int32_t)i;
+        }
     FieldSolver* F = new FieldSolver();
+    F->w = w;
     F->device = device;
     F->st = stream;
     F->n = n;
@@ -429,20 +449,24 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     F->nullspace = remove_null_space;
     F->dir_kind = kind;
     F->grid = (int)std::min<int64_t>(MAX_BLOCKS, std::max<int64_t>(1, (n + FB - 1) / FB));
-    F->grid_rows = (int)std::min<int64_t>(MAX_BLOCKS, std::max<int64_t>(1, (n * RL + FB - 1) / FB));
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess && !F->st) {
+        // a stream of its own (default priority): the iteration batches are
+        // captured into graphs, which the legacy default stream cannot do
+        e = hipStreamCreateWithFlags(&F->st, hipStreamNonBlocking);
+        F->own_stream = e == hipSuccess;
+    }
     auto up = [&](auto** d, const auto& h) {
         if (e != hipSuccess) return;
         e = dmalloc(d, h.size());
         if (e == hipSuccess && !h.empty())
             e = hipMemcpy(*d, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice);
     };
-    up(&F->rp, rp);
-    up(&F->ci, ci);
-    up(&F->kred, Kr);
-    up(&F->kfull, K);
-    up(&F->mass, Mv);
-    for (int d = 0; d < 3; ++d) up(&F->grad[d], B[d]);
+    up(&F->col, ecol);
+    up(&F->kred, ell(Kr));
+    up(&F->kfull, ell(K));
+    up(&F->mass, ell(Mv));
+    for (int d = 0; d < 3; ++d) up(&F->grad[d], ell(B[d]));
     up(&F->dinv_k, dk);
     up(&F->dinv_m, dm);
     up(&F->isd, isd);
@@ -450,7 +474,7 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     for (double** v : vecs)
         if (e == hipSuccess) e = dmalloc(v, (size_t)n);
     if (e == hipSuccess) e = dmalloc(&F->E, (size_t)n * 3);
-    if (e == hipSuccess) e = dmalloc(&F->part, (size_t)std::max(F->grid, F->grid_rows) * 5);
+    if (e == hipSuccess) e = dmalloc(&F->part, (size_t)F->grid * 5);
     if (e == hipSuccess) e = dmalloc(&F->S, 1);
     if (e == hipSuccess) e = hipMemset(F->S, 0, sizeof(CgState));
     if (e == hipSuccess) e = hipMemset(F->phi, 0, n * 8);
@@ -488,33 +512,68 @@ void field_set_warm_start(FieldSolver* F, bool warm) { F->warm = warm; }
         }                                                            \
     } while (0)
 
-// Jacobi-preconditioned CG on (rp, ci, av) for rhs b from F->x (in place).
+static void enqueue_batch(FieldSolver* F, const double* av, const double* dinv) {
+    const int64_t n = F->n;
+    const dim3 g(F->grid), blk(FB);
+    double* part_pq = F->part;
+    double* part2 = F->part + 3 * (size_t)F->grid;
+    for (int k = 0; k < BATCH; ++k) {
+        hipLaunchKernelGGL(k_cg_dir_spmv, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n, part2,
+                           F->grid, part_pq, F->S);
+        hipLaunchKernelGGL(k_cg_update, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n, part_pq,
+                           F->grid, part2, F->S);
+    }
+    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, F->st, F->S);
+}
+
+// The captured batch for operator av (captured on first use).
+static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, hipGraphExec_t* out, std::string& err) {
+    for (auto& b : F->batches)
+        if (b.av == av) {
+            *out = b.exec;
+            return true;
+        }
+    FieldSolver::Batch b;
+    b.av = av;
+    FCHK(hipStreamBeginCapture(F->st, hipStreamCaptureModeThreadLocal));
+    enqueue_batch(F, av, dinv);
+    hipError_t e = hipStreamEndCapture(F->st, &b.graph);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        err = std::string("capturing the CG batch: ") + hipGetErrorString(e);
+        return false;
+    }
+    e = hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(b.graph);
+        err = std::string("instantiating the CG batch: ") + hipGetErrorString(e);
+        return false;
+    }
+    F->batches.push_back(b);
+    *out = b.exec;
+    return true;
+}
+
+// Jacobi-preconditioned CG on the ELL operator av for rhs b from F->x (in place).
 static bool run_cg(FieldSolver* F, const double* av, const double* dinv, const double* b, int* iterations,
                    std::string& err) {
     const int64_t n = F->n;
-    const dim3 g(F->grid), gr(F->grid_rows), blk(FB);
-    double* part_pq = F->part;                       // grid_rows entries
-    double* part2 = F->part + 3 * (size_t)std::max(F->grid, F->grid_rows);  // 2 * grid entries
-    hipLaunchKernelGGL(k_cg_init, gr, blk, 0, F->st, F->rp, F->ci, av, dinv, b, F->x, F->r, F->z, F->p, n, F->part);
-    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), blk, 0, F->st, F->part, F->grid_rows, 1e-12, 1e-14, F->S);
+    const dim3 g(F->grid), blk(FB);
+    hipGraphExec_t exec;
+    if (!batch_exec(F, av, dinv, &exec, err)) return false;
+    hipLaunchKernelGGL(k_cg_init, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n, F->part);
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), blk, 0, F->st, F->part, F->grid, 1e-12, 1e-14, F->S);
     FCHK(hipGetLastError());
     CgState h;
-    int64_t k = 0;
     for (;;) {
-        for (int j = 0; j < BATCH; ++j, ++k) {
-            hipLaunchKernelGGL(k_cg_spmv, gr, blk, 0, F->st, F->rp, F->ci, av, F->p, F->q, n, part_pq, F->S);
-            hipLaunchKernelGGL(k_cg_update, g, blk, 0, F->st, k, F->x, F->r, F->z, F->p, F->q, dinv, n, part_pq,
-                               F->grid_rows, part2, F->S);
-            hipLaunchKernelGGL(k_cg_direction, g, blk, 0, F->st, k, F->p, F->z, n, part2, F->grid, F->S);
-        }
-        FCHK(hipGetLastError());
+        FCHK(hipGraphLaunch(exec, F->st));
         FCHK(hipMemcpyAsync(&h, F->S, sizeof(h), hipMemcpyDeviceToHost, F->st));
         FCHK(hipStreamSynchronize(F->st));
-        if (h.done || k >= F->maxit) break;
+        if (h.done || h.base >= F->maxit) break;
     }
-    if (iterations) *iterations = (int)(h.done ? h.it : k);
+    if (iterations) *iterations = (int)(h.done ? h.it : h.base);
     if (!h.done) {
-        err = "conjugate gradients did not converge in " + std::to_string(k) + " iterations";
+        err = "conjugate gradients did not converge in " + std::to_string(h.base) + " iterations";
         return false;
     }
     return true;
@@ -533,9 +592,8 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
         if (k >= 1) pd[i] = object_potentials ? object_potentials[k - 1] : 0.0;
     }
     FCHK(hipMemcpyAsync(F->phid, pd.data(), n * 8, hipMemcpyHostToDevice, F->st));
-    const dim3 gr(F->grid_rows);
-    hipLaunchKernelGGL(k_spmv, gr, blk, 0, F->st, F->rp, F->ci, F->mass, d_rho, F->b, n);       // b = M rho
-    hipLaunchKernelGGL(k_spmv, gr, blk, 0, F->st, F->rp, F->ci, F->kfull, F->phid, F->t, n);    // K phi_D
+    hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->mass, d_rho, F->b, n);     // b = M rho
+    hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->kfull, F->phid, F->t, n);  // K phi_D
     hipLaunchKernelGGL(k_rhs_dirichlet, g, blk, 0, F->st, F->b, F->t, F->phid, F->isd,
                        F->warm ? (const double*)F->phi : nullptr, F->rhs, F->x, n);
     FCHK(hipGetLastError());
@@ -563,7 +621,7 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
     const dim3 g(F->grid), blk(FB);
     int total = 0;
     for (int c = 0; c < 3; ++c) {
-        hipLaunchKernelGGL(k_spmv, dim3(F->grid_rows), blk, 0, F->st, F->rp, F->ci, F->grad[c], d_phi, F->rhs, n);
+        hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->grad[c], d_phi, F->rhs, n);
         hipLaunchKernelGGL(k_unpack3, g, blk, 0, F->st, F->E, c, F->x, n, F->warm ? 1 : 0);
         int it = 0;
         if (!run_cg(F, F->mass, F->dinv_m, F->rhs, &it, err)) return false;
