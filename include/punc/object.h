@@ -1,6 +1,7 @@
 // Drop-in for the Object interface of punc/include/punc/object.h (PUNC++):
-// the particle path credits absorbed charge to charge/current; the circuit
-// solve that reads them stays on the host.
+// the particle path credits absorbed charge to charge/current; the
+// capacitance-matrix circuit (include/punc/poisson.h) reads and sets charge
+// and potential.
 #ifndef PUNC_GPU_OBJECT_H
 #define PUNC_GPU_OBJECT_H
 
@@ -25,6 +26,23 @@ protected:
 };
 
 using ObjectVector = std::vector<std::shared_ptr<Object>>;
+
+//! A two-port between objects node_a (-1: ground, minus side) and node_b
+//! (plus side) with a value (reference object.h:42-70).
+struct Source {
+    Source(int node_a, int node_b, double value) : node_a(node_a), node_b(node_b), value(value) {}
+    int node_a;
+    int node_b;
+    double value;
+};
+struct VSource : Source {
+    VSource(int a, int b, double v) : Source(a, b, v) {}
+};
+struct ISource : Source {
+    ISource(int a, int b, double v) : Source(a, b, v) {}
+};
+using VSourceVector = std::vector<VSource>;
+using ISourceVector = std::vector<ISource>;
 
 }  // namespace punc
 

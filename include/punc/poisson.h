@@ -51,7 +51,21 @@ public:
         iterations = its[0];
     }
 
+    //! solve_circuit (poisson.cpp:364-376) with the circuit set by CircuitCM:
+    //! the objects' charges are read and, with their potentials, updated.
+    void solve_circuit(Function& phi, const Function& rho, const ObjectVector& objects) {
+        std::vector<double> q(nobj_), v(nobj_);
+        for (size_t j = 0; j < nobj_; ++j) q[j] = objects[j]->charge;
+        check(punc_gpu_poisson_solve_circuit(handle_.get(), rho.values.data(), q.data(), v.data(),
+                                             phi.values.data(), &iterations));
+        for (size_t j = 0; j < nobj_; ++j) {
+            objects[j]->charge = q[j];
+            objects[j]->set_potential(v[j]);
+        }
+    }
+
     punc_field_t* handle() const { return handle_.get(); }
+    size_t num_objects() const { return nobj_; }
     int iterations = 0;  ///< CG iterations of the last solve
 
 private:
@@ -63,6 +77,30 @@ private:
     const Mesh& mesh_;
     size_t nobj_;
     std::shared_ptr<punc_field_t> handle_;
+};
+
+//! CircuitCM (object_CM.cpp:96-357): the capacitance-matrix circuit of a
+//! PoissonSolver's objects with voltage and current sources.
+class CircuitCM {
+public:
+    CircuitCM(PoissonSolver& poisson, const VSourceVector& vsources, const ISourceVector& isources, double dt)
+        : inv_capacitance(poisson.num_objects() * poisson.num_objects()) {
+        std::vector<int> va, vb, ia, ib;
+        std::vector<double> vv, iv;
+        for (const auto& s : vsources) {
+            va.push_back(s.node_a);
+            vb.push_back(s.node_b);
+            vv.push_back(s.value);
+        }
+        for (const auto& s : isources) {
+            ia.push_back(s.node_a);
+            ib.push_back(s.node_b);
+            iv.push_back(s.value);
+        }
+        check(punc_gpu_circuit_set(poisson.handle(), dt, (int)va.size(), va.data(), vb.data(), vv.data(),
+                                   (int)ia.size(), ia.data(), ib.data(), iv.data(), inv_capacitance.data()));
+    }
+    std::vector<double> inv_capacitance;  ///< row-major, n_objects^2
 };
 
 }  // namespace punc

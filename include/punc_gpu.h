@@ -241,11 +241,32 @@ int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* ext
                            const double* object_potentials, double* phi, int* iterations);
 /* phi (ndof) -> E (3*ndof, dof-major x, y, z). */
 int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double* E, int* iterations);
+/* Charge on each object of the potential phi (ndof): the reference's Charge
+ * form eps0 grad(phi).n ds over the object's facets (Charge.ufl), n_objects
+ * values. */
+int punc_gpu_object_charges(punc_field_t* f, const double* phi, double* charges);
+/* Capacitance-matrix circuit (CircuitCM, object_CM.cpp:53-357; charge-sharing
+ * sets object.cpp:68-135): n_objects Laplace solves give the inverse
+ * capacitance matrix (inv_capacitance: n_objects^2 values, row-major, may be
+ * NULL); voltage sources (node_a -1 for ground) and current sources between
+ * objects, dt for the current sources. */
+int punc_gpu_circuit_set(punc_field_t* f, double dt, int n_vsources, const int* v_a, const int* v_b,
+                         const double* v_value, int n_isources, const int* i_a, const int* i_b, const double* i_value,
+                         double* inv_capacitance);
+/* PoissonSolver::solve_circuit (poisson.cpp:364-376): charges (n_objects,
+ * in/out) are the objects' collected charges; potentials (out) the object
+ * potentials the circuit sets; phi (ndof, may be NULL). */
+int punc_gpu_poisson_solve_circuit(punc_field_t* f, const double* rho, double* charges, double* potentials,
+                                   double* phi, int* iterations);
 /* The population's last deposited rho -> phi -> E, written straight into the
  * population's field for the next push (no host round trip); phi and E are
  * copied out when non-NULL; iterations[0..1] (may be NULL): Poisson and E. */
 int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
                              const double* object_potentials, double* phi, double* E, int* iterations);
+/* As punc_gpu_field_solve_pop with the circuit's solve (charges in/out,
+ * potentials out, as punc_gpu_poisson_solve_circuit). */
+int punc_gpu_field_solve_pop_circuit(punc_field_t* f, punc_pop_t* p, double* charges, double* potentials, double* phi,
+                                     double* E, int* iterations);
 
 /* ---- injection --------------------------------------------------------- */
 /* Per species Maxwellian flux source: thermal speed, drift, density n

@@ -79,6 +79,10 @@ _SIGS = {
     "punc_gpu_poisson_solve": (C.c_int, [vp, d_p, d_p, d_p, d_p, i32_p]),
     "punc_gpu_efield_solve": (C.c_int, [vp, d_p, d_p, i32_p]),
     "punc_gpu_field_solve_pop": (C.c_int, [vp, vp, d_p, d_p, d_p, d_p, i32_p]),
+    "punc_gpu_object_charges": (C.c_int, [vp, d_p, d_p]),
+    "punc_gpu_circuit_set": (C.c_int, [vp, C.c_double, C.c_int, i32_p, i32_p, d_p, C.c_int, i32_p, i32_p, d_p, d_p]),
+    "punc_gpu_poisson_solve_circuit": (C.c_int, [vp, d_p, d_p, d_p, d_p, i32_p]),
+    "punc_gpu_field_solve_pop_circuit": (C.c_int, [vp, vp, d_p, d_p, d_p, d_p, i32_p]),
     "punc_gpu_injector_set": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, C.c_double]),
     "punc_gpu_species_instance": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int64,
                                             C.c_int64, C.c_int64]),

@@ -1167,6 +1167,41 @@ extern "C" int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const 
     return 0;
 }
 
+extern "C" int punc_gpu_object_charges(punc_field_t* f, const double* phi, double* charges) {
+    if (!f || !phi || !charges) return fail("object_charges: null argument");
+    const int64_t n = f->mesh->topo.ndof;
+    HIPCHK(hipMemcpy(f->d_in, phi, n * 8, hipMemcpyHostToDevice));
+    std::string err;
+    if (!field_object_charges(f->F, f->d_in, charges, err)) return fail("object_charges: " + err);
+    return 0;
+}
+
+extern "C" int punc_gpu_circuit_set(punc_field_t* f, double dt, int n_vsources, const int* v_a, const int* v_b,
+                                    const double* v_value, int n_isources, const int* i_a, const int* i_b,
+                                    const double* i_value, double* inv_capacitance) {
+    if (!f || n_vsources < 0 || n_isources < 0) return fail("circuit_set: bad argument");
+    if ((n_vsources > 0 && (!v_a || !v_b || !v_value)) || (n_isources > 0 && (!i_a || !i_b || !i_value)))
+        return fail("circuit_set: null source arrays");
+    std::vector<CircuitSource> vs, is;
+    for (int k = 0; k < n_vsources; ++k) vs.push_back({v_a[k], v_b[k], v_value[k]});
+    for (int k = 0; k < n_isources; ++k) is.push_back({i_a[k], i_b[k], i_value[k]});
+    std::string err;
+    if (!field_set_circuit(f->F, dt, vs, is, inv_capacitance, err)) return fail("circuit_set: " + err);
+    return 0;
+}
+
+extern "C" int punc_gpu_poisson_solve_circuit(punc_field_t* f, const double* rho, double* charges,
+                                              double* potentials, double* phi, int* iterations) {
+    if (!f || !rho || !charges || !potentials) return fail("poisson_solve_circuit: null argument");
+    const int64_t n = f->mesh->topo.ndof;
+    HIPCHK(hipMemcpy(f->d_in, rho, n * 8, hipMemcpyHostToDevice));
+    std::string err;
+    if (!field_poisson_circuit(f->F, f->d_in, charges, potentials, iterations, err))
+        return fail("poisson_solve_circuit: " + err);
+    if (phi) HIPCHK(hipMemcpy(phi, field_phi(f->F), n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double* E, int* iterations) {
     if (!f || !phi || !E) return fail("efield_solve: null argument");
     const int64_t n = f->mesh->topo.ndof;
@@ -1177,8 +1212,9 @@ extern "C" int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double*
     return 0;
 }
 
-extern "C" int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
-                                        const double* object_potentials, double* phi, double* E, int* iterations) {
+static int field_solve_pop_common(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
+                                  const double* object_potentials, double* charges, double* potentials, double* phi,
+                                  double* E, int* iterations) {
     if (!f || !p) return fail("field_solve_pop: null argument");
     if (p->mesh != f->mesh) return fail("field_solve_pop: the population lives on another mesh");
     HIPCHK(hipSetDevice(p->device));
@@ -1187,8 +1223,9 @@ extern "C" int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const do
     HIPCHK(hipStreamSynchronize(p->stream));
     std::string err;
     int it_phi = 0, it_e = 0;
-    if (!field_poisson(f->F, p->d_rho, exterior_values, object_potentials, &it_phi, err))
-        return fail("field_solve_pop: " + err);
+    const bool ok = charges ? field_poisson_circuit(f->F, p->d_rho, charges, potentials, &it_phi, err)
+                            : field_poisson(f->F, p->d_rho, exterior_values, object_potentials, &it_phi, err);
+    if (!ok) return fail("field_solve_pop: " + err);
     if (!field_efield(f->F, field_phi(f->F), &it_e, err)) return fail("field_solve_pop: " + err);
     // the field the next push reads: packed dofs and the per-cell copy
     HIPCHK(hipMemcpy(p->d_E, field_E(f->F), n * 24, hipMemcpyDeviceToDevice));
@@ -1202,6 +1239,17 @@ extern "C" int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const do
         iterations[1] = it_e;
     }
     return 0;
+}
+
+extern "C" int punc_gpu_field_solve_pop(punc_field_t* f, punc_pop_t* p, const double* exterior_values,
+                                        const double* object_potentials, double* phi, double* E, int* iterations) {
+    return field_solve_pop_common(f, p, exterior_values, object_potentials, nullptr, nullptr, phi, E, iterations);
+}
+
+extern "C" int punc_gpu_field_solve_pop_circuit(punc_field_t* f, punc_pop_t* p, double* charges, double* potentials,
+                                                double* phi, double* E, int* iterations) {
+    if (!charges || !potentials) return fail("field_solve_pop_circuit: null argument");
+    return field_solve_pop_common(f, p, nullptr, nullptr, charges, potentials, phi, E, iterations);
 }
 
 // ---------------------------------------------------------------------------

@@ -13,6 +13,13 @@ namespace punc_gpu {
 
 struct FieldSolver;
 
+// A voltage or current source between objects node_a (-1: ground) and
+// node_b (reference object.h:42-70).
+struct CircuitSource {
+    int node_a, node_b;
+    double value;
+};
+
 // Dirichlet conditions: the exterior boundary (boundary id 1) when
 // dirichlet_exterior, and the boundaries of objects 0..n_objects-1 (ids 2..).
 // remove_null_space: the constant vector is the null space (no Dirichlet
@@ -35,6 +42,21 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
                    int* iterations, std::string& err);
 // E projection of d_phi (device, ndof) into field_E.
 bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::string& err);
+// Charges on the objects (the Charge form, Charge.ufl) of the potential d_phi
+// (device, ndof): out[j] for j < n_objects (host).
+bool field_object_charges(FieldSolver* F, const double* d_phi, double* out, std::string& err);
+// The capacitance-matrix circuit (CircuitCM, object_CM.cpp:53-357, with the
+// charge-sharing sets of object.cpp:68-135): the inverse capacitance matrix
+// from n_objects Laplace solves (inv_capacitance: n x n, row-major, may be
+// NULL), then the circuit matrix of the sources.
+bool field_set_circuit(FieldSolver* F, double dt, const std::vector<CircuitSource>& vsources,
+                       const std::vector<CircuitSource>& isources, double* inv_capacitance, std::string& err);
+// PoissonSolver::solve_circuit (poisson.cpp:364-376): current sources added
+// to the object charges, a solve with the objects grounded, the image
+// charges and the circuit giving the object potentials and charges, then the
+// corrected solve.  charges: in/out (n_objects); potentials: out.
+bool field_poisson_circuit(FieldSolver* F, const double* d_rho, double* charges, double* potentials, int* iterations,
+                           std::string& err);
 // Iteration limit of every solve (default 20000).
 void field_set_max_iterations(FieldSolver* F, int maxit);
 // Start each solve from the previous solution instead of zero (default off,

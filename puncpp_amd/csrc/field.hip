@@ -241,6 +241,19 @@ __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict_
     }
 }
 
+// Object charges from a potential (the Charge form eps0 grad(phi).n ds over
+// each object's facets, Charge.ufl): a fixed linear functional per object,
+// sum over its entries of w * phi[dof].  One block per object.
+__global__ void __launch_bounds__(FB) k_image_charge(const int64_t* __restrict__ off, const int32_t* __restrict__ dof,
+                                                     const double* __restrict__ w, const double* __restrict__ phi,
+                                                     double* __restrict__ out) {
+    __shared__ double sh[8];
+    double s = 0.0;
+    for (int64_t k = off[blockIdx.x] + threadIdx.x; k < off[blockIdx.x + 1]; k += blockDim.x) s += w[k] * phi[dof[k]];
+    const double t = block_reduce(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 // End of a batch of BATCH iterations.
 __global__ void k_cg_batch_end(CgState* S) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && !S->done) S->base += BATCH;
@@ -280,6 +293,19 @@ struct FieldSolver {
     double *phi = nullptr, *E = nullptr, *part = nullptr;
     CgState* S = nullptr;
     bool own_stream = false;
+    // object charge functionals (CSR over objects) and the circuit
+    int64_t* q_off = nullptr;
+    int32_t* q_dof = nullptr;
+    double* q_w = nullptr;
+    double* q_out = nullptr;
+    double* zero = nullptr;  // a zero charge density (Laplace solves)
+    bool has_circuit = false;
+    double dt = 0.0;
+    std::vector<CircuitSource> vsrc, isrc;
+    std::vector<std::vector<int>> groups, fixed_voltage;
+    std::vector<double> invC;        // n x n, row-major
+    std::vector<double> inv_circuit; // 2n x 2n
+    std::vector<double> circuit_vector;
     // one captured batch of iterations per operator (stiffness, mass)
     struct Batch {
         const double* av;
@@ -316,6 +342,11 @@ static void free_all(FieldSolver* F) {
     hipFree(F->E);
     hipFree(F->part);
     hipFree(F->S);
+    hipFree(F->q_off);
+    hipFree(F->q_dof);
+    hipFree(F->q_w);
+    hipFree(F->q_out);
+    hipFree(F->zero);
     if (F->own_stream && F->st) hipStreamDestroy(F->st);
 }
 
@@ -410,6 +441,54 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
                 if (kd < 0) continue;
                 for (int v = 0; v < 3; ++v) kind[cd[c * 4 + fv[f][v]]] = kd;
             }
+    // Charge functionals: on each facet of object j (cell c, local facet f,
+    // outward unit normal n_f away from vertex f, area A_f) the form gives
+    // sum_k eps0 A_f (g_k . n_f) phi_k over the cell's four dofs.
+    std::vector<std::vector<std::pair<int32_t, double>>> qf((size_t)n_objects);
+    for (int64_t c = 0; c < nc; ++c)
+        for (int f = 0; f < 4; ++f) {
+            const int64_t a = t.adj[c * 4 + f];
+            if (a > -2 || -a - 2 >= n_objects) continue;
+            const double* vc = &t.vcoords[c * 12];
+            const double* P0 = vc + 3 * fv[f][0];
+            const double* P1 = vc + 3 * fv[f][1];
+            const double* P2 = vc + 3 * fv[f][2];
+            double e1[3], e2[3], nn[3];
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = P1[d] - P0[d];
+                e2[d] = P2[d] - P0[d];
+            }
+            nn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            nn[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            nn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            const double len = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            const double area = 0.5 * len;
+            double side = 0.0;  // orientation: away from the opposite vertex
+            for (int d = 0; d < 3; ++d) side += nn[d] * (vc[3 * f + d] - P0[d]);
+            const double sg = side > 0 ? -1.0 : 1.0;
+            for (int d = 0; d < 3; ++d) nn[d] *= sg / len;
+            double g[4][3];
+            tet_gradients(vc, g);
+            for (int k = 0; k < 4; ++k)
+                qf[-a - 2].push_back({cd[c * 4 + k], eps0 * area * (g[k][0] * nn[0] + g[k][1] * nn[1] + g[k][2] * nn[2])});
+        }
+    std::vector<int64_t> qoff(1, 0);
+    std::vector<int32_t> qdof;
+    std::vector<double> qw;
+    for (auto& l : qf) {
+        std::sort(l.begin(), l.end(), [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) {
+            return x.first < y.first;
+        });
+        for (size_t k = 0; k < l.size();) {  // merge equal dofs in facet order
+            double sum = 0.0;
+            size_t e = k;
+            for (; e < l.size() && l[e].first == l[k].first; ++e) sum += l[e].second;
+            qdof.push_back(l[k].first);
+            qw.push_back(sum);
+            k = e;
+        }
+        qoff.push_back((int64_t)qdof.size());
+    }
     std::vector<double> Kr(K), dk((size_t)n), dm((size_t)n);
     std::vector<uint8_t> isd((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
@@ -467,6 +546,9 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     up(&F->kfull, ell(K));
     up(&F->mass, ell(Mv));
     for (int d = 0; d < 3; ++d) up(&F->grad[d], ell(B[d]));
+    up(&F->q_off, qoff);
+    up(&F->q_dof, qdof);
+    up(&F->q_w, qw);
     up(&F->dinv_k, dk);
     up(&F->dinv_m, dm);
     up(&F->isd, isd);
@@ -476,6 +558,9 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     if (e == hipSuccess) e = dmalloc(&F->E, (size_t)n * 3);
     if (e == hipSuccess) e = dmalloc(&F->part, (size_t)F->grid * 5);
     if (e == hipSuccess) e = dmalloc(&F->S, 1);
+    if (e == hipSuccess) e = dmalloc(&F->q_out, (size_t)std::max(n_objects, 1));
+    if (e == hipSuccess) e = dmalloc(&F->zero, (size_t)n);
+    if (e == hipSuccess) e = hipMemset(F->zero, 0, n * 8);
     if (e == hipSuccess) e = hipMemset(F->S, 0, sizeof(CgState));
     if (e == hipSuccess) e = hipMemset(F->phi, 0, n * 8);
     if (e == hipSuccess) e = hipMemset(F->E, 0, n * 24);
@@ -631,6 +716,235 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
     }
     FCHK(hipStreamSynchronize(F->st));
     if (iterations) *iterations = total;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Capacitance-matrix circuit (CircuitCM, object_CM.cpp:53-357; charge-sharing
+// sets, object.cpp:68-135).  Small dense algebra on the host; the solves and
+// the charge functionals on the device.
+// ---------------------------------------------------------------------------
+bool field_object_charges(FieldSolver* F, const double* d_phi, double* out, std::string& err) {
+    if (F->nobj == 0) return true;
+    FCHK(hipSetDevice(F->device));
+    hipLaunchKernelGGL(k_image_charge, dim3(F->nobj), dim3(FB), 0, F->st, F->q_off, F->q_dof, F->q_w, d_phi, F->q_out);
+    FCHK(hipGetLastError());
+    FCHK(hipMemcpyAsync(out, F->q_out, F->nobj * 8, hipMemcpyDeviceToHost, F->st));
+    FCHK(hipStreamSynchronize(F->st));
+    return true;
+}
+
+// Inverse by LU with partial pivoting (the reference: ublas lu_factorize +
+// lu_substitute on the identity); false if singular.
+static bool invert(std::vector<double> A, int n, std::vector<double>& inv) {
+    std::vector<int> piv((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i * n + k]) > fabs(A[p * n + k])) p = i;
+        if (A[p * n + k] == 0.0) return false;
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+        for (int i = k + 1; i < n; ++i) {
+            A[i * n + k] /= A[k * n + k];
+            for (int j = k + 1; j < n; ++j) A[i * n + j] -= A[i * n + k] * A[k * n + j];
+        }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        std::vector<double> b((size_t)n, 0.0);
+        b[c] = 1.0;
+        for (int k = 0; k < n; ++k) std::swap(b[k], b[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= A[i * n + j] * b[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) b[i] -= A[i * n + j] * b[j];
+            b[i] /= A[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) inv[i * n + c] = b[i];
+    }
+    return true;
+}
+
+// get_charge_sharing_set / get_charge_sharing_sets (object.cpp:68-135).
+static void sharing_set(std::vector<CircuitSource>& vs, int node, std::vector<int>& group) {
+    group.push_back(node);
+    size_t i = 0;
+    while (i < vs.size()) {
+        const CircuitSource v = vs[i];
+        if (v.node_a == node) {
+            vs.erase(vs.begin() + i);
+            sharing_set(vs, v.node_b, group);
+        } else if (v.node_b == node) {
+            vs.erase(vs.begin() + i);
+            sharing_set(vs, v.node_a, group);
+        } else {
+            i += 1;
+        }
+    }
+}
+
+static std::vector<std::vector<int>> sharing_sets(std::vector<CircuitSource> vs, int nobj) {
+    std::vector<int> nodes((size_t)nobj);
+    for (int i = 0; i < nobj; ++i) nodes[i] = i;
+    std::vector<std::vector<int>> groups;
+    while (!vs.empty()) {
+        std::vector<int> group;
+        sharing_set(vs, vs[0].node_a, group);
+        groups.push_back(group);
+    }
+    for (auto& g : groups)
+        for (int v : g)
+            if (v != -1) nodes.erase(std::remove(nodes.begin(), nodes.end(), v), nodes.end());
+    groups.erase(std::remove_if(groups.begin(), groups.end(),
+                                [](const std::vector<int>& a) { return std::find(a.begin(), a.end(), -1) != a.end(); }),
+                 groups.end());
+    for (int v : nodes) groups.push_back({v});
+    return groups;
+}
+
+bool field_set_circuit(FieldSolver* F, double dt, const std::vector<CircuitSource>& vsources,
+                       const std::vector<CircuitSource>& isources, double* inv_capacitance, std::string& err) {
+    const int n = F->nobj;
+    if (n == 0) {
+        err = "a circuit needs objects";
+        return false;
+    }
+    for (const auto& v : vsources)
+        if (v.node_b < 0 || v.node_b >= n || v.node_a < -1 || v.node_a >= n) {
+            err = "voltage source node out of range";
+            return false;
+        }
+    for (const auto& v : isources)
+        if (v.node_b < -1 || v.node_b >= n || v.node_a < -1 || v.node_a >= n) {
+            err = "current source node out of range";
+            return false;
+        }
+    // inv_capacitance (object_CM.cpp:53-93): object j at 1, the others at 0,
+    // no charge; capacitance(i, j) = charge on object i of that potential.
+    std::vector<double> C((size_t)n * n), q((size_t)n), pot((size_t)n);
+    const bool warm = F->warm;
+    F->warm = false;
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) pot[i] = i == j ? 1.0 : 0.0;
+        if (!field_poisson(F, F->zero, nullptr, pot.data(), nullptr, err)) {
+            F->warm = warm;
+            return false;
+        }
+        if (!field_object_charges(F, F->phi, q.data(), err)) {
+            F->warm = warm;
+            return false;
+        }
+        for (int i = 0; i < n; ++i) C[i * n + j] = q[i];
+    }
+    F->warm = warm;
+    if (!invert(C, n, F->invC)) {
+        err = "singular capacitance matrix";
+        return false;
+    }
+    if (inv_capacitance) std::copy(F->invC.begin(), F->invC.end(), inv_capacitance);
+    F->dt = dt;
+    F->vsrc = vsources;
+    F->isrc = isources;
+    F->groups = sharing_sets(vsources, n);
+    // objects held at a fixed voltage relative to ground (object_CM.cpp:129-145)
+    F->fixed_voltage.clear();
+    for (size_t i = 0; i < vsources.size(); ++i) {
+        if (vsources[i].node_a != -1) continue;
+        std::vector<int> tmp{vsources[i].node_b};
+        int node = vsources[i].node_b;
+        for (size_t j = 0; j < vsources.size(); ++j)
+            if (vsources[j].node_a == node) {
+                tmp.push_back(vsources[j].node_b);
+                node = vsources[j].node_b;
+            }
+        F->fixed_voltage.push_back(tmp);
+    }
+    F->circuit_vector.assign((size_t)2 * n, 0.0);
+    for (const auto& v : vsources) F->circuit_vector[v.node_b] = v.value;
+    // assemble_matrix (object_CM.cpp:204-248)
+    const auto& iC = F->invC;
+    std::vector<double> M((size_t)4 * n * n, 0.0);
+    auto at = [&](int r, int c) -> double& { return M[(size_t)r * 2 * n + c]; };
+    for (const auto& v : vsources) {
+        at(v.node_b, v.node_b) = 1.0;
+        if (v.node_a != -1) at(v.node_b, v.node_a) = -1.0;
+    }
+    for (const auto& g : F->groups) {
+        for (int j : g) at(n + g[0], n + j) = 1.0;
+        at(g[0], g[0]) = 1.0;
+        for (int j : g) at(g[0], n + j) = -1.0 * iC[g[0] * n + j];
+        for (size_t j = 1; j < g.size(); ++j) {
+            at(n + g[j], g[j]) = 1.0;
+            for (int k : g) at(n + g[j], n + k) = -1.0 * iC[g[j] * n + k];
+        }
+    }
+    for (const auto& fvl : F->fixed_voltage)
+        for (int j : fvl) {
+            at(n + j, j) = 1.0;
+            for (int k : fvl) at(n + j, n + k) = -1.0 * iC[j * n + k];
+        }
+    if (!invert(M, 2 * n, F->inv_circuit)) {
+        err = "singular circuit matrix";
+        return false;
+    }
+    F->has_circuit = true;
+    return true;
+}
+
+bool field_poisson_circuit(FieldSolver* F, const double* d_rho, double* charges, double* potentials, int* iterations,
+                           std::string& err) {
+    if (!F->has_circuit) {
+        err = "no circuit set";
+        return false;
+    }
+    const int n = F->nobj;
+    // pre_solve (object_CM.cpp:40-47, 343-357): current sources, objects grounded
+    for (const auto& s : F->isrc) {
+        const double dQ = s.value * F->dt;
+        if (s.node_a != -1) charges[s.node_a] -= dQ;
+        if (s.node_b != -1) charges[s.node_b] += dQ;
+    }
+    std::vector<double> pot((size_t)n, 0.0), image((size_t)n);
+    int it0 = 0, it1 = 0;
+    if (!field_poisson(F, d_rho, nullptr, pot.data(), &it0, err)) return false;
+    // post_solve (object_CM.cpp:315-341): image charges, assemble_vector, potentials and charges
+    if (!field_object_charges(F, F->phi, image.data(), err)) return false;
+    const auto& iC = F->invC;
+    std::vector<double>& cv = F->circuit_vector;
+    for (const auto& g : F->groups) {
+        double collected = 0.0;
+        for (int j : g) collected += charges[j];
+        cv[n + g[0]] = collected;
+    }
+    for (const auto& g : F->groups) {
+        double c = 0.0;
+        for (int j : g) c += iC[g[0] * n + j] * image[j];
+        cv[g[0]] = c;
+        for (size_t j = 1; j < g.size(); ++j) {
+            c = 0.0;
+            for (int k : g) c += iC[g[j] * n + k] * image[k];
+            cv[n + g[j]] = c;
+        }
+    }
+    for (const auto& fvl : F->fixed_voltage)
+        for (int j : fvl) {
+            double c = 0.0;
+            for (int k : fvl) c += iC[j * n + k] * image[k];
+            cv[n + j] = c;
+        }
+    for (int i = 0; i < n; ++i) {
+        double potential = 0.0, charge = 0.0;
+        for (int j = 0; j < 2 * n; ++j) potential += F->inv_circuit[(size_t)i * 2 * n + j] * cv[j];
+        for (int j = 0; j < 2 * n; ++j) charge += F->inv_circuit[(size_t)(n + i) * 2 * n + j] * cv[j];
+        pot[i] = potential;
+        potentials[i] = potential;
+        charges[i] = charge;
+    }
+    // correction_required: the solve with the circuit's potentials
+    if (!field_poisson(F, d_rho, nullptr, pot.data(), &it1, err)) return false;
+    if (iterations) *iterations = it0 + it1;
     return true;
 }
 

@@ -613,6 +613,62 @@ class FieldSolver:
         self.iterations = (0, it.value)
         return E
 
+    def object_charges(self, phi):
+        """Charge on each object of the potential phi (Charge.ufl)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        q = np.empty(self.n_objects)
+        check(N.lib().punc_gpu_object_charges(self._h, dptr(phi), dptr(q)))
+        return q
+
+    def set_circuit(self, dt, vsources=(), isources=()):
+        """CircuitCM (object_CM.cpp:53-357): sources as (node_a, node_b,
+        value) with node_a = -1 for ground.  Returns the inverse capacitance
+        matrix."""
+        def arrays(src):
+            a = np.array([int(x[0]) for x in src], dtype=np.int32)
+            b = np.array([int(x[1]) for x in src], dtype=np.int32)
+            v = np.array([float(x[2]) for x in src], dtype=np.float64)
+            return a, b, v
+        va, vb, vv = arrays(vsources)
+        ia, ib, iv = arrays(isources)
+        inv_c = np.empty((self.n_objects, self.n_objects))
+        check(N.lib().punc_gpu_circuit_set(self._h, float(dt), len(va), i32ptr(va), i32ptr(vb), dptr(vv), len(ia),
+                                           i32ptr(ia), i32ptr(ib), dptr(iv), dptr(inv_c)))
+        return inv_c
+
+    def solve_circuit(self, rho, objects):
+        """PoissonSolver::solve_circuit: updates each object's charge and
+        potential (objects: n_objects Object instances); returns phi."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        q = np.array([o.charge for o in objects], dtype=np.float64)
+        v = np.empty(self.n_objects)
+        phi = np.empty(self.mesh.ndof)
+        it = N.C.c_int32()
+        check(N.lib().punc_gpu_poisson_solve_circuit(self._h, dptr(rho), dptr(q), dptr(v), dptr(phi),
+                                                     N.C.byref(it)))
+        for o, qi, vi in zip(objects, q, v):
+            o.charge = float(qi)
+            o.set_potential(float(vi))
+        self.iterations = (it.value, 0)
+        return phi
+
+    def solve_population_circuit(self, pop, objects, copy_out=True):
+        """solve_circuit with the population's deposit, E left in the
+        population's field; returns (phi, E) when copy_out."""
+        q = np.array([o.charge for o in objects], dtype=np.float64)
+        v = np.empty(self.n_objects)
+        phi = np.empty(self.mesh.ndof) if copy_out else None
+        E = np.empty((self.mesh.ndof, 3)) if copy_out else None
+        its = (N.C.c_int32 * 2)()
+        check(N.lib().punc_gpu_field_solve_pop_circuit(self._h, pop._h, dptr(q), dptr(v),
+                                                       dptr(phi) if copy_out else None,
+                                                       dptr(E) if copy_out else None, its))
+        for o, qi, vi in zip(objects, q, v):
+            o.charge = float(qi)
+            o.set_potential(float(vi))
+        self.iterations = (its[0], its[1])
+        return (phi, E) if copy_out else None
+
     def solve_population(self, pop, exterior_values=None, object_potentials=None, copy_out=True):
         """The population's deposited rho -> phi -> E, left in the
         population's field for its next push.  Returns (phi, E) when

@@ -1,8 +1,8 @@
 // The reference time loop (interaction/interaction.cpp:342-423) written
 // against the drop-in C++ API (include/punc.h): distribute, field solve on
-// the device (Poisson with a grounded exterior and the object at 0 V, then
-// the E projection), count, push (half step at n == 0), move, update,
-// inject.  Prints one line per step and checks particle-number
+// the device (Poisson with a grounded exterior and a floating object through
+// the capacitance-matrix circuit, then the E projection), count, push (half
+// step at n == 0), move, update, inject.  Prints one line per step and checks particle-number
 // bookkeeping: N(n+1) = N(n) - absorbed + injected, and the diagnostics'
 // conservation properties.
 #include <punc.h>
@@ -35,13 +35,15 @@ int main(int argc, char** argv) {
     ObjectVector objects{std::make_shared<Object>(2)};
     PoissonSolver poisson(mesh, objects, true, 1.0);
     ESolver esolver(poisson);
+    CircuitCM circuit(poisson, VSourceVector{}, ISourceVector{}, 0.05);  // a floating object
     std::vector<double> B{0.0, 0.0, 1.0};
     const double dt = 0.05;
     size_t n_prev = pop.num_of_particles();
     int bad = 0;
+    if (!(circuit.inv_capacitance[0] > 0)) bad++;
     for (int n = 0; n <= steps; ++n) {
         distribute_cg1(pop, rho, dv_inv);
-        poisson.solve(phi, rho, objects);
+        poisson.solve_circuit(phi, rho, objects);
         esolver.solve(E, phi);
         if (n == 0) {
             // the device-resident path gives the same field

@@ -127,3 +127,61 @@ def test_warm_start_same_answer(gpu):
     assert it_w < cold.iterations[0]
     _close(phi_w, phi_c, 1e-9)
     _close(warm.efield(phi_w), cold.efield(phi_c), 1e-8)
+
+
+def _two_object_mesh(n=8):
+    # the sphere's upper facets become a second object (boundary id 3)
+    hm = sphere_mesh(n=n, L=4.0, r=1.5)
+    fv = ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))
+    adj = O.OraclePop(hm, [1.0], [1.0]).localizer()["adj"]
+    for c, f in zip(*np.nonzero(adj == -2)):
+        if hm.xyz[hm.cells[c, list(fv[f])]].mean(axis=0)[2] > 0:
+            hm.facet_bnd[c, f] = 3
+    return hm
+
+
+@pytest.mark.parametrize("case", ["floating", "fixed", "pair"])
+def test_circuit_matches_oracle(gpu, case):
+    from oracle.field import CircuitOracle
+    _print_case("circuit", case=case)
+    hm = _two_object_mesh() if case == "pair" else sphere_mesh(n=8, L=4.0, r=1.5)
+    nobj = 2 if case == "pair" else 1
+    vs = {"floating": [], "fixed": [(-1, 0, 3.0)], "pair": [(0, 1, 2.0)]}[case]
+    isrc = [(-1, 0, 0.5)] if case == "pair" else []
+    mesh = P.Mesh(hm)
+    f = P.FieldSolver(mesh, eps0=1.0, exterior=True, n_objects=nobj)
+    inv_c = f.set_circuit(0.1, vs, isrc)
+    fo = _oracle(hm, 1.0)
+    co = CircuitOracle(fo, nobj, 0.1, vs, isrc)
+    np.testing.assert_allclose(inv_c, co.inv_c, rtol=1e-9)
+    rho = np.random.default_rng(11).normal(size=hm.ndof)
+    objs = [P.Object(2 + j) for j in range(nobj)]
+    for j, o in enumerate(objs):
+        o.charge = 1.0 + j
+    q0 = [o.charge for o in objs]
+    phi = f.solve_circuit(rho, objs)
+    phi_o, pot_o, q_o = co.solve(rho, q0)
+    np.testing.assert_allclose([o.get_potential() for o in objs], pot_o, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose([o.charge for o in objs], q_o, rtol=1e-9, atol=1e-12)
+    _close(phi, phi_o, 1e-9)
+    np.testing.assert_allclose(f.object_charges(phi), fo.object_charges(phi_o, nobj), rtol=1e-8, atol=1e-10)
+
+
+def test_circuit_population_path(gpu):
+    # the deposit of a population through the circuit, E left for the push
+    _print_case("circuit_population")
+    hm = sphere_mesh(n=8, L=4.0, r=1.5)
+    o, mesh, pop = loaded_pair(hm, (5000, 5000))
+    dv = P.element_volume(mesh)
+    P.distribute_cg1(pop, dv)
+    f = P.FieldSolver(mesh, eps0=1.0, exterior=True, n_objects=1)
+    f.set_circuit(0.1)
+    obj = [P.Object(2)]
+    obj[0].charge = -3.0
+    phi, E = f.solve_population_circuit(pop, obj)
+    from oracle.field import CircuitOracle
+    fo = _oracle(hm, 1.0)
+    phi_o, pot_o, q_o = CircuitOracle(fo, 1, 0.1).solve(o.distribute(dv), [-3.0])
+    assert obj[0].get_potential() == pytest.approx(pot_o[0], rel=1e-9)
+    _close(phi, phi_o, 1e-9)
+    _close(E, fo.efield(phi_o), 1e-8)
