@@ -71,7 +71,7 @@ def synthetic_field(hm, seed=1):
     return np.ascontiguousarray(E)
 
 
-def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
+def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, extras=True):
     import puncpp_amd as P
     from puncpp_amd import _native as N
 
@@ -163,9 +163,9 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None):
     t_out = time.perf_counter()
     xfer = {"field_in_ms": 1e3 * (t_mid - t_in) / 5, "rho_out_ms": 1e3 * (t_out - t_mid) / 5,
             "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
-    diag = run_diagnostics(pop, mesh, dv)
-    flux_mc = run_flux_mc(mesh)
-    field = run_field_solve(pop, mesh)
+    diag = run_diagnostics(pop, mesh, dv) if extras else None
+    flux_mc = run_flux_mc(mesh) if extras else None
+    field = run_field_solve(pop, mesh) if extras else None
     elapsed = t2 - t1
     n_avg = 0.5 * (n_start + n_end)
     value = n_avg * steps / elapsed
@@ -246,9 +246,25 @@ def run_field_solve(pop, mesh, reps=3):
         wms.append(1e3 * (time.perf_counter() - t))
     log(f"[bench] field solve, warm start after a step: {np.mean(wms):.2f} ms (Poisson {f.iterations[0]} + "
         f"E {f.iterations[1]} CG iterations)")
+    # the fused step's potential energy (on since the field solve set phi):
+    # the particle kernel's average with and without it, over 6 steps each
+    kern = {}
+    for on in (True, False):
+        if not on:
+            N.check(lib.punc_gpu_set_potential(pop._h, None))
+        N.check(lib.punc_gpu_sort(pop._h))
+        N.check(lib.punc_gpu_timing(pop._h, 1))
+        for k in range(6):
+            N.check(lib.punc_gpu_step(pop._h, 2, 0.02, N.dptr(B), 0.02, None, 1, 0, 20000 + k))
+        ms_tot, nl = N.C.c_double(), N.C.c_int64()
+        N.check(lib.punc_gpu_timing_read(pop._h, N.C.byref(ms_tot), N.C.byref(nl)))
+        N.check(lib.punc_gpu_timing(pop._h, 0))
+        kern["pe_on" if on else "pe_off"] = ms_tot.value / max(nl.value, 1)
+    log(f"[bench] fused step kernel with the potential energy {kern['pe_on']:.3f} ms, without {kern['pe_off']:.3f} ms")
     return {"ms": ms, "dofs": mesh.ndof, "poisson_iterations": its[0], "efield_iterations": its[1],
             "warm_start_ms": float(np.mean(wms)), "warm_poisson_iterations": f.iterations[0],
-            "warm_efield_iterations": f.iterations[1], "preconditioner": "Jacobi", "rtol": 1e-12}
+            "warm_efield_iterations": f.iterations[1], "preconditioner": "Jacobi", "rtol": 1e-12,
+            "step_kernel_ms_with_potential_energy": kern["pe_on"], "step_kernel_ms_without": kern["pe_off"]}
 
 
 def run_diagnostics(pop, mesh, dv, reps=5):

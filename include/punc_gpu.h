@@ -316,6 +316,17 @@ int punc_gpu_graph_launch(punc_pop_t* p, int count);
 int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t* n_injected);
 /* rho (ndof) from the last deposit. */
 int punc_gpu_get_rho(punc_pop_t* p, double* rho);
+/* The potential phi (ndof) at the particles' positions at the start of each
+ * fused step: with it set, every punc_gpu_step / graph replay also computes
+ * particle_potential_energy_cg1 (diagnostics.h:379-412) of the positions it
+ * starts from - the pair the reference's loop evaluates before the push
+ * (interaction.cpp:368-371) - inside the particle kernel, read back with
+ * punc_gpu_step_potential_energy.  phi == NULL switches it off.
+ * punc_gpu_field_solve_pop sets it from the solved potential. */
+int punc_gpu_set_potential(punc_pop_t* p, const double* phi);
+/* Potential energy computed by the last fused step (NaN when no potential
+ * was set). */
+int punc_gpu_step_potential_energy(punc_pop_t* p, double* PE);
 
 /* ---- adjoint ----------------------------------------------------------- */
 /* Reverse mode through K steps of boris_cg1 + move (+ the relocation walk)

@@ -94,6 +94,8 @@ _SIGS = {
                                 C.c_uint64]),
     "punc_gpu_step_results": (C.c_int, [vp, d_p, i64_p, i64_p]),
     "punc_gpu_get_rho": (C.c_int, [vp, d_p]),
+    "punc_gpu_set_potential": (C.c_int, [vp, d_p]),
+    "punc_gpu_step_potential_energy": (C.c_int, [vp, d_p]),
     "punc_gpu_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "punc_gpu_get_launch": (C.c_int, [vp, i32_p, i32_p, i32_p]),
     "punc_gpu_autotune": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, d_p, C.c_int, C.c_uint64,

@@ -156,6 +156,10 @@ struct punc_pop_s {
     double* d_rho_ema = nullptr;   // ndof
     double* d_cvol = nullptr;      // nc
     double4* d_cellphi = nullptr;  // nc: phi at each cell's four dofs
+    // fused-step potential energy (phi held in the field rows' slots 12..15)
+    bool pe_on = false;
+    double* d_pe_part = nullptr;  // ke_np
+    double* d_pe = nullptr;       // 1: the last fused step's PE (NaN when off)
     int64_t dens_len = 0;          // entries per species class of the last density (0: none)
     int64_t dens_ema_len = 0;
     // timing
@@ -475,6 +479,8 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_rho_ema);
     hipFree(P->d_cvol);
     hipFree(P->d_cellphi);
+    hipFree(P->d_pe_part);
+    hipFree(P->d_pe);
     for (auto& c : P->coef_cache) hipFree(c.d);
     hipFree(P->d_scount);
     hipFree(P->d_soffset);
@@ -567,6 +573,8 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_rho, ndof));
     PCHK(dalloc(&P->d_dv_inv, ndof));
     PCHK(dalloc(&P->d_ke_part, P->ke_np));
+    PCHK(dalloc(&P->d_pe_part, P->ke_np));
+    PCHK(dalloc(&P->d_pe, 1));
     PCHK(dalloc(&P->d_ke, 4));
     const int64_t nc = m->topo.nc;
     PCHK(dalloc(&P->d_cellE, nc * 16));
@@ -1230,8 +1238,12 @@ static int field_solve_pop_common(punc_field_t* f, punc_pop_t* p, const double* 
     // the field the next push reads: packed dofs and the per-cell copy
     HIPCHK(hipMemcpy(p->d_E, field_E(f->F), n * 24, hipMemcpyDeviceToDevice));
     HIPCHK(launch_cell_field(p->mesh->dm, p->d_E, p->d_cellE, p->stream));
+    // the potential too: the next fused step reports the potential energy
+    HIPCHK(launch_cell_potential(p->mesh->dm, field_phi(f->F), p->d_cellE, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     p->field_set = true;
+    if (!p->pe_on) graph_free(p);
+    p->pe_on = true;
     if (phi) HIPCHK(hipMemcpy(phi, field_phi(f->F), n * 8, hipMemcpyDeviceToHost));
     if (E) HIPCHK(hipMemcpy(E, field_E(f->F), n * 24, hipMemcpyDeviceToHost));
     if (iterations) {
@@ -1899,9 +1911,12 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
     for (int i = 0; i < 3; ++i) L.Ld[i] = a.Ld[i];
     L.locate = true;
     L.deposit = true;
+    if (P->pe_on) L.pe_part = P->d_pe_part;
     HIPCHK(timed_launch(P, L, true));
     HIPCHK(launch_step_end(P->d_state, P->stream));
     HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
+    if (P->pe_on) HIPCHK(launch_sum_partials(P->d_pe_part, L.grid, P->d_pe, P->stream));
+    else HIPCHK(launch_fill_f64(P->d_pe, 1, NAN, P->stream));
     HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_H, P->d_S, P->grid, P->stream));
     if (inj) {
         HIPCHK(hipStreamWaitEvent(P->stream, P->ev_join, 0));
@@ -2015,6 +2030,29 @@ extern "C" int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbe
         if (r) return r;
         *n_injected = s.inj_kept;
     }
+    return 0;
+}
+
+static int diag_alloc(punc_pop_s* P);
+
+extern "C" int punc_gpu_set_potential(punc_pop_t* p, const double* phi) {
+    if (!p) return fail("set_potential: null population");
+    HIPCHK(hipSetDevice(p->device));
+    int r = diag_alloc(p);
+    if (r) return r;
+    if (phi) HIPCHK(hipMemcpyAsync(p->d_phi, phi, p->mesh->topo.ndof * 8, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(launch_cell_potential(p->mesh->dm, phi ? p->d_phi : nullptr, p->d_cellE, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->pe_on != (phi != nullptr)) graph_free(p);  // a captured step holds the other kernel
+    p->pe_on = phi != nullptr;
+    return 0;
+}
+
+extern "C" int punc_gpu_step_potential_energy(punc_pop_t* p, double* PE) {
+    if (!p || !PE) return fail("step_potential_energy: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipMemcpyAsync(PE, p->d_pe, 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }
 

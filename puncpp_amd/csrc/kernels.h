@@ -25,6 +25,7 @@ struct ParticleLaunch {
     bool locate;
     bool deposit;
     int dep_mode;  // 0: per-particle atomics, 1: wavefront run reduction, 2: + LDS table
+    double* pe_part;  // fused step: per-block potential energy (nullptr: not computed)
     int grid;
     int block;
 };
@@ -63,6 +64,9 @@ hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st);
 // E3: 3 FP64 components per dof, packed.
 hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st);
 hipError_t launch_expand_field(const double* E3, double4* E, int64_t ndof, hipStream_t st);
+// phi (ndof, device) at each cell's dofs into slots 12..15 of the cell's field
+// row, read by the fused step's potential energy; nullptr clears them.
+hipError_t launch_cell_potential(const DevMesh& M, const double* phi, double* cellE, hipStream_t st);
 hipError_t launch_kinetic(DevState* S, const SpCoef* coef, int cell_bits, double* ke_part, int grid,
                           hipStream_t st);
 hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);

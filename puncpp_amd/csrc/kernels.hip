@@ -590,13 +590,16 @@ __device__ inline SpCoef* step_coef_lds() {
     return (SpCoef*)(dyn_smem + (DMODE >= 2 ? LDS_TABLE_BYTES / 8 : 0));
 }
 
-template <int PUSH, int MOVE, int DMODE>
+// PE: also the particle potential energy at the step's starting positions
+// (particle_potential_energy_cg1, diagnostics.h:379-412), phi read from
+// slots 12..15 of the cell's field row: per-block sums into pe_part.
+template <int PUSH, int MOVE, int DMODE, bool PE>
 __global__ void __launch_bounds__(256)
 k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ gcoef, const int nsp,
             const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
-            WalkRec* __restrict__ wrec) {
+            WalkRec* __restrict__ wrec, double* __restrict__ pe_part) {
     typedef double v2d __attribute__((ext_vector_type(2)));
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     __shared__ double sh[8];
@@ -612,7 +615,10 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     int64_t s, e;
     chunk_range(n, s, e);
     if (s >= e) {  // block-uniform: nothing in this block's chunk
-        if (threadIdx.x == 0) ke_part[blockIdx.x] = 0.0;
+        if (threadIdx.x == 0) {
+            ke_part[blockIdx.x] = 0.0;
+            if (PE) pe_part[blockIdx.x] = 0.0;
+        }
         return;
     }
     // A pending cell order: particle j of this step is cur[perm[j]], and the
@@ -623,7 +629,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     const int32_t cmask = (1 << cell_bits) - 1;
     const double tol = M.inside_tol;
     const double X = M.coord_bound;
-    double ke = 0.0;
+    double ke = 0.0, pe = 0.0;
     constexpr bool use_lds = DMODE >= 2;
     if (threadIdx.x == 0) wcnt = 0;
     if (use_lds) lds_clear(lds_table());
@@ -668,6 +674,14 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             ef[2 * m] = t.x;
             ef[2 * m + 1] = t.y;
         }
+        double ph[4];
+        if (PE) {
+            const v2d a = ep[6], b = ep[7];
+            ph[0] = a.x;
+            ph[1] = a.y;
+            ph[2] = b.x;
+            ph[3] = b.y;
+        }
         __builtin_amdgcn_sched_barrier(0);
         // Prefetch: the order entries of the group after next, then the next
         // group (clamped index: lanes past the end reload the chunk's last
@@ -682,6 +696,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         double l[4];
         bary4(rb, x, y, z, l);
         const SpCoef& K = coef[sp];
+        if (PE && active) {
+            double phi_x = 0.0;
+            for (int k = 0; k < 4; ++k) phi_x += l[k] * ph[k];
+            pe += 0.5 * K.q * phi_x;
+        }
         double ke_l = 0.0;
         push_velocity<PUSH>(K, l, ef, vx, vy, vz, ke_l);
         // move / move_periodic (pusher.h:658, 676-677)
@@ -738,6 +757,10 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     }
     const double t = block_sum(ke, sh);  // synchronizes: wcnt and the segment are complete
     if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
+    if (PE) {
+        const double tp = block_sum(pe, sh);
+        if (threadIdx.x == 0) pe_part[blockIdx.x] = tp;
+    }
     // Phase 2: the walks of this block's list.  A.cs is rewritten only when
     // the cell changes.
     const int64_t we = s + wcnt;
@@ -804,7 +827,17 @@ __global__ void k_cell_field(const DevMesh M, const double* __restrict__ E3, dou
             o[3 * k + 1] = v[1];
             o[3 * k + 2] = v[2];
         }
-        for (int k = 12; k < CELL_FIELD_STRIDE; ++k) o[k] = 0.0;
+        // slots 12..15: phi at the four dofs (k_cell_potential), kept
+    }
+}
+
+// phi at the cell's four dofs into slots 12..15 of its field row (the fused
+// step's potential energy); phi == nullptr clears them.
+__global__ void k_cell_potential(const DevMesh M, const double* __restrict__ phi, double* __restrict__ cellE) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < M.nc; c += (int64_t)gridDim.x * blockDim.x) {
+        const CellRec& R = M.rec[c];
+        double* o = cellE + c * CELL_FIELD_STRIDE + 12;
+        for (int k = 0; k < 4; ++k) o[k] = phi ? phi[R.dof[k]] : 0.0;
     }
 }
 
@@ -1797,9 +1830,14 @@ template <int PUSH, int MOVE, int DMODE>
 static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
     if (L.nsp < 1 || L.nsp > MAX_SPECIES) return hipErrorInvalidValue;
     const size_t shmem = (DMODE >= 2 ? LDS_TABLE_BYTES : 0) + (size_t)L.nsp * sizeof(SpCoef);
-    hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh, L.state,
-                       L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw, L.ke_part,
-                       L.holes, L.absorbed, (WalkRec*)L.walk);
+    if (L.pe_part)
+        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, true>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
+                           L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, L.pe_part);
+    else
+        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, false>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
+                           L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, nullptr);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1834,6 +1872,12 @@ hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st) {
 
 hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st) {
     hipLaunchKernelGGL(k_cell_field, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, E3, cellE);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_cell_potential(const DevMesh& M, const double* phi, double* cellE, hipStream_t st) {
+    hipLaunchKernelGGL(k_cell_potential, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, phi, cellE);
     LAUNCH_CHECK();
     return hipSuccess;
 }

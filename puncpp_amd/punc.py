@@ -463,6 +463,23 @@ class Population:
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
 
+    def set_potential(self, phi):
+        """phi (ndof) for the potential energy each fused step computes at
+        its starting positions (None switches it off)."""
+        if phi is None:
+            check(N.lib().punc_gpu_set_potential(self._h, None))
+            return
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        if phi.shape != (self.mesh.ndof,):
+            raise ValueError(f"phi must hold {self.mesh.ndof} dof values")
+        check(N.lib().punc_gpu_set_potential(self._h, dptr(phi)))
+
+    def step_potential_energy(self):
+        """Potential energy computed by the last fused step (NaN if off)."""
+        v = N.C.c_double()
+        check(N.lib().punc_gpu_step_potential_energy(self._h, N.C.byref(v)))
+        return v.value
+
     def save_file(self, fname, binary=True):
         """Population::save_file (population.h:807-839): every particle as
         x[3], v[3], q, m - 64-byte binary records or "%.17g" text lines - in

@@ -182,3 +182,44 @@ def test_large_density_properties(gpu):
     # drifting Maxwellian speeds, a = |drift| / vth:
     # E|v| = vth (sqrt(2/pi) exp(-a^2/2) + (a + 1/a) erf(a/sqrt(2)))
     assert s[0] == pytest.approx(1.66144, rel=1e-3) and s[2] == pytest.approx(0.505, rel=1e-3)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_step_potential_energy(gpu, graph):
+    # the potential energy computed inside the fused step equals the oracle's
+    # particle_potential_energy_cg1 at the positions the step starts from,
+    # and the step's particles are unchanged by computing it
+    from helpers import assert_same_sets, loaded_pair, synthetic_field
+    from puncpp_amd import _native as N
+    _print_case("fused_step_pe", graph=graph)
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    o, mesh, pop = loaded_pair(hm, (4000, 4000))
+    E = synthetic_field(hm, scale=0.2)
+    phi = hm.xyz @ np.array([0.2, -0.5, 0.3]) + np.random.default_rng(6).normal(size=hm.ndof) * 0.1
+    dv = P.element_volume(mesh)
+    B = np.array([0.0, 0.0, 1.0])
+    L = N.lib()
+    N.check(L.punc_gpu_set_field(pop._h, N.dptr(E)))
+    N.check(L.punc_gpu_distribute(pop._h, N.dptr(dv), None))
+    pop.set_potential(phi)
+    dt = 0.05
+    q = np.array([QE, QI])
+    for n in range(3):
+        pe_o = o.potential_energy(phi)
+        if graph:
+            if n == 0:
+                N.check(L.punc_gpu_step_graph(pop._h, 2, dt, N.dptr(B), dt, None, 0, 0, 0))
+            N.check(L.punc_gpu_graph_launch(pop._h, 1))
+        else:
+            N.check(L.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 0, 0, n))
+        pe = pop.step_potential_energy()
+        st = o.get_particles()
+        scale = 0.5 * np.sum(np.abs(q[st["species"]])) * np.abs(phi).max()
+        assert abs(pe - pe_o) <= 1e-11 * scale, (pe, pe_o)
+        o.boris(E, B, dt)
+        o.move(dt)
+        o.update()
+        assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what=f"pe step {n}")
+    pop.set_potential(None)
+    N.check(L.punc_gpu_step(pop._h, 2, dt, N.dptr(B), dt, None, 0, 0, 9))
+    assert np.isnan(pop.step_potential_energy())
