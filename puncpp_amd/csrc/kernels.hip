@@ -215,24 +215,29 @@ __device__ inline void lds_clear(LdsTable t) {
 }
 
 __device__ inline void lds_add(LdsTable t, double* __restrict__ rho_raw, int32_t dof, double v) {
+    // Linear probing from the hashed slot.  One exit for the common case (the
+    // slot found or claimed within the probe limit), so the add below runs
+    // once for the whole wavefront.
     uint32_t h = ((uint32_t)dof * 0x9E3779B1u) >> (32 - 11);  // 11 = log2(LDS_SLOTS)
-    for (int probe = 0; probe < 16; ++probe) {
-        const int32_t k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (k == dof) {
-            unsafeAtomicAdd(&t.val[h], v);
-            return;
-        }
+    int probe = 0;
+#pragma unroll 1
+    for (;;) {
+        int32_t k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (k == LDS_EMPTY) {
-            const int32_t old = atomicCAS(&t.key[h], LDS_EMPTY, dof);
-            if (old == LDS_EMPTY || old == dof) {
-                unsafeAtomicAdd(&t.val[h], v);
-                if (old == LDS_EMPTY) atomicAdd(t.count, 1);
-                return;
+            k = atomicCAS(&t.key[h], LDS_EMPTY, dof);
+            if (k == LDS_EMPTY) {
+                atomicAdd(t.count, 1);
+                k = dof;
             }
         }
+        if (k == dof) break;
         h = (h + 1) & (LDS_SLOTS - 1);
+        if (++probe == 16) {
+            unsafeAtomicAdd(&rho_raw[dof], v);  // table region full: straight to HBM
+            return;
+        }
     }
-    unsafeAtomicAdd(&rho_raw[dof], v);  // table region full: straight to HBM
+    unsafeAtomicAdd(&t.val[h], v);
 }
 
 // Every thread of the block must call this (it synchronizes).
