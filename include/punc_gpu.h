@@ -234,6 +234,17 @@ int punc_gpu_field_set_max_iterations(punc_field_t* f, int maxit);
 /* warm != 0: each solve starts from the previous solution (the reference's
  * solvers start from zero; the result agrees to the solver tolerance). */
 int punc_gpu_field_set_warm_start(punc_field_t* f, int warm);
+/* Preconditioner of the Poisson solves: 1 smoothed-aggregation algebraic
+ * multigrid, the default whenever its hierarchy can be built (the reference's
+ * PETSc solver takes hypre's AMG, poisson.cpp:299-304); 0 Jacobi.  Setting 1
+ * fails when there is no hierarchy (operators of at most 256 dofs, or
+ * coarsening that stalls).  The E projection is always Jacobi-preconditioned
+ * (a mass matrix).  No counterpart in the reference's API beyond that choice
+ * of PETSc preconditioner. */
+int punc_gpu_field_set_preconditioner(punc_field_t* f, int kind);
+/* kind: current preconditioner; levels: multigrid levels above the dense
+ * coarsest solve; coarse: that solve's size (any pointer may be NULL). */
+int punc_gpu_field_get_preconditioner(punc_field_t* f, int* kind, int* levels, int64_t* coarse);
 /* rho (ndof) -> phi (ndof).  exterior_values: one value per dof, read at the
  * exterior Dirichlet dofs (NULL: 0); object_potentials: n_objects values
  * (NULL: 0).  iterations may be NULL. */

@@ -76,6 +76,9 @@ _SIGS = {
     "punc_gpu_field_destroy": (None, [vp]),
     "punc_gpu_field_set_max_iterations": (C.c_int, [vp, C.c_int]),
     "punc_gpu_field_set_warm_start": (C.c_int, [vp, C.c_int]),
+    "punc_gpu_field_set_preconditioner": (C.c_int, [vp, C.c_int]),
+    "punc_gpu_field_get_preconditioner": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                    C.POINTER(C.c_int64)]),
     "punc_gpu_poisson_solve": (C.c_int, [vp, d_p, d_p, d_p, d_p, i32_p]),
     "punc_gpu_efield_solve": (C.c_int, [vp, d_p, d_p, i32_p]),
     "punc_gpu_field_solve_pop": (C.c_int, [vp, vp, d_p, d_p, d_p, d_p, i32_p]),

@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["kernels.hip", "capi.hip", "adjoint.hip", "field.hip", "mesh_host.cpp"]
+SOURCES = ["kernels.hip", "capi.hip", "adjoint.hip", "field.hip", "mesh_host.cpp", "amg.cpp"]
 
 
 def _run(cmd, verbose):

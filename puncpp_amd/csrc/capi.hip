@@ -1163,6 +1163,20 @@ extern "C" int punc_gpu_field_set_warm_start(punc_field_t* f, int warm) {
     return 0;
 }
 
+extern "C" int punc_gpu_field_set_preconditioner(punc_field_t* f, int kind) {
+    if (!f) return fail("field_set_preconditioner: null field");
+    std::string err;
+    if (!field_set_preconditioner(f->F, kind, err)) return fail("field_set_preconditioner: " + err);
+    return 0;
+}
+
+extern "C" int punc_gpu_field_get_preconditioner(punc_field_t* f, int* kind, int* levels, int64_t* coarse) {
+    if (!f) return fail("field_get_preconditioner: null field");
+    const int k = field_preconditioner(f->F, levels, coarse);
+    if (kind) *kind = k;
+    return 0;
+}
+
 extern "C" int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* exterior_values,
                                       const double* object_potentials, double* phi, int* iterations) {
     if (!f || !rho || !phi) return fail("poisson_solve: null argument");

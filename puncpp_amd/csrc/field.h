@@ -63,5 +63,12 @@ void field_set_max_iterations(FieldSolver* F, int maxit);
 // as the reference's solvers start from zero); the answer is the same to the
 // solver tolerance, reached in fewer iterations when the field changes little.
 void field_set_warm_start(FieldSolver* F, bool warm);
+// Preconditioner of the Poisson (stiffness) solves: 1 smoothed-aggregation
+// multigrid (the default when its hierarchy could be built), 0 Jacobi.  The E
+// projection (mass matrix) is always Jacobi-preconditioned.
+bool field_set_preconditioner(FieldSolver* F, int kind, std::string& err);
+// The current preconditioner (0/1); levels: multigrid levels above the dense
+// coarsest solve, coarse: its size (0 without a hierarchy).
+int field_preconditioner(const FieldSolver* F, int* levels, int64_t* coarse);
 
 }  // namespace punc_gpu

@@ -20,6 +20,8 @@
 // without host synchronisation.
 #include "field.h"
 
+#include "amg.h"
+
 #include <math.h>
 
 #include <algorithm>
@@ -33,7 +35,8 @@ namespace {
 
 constexpr int FB = 256;         // threads per block
 constexpr int MAX_BLOCKS = 1024;  // also the partials every block of the next kernel adds
-constexpr int BATCH = 32;       // CG iterations between convergence checks
+constexpr int BATCH = 32;       // CG iterations between convergence checks (Jacobi; even)
+constexpr int BATCH_AMG = 8;    // the same with the multigrid preconditioner (even)
 
 struct CgState {
     double rz[2];  // r.z entering iteration k at [k & 1]
@@ -81,6 +84,19 @@ __device__ inline double row_dot(const int32_t* __restrict__ col, const double* 
     return s;
 }
 
+// The same for NC interleaved vectors (x[NC * j + c]): the operator applied
+// to each component (the block-diagonal matrix of a vector CG1 space).
+template <int NC>
+__device__ inline void row_dot_nc(const int32_t* __restrict__ col, const double* __restrict__ av, int w, int64_t n,
+                                  const double* __restrict__ x, int64_t i, double out[NC]) {
+    for (int c = 0; c < NC; ++c) out[c] = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const double a = av[k * n + i];
+        const int64_t j = col[k * n + i];
+        for (int c = 0; c < NC; ++c) out[c] += a * x[NC * j + c];
+    }
+}
+
 __global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ col, int w, const double* __restrict__ av,
                                              const double* __restrict__ x, double* __restrict__ y, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -99,10 +115,22 @@ __global__ void k_rhs_dirichlet(const double* __restrict__ b, const double* __re
     }
 }
 
-// Ec[i] = E3[3i + c] (warm start of a component solve), or 0.
-__global__ void k_unpack3(const double* __restrict__ E3, int c, double* __restrict__ e, int64_t n, int warm) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        e[i] = warm ? E3[3 * i + c] : 0.0;
+// rhs[3i + c] = (B_c phi)_i: the three gradient matrices share the sparsity.
+__global__ void __launch_bounds__(FB) k_grad3(const int32_t* __restrict__ col, int w, const double* __restrict__ g0,
+                                              const double* __restrict__ g1, const double* __restrict__ g2,
+                                              const double* __restrict__ phi, double* __restrict__ rhs, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < w; ++k) {
+            const double f = phi[col[k * n + i]];
+            s0 += g0[k * n + i] * f;
+            s1 += g1[k * n + i] * f;
+            s2 += g2[k * n + i] * f;
+        }
+        rhs[3 * i] = s0;
+        rhs[3 * i + 1] = s1;
+        rhs[3 * i + 2] = s2;
+    }
 }
 
 // Per-block sums of v, then the mean (one block).
@@ -128,7 +156,11 @@ __global__ void k_remove_mean(double* __restrict__ v, int64_t n, const CgState* 
         v[i] -= m;
 }
 
-// CG start: r = b - A x (x given), z = D^-1 r, p = z; per-block r.z, r.r, b.b.
+// CG start: r = b - A x (x given), z = D^-1 r, p = z; per-block r.z, r.r, b.b
+// (dinv null: r, r.r and b.b; the V-cycle writes z and r.z).  NC: the
+// vectors hold NC interleaved components (one CG on the block-diagonal
+// system); A and D^-1 are those of one component.
+template <int NC>
 __global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ col, int w, const double* __restrict__ av,
                                                 const double* __restrict__ dinv, const double* __restrict__ b,
                                                 const double* __restrict__ x, double* __restrict__ r,
@@ -137,18 +169,25 @@ __global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ col,
     __shared__ double sh[8];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double ri = b[i] - row_dot(col, av, w, n, x, i);
-        const double zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        a0 += ri * zi;
-        a1 += ri * ri;
-        a2 += b[i] * b[i];
+        double ax[NC];
+        row_dot_nc<NC>(col, av, w, n, x, i, ax);
+        for (int c = 0; c < NC; ++c) {
+            const int64_t m = NC * i + c;
+            const double ri = b[m] - ax[c];
+            r[m] = ri;
+            if (dinv) {  // Jacobi; without it the V-cycle writes z and r.z
+                const double zi = dinv[i] * ri;
+                z[m] = zi;
+                p[m] = zi;
+                a0 += ri * zi;
+            }
+            a1 += ri * ri;
+            a2 += b[m] * b[m];
+        }
     }
     const double t0 = block_reduce(a0, sh), t1 = block_reduce(a1, sh), t2 = block_reduce(a2, sh);
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = t0;
+        if (dinv) part[3 * blockIdx.x] = t0;
         part[3 * blockIdx.x + 1] = t1;
         part[3 * blockIdx.x + 2] = t2;
     }
@@ -179,6 +218,7 @@ __global__ void __launch_bounds__(FB) k_cg_init_finish(const double* __restrict_
 // the updated p needs neighbours' p; A z only needs z, complete since the
 // last kernel).  Per-block p.q.  Block 0 records rz_t and, at convergence,
 // the flag and the iteration count.
+template <int NC>
 __global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const int32_t* __restrict__ col, int w,
                                                     const double* __restrict__ av, const double* __restrict__ z,
                                                     double* __restrict__ p, double* __restrict__ q, int64_t n,
@@ -203,19 +243,25 @@ __global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const int32_t* __
     }
     double a = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double az = row_dot(col, av, w, n, z, i);
-        const double pi = first ? z[i] : z[i] + beta * p[i];
-        const double qi = first ? az : az + beta * q[i];
-        p[i] = pi;
-        q[i] = qi;
-        a += pi * qi;
+        double az[NC];
+        row_dot_nc<NC>(col, av, w, n, z, i, az);
+        for (int c = 0; c < NC; ++c) {
+            const int64_t m = NC * i + c;
+            const double pi = first ? z[m] : z[m] + beta * p[m];
+            const double qi = first ? az[c] : az[c] + beta * q[m];
+            p[m] = pi;
+            q[m] = qi;
+            a += pi * qi;
+        }
     }
     const double t = block_reduce(a, sh);
     if (threadIdx.x == 0) part_pq[blockIdx.x] = t;
 }
 
 // Step 2: alpha = rz_t / p.q (every block adds the partials); x += alpha p,
-// r -= alpha q, z = D^-1 r; per-block r.z and r.r.
+// r -= alpha q, z = D^-1 r; per-block r.z and r.r (dinv null: r.r only, the
+// multigrid V-cycle that follows writes z and r.z).
+template <int NC>
 __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict__ x, double* __restrict__ r,
                                                   double* __restrict__ z, const double* __restrict__ p,
                                                   const double* __restrict__ q, const double* __restrict__ dinv,
@@ -226,19 +272,119 @@ __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict_
     const double alpha = S->rz[k & 1] / sum_partials(part_pq, cnt_pq, 1, 0, sh);
     double a0 = 0.0, a1 = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
-        const double zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        a0 += ri * zi;
-        a1 += ri * ri;
+        for (int c = 0; c < NC; ++c) {
+            const int64_t m = NC * i + c;
+            x[m] += alpha * p[m];
+            const double ri = r[m] - alpha * q[m];
+            r[m] = ri;
+            if (dinv) {  // Jacobi; without it the V-cycle writes z and r.z
+                const double zi = dinv[i] * ri;
+                z[m] = zi;
+                a0 += ri * zi;
+            }
+            a1 += ri * ri;
+        }
     }
     const double t0 = block_reduce(a0, sh), t1 = block_reduce(a1, sh);
     if (threadIdx.x == 0) {
-        part2[2 * blockIdx.x] = t0;
+        if (dinv) part2[2 * blockIdx.x] = t0;
         part2[2 * blockIdx.x + 1] = t1;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid V-cycle (the preconditioner z = M r of the stiffness solves).
+// Level l holds A_l, the prolongation P_l from level l+1 and R_l = P_l^T in
+// CSR, and 1/diag(A_l).  Going down, b_l -> x_l = w D^-1 b_l, r_l = b_l -
+// A_l x_l, b_{l+1} = R_l r_l; the coarsest level applies its dense inverse;
+// going up, x_l += P_l y_{l+1}, y_l = x_l + w D^-1 (b_l - A_l x_l).
+// A row is handled by a group of G lanes (G a power of two near the
+// matrix's mean row length): lane g of the group takes entries g, g + G, ...
+// of the row, contiguous across the group, and the group adds its partial
+// sums with lane shuffles.  With S given, every kernel returns at once once
+// the solve has converged.
+// ---------------------------------------------------------------------------
+struct CsrDev {
+    const int32_t* rp;  // rows + 1
+    const int32_t* ci;
+    const double* v;
+    int64_t rows;
+};
+
+// The group's sum of v[k] * x[ci[k]] * (SCALE ? s[ci[k]] : 1) over row i.
+template <int G, bool SCALE>
+__device__ inline double group_row(const CsrDev& M, int64_t i, int sub, const double* __restrict__ x,
+                                   const double* __restrict__ sc) {
+    double s = 0.0;
+    for (int32_t k = M.rp[i] + sub; k < M.rp[i + 1]; k += G) {
+        const int32_t j = M.ci[k];
+        s += SCALE ? M.v[k] * (sc[j] * x[j]) : M.v[k] * x[j];
+    }
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    return s;
+}
+
+// MODE 0 (pre-smooth): x = w D^-1 b, r = b - A x (out = r, aux = x)
+// MODE 1 (apply): out = M in
+// MODE 2 (prolong): out += M in
+template <int G, int MODE>
+__global__ void __launch_bounds__(FB) k_amg_rows(const CsrDev M, const double* __restrict__ dinv, double omega,
+                                                 const double* __restrict__ in, double* __restrict__ out,
+                                                 double* __restrict__ aux, const CgState* __restrict__ S) {
+    if (S && S->done) return;
+    const int sub = threadIdx.x & (G - 1);
+    const int64_t step = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; i < M.rows; i += step) {
+        if (MODE == 0) {
+            const double ax = omega * group_row<G, true>(M, i, sub, in, dinv);
+            if (sub == 0) {
+                aux[i] = omega * dinv[i] * in[i];
+                out[i] = in[i] - ax;
+            }
+        } else {
+            const double t = group_row<G, false>(M, i, sub, in, nullptr);
+            if (sub == 0) out[i] = MODE == 2 ? out[i] + t : t;
+        }
+    }
+}
+
+// Post-smooth: y = x + w D^-1 (b - A x); with part, per-block b.y at
+// part[stride * block + off].
+template <int G>
+__global__ void __launch_bounds__(FB) k_amg_post(const CsrDev M, const double* __restrict__ dinv, double omega,
+                                                 const double* __restrict__ b, const double* __restrict__ x,
+                                                 double* __restrict__ y, double* part, int stride, int off,
+                                                 const CgState* __restrict__ S) {
+    __shared__ double sh[8];
+    if (S && S->done) return;
+    const int sub = threadIdx.x & (G - 1);
+    const int64_t step = ((int64_t)gridDim.x * blockDim.x) / G;
+    double a = 0.0;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; i < M.rows; i += step) {
+        const double ax = group_row<G, false>(M, i, sub, x, nullptr);
+        if (sub == 0) {
+            const double yi = x[i] + omega * dinv[i] * (b[i] - ax);
+            y[i] = yi;
+            a += b[i] * yi;
+        }
+    }
+    if (part) {
+        const double t = block_reduce(a, sh);
+        if (threadIdx.x == 0) part[(int64_t)stride * blockIdx.x + off] = t;
+    }
+}
+
+// y = C b for the dense coarsest inverse C (n x n, row-major): one wavefront per row.
+__global__ void __launch_bounds__(FB) k_amg_dense(const double* __restrict__ C, int64_t n, const double* __restrict__ b,
+                                                  double* __restrict__ y, const CgState* __restrict__ S) {
+    if (S && S->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    if (row >= n) return;  // wavefront-uniform
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s += C[row * n + j] * b[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) y[row] = s;
 }
 
 // Object charges from a potential (the Charge form eps0 grad(phi).n ds over
@@ -254,15 +400,9 @@ __global__ void __launch_bounds__(FB) k_image_charge(const int64_t* __restrict__
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// End of a batch of BATCH iterations.
-__global__ void k_cg_batch_end(CgState* S) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !S->done) S->base += BATCH;
-}
-
-// E3[3i + c] = Ec[i].
-__global__ void k_pack3(const double* __restrict__ e, int c, double* __restrict__ E3, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        E3[3 * i + c] = e[i];
+// End of a batch of nb iterations.
+__global__ void k_cg_batch_end(CgState* S, int nb) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !S->done) S->base += nb;
 }
 
 template <typename T>
@@ -306,9 +446,28 @@ struct FieldSolver {
     std::vector<double> invC;        // n x n, row-major
     std::vector<double> inv_circuit; // 2n x 2n
     std::vector<double> circuit_vector;
+    // multigrid preconditioner of the stiffness solves (levels: see k_amg_pre)
+    struct AmgDev {
+        int64_t n = 0, nc = 0;  // rows of this level and of the next
+        CsrDev A{}, P{}, R{};
+        int ga = 1, gp = 1, gr = 1;  // lanes per row of each
+        int32_t* mem_i[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        double* mem_d[3] = {nullptr, nullptr, nullptr};
+        double* dinv = nullptr;
+        double omega = 0.0;
+        double *b = nullptr, *x = nullptr, *r = nullptr, *y = nullptr;  // level 0: b, y are the CG r, z
+    };
+    std::vector<AmgDev> amg;
+    double *coarse_inv = nullptr, *cb = nullptr, *cy = nullptr;
+    int64_t ncoarse = 0;
+    bool amg_ok = false;   // a hierarchy was built
+    bool use_amg = false;  // the stiffness solves use it
+    std::string amg_note;  // why not, when it was not built
     // one captured batch of iterations per operator (stiffness, mass)
     struct Batch {
         const double* av;
+        bool amg;
+        int nc;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -321,6 +480,22 @@ static void free_all(FieldSolver* F) {
         hipGraphDestroy(b.graph);
     }
     F->batches.clear();
+    for (size_t l = 0; l < F->amg.size(); ++l) {
+        auto& L = F->amg[l];
+        for (auto* m : L.mem_i) hipFree(m);
+        for (auto* m : L.mem_d) hipFree(m);
+        if (l > 0) {  // level 0's b and y are the CG vectors
+            hipFree(L.b);
+            hipFree(L.y);
+        }
+        hipFree(L.dinv);
+        hipFree(L.x);
+        hipFree(L.r);
+    }
+    F->amg.clear();
+    hipFree(F->coarse_inv);
+    hipFree(F->cb);
+    hipFree(F->cy);
     hipFree(F->col);
     hipFree(F->kred);
     hipFree(F->kfull);
@@ -375,6 +550,69 @@ static double tet_gradients(const double* vc, double g[4][3]) {
         for (int d = 0; d < 3; ++d) g[k + 1][d] = inv[k][d];
     for (int d = 0; d < 3; ++d) g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
     return fabs(det) / 6.0;
+}
+
+// Lanes per row for a CSR matrix: the largest power of two not above its
+// mean row length, at most 64.
+static int lanes_per_row(const Csr& A) {
+    const double mean = A.nrow ? (double)A.ci.size() / (double)A.nrow : 1.0;
+    int g = 1;
+    while (g < 64 && 2.0 * g <= mean) g *= 2;
+    return g;
+}
+
+static hipError_t upload_amg(FieldSolver* F, const AmgHierarchy& H) {
+    hipError_t e = hipSuccess;
+    auto up = [&](auto** d, const auto& h) {
+        if (e != hipSuccess) return;
+        e = dmalloc(d, h.size());
+        if (e == hipSuccess && !h.empty())
+            e = hipMemcpy(*d, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice);
+    };
+    auto vec = [&](double** d, int64_t len) {
+        if (e == hipSuccess) e = dmalloc(d, (size_t)len);
+    };
+    auto csr = [&](const Csr& h, CsrDev& d, int32_t** ip, int32_t** ic, double** dv) {
+        if (h.ci.size() >= ((size_t)1 << 31) && e == hipSuccess) e = hipErrorInvalidValue;  // 32-bit row pointers
+        std::vector<int32_t> rp(h.rp.begin(), h.rp.end());
+        up(ip, rp);
+        up(ic, h.ci);
+        up(dv, h.v);
+        d.rp = *ip;
+        d.ci = *ic;
+        d.v = *dv;
+        d.rows = h.nrow;
+    };
+    const size_t L = H.levels.size();
+    F->amg.resize(L);
+    for (size_t l = 0; l < L; ++l) {
+        const AmgLevel& h = H.levels[l];
+        FieldSolver::AmgDev& d = F->amg[l];
+        d.n = h.A.nrow;
+        d.nc = h.P.ncol;
+        d.omega = h.omega;
+        csr(h.A, d.A, &d.mem_i[0], &d.mem_i[1], &d.mem_d[0]);
+        csr(h.P, d.P, &d.mem_i[2], &d.mem_i[3], &d.mem_d[1]);
+        csr(h.R, d.R, &d.mem_i[4], &d.mem_i[5], &d.mem_d[2]);
+        d.ga = lanes_per_row(h.A);
+        d.gp = lanes_per_row(h.P);
+        d.gr = lanes_per_row(h.R);
+        up(&d.dinv, h.dinv);
+        vec(&d.x, d.n);
+        vec(&d.r, d.n);
+        if (l == 0) {
+            d.b = F->r;
+            d.y = F->z;
+        } else {
+            vec(&d.b, d.n);
+            vec(&d.y, d.n);
+        }
+    }
+    F->ncoarse = H.ncoarse;
+    up(&F->coarse_inv, H.coarse_inv);
+    vec(&F->cb, H.ncoarse);
+    vec(&F->cy, H.ncoarse);
+    return e;
 }
 
 FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, bool dirichlet_exterior, int n_objects,
@@ -552,9 +790,12 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     up(&F->dinv_k, dk);
     up(&F->dinv_m, dm);
     up(&F->isd, isd);
-    double** vecs[] = {&F->b, &F->t, &F->phid, &F->rhs, &F->x, &F->r, &F->z, &F->p, &F->q, &F->phi};
+    double** vecs[] = {&F->b, &F->t, &F->phid, &F->phi};
     for (double** v : vecs)
         if (e == hipSuccess) e = dmalloc(v, (size_t)n);
+    double** vecs3[] = {&F->rhs, &F->x, &F->r, &F->z, &F->p, &F->q};  // also the E solve's three components
+    for (double** v : vecs3)
+        if (e == hipSuccess) e = dmalloc(v, (size_t)n * 3);
     if (e == hipSuccess) e = dmalloc(&F->E, (size_t)n * 3);
     if (e == hipSuccess) e = dmalloc(&F->part, (size_t)F->grid * 5);
     if (e == hipSuccess) e = dmalloc(&F->S, 1);
@@ -564,6 +805,30 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     if (e == hipSuccess) e = hipMemset(F->S, 0, sizeof(CgState));
     if (e == hipSuccess) e = hipMemset(F->phi, 0, n * 8);
     if (e == hipSuccess) e = hipMemset(F->E, 0, n * 24);
+    // Multigrid hierarchy of the eliminated stiffness operator (its explicit
+    // zeros dropped); the Dirichlet rows are identity rows, left out of the
+    // aggregates.  Without one the stiffness solves stay Jacobi-preconditioned.
+    if (e == hipSuccess) {
+        Csr A;
+        A.nrow = A.ncol = n;
+        A.rp.assign((size_t)n + 1, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
+                if (Kr[j] != 0.0 || ci[j] == i) {
+                    A.ci.push_back(ci[j]);
+                    A.v.push_back(Kr[j]);
+                }
+            A.rp[i + 1] = (int64_t)A.ci.size();
+        }
+        AmgHierarchy H;
+        std::string note;
+        if (amg_build(A, isd, remove_null_space, H, note) && !H.levels.empty()) {
+            e = upload_amg(F, H);
+            F->amg_ok = F->use_amg = e == hipSuccess;
+        } else {
+            F->amg_note = H.levels.empty() && note.empty() ? "operator small enough for Jacobi" : note;
+        }
+    }
     if (e != hipSuccess) {
         err = std::string("device allocation: ") + hipGetErrorString(e);
         free_all(F);
@@ -587,6 +852,23 @@ double* field_phi(FieldSolver* F) { return F->phi; }
 double* field_E(FieldSolver* F) { return F->E; }
 void field_set_max_iterations(FieldSolver* F, int maxit) { F->maxit = std::max(1, maxit); }
 void field_set_warm_start(FieldSolver* F, bool warm) { F->warm = warm; }
+bool field_set_preconditioner(FieldSolver* F, int kind, std::string& err) {
+    if (kind != 0 && kind != 1) {
+        err = "preconditioner: 0 (Jacobi) or 1 (multigrid)";
+        return false;
+    }
+    if (kind == 1 && !F->amg_ok) {
+        err = "no multigrid hierarchy for this operator: " + F->amg_note;
+        return false;
+    }
+    F->use_amg = kind == 1;
+    return true;
+}
+int field_preconditioner(const FieldSolver* F, int* levels, int64_t* coarse) {
+    if (levels) *levels = (int)F->amg.size();
+    if (coarse) *coarse = F->ncoarse;
+    return F->use_amg ? 1 : 0;
+}
 
 #define FCHK(x)                                                      \
     do {                                                             \
@@ -597,31 +879,95 @@ void field_set_warm_start(FieldSolver* F, bool warm) { F->warm = warm; }
         }                                                            \
     } while (0)
 
-static void enqueue_batch(FieldSolver* F, const double* av, const double* dinv) {
+template <int MODE>
+static void launch_rows(int g, dim3 grid, hipStream_t st, const CsrDev& M, const double* dinv, double omega,
+                        const double* in, double* out, double* aux, const CgState* S) {
+    const dim3 blk(FB);
+    switch (g) {
+#define PUNC_AMG_G(G) \
+    case G: hipLaunchKernelGGL((k_amg_rows<G, MODE>), grid, blk, 0, st, M, dinv, omega, in, out, aux, S); break;
+        PUNC_AMG_G(1) PUNC_AMG_G(2) PUNC_AMG_G(4) PUNC_AMG_G(8) PUNC_AMG_G(16) PUNC_AMG_G(32) PUNC_AMG_G(64)
+#undef PUNC_AMG_G
+    }
+}
+
+static void launch_post(int g, dim3 grid, hipStream_t st, const CsrDev& M, const double* dinv, double omega,
+                        const double* b, const double* x, double* y, double* part, int stride, int off,
+                        const CgState* S) {
+    const dim3 blk(FB);
+    switch (g) {
+#define PUNC_AMG_G(G) \
+    case G: hipLaunchKernelGGL((k_amg_post<G>), grid, blk, 0, st, M, dinv, omega, b, x, y, part, stride, off, S); break;
+        PUNC_AMG_G(1) PUNC_AMG_G(2) PUNC_AMG_G(4) PUNC_AMG_G(8) PUNC_AMG_G(16) PUNC_AMG_G(32) PUNC_AMG_G(64)
+#undef PUNC_AMG_G
+    }
+}
+
+// Blocks for rows x lanes threads (grid-stride beyond 4 * MAX_BLOCKS).
+static dim3 grid_rows(int64_t rows, int lanes) {
+    return dim3((unsigned)std::min<int64_t>(4 * MAX_BLOCKS, std::max<int64_t>(1, (rows * lanes + FB - 1) / FB)));
+}
+
+// z = M r by one V-cycle; per-block r.z into part[stride * block + off]
+// (level 0's post-smooth runs on F->grid blocks, the grid of the CG kernels,
+// whose partials the next CG kernel adds).
+static void enqueue_vcycle(FieldSolver* F, const CgState* S, double* part, int stride, int off) {
+    const size_t L = F->amg.size();
+    for (size_t l = 0; l < L; ++l) {
+        const auto& d = F->amg[l];
+        double* bnext = l + 1 < L ? F->amg[l + 1].b : F->cb;
+        launch_rows<0>(d.ga, grid_rows(d.n, d.ga), F->st, d.A, d.dinv, d.omega, d.b, d.r, d.x, S);
+        launch_rows<1>(d.gr, grid_rows(d.nc, d.gr), F->st, d.R, nullptr, 0.0, d.r, bnext, nullptr, S);
+    }
+    hipLaunchKernelGGL(k_amg_dense, dim3((unsigned)((F->ncoarse + 3) / 4)), dim3(FB), 0, F->st, F->coarse_inv,
+                       F->ncoarse, F->cb, F->cy, S);
+    for (size_t l = L; l-- > 0;) {
+        const auto& d = F->amg[l];
+        const double* ynext = l + 1 < L ? F->amg[l + 1].y : F->cy;
+        launch_rows<2>(d.gp, grid_rows(d.n, d.gp), F->st, d.P, nullptr, 0.0, ynext, d.x, nullptr, S);
+        launch_post(d.ga, l == 0 ? dim3(F->grid) : grid_rows(d.n, d.ga), F->st, d.A, d.dinv, d.omega, d.b, d.x, d.y,
+                    l == 0 ? part : nullptr, stride, off, S);
+    }
+}
+
+// dinv null: the multigrid preconditioner.  nc: interleaved components (1, 3).
+static void enqueue_batch(FieldSolver* F, const double* av, const double* dinv, int nc) {
     const int64_t n = F->n;
     const dim3 g(F->grid), blk(FB);
     double* part_pq = F->part;
     double* part2 = F->part + 3 * (size_t)F->grid;
-    for (int k = 0; k < BATCH; ++k) {
-        hipLaunchKernelGGL(k_cg_dir_spmv, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n, part2,
-                           F->grid, part_pq, F->S);
-        hipLaunchKernelGGL(k_cg_update, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n, part_pq,
-                           F->grid, part2, F->S);
+    const int nb = dinv ? BATCH : BATCH_AMG;
+    for (int k = 0; k < nb; ++k) {
+        if (nc == 3) {
+            hipLaunchKernelGGL(k_cg_dir_spmv<3>, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n,
+                               part2, F->grid, part_pq, F->S);
+            hipLaunchKernelGGL(k_cg_update<3>, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n,
+                               part_pq, F->grid, part2, F->S);
+            continue;
+        }
+        hipLaunchKernelGGL(k_cg_dir_spmv<1>, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n,
+                           part2, F->grid, part_pq, F->S);
+        hipLaunchKernelGGL(k_cg_update<1>, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n,
+                           part_pq, F->grid, part2, F->S);
+        if (!dinv) enqueue_vcycle(F, F->S, part2, 2, 0);
     }
-    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, F->st, F->S);
+    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, F->st, F->S, nb);
 }
 
 // The captured batch for operator av (captured on first use).
-static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, hipGraphExec_t* out, std::string& err) {
+static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, int nc, hipGraphExec_t* out,
+                       std::string& err) {
     for (auto& b : F->batches)
-        if (b.av == av) {
+        if (b.av == av && b.amg == (dinv == nullptr) && b.nc == nc) {
             *out = b.exec;
             return true;
         }
     FieldSolver::Batch b;
     b.av = av;
+    b.amg = dinv == nullptr;
+    b.nc = nc;
     FCHK(hipStreamBeginCapture(F->st, hipStreamCaptureModeThreadLocal));
-    enqueue_batch(F, av, dinv);
+    enqueue_batch(F, av, dinv, nc);
     hipError_t e = hipStreamEndCapture(F->st, &b.graph);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -639,14 +985,26 @@ static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, hip
     return true;
 }
 
-// Jacobi-preconditioned CG on the ELL operator av for rhs b from F->x (in place).
+// Preconditioned CG on the ELL operator av for rhs b from F->x (in place):
+// the multigrid V-cycle for the stiffness operator when its hierarchy is in
+// use, Jacobi (dinv) otherwise.
+// nc = 3: the vectors hold three interleaved components, solved as one CG
+// on the block-diagonal system (Jacobi only).
 static bool run_cg(FieldSolver* F, const double* av, const double* dinv, const double* b, int* iterations,
-                   std::string& err) {
+                   std::string& err, int nc = 1) {
     const int64_t n = F->n;
     const dim3 g(F->grid), blk(FB);
+    if (av == F->kred && F->use_amg && nc == 1) dinv = nullptr;
     hipGraphExec_t exec;
-    if (!batch_exec(F, av, dinv, &exec, err)) return false;
-    hipLaunchKernelGGL(k_cg_init, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n, F->part);
+    if (!batch_exec(F, av, dinv, nc, &exec, err)) return false;
+    if (nc == 3) {
+        hipLaunchKernelGGL(k_cg_init<3>, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n,
+                           F->part);
+    } else {
+        hipLaunchKernelGGL(k_cg_init<1>, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n,
+                           F->part);
+        if (!dinv) enqueue_vcycle(F, nullptr, F->part, 3, 0);
+    }
     hipLaunchKernelGGL(k_cg_init_finish, dim3(1), blk, 0, F->st, F->part, F->grid, 1e-12, 1e-14, F->S);
     FCHK(hipGetLastError());
     CgState h;
@@ -704,18 +1062,20 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
     FCHK(hipSetDevice(F->device));
     const int64_t n = F->n;
     const dim3 g(F->grid), blk(FB);
-    int total = 0;
-    for (int c = 0; c < 3; ++c) {
-        hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->grad[c], d_phi, F->rhs, n);
-        hipLaunchKernelGGL(k_unpack3, g, blk, 0, F->st, F->E, c, F->x, n, F->warm ? 1 : 0);
-        int it = 0;
-        if (!run_cg(F, F->mass, F->dinv_m, F->rhs, &it, err)) return false;
-        total += it;
-        hipLaunchKernelGGL(k_pack3, g, blk, 0, F->st, F->x, c, F->E, n);
-        FCHK(hipGetLastError());
-    }
+    // the vector right-hand side (-grad(phi).v dx, dof-major like E) and one
+    // CG on the vector space's block-diagonal mass matrix, as the reference
+    // solves the vector system at once (efield.cpp:70-75)
+    hipLaunchKernelGGL(k_grad3, g, blk, 0, F->st, F->col, F->w, F->grad[0], F->grad[1], F->grad[2], d_phi, F->rhs, n);
+    if (F->warm)
+        FCHK(hipMemcpyAsync(F->x, F->E, n * 24, hipMemcpyDeviceToDevice, F->st));
+    else
+        FCHK(hipMemsetAsync(F->x, 0, n * 24, F->st));
+    FCHK(hipGetLastError());
+    int it = 0;
+    if (!run_cg(F, F->mass, F->dinv_m, F->rhs, &it, err, 3)) return false;
+    FCHK(hipMemcpyAsync(F->E, F->x, n * 24, hipMemcpyDeviceToDevice, F->st));
     FCHK(hipStreamSynchronize(F->st));
-    if (iterations) *iterations = total;
+    if (iterations) *iterations = it;
     return true;
 }
 

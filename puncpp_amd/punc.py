@@ -578,9 +578,12 @@ class FieldSolver:
     exterior boundary (id 1); n_objects: objects (ids 2..) held at given
     potentials; remove_null_space: no Dirichlet condition (periodic box), the
     potential has zero mean.  Tolerances as the reference's Krylov solvers
-    (relative 1e-12, absolute 1e-14)."""
+    (relative 1e-12, absolute 1e-14).  preconditioner: "amg" (smoothed-
+    aggregation multigrid, the default where a hierarchy can be built; the
+    reference's PETSc solver takes hypre's AMG) or "jacobi"."""
 
-    def __init__(self, mesh, eps0=1.0, exterior=True, n_objects=0, remove_null_space=False, warm_start=False):
+    def __init__(self, mesh, eps0=1.0, exterior=True, n_objects=0, remove_null_space=False, warm_start=False,
+                 preconditioner=None):
         self.mesh = mesh
         self.n_objects = int(n_objects)
         h = N.vp()
@@ -589,6 +592,24 @@ class FieldSolver:
         self._h = h
         self.iterations = (0, 0)
         self.set_warm_start(warm_start)
+        if preconditioner is not None:
+            self.set_preconditioner(preconditioner)
+
+    _PRECONDITIONERS = {"jacobi": 0, "amg": 1}
+
+    def set_preconditioner(self, kind):
+        """Preconditioner of the Poisson solves, "amg" or "jacobi"; "amg"
+        raises the library's error when no hierarchy could be built."""
+        if kind not in self._PRECONDITIONERS:
+            raise ValueError(f"preconditioner must be one of {sorted(self._PRECONDITIONERS)}")
+        check(N.lib().punc_gpu_field_set_preconditioner(self._h, self._PRECONDITIONERS[kind]))
+
+    @property
+    def preconditioner(self):
+        """(kind, multigrid levels, coarsest dense size)."""
+        k, lv, nc = N.C.c_int(), N.C.c_int(), N.C.c_int64()
+        check(N.lib().punc_gpu_field_get_preconditioner(self._h, N.C.byref(k), N.C.byref(lv), N.C.byref(nc)))
+        return ("amg" if k.value == 1 else "jacobi", lv.value, nc.value)
 
     def set_warm_start(self, warm):
         """Start each solve from the previous solution (the reference starts

@@ -185,3 +185,49 @@ def test_circuit_population_path(gpu):
     assert obj[0].get_potential() == pytest.approx(pot_o[0], rel=1e-9)
     _close(phi, phi_o, 1e-9)
     _close(E, fo.efield(phi_o), 1e-8)
+
+
+@pytest.mark.parametrize("case", ["object", "periodic"])
+def test_multigrid_and_jacobi_agree(gpu, case):
+    # the multigrid-preconditioned solve (the default) and the Jacobi one
+    # reach the oracle's solution; the multigrid one in far fewer iterations
+    _print_case("multigrid", case=case)
+    if case == "object":
+        hm = sphere_mesh(n=12, L=4.0, r=1.5)
+        kw, okw = dict(exterior=True, n_objects=1), dict(exterior=True, object_potentials=[0.7])
+        skw = dict(object_potentials=[0.7])
+    else:
+        hm = P.box_mesh(14, [0.0] * 3, [2 * np.pi] * 3, periodic=True)
+        kw, okw, skw = dict(exterior=False, remove_null_space=True), dict(exterior=False, remove_null_space=True), {}
+    mesh = P.Mesh(hm)
+    rho = np.random.default_rng(21).normal(size=hm.ndof)
+    f = P.FieldSolver(mesh, **kw)
+    kind, levels, coarse = f.preconditioner
+    assert kind == "amg" and levels >= 1 and 0 < coarse <= 2048
+    phi_m = f.solve(rho, **skw)
+    it_m = f.iterations[0]
+    f.set_preconditioner("jacobi")
+    assert f.preconditioner[0] == "jacobi"
+    phi_j = f.solve(rho, **skw)
+    it_j = f.iterations[0]
+    print(f"iterations multigrid {it_m} jacobi {it_j}", flush=True)
+    assert it_m < it_j / 2
+    ref = _oracle(hm, 1.0).poisson(rho, **okw)
+    _close(phi_m, ref, 1e-9)
+    _close(phi_j, ref, 1e-9)
+    f.set_preconditioner("amg")  # back again: its captured batch is reused
+    _close(f.solve(rho, **skw), ref, 1e-9)
+    assert f.iterations[0] == it_m
+
+
+def test_multigrid_small_operator_stays_jacobi(gpu):
+    # at most 256 dofs: no hierarchy, Jacobi, and asking for multigrid fails
+    _print_case("multigrid_small")
+    hm = P.box_mesh(4, [0.0] * 3, [1.0] * 3)
+    f = P.FieldSolver(P.Mesh(hm), exterior=True)
+    assert f.preconditioner == ("jacobi", 0, 0)
+    from puncpp_amd._native import PuncGpuError
+    with pytest.raises(PuncGpuError):
+        f.set_preconditioner("amg")
+    rho = np.random.default_rng(5).normal(size=hm.ndof)
+    _close(f.solve(rho), _oracle(hm, 1.0).poisson(rho, exterior=True), 1e-9)
