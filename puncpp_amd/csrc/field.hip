@@ -7,13 +7,15 @@
 //   mass       M_ij = V (1 + delta_ij) / 20          (L = rho v dx, and the E projection)
 //   gradient   B_c,ij = -(V / 4) (g_j)_c             (L = -grad(phi).v dx, component c)
 // summed into matrices with one sparsity (the dof graph of the cells), kept
-// on the device in ELL layout.
+// on the device in ELL layout for the one-off products (right-hand sides)
+// and in CSR for the operators the conjugate-gradient iterations apply.
 // Dirichlet dofs are eliminated symmetrically: the operator keeps the free
 // rows and columns and has unit rows at the Dirichlet dofs; the right-hand
 // side is b - K phi_D on free dofs and phi_D on Dirichlet dofs.  This has
 // the solution of the reference's row-replaced system (DirichletBC::apply)
 // and keeps the operator symmetric positive definite, so the solves are
-// Jacobi-preconditioned conjugate gradients.  Every reduction is a set of
+// preconditioned conjugate gradients: smoothed-aggregation multigrid for the
+// Poisson operator (amg.cpp), Jacobi for the mass matrix.  Every reduction is a set of
 // per-block partials that every block of the next kernel adds in block order
 // (deterministic; no atomics or fences - the kernel boundary orders the
 // memory), and the CG scalars stay on the device: iterations run in batches
@@ -28,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 namespace punc_gpu {
 
@@ -84,15 +87,46 @@ __device__ inline double row_dot(const int32_t* __restrict__ col, const double* 
     return s;
 }
 
-// The same for NC interleaved vectors (x[NC * j + c]): the operator applied
-// to each component (the block-diagonal matrix of a vector CG1 space).
-template <int NC>
-__device__ inline void row_dot_nc(const int32_t* __restrict__ col, const double* __restrict__ av, int w, int64_t n,
-                                  const double* __restrict__ x, int64_t i, double out[NC]) {
+// CSR matrix on the device (32-bit row pointers).
+struct CsrDev {
+    const int32_t* rp;  // rows + 1
+    const int32_t* ci;
+    const double* v;
+    int64_t rows;
+};
+
+// Row i of M applied to NC interleaved vectors (x[NC * j + c]) by a group of
+// G lanes (sub: this lane's place in the group): lane g takes entries
+// g, g + G, ... of the row; the group's sums are added with lane shuffles,
+// so every lane of the group returns the row's values.
+template <int NC, int G>
+__device__ inline void group_row_nc(const CsrDev& M, int64_t i, int sub, const double* __restrict__ x, double out[NC]) {
     for (int c = 0; c < NC; ++c) out[c] = 0.0;
-    for (int k = 0; k < w; ++k) {
-        const double a = av[k * n + i];
-        const int64_t j = col[k * n + i];
+    for (int32_t k = M.rp[i] + sub; k < M.rp[i + 1]; k += G) {
+        const double a = M.v[k];
+        const int64_t j = M.ci[k];
+        for (int c = 0; c < NC; ++c) out[c] += a * x[NC * j + c];
+    }
+    for (int o = G / 2; o > 0; o >>= 1)
+        for (int c = 0; c < NC; ++c) out[c] += __shfl_xor(out[c], o, G);
+}
+
+// ELL matrix on the device (layout of row_dot).
+struct EllDev {
+    const int32_t* col;
+    const double* v;
+    int w;
+    int64_t rows;
+};
+
+// The same for an ELL matrix, one lane per row (G = 1).
+template <int NC, int G>
+__device__ inline void group_row_nc(const EllDev& M, int64_t i, int sub, const double* __restrict__ x, double out[NC]) {
+    static_assert(G == 1, "ELL rows take one lane");
+    for (int c = 0; c < NC; ++c) out[c] = 0.0;
+    for (int k = 0; k < M.w; ++k) {
+        const double a = M.v[k * M.rows + i];
+        const int64_t j = M.col[k * M.rows + i];
         for (int c = 0; c < NC; ++c) out[c] += a * x[NC * j + c];
     }
 }
@@ -160,17 +194,19 @@ __global__ void k_remove_mean(double* __restrict__ v, int64_t n, const CgState* 
 // (dinv null: r, r.r and b.b; the V-cycle writes z and r.z).  NC: the
 // vectors hold NC interleaved components (one CG on the block-diagonal
 // system); A and D^-1 are those of one component.
-template <int NC>
-__global__ void __launch_bounds__(FB) k_cg_init(const int32_t* __restrict__ col, int w, const double* __restrict__ av,
-                                                const double* __restrict__ dinv, const double* __restrict__ b,
-                                                const double* __restrict__ x, double* __restrict__ r,
-                                                double* __restrict__ z, double* __restrict__ p, int64_t n,
-                                                double* part) {
+template <int NC, int G, typename Op>
+__global__ void __launch_bounds__(FB) k_cg_init(const Op A, const double* __restrict__ dinv,
+                                                const double* __restrict__ b, const double* __restrict__ x,
+                                                double* __restrict__ r, double* __restrict__ z,
+                                                double* __restrict__ p, double* part) {
     __shared__ double sh[8];
+    const int sub = threadIdx.x & (G - 1);
+    const int64_t step = ((int64_t)gridDim.x * blockDim.x) / G;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; i < A.rows; i += step) {
         double ax[NC];
-        row_dot_nc<NC>(col, av, w, n, x, i, ax);
+        group_row_nc<NC, G>(A, i, sub, x, ax);
+        if (sub != 0) continue;
         for (int c = 0; c < NC; ++c) {
             const int64_t m = NC * i + c;
             const double ri = b[m] - ax[c];
@@ -218,10 +254,9 @@ __global__ void __launch_bounds__(FB) k_cg_init_finish(const double* __restrict_
 // the updated p needs neighbours' p; A z only needs z, complete since the
 // last kernel).  Per-block p.q.  Block 0 records rz_t and, at convergence,
 // the flag and the iteration count.
-template <int NC>
-__global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const int32_t* __restrict__ col, int w,
-                                                    const double* __restrict__ av, const double* __restrict__ z,
-                                                    double* __restrict__ p, double* __restrict__ q, int64_t n,
+template <int NC, int G, typename Op>
+__global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const Op A, const double* __restrict__ z,
+                                                    double* __restrict__ p, double* __restrict__ q,
                                                     const double* __restrict__ part2, int cnt2, double* part_pq,
                                                     CgState* S) {
     __shared__ double sh[8];
@@ -242,9 +277,12 @@ __global__ void __launch_bounds__(FB) k_cg_dir_spmv(int64_t k, const int32_t* __
         if (blockIdx.x == 0 && threadIdx.x == 0) S->rz[k & 1] = rz;
     }
     double a = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x & (G - 1);
+    const int64_t step = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; i < A.rows; i += step) {
         double az[NC];
-        row_dot_nc<NC>(col, av, w, n, z, i, az);
+        group_row_nc<NC, G>(A, i, sub, z, az);
+        if (sub != 0) continue;
         for (int c = 0; c < NC; ++c) {
             const int64_t m = NC * i + c;
             const double pi = first ? z[m] : z[m] + beta * p[m];
@@ -304,13 +342,6 @@ __global__ void __launch_bounds__(FB) k_cg_update(int64_t k, double* __restrict_
 // sums with lane shuffles.  With S given, every kernel returns at once once
 // the solve has converged.
 // ---------------------------------------------------------------------------
-struct CsrDev {
-    const int32_t* rp;  // rows + 1
-    const int32_t* ci;
-    const double* v;
-    int64_t rows;
-};
-
 // The group's sum of v[k] * x[ci[k]] * (SCALE ? s[ci[k]] : 1) over row i.
 template <int G, bool SCALE>
 __device__ inline double group_row(const CsrDev& M, int64_t i, int sub, const double* __restrict__ x,
@@ -425,7 +456,7 @@ struct FieldSolver {
     std::vector<int32_t> dir_kind;  // per dof: -1 free, 0 exterior, 1+j object j
     // device
     int32_t* col = nullptr;  // ELL columns (w * n)
-    double *kred = nullptr, *kfull = nullptr, *mass = nullptr, *grad[3] = {nullptr, nullptr, nullptr};
+    double *kfull = nullptr, *mass = nullptr, *grad[3] = {nullptr, nullptr, nullptr};
     double *dinv_k = nullptr, *dinv_m = nullptr;
     uint8_t* isd = nullptr;
     double *b = nullptr, *t = nullptr, *phid = nullptr, *rhs = nullptr;
@@ -446,7 +477,14 @@ struct FieldSolver {
     std::vector<double> invC;        // n x n, row-major
     std::vector<double> inv_circuit; // 2n x 2n
     std::vector<double> circuit_vector;
-    // multigrid preconditioner of the stiffness solves (levels: see k_amg_pre)
+    // the CG operators: the eliminated stiffness in CSR without its explicit
+    // zeros (gk lanes per row), the mass matrix in ELL
+    CsrDev kcsr{};
+    EllDev mell{};  // the mass matrix's ELL arrays (one lane per row measured faster there)
+    int gk = 1;
+    int32_t* csr_i[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* csr_d[2] = {nullptr, nullptr};
+    // multigrid preconditioner of the stiffness solves (levels: see k_amg_rows)
     struct AmgDev {
         int64_t n = 0, nc = 0;  // rows of this level and of the next
         CsrDev A{}, P{}, R{};
@@ -493,11 +531,12 @@ static void free_all(FieldSolver* F) {
         hipFree(L.r);
     }
     F->amg.clear();
+    for (auto* m : F->csr_i) hipFree(m);
+    for (auto* m : F->csr_d) hipFree(m);
     hipFree(F->coarse_inv);
     hipFree(F->cb);
     hipFree(F->cy);
     hipFree(F->col);
-    hipFree(F->kred);
     hipFree(F->kfull);
     hipFree(F->mass);
     for (auto& g : F->grad) hipFree(g);
@@ -780,7 +819,6 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
             e = hipMemcpy(*d, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice);
     };
     up(&F->col, ecol);
-    up(&F->kred, ell(Kr));
     up(&F->kfull, ell(K));
     up(&F->mass, ell(Mv));
     for (int d = 0; d < 3; ++d) up(&F->grad[d], ell(B[d]));
@@ -808,18 +846,30 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     // Multigrid hierarchy of the eliminated stiffness operator (its explicit
     // zeros dropped); the Dirichlet rows are identity rows, left out of the
     // aggregates.  Without one the stiffness solves stay Jacobi-preconditioned.
+    Csr A;  // the eliminated stiffness without its explicit zeros
+    A.nrow = A.ncol = n;
+    A.rp.assign((size_t)n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
+            if (Kr[j] != 0.0 || ci[j] == i) {
+                A.ci.push_back(ci[j]);
+                A.v.push_back(Kr[j]);
+            }
+        A.rp[i + 1] = (int64_t)A.ci.size();
+    }
+    auto upcsr = [&](const Csr& h, CsrDev& d, int32_t** ip, int32_t** ic, double** dv) {
+        up(ip, std::vector<int32_t>(h.rp.begin(), h.rp.end()));
+        up(ic, h.ci);
+        up(dv, h.v);
+        d.rp = *ip;
+        d.ci = *ic;
+        d.v = *dv;
+        d.rows = h.nrow;
+    };
+    upcsr(A, F->kcsr, &F->csr_i[0], &F->csr_i[1], &F->csr_d[0]);
+    F->gk = std::min(lanes_per_row(A), 16);
+    F->mell = EllDev{F->col, F->mass, F->w, n};
     if (e == hipSuccess) {
-        Csr A;
-        A.nrow = A.ncol = n;
-        A.rp.assign((size_t)n + 1, 0);
-        for (int64_t i = 0; i < n; ++i) {
-            for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
-                if (Kr[j] != 0.0 || ci[j] == i) {
-                    A.ci.push_back(ci[j]);
-                    A.v.push_back(Kr[j]);
-                }
-            A.rp[i + 1] = (int64_t)A.ci.size();
-        }
         AmgHierarchy H;
         std::string note;
         if (amg_build(A, isd, remove_null_space, H, note) && !H.levels.empty()) {
@@ -930,44 +980,84 @@ static void enqueue_vcycle(FieldSolver* F, const CgState* S, double* part, int s
     }
 }
 
-// dinv null: the multigrid preconditioner.  nc: interleaved components (1, 3).
-static void enqueue_batch(FieldSolver* F, const double* av, const double* dinv, int nc) {
-    const int64_t n = F->n;
+// A CG operator: the CSR stiffness (G lanes per row) or the ELL mass matrix.
+struct CgOp {
+    const CsrDev* csr;
+    const EllDev* ell;
+    int G;
+    const double* key() const { return csr ? csr->v : ell->v; }
+};
+
+template <int NC, int G, typename Op>
+static void launch_dir_update(FieldSolver* F, int64_t k, const Op& A, const double* dinv, double* part_pq,
+                              double* part2) {
     const dim3 g(F->grid), blk(FB);
+    hipLaunchKernelGGL((k_cg_dir_spmv<NC, G, Op>), g, blk, 0, F->st, k, A, F->z, F->p, F->q, part2, F->grid, part_pq,
+                       F->S);
+    hipLaunchKernelGGL(k_cg_update<NC>, g, blk, 0, F->st, k, F->x, F->r, F->z, F->p, F->q, dinv, F->n, part_pq,
+                       F->grid, part2, F->S);
+}
+
+template <int NC, int G, typename Op>
+static void launch_init(FieldSolver* F, const Op& A, const double* dinv, const double* b) {
+    hipLaunchKernelGGL((k_cg_init<NC, G, Op>), dim3(F->grid), dim3(FB), 0, F->st, A, dinv, b, F->x, F->r, F->z, F->p,
+                       F->part);
+}
+
+// Calls fn(components, lanes, operator) with the first two as integral
+// constants: ELL with one lane per row, CSR with op.G lanes.
+template <int NC, typename Fn>
+static void dispatch_nc(const CgOp& op, Fn&& fn) {
+    using std::integral_constant;
+    if (op.ell) {
+        fn(integral_constant<int, NC>(), integral_constant<int, 1>(), *op.ell);
+        return;
+    }
+    switch (op.G) {
+        case 1: fn(integral_constant<int, NC>(), integral_constant<int, 1>(), *op.csr); break;
+        case 2: fn(integral_constant<int, NC>(), integral_constant<int, 2>(), *op.csr); break;
+        case 4: fn(integral_constant<int, NC>(), integral_constant<int, 4>(), *op.csr); break;
+        case 8: fn(integral_constant<int, NC>(), integral_constant<int, 8>(), *op.csr); break;
+        default: fn(integral_constant<int, NC>(), integral_constant<int, 16>(), *op.csr); break;
+    }
+}
+
+template <typename Fn>
+static void dispatch_op(const CgOp& op, int nc, Fn&& fn) {
+    if (nc == 3)
+        dispatch_nc<3>(op, fn);
+    else
+        dispatch_nc<1>(op, fn);
+}
+
+// dinv null: the multigrid preconditioner.  nc: interleaved components (1, 3).
+static void enqueue_batch(FieldSolver* F, const CgOp& op, const double* dinv, int nc) {
     double* part_pq = F->part;
     double* part2 = F->part + 3 * (size_t)F->grid;
     const int nb = dinv ? BATCH : BATCH_AMG;
     for (int k = 0; k < nb; ++k) {
-        if (nc == 3) {
-            hipLaunchKernelGGL(k_cg_dir_spmv<3>, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n,
-                               part2, F->grid, part_pq, F->S);
-            hipLaunchKernelGGL(k_cg_update<3>, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n,
-                               part_pq, F->grid, part2, F->S);
-            continue;
-        }
-        hipLaunchKernelGGL(k_cg_dir_spmv<1>, g, blk, 0, F->st, (int64_t)k, F->col, F->w, av, F->z, F->p, F->q, n,
-                           part2, F->grid, part_pq, F->S);
-        hipLaunchKernelGGL(k_cg_update<1>, g, blk, 0, F->st, (int64_t)k, F->x, F->r, F->z, F->p, F->q, dinv, n,
-                           part_pq, F->grid, part2, F->S);
+        dispatch_op(op, nc, [&](auto nc_c, auto g_c, const auto& A) {
+            launch_dir_update<decltype(nc_c)::value, decltype(g_c)::value>(F, (int64_t)k, A, dinv, part_pq, part2);
+        });
         if (!dinv) enqueue_vcycle(F, F->S, part2, 2, 0);
     }
     hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, F->st, F->S, nb);
 }
 
-// The captured batch for operator av (captured on first use).
-static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, int nc, hipGraphExec_t* out,
+// The captured batch for operator op (captured on first use).
+static bool batch_exec(FieldSolver* F, const CgOp& op, const double* dinv, int nc, hipGraphExec_t* out,
                        std::string& err) {
     for (auto& b : F->batches)
-        if (b.av == av && b.amg == (dinv == nullptr) && b.nc == nc) {
+        if (b.av == op.key() && b.amg == (dinv == nullptr) && b.nc == nc) {
             *out = b.exec;
             return true;
         }
     FieldSolver::Batch b;
-    b.av = av;
+    b.av = op.key();
     b.amg = dinv == nullptr;
     b.nc = nc;
     FCHK(hipStreamBeginCapture(F->st, hipStreamCaptureModeThreadLocal));
-    enqueue_batch(F, av, dinv, nc);
+    enqueue_batch(F, op, dinv, nc);
     hipError_t e = hipStreamEndCapture(F->st, &b.graph);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -985,27 +1075,20 @@ static bool batch_exec(FieldSolver* F, const double* av, const double* dinv, int
     return true;
 }
 
-// Preconditioned CG on the ELL operator av for rhs b from F->x (in place):
-// the multigrid V-cycle for the stiffness operator when its hierarchy is in
-// use, Jacobi (dinv) otherwise.
-// nc = 3: the vectors hold three interleaved components, solved as one CG
-// on the block-diagonal system (Jacobi only).
-static bool run_cg(FieldSolver* F, const double* av, const double* dinv, const double* b, int* iterations,
+// Preconditioned CG on operator op for rhs b from F->x (in place): the
+// multigrid V-cycle for the stiffness operator when its hierarchy is in use,
+// Jacobi (dinv) otherwise.  nc = 3: the vectors hold three interleaved
+// components, solved as one CG on the block-diagonal system (Jacobi only).
+static bool run_cg(FieldSolver* F, const CgOp& op, const double* dinv, const double* b, int* iterations,
                    std::string& err, int nc = 1) {
-    const int64_t n = F->n;
-    const dim3 g(F->grid), blk(FB);
-    if (av == F->kred && F->use_amg && nc == 1) dinv = nullptr;
+    if (op.csr == &F->kcsr && F->use_amg && nc == 1) dinv = nullptr;
     hipGraphExec_t exec;
-    if (!batch_exec(F, av, dinv, nc, &exec, err)) return false;
-    if (nc == 3) {
-        hipLaunchKernelGGL(k_cg_init<3>, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n,
-                           F->part);
-    } else {
-        hipLaunchKernelGGL(k_cg_init<1>, g, blk, 0, F->st, F->col, F->w, av, dinv, b, F->x, F->r, F->z, F->p, n,
-                           F->part);
-        if (!dinv) enqueue_vcycle(F, nullptr, F->part, 3, 0);
-    }
-    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), blk, 0, F->st, F->part, F->grid, 1e-12, 1e-14, F->S);
+    if (!batch_exec(F, op, dinv, nc, &exec, err)) return false;
+    dispatch_op(op, nc, [&](auto nc_c, auto g_c, const auto& A) {
+        launch_init<decltype(nc_c)::value, decltype(g_c)::value>(F, A, dinv, b);
+    });
+    if (!dinv) enqueue_vcycle(F, nullptr, F->part, 3, 0);
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(FB), 0, F->st, F->part, F->grid, 1e-12, 1e-14, F->S);
     FCHK(hipGetLastError());
     CgState h;
     for (;;) {
@@ -1046,7 +1129,7 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
         hipLaunchKernelGGL(k_remove_mean, g, blk, 0, F->st, F->rhs, n, F->S);
         FCHK(hipGetLastError());
     }
-    if (!run_cg(F, F->kred, F->dinv_k, F->rhs, iterations, err)) return false;
+    if (!run_cg(F, CgOp{&F->kcsr, nullptr, F->gk}, F->dinv_k, F->rhs, iterations, err)) return false;
     if (F->nullspace) {
         hipLaunchKernelGGL(k_partial_sum, g, blk, 0, F->st, F->x, n, F->part);
         hipLaunchKernelGGL(k_finish_mean, dim3(1), blk, 0, F->st, F->part, F->grid, n, F->S);
@@ -1072,7 +1155,7 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
         FCHK(hipMemsetAsync(F->x, 0, n * 24, F->st));
     FCHK(hipGetLastError());
     int it = 0;
-    if (!run_cg(F, F->mass, F->dinv_m, F->rhs, &it, err, 3)) return false;
+    if (!run_cg(F, CgOp{nullptr, &F->mell, 1}, F->dinv_m, F->rhs, &it, err, 3)) return false;
     FCHK(hipMemcpyAsync(F->E, F->x, n * 24, hipMemcpyDeviceToDevice, F->st));
     FCHK(hipStreamSynchronize(F->st));
     if (iterations) *iterations = it;
