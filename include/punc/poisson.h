@@ -10,6 +10,8 @@
 #include <punc/population.h>
 
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace punc {
@@ -26,6 +28,17 @@ public:
         check(punc_gpu_field_create(mesh.handle(), eps0, exterior_dirichlet ? 1 : 0, (int)objects.size(),
                                     remove_null_space ? 1 : 0, &h));
         handle_.reset(h, punc_gpu_field_destroy);
+    }
+
+    //! The preconditioner of the solves, by the reference's PETSc names
+    //! (poisson.cpp:299-304): "hypre_amg" or "amg" (smoothed-aggregation
+    //! multigrid, the default where a hierarchy can be built) or "jacobi".
+    void set_preconditioner(const std::string& name) {
+        int kind = -1;
+        if (name == "hypre_amg" || name == "amg") kind = 1;
+        if (name == "jacobi") kind = 0;
+        if (kind < 0) throw std::invalid_argument("preconditioner: hypre_amg, amg or jacobi");
+        check(punc_gpu_field_set_preconditioner(handle_.get(), kind));
     }
 
     //! solve (poisson.cpp:349-362): phi from rho, the objects at their

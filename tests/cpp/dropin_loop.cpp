@@ -51,6 +51,15 @@ int main(int argc, char** argv) {
             poisson.solve_population(pop, objects, &phi2, &E2);
             for (int64_t d = 0; d < 3 * mesh.ndof; ++d)
                 if (std::fabs(E2.values[d] - E.values[d]) > 1e-9 * (1.0 + std::fabs(E.values[d]))) bad++;
+            // and so does the Jacobi-preconditioned solve
+            Function phi3(mesh, 1);
+            poisson.set_preconditioner("jacobi");
+            poisson.solve(phi3, rho, objects);
+            poisson.set_preconditioner("hypre_amg");
+            double pmax = 0.0;
+            for (int64_t d = 0; d < mesh.ndof; ++d) pmax = std::max(pmax, std::fabs(phi2.values[d]));
+            for (int64_t d = 0; d < mesh.ndof; ++d)
+                if (std::fabs(phi3.values[d] - phi2.values[d]) > 1e-9 * pmax) bad++;
         }
         double qsum = 0.0;
         for (int64_t d = 0; d < mesh.ndof; ++d) qsum += rho.values[d] / dv_inv[d];
