@@ -1536,10 +1536,8 @@ k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ 
         const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
         if (DG0) {
             if (key >= 0 && tail) lds_add(t, raw, key, (double)(lane - hl + 1));
-        } else {
-            run_scan4(d, lane, hl);
-            if (key >= 0 && tail)
-                for (int k = 0; k < 4; ++k) lds_add(t, raw, dof[k], d[k]);
+        } else if (key >= 0) {
+            for (int k = 0; k < 4; ++k) lds_add(t, raw, dof[k], d[k]);
         }
         // one decision for the whole block (the count read is a heuristic)
         if (__syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8)) lds_flush(t, raw);
