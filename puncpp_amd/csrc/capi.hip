@@ -1531,7 +1531,8 @@ static int density_common(punc_pop_s* P, bool dg0, const double* dv_inv, double 
         scale = P->d_dv_inv;
     }
     HIPCHK(launch_fill_f64(P->d_draw, 2 * len, 0.0, P->stream));
-    HIPCHK(launch_density(P->mesh->dm, P->d_state, coef, P->cell_bits, dg0, P->d_draw, diag_grid(P), P->stream));
+    HIPCHK(launch_density(P->mesh->dm, P->d_state, coef, P->nsp, P->cell_bits, dg0, P->d_draw, diag_grid(P),
+                          P->stream));
     HIPCHK(launch_density_scale(dg0, P->d_draw, scale, we, wi, P->d_dens, P->d_dens + len, len, P->stream));
     P->dens_len = len;
     if (ne) HIPCHK(hipMemcpyAsync(ne, P->d_dens, len * 8, hipMemcpyDeviceToHost, P->stream));

@@ -134,7 +134,7 @@ hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStrea
 // into part (6 * grid), merged into out (6).
 hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
                                    const double* phi, double4* cellPhi, double* part, int grid, hipStream_t st);
-hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits, bool dg0,
+hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int nsp, int cell_bits, bool dg0,
                           double* raw, int grid, hipStream_t st);
 hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale, double we, double wi,
                                 double* ne, double* ni, int64_t n, hipStream_t st);

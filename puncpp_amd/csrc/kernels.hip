@@ -1498,8 +1498,11 @@ k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __re
 // flushed to raw with one global atomic per occupied slot.
 template <bool DG0>
 __global__ void __launch_bounds__(256)
-k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
+k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
           double* __restrict__ raw) {
+    __shared__ int32_t cls_of[MAX_SPECIES];  // class per species: DG0 1 for q > 0, CG1 0 for q < 0
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x)
+        cls_of[k] = DG0 ? (coef[k].q > 0 ? 1 : 0) : (coef[k].q < 0 ? 0 : 1);
     const GArrays A = gview(S->cur);
     int64_t s, e;
     chunk_range(S->n, s, e);
@@ -1509,38 +1512,41 @@ k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ 
     lds_clear(t);
     __syncthreads();
     const int lane = threadIdx.x & 63;
+    // the next group's cell ids are loaded one group ahead
+    int32_t cs_next = s + threadIdx.x < e ? A.cs[s + threadIdx.x] : 0;
+    int iter = 0;
     for (int64_t b = s; b < e; b += blockDim.x) {  // block-uniform trip count
         const int64_t i = b + threadIdx.x;
+        const int32_t cs = cs_next;
+        if (i + blockDim.x < e) cs_next = A.cs[i + blockDim.x];
         int32_t key = -1;
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         int32_t dof[4] = {0, 0, 0, 0};
         if (i < e) {
-            const int32_t cs = A.cs[i];
             const int32_t c = cs & cmask;
-            const double q = coef[cs >> cell_bits].q;
-            if (DG0) {
-                key = 2 * c + (q > 0 ? 1 : 0);
-            } else {
-                const int cls = q < 0 ? 0 : 1;
-                key = 2 * c + cls;
+            const int cls = cls_of[cs >> cell_bits];
+            key = 2 * c + cls;
+            if (!DG0) {
                 const CellRec& R = M.rec[c];
                 bary4(R.bary, A.x[i], A.y[i], A.z[i], d);
                 for (int k = 0; k < 4; ++k) dof[k] = 2 * R.dof[k] + cls;
             }
         }
-        const int32_t kprev = __shfl_up(key, 1, 64);
-        const bool head = lane == 0 || kprev != key;
-        const unsigned long long heads = __ballot(head);
-        const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
-        const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
-        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
         if (DG0) {
+            const int32_t kprev = __shfl_up(key, 1, 64);
+            const bool head = lane == 0 || kprev != key;
+            const unsigned long long heads = __ballot(head);
+            const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+            const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
+            const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
             if (key >= 0 && tail) lds_add(t, raw, key, (double)(lane - hl + 1));
         } else if (key >= 0) {
             for (int k = 0; k < 4; ++k) lds_add(t, raw, dof[k], d[k]);
         }
-        // one decision for the whole block (the count read is a heuristic)
-        if (__syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8)) lds_flush(t, raw);
+        // every 8 groups, one decision for the whole block (the count read is
+        // a heuristic; a full table region sends adds straight to HBM)
+        if ((++iter & 7) == 0 && __syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8))
+            lds_flush(t, raw);
     }
     lds_flush(t, raw);
 }
@@ -2119,12 +2125,13 @@ hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* 
     return hipSuccess;
 }
 
-hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits, bool dg0,
+hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int nsp, int cell_bits, bool dg0,
                           double* raw, int grid, hipStream_t st) {
+    if (nsp < 1 || nsp > MAX_SPECIES) return hipErrorInvalidValue;
     if (dg0)
-        hipLaunchKernelGGL(k_density<true>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, cell_bits, raw);
+        hipLaunchKernelGGL(k_density<true>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, nsp, cell_bits, raw);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, cell_bits, raw);
+        hipLaunchKernelGGL(k_density<false>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, nsp, cell_bits, raw);
     LAUNCH_CHECK();
     return hipSuccess;
 }
