@@ -52,9 +52,8 @@ def measured_traffic(cfg, tune):
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     shape = f"{tune['grid']},{tune['block']},{tune['deposit_mode']}"
-    if t.get("launch") != shape:
-        return None
-    return t["traffic_bytes"]
+    rec = t.get("launches", {}).get(shape)
+    return rec["traffic_bytes"] if rec else None
 
 
 def log(*a):

@@ -366,12 +366,13 @@ int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, doub
  * mode (0: four FP64 atomics per particle; 1: wavefront runs of equal cells
  * are summed across lanes first; 2: run sums go into a per-block LDS table
  * keyed by dof, flushed with one atomic per occupied slot; 3: every lane
- * adds its four contributions to that LDS table directly).  Default:
+ * adds its four contributions to that LDS table directly; 4: lanes of a
+ * quad with the same cell are summed first, then added as in 3).  Default:
  * 128 x CUs, 256, mode 3. */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
 /* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every
- * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0..3) on a
+ * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0..4) on a
  * snapshot of the population, restores the snapshot (particles, counts,
  * absorbed counters, rho, KE), and keeps the fastest shape.  A captured step
  * graph is dropped.  No counterpart in the reference. */

@@ -2086,7 +2086,7 @@ extern "C" int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int depos
     if (!p) return fail("set_launch: null population");
     if (grid < 1 || grid > p->ke_np) return fail("set_launch: grid out of range");
     if (block != 64 && block != 128 && block != 256) return fail("set_launch: block must be 64, 128 or 256");
-    if (deposit_mode < 0 || deposit_mode > 3) return fail("set_launch: deposit_mode must be 0, 1, 2 or 3");
+    if (deposit_mode < 0 || deposit_mode > 4) return fail("set_launch: deposit_mode must be 0 to 4");
     graph_free(p);  // a captured graph holds the old launch shape
     p->grid = grid;
     p->block = block;
@@ -2236,7 +2236,7 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     hipEventCreate(&e1);
     const int grids[] = {4, 8, 16, 32, 64, 128};
     const int blocks[] = {128, 256};
-    const int modes[] = {0, 1, 2, 3};
+    const int modes[] = {0, 1, 2, 3, 4};
     int rc = 0;
     for (int gm : grids) {
         for (int bl : blocks) {

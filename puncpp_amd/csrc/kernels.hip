@@ -320,6 +320,31 @@ __device__ inline void deposit_cg1(const DevMesh& M, double* __restrict__ rho_ra
         for (int k = 0; k < 4; ++k) lds_add(t, rho_raw, dof[k], d[k]);
         return;
     }
+    if (mode == 4) {
+        // Lanes of a quad with the same key are first summed into the first
+        // lane of their stretch (two DPP steps inside the quad: from lane+1,
+        // then from lane+2), so that fewer lanes of a wavefront add to the
+        // same LDS slot at once; those lanes then add as in mode 3.
+        const int ql = threadIdx.x & 3;
+        const int32_t k1 = __builtin_amdgcn_update_dpp(0, key, 0xF9, 0xf, 0xf, true);  // quad_perm(1,2,3,3)
+        const int32_t k2 = __builtin_amdgcn_update_dpp(0, key, 0xEE, 0xf, 0xf, true);  // quad_perm(2,3,2,3)
+        const int32_t km = __builtin_amdgcn_update_dpp(0, key, 0x90, 0xf, 0xf, true);  // quad_perm(0,0,1,2)
+        const bool s1 = ql < 3 && k1 == key;
+        const bool s2 = ql < 2 && s1 && k2 == key;
+        for (int k = 0; k < 4; ++k) {
+            const double t1 = dpp_f64<0xF9, 0xf>(d[k]);
+            d[k] += s1 ? t1 : 0.0;
+        }
+        for (int k = 0; k < 4; ++k) {
+            const double t2 = dpp_f64<0xEE, 0xf>(d[k]);
+            d[k] += s2 ? t2 : 0.0;
+        }
+        const bool head = ql == 0 || km != key;
+        if (key < 0 || !head) return;
+        const LdsTable t = lds_table();
+        for (int k = 0; k < 4; ++k) lds_add(t, rho_raw, dof[k], d[k]);
+        return;
+    }
     if (mode >= 1) {
         // A run is a maximal stretch of consecutive lanes with one key; the
         // same key may reappear in a later run, which is summed separately.
@@ -1824,6 +1849,7 @@ static hipError_t launch_pml(const ParticleLaunch& L, hipStream_t st) {
         case 1: return launch_particles_t<PUSH, MOVE, LOCATE, true, 1>(L, st);
         case 2: return launch_particles_t<PUSH, MOVE, LOCATE, true, 2>(L, st);
         case 3: return launch_particles_t<PUSH, MOVE, LOCATE, true, 3>(L, st);
+        case 4: return launch_particles_t<PUSH, MOVE, LOCATE, true, 4>(L, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1859,6 +1885,7 @@ static hipError_t launch_step_pm(const ParticleLaunch& L, hipStream_t st) {
         case 1: return launch_step_t<PUSH, MOVE, 1>(L, st);
         case 2: return launch_step_t<PUSH, MOVE, 2>(L, st);
         case 3: return launch_step_t<PUSH, MOVE, 3>(L, st);
+        case 4: return launch_step_t<PUSH, MOVE, 4>(L, st);
     }
     return hipErrorInvalidValue;
 }

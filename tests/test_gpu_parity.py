@@ -127,7 +127,7 @@ def test_distribute_cg1(gpu):
     assert tot == pytest.approx(5000 * QE + 5000 * QI, abs=1e-9)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_distribute_unsorted_modes(gpu, mode):
     """Particles in random order: equal cells recur in non-adjacent lanes."""
     _print_case("distribute_unsorted", mode=mode)
@@ -214,11 +214,12 @@ def test_sort_preserves_set(gpu):
     assert np.all(np.diff(after["cell"]) >= 0)
 
 
-@pytest.mark.parametrize("pusher,mode,grid", [(1, 2, 0), (2, 2, 0), (2, 3, 0), (1, 3, 7), (2, 1, 3)])
+@pytest.mark.parametrize("pusher,mode,grid", [(1, 2, 0), (2, 2, 0), (2, 3, 0), (1, 3, 7), (2, 1, 3), (2, 4, 0),
+                                               (1, 4, 7)])
 def test_fused_step_matches_sequence(gpu, pusher, mode, grid):
     """punc_gpu_step (push+move+update+inject+deposit) against the oracle's
     reference-ordered sequence (interaction.cpp:384-422, then distribute), for
-    deposit modes 1-3; grid 0 keeps the default grid, a small grid gives each
+    deposit modes 1-4; grid 0 keeps the default grid, a small grid gives each
     block a long chunk (several prefetched groups and a long walk list)."""
     _print_case("fused_step", pusher=pusher, mode=mode, grid=grid)
     hm = sphere_mesh(n=6, L=3.0, r=1.0)
@@ -333,7 +334,7 @@ def test_graph_replay_matches_eager(gpu):
         assert np.allclose(ra, rb, rtol=1e-11, atol=1e-11 * np.abs(ra).max())
 
 
-@pytest.mark.parametrize("mode,graph", [(2, False), (3, False), (2, True)])
+@pytest.mark.parametrize("mode,graph", [(2, False), (3, False), (2, True), (4, True)])
 def test_fused_step_after_sort(gpu, mode, graph):
     """A re-sort leaves its cell order pending and the next fused step reads
     the particles through it (writing the alternate buffers); the result is
