@@ -51,6 +51,14 @@ void density_dg0(const Space&, PopulationType& pop, const std::vector<Species>& 
                                ni.values.data()));
 }
 
+//! ema(f, g, dt, tau) (diagnostics.h:573-588) in the reference's form:
+//! g = w f + (1 - w) g, w = 1 - exp(-dt/tau), over the values of two
+//! Functions of one space (computed on the GPU).
+inline void ema(const Function& f, Function& g, double dt, double tau) {
+    if (f.values.size() != g.values.size()) throw std::runtime_error("ema: f and g differ in length");
+    check(punc_gpu_ema((int64_t)g.values.size(), f.values.data(), g.values.data(), dt, tau));
+}
+
 //! ema(rho, rho_ema, dt, tau) (diagnostics.h:573-588) for the population's
 //! deposited charge density: the average lives on the device and is copied
 //! into g.  reset zeroes it first (a freshly constructed rho_ema).

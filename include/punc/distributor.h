@@ -5,6 +5,7 @@
 
 #include <punc/population.h>
 
+#include <stdexcept>
 #include <vector>
 
 namespace punc {
@@ -14,6 +15,13 @@ inline std::vector<double> element_volume(const Mesh& mesh, bool voronoi = true)
     std::vector<double> dv((size_t)mesh.ndof);
     check(punc_gpu_element_volume(mesh.handle(), voronoi ? 1 : 0, dv.data()));
     return dv;
+}
+
+//! element_volume(V, voronoi) in the reference's form (distributor.h:52),
+//! V a scalar CG1 space (CG1_space(mesh)).
+inline std::vector<double> element_volume(const FunctionSpace& V, bool voronoi = true) {
+    if (V.dg0() || V.value_dim() != 1) throw std::runtime_error("element_volume: V must be a scalar CG1 space");
+    return element_volume(V.mesh(), voronoi);
 }
 
 //! distribute_cg1 (distributor.h:168-210): rho_j = dv_inv_j * sum_p q_p lambda_j(x_p).

@@ -36,6 +36,29 @@ inline void create_flux(const Mesh& mesh, std::vector<Species>& species, uint64_
     }
 }
 
+//! create_flux(species, facets) in the reference's form (injector.h:144,
+//! called at interaction.cpp:119): the tables over the given exterior facets,
+//! Monte Carlo where needed from the Philox stream with seed 0 (species i
+//! uses substream i; the reference's stream is unseeded).
+inline void create_flux(std::vector<Species>& species, std::vector<ExteriorFacet>& facets) {
+    const size_t nf = facets.size();
+    std::vector<double> normal(3 * nf), area(nf);
+    for (size_t j = 0; j < nf; ++j) {
+        area[j] = facets[j].area;
+        for (int k = 0; k < 3; ++k) normal[3 * j + k] = facets[j].normal[k];
+    }
+    for (size_t i = 0; i < species.size(); ++i) {
+        auto& s = species[i];
+        const VelocityPdf& v = velocity_pdf(s);
+        s.vdf->num_particles.assign(nf, 0.0);
+        s.vdf->pdf_max.assign(nf, 0.0);
+        std::vector<double> vd = s.vdf->vd();
+        check(punc_gpu_vdf_flux_facets((int64_t)nf, normal.data(), area.data(), v.kind, s.vdf->vth(), vd.data(),
+                                       v.kappa, v.alpha, v.vdf_range, 0, (int)i, 500000,
+                                       s.vdf->num_particles.data(), s.vdf->pdf_max.data()));
+    }
+}
+
 //! inject_particles (injector.h:157-232) with the create_flux tables.
 template <typename PopulationType>
 void inject_particles(PopulationType& pop, std::vector<Species>& species,

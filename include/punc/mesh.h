@@ -123,6 +123,32 @@ private:
     }
 };
 
+//! The function spaces the particle path uses (the stand-in for
+//! df::FunctionSpace): CG1 or DG0, scalar or 3-vector, on a Mesh.  A space
+//! fixes the length and layout of a Function's values.
+class FunctionSpace {
+public:
+    FunctionSpace(const Mesh& mesh, int value_dim, bool dg0) : mesh_(&mesh), value_dim_(value_dim), dg0_(dg0) {}
+    const Mesh& mesh() const { return *mesh_; }
+    int value_dim() const { return value_dim_; }
+    bool dg0() const { return dg0_; }
+    //! Number of values (df::FunctionSpace::dim).
+    int64_t dim() const { return (dg0_ ? mesh_->num_cells() : mesh_->ndof) * value_dim_; }
+
+private:
+    const Mesh* mesh_;
+    int value_dim_;
+    bool dg0_;
+};
+
+//! CG1_space, CG1_vector_space, DG0_space, DG0_vector_space (reference
+//! poisson.h:146-174).  A periodic mesh carries its constraint in the
+//! vertex -> dof map, so CG1_space takes no constraint argument.
+inline FunctionSpace CG1_space(const Mesh& mesh) { return FunctionSpace(mesh, 1, false); }
+inline FunctionSpace CG1_vector_space(const Mesh& mesh) { return FunctionSpace(mesh, 3, false); }
+inline FunctionSpace DG0_space(const Mesh& mesh) { return FunctionSpace(mesh, 1, true); }
+inline FunctionSpace DG0_vector_space(const Mesh& mesh) { return FunctionSpace(mesh, 3, true); }
+
 //! A CG1 function on a mesh: ndof values per component (the stand-in for
 //! df::Function on the particle path; E has 3 components, rho has 1).
 class Function {
@@ -130,6 +156,9 @@ public:
     //! dg0: one value per cell and component (cell-major) instead of per dof.
     Function(const Mesh& mesh, int value_dim, bool dg0 = false)
         : value_dim(value_dim), dg0(dg0), values((dg0 ? mesh.num_cells() : mesh.ndof) * value_dim, 0.0) {}
+    //! A zero function in V, as df::Function(std::make_shared<const df::FunctionSpace>(V)).
+    explicit Function(const FunctionSpace& V) : Function(V.mesh(), V.value_dim(), V.dg0()) {}
+    explicit Function(const std::shared_ptr<const FunctionSpace>& V) : Function(*V) {}
     int value_dim;
     bool dg0;
     std::vector<double> values;  ///< dof-major: values[dof*value_dim + component]

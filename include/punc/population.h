@@ -162,6 +162,14 @@ public:
         return out;
     }
 
+    //! Particles relocated within 1e-12 of a facet plane of their cell since
+    //! the last call (reporting only; the walk is relocate_fast's).
+    std::size_t num_near_face() {
+        int64_t k = 0;
+        check(punc_gpu_near_face(handle(), &k));
+        return (std::size_t)k;
+    }
+
     //! Re-sort particles by cell (keeps per-cell data reads coalesced).
     void sort() { check(punc_gpu_sort(handle())); }
 

@@ -108,6 +108,14 @@ int punc_gpu_vdf_flux(punc_mesh_t* m, int kind, double vth, const double vd[3], 
                       double vdf_range, uint64_t seed, int rng_species, int64_t n_iter, double* num,
                       double* pdf_max);
 
+/* create_flux(species, facets) (injector.cpp:104-171) for one species over
+ * nf facets given by their inward unit normals (nf*3) and areas (nf), as the
+ * reference's form takes the facet list rather than a mesh; otherwise as
+ * punc_gpu_vdf_flux (the Monte Carlo runs on the current device). */
+int punc_gpu_vdf_flux_facets(int64_t nf, const double* normal, const double* area, int kind, double vth,
+                             const double vd[3], double kappa, double alpha, double vdf_range, uint64_t seed,
+                             int rng_species, int64_t n_iter, double* num, double* pdf_max);
+
 /* ---- population -------------------------------------------------------- */
 /* A population of nspecies species with weighted charge q[s] and mass m[s]
  * (Species::q, Species::m, population.cpp:61-64), room for capacity particles.
@@ -207,6 +215,10 @@ int punc_gpu_statistics(punc_pop_t* p, double* stats);
  * (zero when first used; reset != 0 zeroes it first).  g (ndof) receives
  * rho_ema when non-NULL. */
 int punc_gpu_rho_ema(punc_pop_t* p, double dt, double tau, int reset, double* g);
+/* ema(f, g, dt, tau) (diagnostics.h:573-588) over caller arrays of n values:
+ * g = w f + (1 - w) g with w = 1 - exp(-dt/tau), computed on the current
+ * device (the reference's form, for Functions that live on the host). */
+int punc_gpu_ema(int64_t n, const double* f, double* g, double dt, double tau);
 /* ema(ne, ne_ema, dt, tau) and ema(ni, ni_ema, dt, tau) over the last density
  * computed by punc_gpu_density_cg1/dg0, the averages kept on the device
  * (reset != 0, needed when the density's length changed, zeroes them). */
@@ -325,6 +337,14 @@ int punc_gpu_graph_launch(punc_pop_t* p, int count);
 /* KE of the last push, absorbed counts since the last read (nspecies*64), and
  * particles injected by the last step.  Any pointer may be NULL. */
 int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t* n_injected);
+/* Particles that a relocation (punc_gpu_update or a fused step, eager or
+ * replayed) left within 1e-12 of a facet plane of their cell, |proj| as
+ * relocate_fast evaluates it (population.h:547-565), summed since the last
+ * call (which resets the count).  Where |proj| is that small the walk's
+ * decision rests on rounding: these are the particles whose cell another
+ * evaluation order could place differently.  Reporting only; the relocation
+ * itself is unchanged. */
+int punc_gpu_near_face(punc_pop_t* p, int64_t* n_near);
 /* rho (ndof) from the last deposit. */
 int punc_gpu_get_rho(punc_pop_t* p, double* rho);
 /* The potential phi (ndof) at the particles' positions at the start of each

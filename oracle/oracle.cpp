@@ -285,6 +285,9 @@ bool inside(const Cell& c, const double* p) {
 }
 
 int64_t locate(const Pop& P, const double* p) {
+    // non-finite points are outside the mesh (a NaN passes every plane test)
+    for (int d = 0; d < 3; ++d)
+        if (!std::isfinite(p[d])) return -1;
     int b[3];
     for (int d = 0; d < 3; ++d) {
         double f = floor((p[d] - P.glo[d]) / P.gh[d]);
@@ -299,6 +302,8 @@ int64_t locate(const Pop& P, const double* p) {
 }
 
 int64_t locate_brute(const Pop& P, const double* p) {
+    for (int d = 0; d < 3; ++d)
+        if (!std::isfinite(p[d])) return -1;
     for (int64_t c = 0; c < P.nc; ++c)
         if (inside(P.cells[c], p)) return c;
     return -1;

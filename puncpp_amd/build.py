@@ -10,6 +10,7 @@ serial CPU path.  Packed-FP32 instruction selection is disabled.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -34,9 +35,9 @@ def _run(cmd, verbose):
     return r
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, src + ".o")
@@ -44,8 +45,13 @@ def build(verbose=False):
             cmd = [HIPCC] + DEVICE + COMMON + ["-c", path, "-o", obj]
         else:
             cmd = [HIPCC, "-x", "c++"] + COMMON + ["-c", path, "-o", obj]
-        _run(cmd, verbose)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: one compiler process each
+    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB)

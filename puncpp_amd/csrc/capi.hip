@@ -128,6 +128,9 @@ struct punc_pop_s {
     // captured step
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // Set when a captured step was dropped because a buffer, table or the
+    // species set it captured changed (see invalidate_graph).
+    bool graph_dropped = false;
     // injection
     std::vector<int> inj_on;
     std::vector<SpInstance> inst;    // per species instance binding
@@ -303,6 +306,7 @@ static int mesh_ensure_device(punc_mesh_s* M) {
     D.ndof = M->host.ndof;
     D.inside_tol = T.inside_tol;
     D.coord_bound = T.coord_bound;
+    D.near_lam = T.near_lam;
     for (int d = 0; d < 3; ++d) {
         D.gn[d] = T.grid_n[d];
         D.glo[d] = T.grid_lo[d];
@@ -397,42 +401,54 @@ extern "C" int punc_gpu_vdf_info(int kind, double vth, const double vd[3], doubl
     return 0;
 }
 
-// create_flux for one species: closed forms where the distribution has them,
-// the device Monte Carlo otherwise.
-static int vdf_flux_tables(punc_mesh_s* M, const VdfParams& F, uint64_t seed, int rng_sp, int64_t n_iter,
-                           double* num, double* pdf_max) {
-    const auto& ext = M->topo.ext;
-    const int64_t nf = (int64_t)ext.size();
+// create_flux for one species over nf facets (inward unit normals, 3 per
+// facet, and areas, host arrays): closed forms where the distribution has
+// them, the device Monte Carlo otherwise (on `device`, or the current device
+// when it is negative).
+static int vdf_flux_facets(int device, int64_t nf, const double* normal, const double* area, const VdfParams& F,
+                           uint64_t seed, int rng_sp, int64_t n_iter, double* num, double* pdf_max) {
     const bool cn = vdf_has_flux_num(F), cm = vdf_has_flux_max(F);
     for (int64_t j = 0; j < nf; ++j) {
-        if (cn) num[j] = vdf_flux_num(F, ext[j].normal, ext[j].area);
-        if (cm) pdf_max[j] = vdf_flux_max(F, ext[j].normal);
+        if (cn) num[j] = vdf_flux_num(F, normal + 3 * j, area[j]);
+        if (cm) pdf_max[j] = vdf_flux_max(F, normal + 3 * j);
     }
     if ((cn && cm) || nf == 0) return 0;
     if (n_iter < 1) return fail("vdf_flux: n_iter must be positive");
-    HIPCHK(hipSetDevice(M->device));
-    int r = mesh_ensure_device(M);
-    if (r) return r;
-    std::vector<double> area((size_t)nf);
-    for (int64_t j = 0; j < nf; ++j) area[j] = ext[j].area;
-    double *d_area = nullptr, *d_num = nullptr, *d_max = nullptr;
+    if (device >= 0) HIPCHK(hipSetDevice(device));
+    double *d_normal = nullptr, *d_area = nullptr, *d_num = nullptr, *d_max = nullptr;
     auto cleanup = [&]() {
+        hipFree(d_normal);
         hipFree(d_area);
         hipFree(d_num);
         hipFree(d_max);
     };
-    hipError_t e = dalloc(&d_area, nf);
+    hipError_t e = dalloc(&d_normal, 3 * nf);
+    if (e == hipSuccess) e = dalloc(&d_area, nf);
     if (e == hipSuccess) e = dalloc(&d_num, nf);
     if (e == hipSuccess) e = dalloc(&d_max, nf);
-    if (e == hipSuccess) e = hipMemcpy(d_area, area.data(), nf * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_normal, normal, 3 * nf * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_area, area, nf * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = launch_flux_mc(M->d_fnormal, d_area, nf, F, seed, rng_sp, n_iter, !cn, !cm, d_num, d_max, nullptr);
+        e = launch_flux_mc(d_normal, d_area, nf, F, seed, rng_sp, n_iter, !cn, !cm, d_num, d_max, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess && !cn) e = hipMemcpy(num, d_num, nf * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess && !cm) e = hipMemcpy(pdf_max, d_max, nf * 8, hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return hipfail(e, "vdf_flux");
     return 0;
+}
+
+// create_flux for one species over the mesh's exterior facets.
+static int vdf_flux_tables(punc_mesh_s* M, const VdfParams& F, uint64_t seed, int rng_sp, int64_t n_iter,
+                           double* num, double* pdf_max) {
+    const auto& ext = M->topo.ext;
+    const int64_t nf = (int64_t)ext.size();
+    std::vector<double> normal((size_t)nf * 3), area((size_t)nf);
+    for (int64_t j = 0; j < nf; ++j) {
+        for (int k = 0; k < 3; ++k) normal[3 * j + k] = ext[j].normal[k];
+        area[j] = ext[j].area;
+    }
+    return vdf_flux_facets(M->device, nf, normal.data(), area.data(), F, seed, rng_sp, n_iter, num, pdf_max);
 }
 
 extern "C" int punc_gpu_vdf_flux(punc_mesh_t* m, int kind, double vth, const double vd[3], double kappa,
@@ -445,10 +461,33 @@ extern "C" int punc_gpu_vdf_flux(punc_mesh_t* m, int kind, double vth, const dou
     return vdf_flux_tables(m, F, seed, rng_species, n_iter, num, pdf_max);
 }
 
+extern "C" int punc_gpu_vdf_flux_facets(int64_t nf, const double* normal, const double* area, int kind, double vth,
+                                        const double vd[3], double kappa, double alpha, double vdf_range,
+                                        uint64_t seed, int rng_species, int64_t n_iter, double* num,
+                                        double* pdf_max) {
+    if (nf < 0) return fail("vdf_flux_facets: negative facet count");
+    if (nf > 0 && (!normal || !area || !num || !pdf_max)) return fail("vdf_flux_facets: null argument");
+    if (!vd) return fail("vdf_flux_facets: null drift");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("vdf_flux_facets: " + err);
+    return vdf_flux_facets(-1, nf, normal, area, F, seed, rng_species, n_iter, num, pdf_max);
+}
+
 // ---------------------------------------------------------------------------
 // Population
 // ---------------------------------------------------------------------------
 static void graph_free(punc_pop_s* P);
+
+// A captured step holds raw device pointers (species coefficients, injection
+// tables, candidate buffers) and the species count of its capture: whatever
+// frees or rebuilds one of them, or changes what the replay should compute,
+// drops the graph first, so that a later punc_gpu_graph_launch fails instead
+// of reading freed memory or replaying stale parameters.
+static void invalidate_graph(punc_pop_s* P) {
+    if (P->graph_exec) P->graph_dropped = true;
+    graph_free(P);
+}
 
 static void pop_free(punc_pop_s* P) {
     graph_free(P);
@@ -637,6 +676,7 @@ extern "C" int punc_gpu_pop_add_species(punc_pop_t* p, double q, double mass, in
     if (p->nsp >= p->max_sp) return fail("pop_add_species: species limit of this population reached");
     HIPCHK(hipSetDevice(p->device));
     HIPCHK(hipStreamSynchronize(p->stream));
+    invalidate_graph(p);
     p->q.push_back(q);
     p->m.push_back(mass);
     p->nsp += 1;
@@ -910,6 +950,7 @@ static int get_coef(punc_pop_s* P, int kind, double dt, const double* B, const S
     }
     if (P->coef_cache.size() >= 8) {
         hipStreamSynchronize(P->stream);
+        invalidate_graph(P);
         for (auto& c : P->coef_cache) hipFree(c.d);
         P->coef_cache.clear();
     }
@@ -1256,7 +1297,7 @@ static int field_solve_pop_common(punc_field_t* f, punc_pop_t* p, const double* 
     HIPCHK(launch_cell_potential(p->mesh->dm, field_phi(f->F), p->d_cellE, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     p->field_set = true;
-    if (!p->pe_on) graph_free(p);
+    if (!p->pe_on) invalidate_graph(p);
     p->pe_on = true;
     if (phi) HIPCHK(hipMemcpy(phi, field_phi(f->F), n * 8, hipMemcpyDeviceToHost));
     if (E) HIPCHK(hipMemcpy(E, field_E(f->F), n * 24, hipMemcpyDeviceToHost));
@@ -1589,6 +1630,22 @@ extern "C" int punc_gpu_rho_ema(punc_pop_t* p, double dt, double tau, int reset,
     return 0;
 }
 
+extern "C" int punc_gpu_ema(int64_t n, const double* f, double* g, double dt, double tau) {
+    if (n < 0) return fail("ema: negative length");
+    if (n == 0) return 0;
+    if (!f || !g) return fail("ema: null argument");
+    const double w = 1.0 - exp(-dt / tau);
+    double* d = nullptr;
+    HIPCHK(dalloc(&d, 2 * (size_t)n));
+    hipError_t e = hipMemcpy(d, f, n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, g, n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_ema(d, d + n, w, n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(g, d + n, n * 8, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return hipfail(e, "ema");
+    return 0;
+}
+
 extern "C" int punc_gpu_density_ema(punc_pop_t* p, double dt, double tau, int reset, double* ne_g,
                                     double* ni_g) {
     if (!p) return fail("density_ema: null population");
@@ -1688,6 +1745,7 @@ extern "C" int punc_gpu_injector_set_vdf(punc_pop_t* p, int species, int kind, d
     memcpy(&p->flux_num[species * nf], tn.data(), nf * 8);
     memcpy(&p->flux_max[species * nf], tm.data(), nf * 8);
     p->inj_dt = -1.0;  // rebuild the pair table
+    invalidate_graph(p);
     return 0;
 }
 
@@ -1710,6 +1768,7 @@ extern "C" int punc_gpu_species_instance(punc_pop_t* p, int species, int rng_spe
     I.f0 = facet_begin;
     I.f1 = facet_end;
     p->inj_dt = -1.0;
+    invalidate_graph(p);
     return 0;
 }
 
@@ -1744,6 +1803,7 @@ static int inject_prepare(punc_pop_s* P, double dt) {
     const int64_t np = (int64_t)psp.size();
     cap = std::max<int64_t>(cap, 1024);
     HIPCHK(hipStreamSynchronize(P->stream));
+    invalidate_graph(P);  // the tables below are freed and rebuilt
     hipFree(P->d_num);
     hipFree(P->d_pmax);
     hipFree(P->d_spi);
@@ -1997,6 +2057,7 @@ extern "C" int punc_gpu_step_graph(punc_pop_t* p, int pusher, double push_dt, co
     if (r) return r;
     HIPCHK(hipStreamSynchronize(p->stream));
     graph_free(p);
+    p->graph_dropped = false;
     HIPCHK(launch_set_step(p->d_state, first_step, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     const bool was_timing = p->timing;
@@ -2022,7 +2083,11 @@ extern "C" int punc_gpu_step_graph(punc_pop_t* p, int pusher, double push_dt, co
 
 extern "C" int punc_gpu_graph_launch(punc_pop_t* p, int count) {
     if (!p) return fail("graph_launch: null population");
-    if (!p->graph_exec) return fail("graph_launch: no captured step (punc_gpu_step_graph)");
+    if (!p->graph_exec)
+        return fail(p->graph_dropped ? "graph_launch: the captured step was dropped because the population's species, "
+                                       "injection tables or coefficients changed since the capture; capture again "
+                                       "(punc_gpu_step_graph)"
+                                     : "graph_launch: no captured step (punc_gpu_step_graph)");
     HIPCHK(hipSetDevice(p->device));
     for (int k = 0; k < count; ++k) {
         HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
@@ -2048,6 +2113,18 @@ extern "C" int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbe
     return 0;
 }
 
+extern "C" int punc_gpu_near_face(punc_pop_t* p, int64_t* n_near) {
+    if (!p || !n_near) return fail("near_face: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    DevState s;
+    int r = read_state(p, s);
+    if (r) return r;
+    *n_near = s.n_near;
+    HIPCHK(launch_fill_u64((unsigned long long*)&p->d_state->n_near, 1, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
 static int diag_alloc(punc_pop_s* P);
 
 extern "C" int punc_gpu_set_potential(punc_pop_t* p, const double* phi) {
@@ -2058,7 +2135,7 @@ extern "C" int punc_gpu_set_potential(punc_pop_t* p, const double* phi) {
     if (phi) HIPCHK(hipMemcpyAsync(p->d_phi, phi, p->mesh->topo.ndof * 8, hipMemcpyHostToDevice, p->stream));
     HIPCHK(launch_cell_potential(p->mesh->dm, phi ? p->d_phi : nullptr, p->d_cellE, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
-    if (p->pe_on != (phi != nullptr)) graph_free(p);  // a captured step holds the other kernel
+    if (p->pe_on != (phi != nullptr)) invalidate_graph(p);  // a captured step holds the other kernel
     p->pe_on = phi != nullptr;
     return 0;
 }

@@ -42,7 +42,8 @@ struct DevState {
     // the alternate buffers; other operations apply it first (gather + swap).
     const uint32_t* perm;
     int64_t perm_pending;
-    int64_t pad[4];
+    int64_t n_near;    // relocated particles near a facet of their cell (NEAR_FACE_DIST), since last read
+    int64_t pad[3];
 };
 
 // Per-species coefficients for one call (depends on dt and B).
@@ -66,6 +67,7 @@ struct DevMesh {
     int64_t ndof;
     double inside_tol;   // 0: no "certainly inside" shortcut
     double coord_bound;  // margin valid for |x|, |y|, |z| <= coord_bound
+    double near_lam;     // below it a barycentric coordinate may be near a facet (MeshTopo::near_lam)
     int gn[3];
     double glo[3];
     double gh[3];

@@ -144,6 +144,31 @@ __device__ inline int32_t walk_bary(const DevMesh& M, int32_t c, double x, doubl
     return WALK_FAILED;
 }
 
+// Near-face test (counted and reported, never acted on): some facet plane of
+// cell c gives |proj| < NEAR_FACE_DIST at the point, proj evaluated as the
+// walk evaluates it.  l: the computed barycentric coordinates there; inside
+// the margin's coordinate range only points with a coordinate below
+// near_lam can be that close, so the planes are read for those alone.
+__device__ inline bool near_face_test(const DevMesh& M, int32_t c, double x, double y, double z,
+                                      const double l[4], bool in_range) {
+    if (in_range) {
+        const double lm = fmin(fmin(l[0], l[1]), fmin(l[2], l[3]));
+        if (!(lm < M.near_lam)) return false;
+    }
+    const double* p = M.planes[c].p;
+    bool nf = false;
+    for (int i = 0; i < 4; ++i) nf = nf || fabs(plane_proj(p + 4 * i, x, y, z)) < NEAR_FACE_DIST;
+    return nf;
+}
+
+// Adds the wavefront's near-face particles to S->n_near (one atomic per
+// wavefront).  Every lane of the wavefront must call this together.
+__device__ inline void count_near(DevState* __restrict__ S, bool nf) {
+    const unsigned long long m = __ballot(nf);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1)
+        atomicAdd((unsigned long long*)&S->n_near, (unsigned long long)__popcll(m));
+}
+
 // CG1 deposit of one particle per lane (distribute_cg1, distributor.h:189-203:
 // accum_i += q*lambda_i, then rho[dof_i] += accum_i).  key = cell, or -1 for
 // lanes with nothing to deposit.  mode 0: four FP64 atomics per particle.
@@ -471,6 +496,7 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, dq = 0.0;
         int32_t leave_key = -1;  // particle left: species*MAX_BND + bnd, or ABS_LOST
         bool dready = false;     // dl/ddof already hold the deposit weights and dofs
+        bool near = false;       // relocated near a facet of its cell
         double dl[4] = {0.0, 0.0, 0.0, 0.0};
         int32_t ddof[4] = {0, 0, 0, 0};
         if (active) {
@@ -530,6 +556,17 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             }
             const int32_t c2 = stay ? c : walk(M, c, x, y, z);
             dready = stay;
+            if (c2 >= 0) {
+                const bool in_range = M.inside_tol > 0.0 && fabs(x) <= M.coord_bound &&
+                                      fabs(y) <= M.coord_bound && fabs(z) <= M.coord_bound;
+                double ln[4];
+                if (stay) {
+                    for (int k = 0; k < 4; ++k) ln[k] = dl[k];
+                } else {
+                    bary4(M.rec[c2].bary, x, y, z, ln);
+                }
+                near = near_face_test(M, c2, x, y, z, ln, in_range);
+            }
             if (c2 < 0) {
                 alive = false;
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
@@ -547,7 +584,10 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             dq = coef[sp].q;
         }
         }  // active
-        if (LOCATE) record_leavers(S, holes, absorbed, leave_key, i);
+        if (LOCATE) {
+            record_leavers(S, holes, absorbed, leave_key, i);
+            count_near(S, near);
+        }
         if (DEPOSIT) deposit_cg1<DMODE>(M, rho_raw, dkey, dx_, dy_, dz_, dq, dready, dl, ddof);
         if (use_lds) {
             __syncthreads();
@@ -759,14 +799,17 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         int32_t dkey = -1;  // deposit cell (-1: nothing to deposit in this phase)
         int32_t ddof[4];
         bool walker = false;
+        bool near = false;
         if (active) {
             if (in_range && certainly_inside(dl, tol)) {
                 dkey = c;
                 for (int k = 0; k < 4; ++k) ddof[k] = rd[k];
+                near = near_face_test(M, c, x, y, z, dl, true);
             } else {
                 walker = true;
             }
         }
+        count_near(S, near);
         // Wavefront-aggregated append to this block's walk-list segment.
         const unsigned long long wm = __ballot(walker);
         if (wm) {
@@ -803,6 +846,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         int32_t ddof[4] = {0, 0, 0, 0};
         int32_t leave_key = -1;
         int64_t i = 0;
+        bool near = false;
         if (j < we) {
             const WalkRec r = wrec[j];
             i = r.i;
@@ -816,9 +860,12 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 if (c2 != c) O.cs[i] = (sp << cell_bits) | c2;
                 dkey = c2;
                 dq = coef[sp].q;
+                const bool in_range = tol > 0.0 && fabs(r.x) <= X && fabs(r.y) <= X && fabs(r.z) <= X;
+                near = near_face_test(M, c2, r.x, r.y, r.z, dl, in_range);
             }
         }
         record_leavers(S, holes, absorbed, leave_key, i);
+        count_near(S, near);
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, dq, true, dl, ddof);
     }
     if (use_lds) lds_flush(lds_table(), rho_raw);

@@ -391,7 +391,7 @@ bool build_topology(const HostMesh& m, MeshTopo& t, std::string& err) {
         for (double v : m.xyz) X = std::max(X, fabs(v));
         X = 2.0 * X + 1.0;
         const double u = 0x1.0p-53;
-        double worst = 0.0;
+        double worst = 0.0, near = 0.0;
         for (int64_t c = 0; c < nc && worst < 1e300; ++c) {
             const double* A = t.rec[c].bary;
             const double* P = &t.vcoords[c * 12];
@@ -426,11 +426,20 @@ bool build_topology(const HostMesh& m, MeshTopo& t, std::string& err) {
                         eta = std::max(eta, fabs(v) + ep);
                 }
                 worst = (h > 0) ? std::max(worst, el[i] + alpha * S + (S * eta + ep) / h) : 1e300;
+                // |computed proj_i| < D: the exact stored-plane value is below
+                // D + ep, and mu_i h = -pi_i(x) + sum_{j != i} mu_j pi_i(P_j),
+                // so mu_i < (D + ep + S eta)/h and the computed
+                // lambda_i < mu_i + el_i + alpha S.
+                if (h > 0) near = std::max(near, (NEAR_FACE_DIST + ep + S * eta) / h + el[i] + alpha * S);
             }
         }
         t.inside_tol = 2 * worst;
         t.coord_bound = X;
-        if (!(t.inside_tol < 1e-6)) t.inside_tol = 0;  // degenerate mesh: always walk
+        t.near_lam = 2 * near;
+        if (!(t.inside_tol < 1e-6)) {  // degenerate mesh: always walk, always test the planes
+            t.inside_tol = 0;
+            t.near_lam = INFINITY;
+        }
     }
     t.bnd_ids.clear();
     for (int b = 1; b < 64; ++b)

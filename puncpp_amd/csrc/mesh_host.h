@@ -59,6 +59,12 @@ struct MeshTopo {
     // build_topology).  0 disables the shortcut.
     double inside_tol = 0;
     double coord_bound = 0;  // |coordinate| bound the margin was derived for
+    // Every point whose computed facet projection |proj_i| is below
+    // NEAR_FACE_DIST (within 1e-12 of a facet plane) has a computed
+    // barycentric coordinate lambda_i below near_lam (rounding bounds as for
+    // inside_tol); points above it in all four are not near a facet.
+    // Infinity when the margin is disabled.
+    double near_lam = 0;
     double volume = 0;                // total mesh volume
     // Uniform bucket grid for point location.
     int grid_n[3] = {1, 1, 1};

@@ -54,6 +54,10 @@ PUNC_HD int32_t cs_species(int32_t cs, int cell_bits) { return cs >> cell_bits; 
 // Relocation walk outcome when the hop cap is exceeded (never a valid -bnd id).
 constexpr int32_t WALK_FAILED = -(1 << 30);
 constexpr int WALK_MAX_HOPS = 100000;
+// A particle whose position gives |proj| below this for some facet plane of
+// its cell is "near a face": counted and reported (a walk decision there
+// rests on rounding, and the reference's own location may differ).
+constexpr double NEAR_FACE_DIST = 1e-12;
 
 // ----------------------------------------------------------------------------
 // Counter-based RNG: Philox4x64-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
@@ -200,6 +204,9 @@ PUNC_HD int32_t locate_in_grid(const CellPlanes* planes, const int32_t* bucket_s
                                const int32_t* bucket_cells, const int* gn, const double* glo,
                                const double* gh, double x, double y, double z,
                                int32_t cell_lo = 0, int32_t cell_hi = 0x7fffffff) {
+    // A non-finite coordinate is outside every cell (a NaN would pass every
+    // "proj > 0" test and land in the first candidate).
+    if (!(fabs(x) <= 1.79e308 && fabs(y) <= 1.79e308 && fabs(z) <= 1.79e308)) return -1;
     double q[3] = {x, y, z};
     int b[3];
     for (int d = 0; d < 3; ++d) {

@@ -463,6 +463,13 @@ class Population:
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
 
+    def near_face_count(self):
+        """Particles relocated within 1e-12 of a facet plane of their cell
+        (the walk's proj, population.h:547-565) since the last call."""
+        k = N.C.c_int64()
+        check(N.lib().punc_gpu_near_face(self._h, N.C.byref(k)))
+        return k.value
+
     def set_potential(self, phi):
         """phi (ndof) for the potential energy each fused step computes at
         its starting positions (None switches it off)."""

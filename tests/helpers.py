@@ -81,3 +81,21 @@ def rho_scale(o, hm, dv_inv):
     mag = np.zeros(hm.ndof)
     np.add.at(mag, dofs.ravel(), (np.abs(lam) * q[:, None]).ravel())
     return mag * dv_inv
+
+
+def first_positive_meshes():
+    """Two tetrahedra T0, T1 sharing facet BCD, and the point p = (-0.05,
+    0.6, 0.6) inside T1 that lies on the positive side of two facet planes of
+    T0: BCD (the neighbour, proj 0.087) and ACD (an object facet, boundary
+    id 2, proj 0.05).  The two vertex labelings give the object facet the
+    higher local index (labeling 0) or the lower one (labeling 1): the
+    first-positive walk from T0 (population.h:547-565) then ends in T1 or on
+    the object, where an argmax walk would end in T1 both times and a
+    last-positive walk the other way round."""
+    A, B, Cc, D, E = (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (-0.5, 1.0, 1.0)
+    m0 = P.HostMesh(np.array([A, B, Cc, D, E]), np.array([[0, 1, 2, 3], [1, 2, 3, 4]]),
+                    np.array([[0, 2, 1, 1], [1, 1, 1, 0]], dtype=np.int32))
+    m1 = P.HostMesh(np.array([B, A, Cc, D, E]), np.array([[0, 1, 2, 3], [0, 2, 3, 4]]),
+                    np.array([[2, 0, 1, 1], [1, 1, 1, 0]], dtype=np.int32))
+    p = np.array([-0.05, 0.6, 0.6])
+    return (m0, 1), (m1, -2), p

@@ -117,6 +117,39 @@ def test_locate_matches_oracle_brute_force():
     assert (got < 0).any() and (got >= 0).any()
 
 
+def test_locate_non_finite_is_outside():
+    """NaN or infinite coordinates are outside every cell in both the
+    library's point location and the oracle's (a NaN passes every plane test,
+    and its bucket index would be out of range)."""
+    hm = P.box_mesh(4, [-2.0] * 3, [2.0] * 3, sphere_r=0.8)
+    m = P.Mesh(hm)
+    o = O.OraclePop(hm, [QE], [ME])
+    bad = np.array([[np.nan, 0.1, 0.1], [0.1, np.inf, 0.1], [0.1, 0.1, -np.inf], [np.nan] * 3])
+    assert (m.locate(bad) == -1).all()
+    for x in bad:
+        assert o.locate(x) == -1 and o.locate(x, brute=True) == -1
+    assert o.add_particles(0, bad, np.zeros_like(bad)) == 0
+
+
+def test_facet_flux_form_matches_mesh_form():
+    """create_flux over a facet list (the reference's form) gives the mesh
+    form's closed-form tables bit for bit."""
+    hm = P.box_mesh(3, [-1.0] * 3, [1.0] * 3, sphere_r=0.5)
+    m = P.Mesh(hm)
+    ef = m.exterior_facet_arrays()
+    nf = m.num_exterior
+    for vth, vd in [(1.0, [0, 0, 0.5]), (0.3, [0.2, -0.1, 0.0])]:
+        vd = np.array(vd, dtype=np.float64)
+        a_num, a_max, b_num, b_max = (np.empty(nf) for _ in range(4))
+        N.check(N.lib().punc_gpu_vdf_flux(m._h, 0, vth, N.dptr(vd), 0.0, 0.0, 5.0, 0, 0, 500000,
+                                          N.dptr(a_num), N.dptr(a_max)))
+        nrm = np.ascontiguousarray(ef["normal"])
+        area = np.ascontiguousarray(ef["area"])
+        N.check(N.lib().punc_gpu_vdf_flux_facets(nf, N.dptr(nrm), N.dptr(area), 0, vth, N.dptr(vd), 0.0, 0.0,
+                                                 5.0, 0, 0, 500000, N.dptr(b_num), N.dptr(b_max)))
+        assert np.array_equal(a_num, b_num) and np.array_equal(a_max, b_max)
+
+
 def test_maxwellian_flux_matches_oracle():
     hm = P.box_mesh(3, [-1.0] * 3, [1.0] * 3)
     m = P.Mesh(hm)

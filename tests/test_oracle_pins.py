@@ -15,7 +15,7 @@ from scipy import integrate, optimize
 
 import oracle as O
 import puncpp_amd as P
-from helpers import MI, ME, QE, QI
+from helpers import MI, ME, QE, QI, first_positive_meshes
 
 # ---------------------------------------------------------------------------
 # RNG
@@ -650,3 +650,156 @@ def test_load_statistics():
     assert np.allclose(st["v"].std(0), 1.3, rtol=0.02)
     # density 1 + 0.2 cos(x): E[cos x] = 0.2 * E[cos^2]/E[1] = 0.1
     assert np.mean(np.cos(st["x"][:, 0])) == pytest.approx(0.1, abs=0.01)
+
+
+# ---------------------------------------------------------------------------
+# Pins a plausible slip would fail: Boris' returned energy, the particle
+# counts and kinetic energy recomputed from the particle state, the facet
+# point distribution, and the first-positive-facet walk order.
+# ---------------------------------------------------------------------------
+
+
+def _boris_ke_closed_form(st, Ep, dt):
+    """sum 0.5 m |v_old + (0.5 dt q/m) E(x_old)|^2 (pusher.h:343-350)."""
+    qm = np.where(st["species"] == 0, QE / ME, QI / MI)
+    m = np.where(st["species"] == 0, ME, MI)
+    vminus = st["v"] + (0.5 * dt) * qm[:, None] * Ep
+    return np.sum(0.5 * m[:, None] * vminus * vminus), m
+
+
+def test_boris_ke_uniform_field_closed_form(small):
+    hm, _ = small
+    o = _two_species(hm, n=400)
+    st0 = o.get_particles()
+    E0 = np.array([0.8, -0.5, 1.1])
+    E = np.tile(E0, (hm.ndof, 1))
+    B = np.array([0.0, 0.0, 1.0])
+    dt = 0.2
+    ke = o.boris(E, B, dt)
+    st1 = o.get_particles()
+    ref, m = _boris_ke_closed_form(st0, np.tile(E0, (st0["x"].shape[0], 1)), dt)
+    assert ke == pytest.approx(ref, rel=1e-12)
+    # the energies a slip would return are far outside that tolerance: the
+    # old velocity, the new velocity, and accel's 0.5 m v.(v + a E) form.
+    # (v+ = rotation of v- has the same norm: not a distinguishable slip.)
+    ke_old = np.sum(0.5 * m[:, None] * st0["v"] ** 2)
+    ke_new = np.sum(0.5 * m[:, None] * st1["v"] ** 2)
+    qm = np.where(st0["species"] == 0, QE / ME, QI / MI)
+    a = 0.5 * dt * qm[:, None] * E0[None, :]
+    ke_accel_form = np.sum(0.5 * m[:, None] * st0["v"] * (st0["v"] + a))
+    for wrong in (ke_old, ke_new, ke_accel_form):
+        assert abs(wrong - ref) > 1e-6 * abs(ref)
+
+
+def test_boris_ke_linear_field_at_old_position(small):
+    hm, _ = small
+    o = _two_species(hm, n=300)
+    st0 = o.get_particles()
+    G = np.array([[0.3, 0.2, -0.3], [0.1, 0.4, 0.1], [-0.2, 0.0, 0.5]])
+    c = np.array([0.5, -0.5, 0.25])
+    E = hm.xyz @ G.T + c
+    dt = 0.3
+    ke = o.boris(E, np.array([0.2, -0.1, 1.0]), dt)
+    Ep = st0["x"] @ G.T + c  # linear fields are interpolated exactly
+    ref, _ = _boris_ke_closed_form(st0, Ep, dt)
+    assert ke == pytest.approx(ref, rel=1e-11)
+
+
+def test_kinetic_energy_and_counts_recomputed():
+    """kinetic_energy (diagnostics.h:248-265) and num_of_positives/negatives
+    (population.h:701-744) against numpy over the particle state, with a
+    neutral species that belongs to neither count."""
+    hm = P.box_mesh(4, [-2.0] * 3, [2.0] * 3, sphere_r=0.8)
+    qs, ms = [QE, QI, 0.0, -2.0], [ME, MI, 3.0, 7.0]
+    o = O.OraclePop(hm, qs, ms)
+    for s, (n, vth) in enumerate(((700, 1.0), (500, 0.2), (300, 0.5), (200, 0.7))):
+        assert o.load(s, n, 5, 0.0, 0.0, 0, vth, [0.1, 0.0, 0.5]) == 0
+    st = o.get_particles()
+    q = np.array(qs)[st["species"]]
+    m = np.array(ms)[st["species"]]
+    assert o.size() == st["x"].shape[0] == 1700
+    assert o.num_of_positives() == int((q > 0).sum()) == 500
+    assert o.num_of_negatives() == int((q < 0).sum()) == 900
+    ke = np.sum(0.5 * m[:, None] * st["v"] * st["v"])
+    assert o.kinetic_energy() == pytest.approx(ke, rel=1e-13)
+    # a slip (q/m for m, dropping the 1/2, or one component) would be far off
+    assert abs(np.sum(m[:, None] * st["v"] ** 2) - ke) > 0.5 * ke
+
+
+def _facet_barycentric(o, hm, tol=1e-6):
+    """Barycentric coordinates (w.r.t. the facet's vertices v0, v1, v2) of
+    particles injected with a negligible push, each matched to the exterior
+    facet of its cell whose plane it lies on."""
+    f = o.exterior_facets()
+    st = o.get_particles()
+    by_cell = {}
+    for j, c in enumerate(f["cell"]):
+        by_cell.setdefault(int(c), []).append(j)
+    out = []
+    for x, c in zip(st["x"], st["cell"]):
+        best, bd = None, np.inf
+        for j in by_cell[int(c)]:
+            d = abs(np.dot(x - f["vertices"][j, 0:3], f["normal"][j]))
+            if d < bd:
+                best, bd = j, d
+        assert bd < tol
+        v = f["vertices"][best].reshape(3, 3)
+        M = np.stack([v[1] - v[0], v[2] - v[0]], axis=1)
+        b12, *_ = np.linalg.lstsq(M, x - v[0], rcond=None)
+        out.append([1.0 - b12.sum(), b12[0], b12[1]])
+    return np.array(out)
+
+
+def test_injected_facet_points_uniform_on_triangle():
+    """random_facet_points (injector.cpp:78-102): r1 along v0v1, then
+    sqrt(r2) toward v2 gives points uniform on the triangle, so every
+    barycentric coordinate follows Beta(1, 2) (CDF 1 - (1 - b)^2).  With r2
+    in place of sqrt(r2) the v2 coordinate would be uniform instead."""
+    from scipy import stats
+    hm = P.box_mesh(1, [-100.0] * 3, [100.0] * 3)
+    o = O.OraclePop(hm, [QE], [ME])
+    o.set_injector(0, 1.0, [0.0, 0.0, 0.0], 1e9, 5.0)
+    dt = 1e-9  # pushes of ~1e-9 against facets of size 200
+    assert o.inject(dt, 21, 0) > 30000
+    b = _facet_barycentric(o, hm)
+    assert np.all(b > -1e-9) and np.all(b < 1 + 1e-9)
+    cdf = lambda x: 1.0 - (1.0 - np.clip(x, 0.0, 1.0)) ** 2
+    for k in range(3):
+        d, p = stats.kstest(b[:, k], cdf)
+        assert p > 1e-3, (k, d, p)
+    # the v0/v1 split is uniform: b1 / (b0 + b1) ~ U(0, 1)
+    d, p = stats.kstest(b[:, 1] / (b[:, 0] + b[:, 1]), "uniform")
+    assert p > 1e-3, (d, p)
+    # power of the same test against the slip: r2 instead of sqrt(r2)
+    rng = np.random.default_rng(0)
+    wrong_b2 = 1.0 - rng.uniform(size=b.shape[0])
+    assert stats.kstest(wrong_b2, cdf)[1] < 1e-10
+
+
+def test_relocate_first_positive_facet_order():
+    (m0, want0), (m1, want1), p = first_positive_meshes()
+    for hm, want in ((m0, want0), (m1, want1)):
+        o = O.OraclePop(hm, [QE], [ME])
+        assert o.locate(p, brute=True) == 1  # p lies in T1
+        adj = o.localizer()["adj"]
+        planes = o.localizer()["planes"].reshape(-1, 4, 4)
+        proj = planes[0, :, 0] + planes[0, :, 1:] @ p
+        pos = np.nonzero(proj > 0)[0]
+        assert len(pos) == 2 and sorted(adj[0, pos].tolist()) == [-2, 1]
+        assert o.relocate(p, 0) == want
+        # the walk a slip would do differs from the expected answer
+        argmax_next = adj[0, int(np.argmax(proj))]
+        last_next = adj[0, int(pos[-1])]
+        assert argmax_next == 1 and last_next != (want if want < 0 else 1)
+    # the full update path agrees: the particle is absorbed by the object
+    # under labeling 1 and kept in T1 under labeling 0
+    for hm, want in ((m0, want0), (m1, want1)):
+        o = O.OraclePop(hm, [QE], [ME])
+        x0 = np.array([[0.25, 0.25, 0.25]])
+        assert o.add_particles(0, x0, (p[None, :] - x0) / 0.5) == 1
+        o.move(0.5)
+        ab, ch, lost = o.update()
+        if want < 0:
+            assert o.size() == 0 and ab[0, 2] == 1 and ch[2] == QE
+        else:
+            assert o.size() == 1 and o.get_particles()["cell"][0] == 1
