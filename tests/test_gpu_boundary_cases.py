@@ -134,7 +134,7 @@ def test_graph_dropped_when_captured_state_changes(gpu):
     P.create_flux(sp, mesh)
     pop = P.Population(mesh, sp, capacity=200000)
     P.load_particles(pop, sp, seed=1)
-    P._register_injectors(pop, sp)
+    P.punc._register_injectors(pop, sp)
     E = np.zeros((hm.ndof, 3))
     N.check(N.lib().punc_gpu_set_field(pop._h, N.dptr(E)))
     N.check(N.lib().punc_gpu_distribute(pop._h, N.dptr(P.element_volume(mesh)), None))
@@ -176,7 +176,7 @@ def test_graph_dropped_when_captured_state_changes(gpu):
     N.check(L.punc_gpu_set_field(g2._h, N.dptr(E)))
     N.check(L.punc_gpu_distribute(g2._h, N.dptr(P.element_volume(mesh)), None))
     N.check(L.punc_gpu_injector_set(g2._h, 0, 1.1, N.dptr(np.array([0.0, 0.0, 0.4])), 30.0, 5.0))
-    P._register_injectors(g2, sp[1:])
+    P.punc._register_injectors(g2, sp[1:])
     N.check(L.punc_gpu_step_graph(pop._h, 2, 0.05, N.dptr(B), 0.05, None, 1, 7, 500))
     N.check(L.punc_gpu_graph_launch(pop._h, 2))
     for step in (500, 501):
