@@ -116,6 +116,18 @@ int punc_gpu_vdf_flux_facets(int64_t nf, const double* normal, const double* are
                              const double vd[3], double kappa, double alpha, double vdf_range, uint64_t seed,
                              int rng_species, int64_t n_iter, double* num, double* pdf_max);
 
+/* punc_gpu_vdf_flux_facets without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * facets and the tables are staged through a fixed device array that each
+ * GPU holds from the first call on it until the process ends, at most
+ * 131072 facets per call (larger nf is rejected with an error).  Calls that
+ * overlap on one GPU share that array and race.  Same arguments and results
+ * as punc_gpu_vdf_flux_facets, bit for bit. */
+int punc_gpu_vdf_flux_facets_noalloc(int64_t nf, const double* normal, const double* area, int kind, double vth,
+                                     const double vd[3], double kappa, double alpha, double vdf_range,
+                                     uint64_t seed, int rng_species, int64_t n_iter, double* num, double* pdf_max);
+
 /* ---- population -------------------------------------------------------- */
 /* A population of nspecies species with weighted charge q[s] and mass m[s]
  * (Species::q, Species::m, population.cpp:61-64), room for capacity particles.

@@ -100,6 +100,9 @@ _SIGS = {
     "punc_gpu_vdf_flux_facets": (C.c_int, [C.c_int64, d_p, d_p, C.c_int, C.c_double, d_p, C.c_double, C.c_double,
                                            C.c_double, C.c_uint64, C.c_int, C.c_int64, d_p, d_p]),
     "punc_gpu_ema": (C.c_int, [C.c_int64, d_p, d_p, C.c_double, C.c_double]),
+    "punc_gpu_vdf_flux_facets_noalloc": (C.c_int, [C.c_int64, d_p, d_p, C.c_int, C.c_double, d_p, C.c_double,
+                                                   C.c_double, C.c_double, C.c_uint64, C.c_int, C.c_int64, d_p,
+                                                   d_p]),
     "punc_gpu_get_rho": (C.c_int, [vp, d_p]),
     "punc_gpu_set_potential": (C.c_int, [vp, d_p]),
     "punc_gpu_step_potential_energy": (C.c_int, [vp, d_p]),

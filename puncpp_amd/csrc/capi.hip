@@ -1,6 +1,8 @@
 // C ABI of libpunc_gpu.so: handles, device memory, and the operator entry
 // points declared in include/punc_gpu.h.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <math.h>
 #include <string.h>
 
@@ -472,6 +474,78 @@ extern "C" int punc_gpu_vdf_flux_facets(int64_t nf, const double* normal, const 
     std::string err;
     if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("vdf_flux_facets: " + err);
     return vdf_flux_facets(-1, nf, normal, area, F, seed, rng_species, n_iter, num, pdf_max);
+}
+
+// ---------------------------------------------------------------------------
+// The flux tables without any allocation.  Callers that cannot allocate
+// device memory (code under HIP graph capture, tight serving loops) stage the
+// facets and the two tables through a file-scope device array of fixed size.
+// Each GPU holds its own copy of the array, at its own address, from the
+// first call on that GPU until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the array's
+// name is not a device address in host code).  Every call on one GPU uses the
+// same array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_FLUX_MAX_FACETS = 131072;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+// per facet: 3 normal components, the area, and the two results (6 MiB)
+static __device__ __align__(256) double scratch[6 * NOALLOC_FLUX_MAX_FACETS];
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The current device's scratch address, looked up once per device.
+static int noalloc_scratch(double** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail("vdf_flux_facets_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        double* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hipfail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail("vdf_flux_facets_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int punc_gpu_vdf_flux_facets_noalloc(int64_t nf, const double* normal, const double* area, int kind,
+                                                double vth, const double vd[3], double kappa, double alpha,
+                                                double vdf_range, uint64_t seed, int rng_species, int64_t n_iter,
+                                                double* num, double* pdf_max) {
+    if (nf < 0) return fail("vdf_flux_facets_noalloc: negative facet count");
+    if (nf > NOALLOC_FLUX_MAX_FACETS)
+        return fail("vdf_flux_facets_noalloc: " + std::to_string(nf) + " facets exceed the scratch maximum of " +
+                    std::to_string(NOALLOC_FLUX_MAX_FACETS) + " (use punc_gpu_vdf_flux_facets)");
+    if (nf > 0 && (!normal || !area || !num || !pdf_max)) return fail("vdf_flux_facets_noalloc: null argument");
+    if (!vd) return fail("vdf_flux_facets_noalloc: null drift");
+    VdfParams F;
+    std::string err;
+    if (!vdf_setup(kind, vth, vd, kappa, alpha, vdf_range, F, err)) return fail("vdf_flux_facets_noalloc: " + err);
+    const bool cn = vdf_has_flux_num(F), cm = vdf_has_flux_max(F);
+    for (int64_t j = 0; j < nf; ++j) {
+        if (cn) num[j] = vdf_flux_num(F, normal + 3 * j, area[j]);
+        if (cm) pdf_max[j] = vdf_flux_max(F, normal + 3 * j);
+    }
+    if ((cn && cm) || nf == 0) return 0;
+    if (n_iter < 1) return fail("vdf_flux_facets_noalloc: n_iter must be positive");
+    double* base = nullptr;
+    int r = noalloc_scratch(&base);
+    if (r) return r;
+    double* d_normal = base;
+    double* d_area = base + 3 * nf;
+    double* d_num = base + 4 * nf;
+    double* d_max = base + 5 * nf;
+    // every entry the kernel reads is written here; the results are written
+    // by the kernel for every facet, so nothing needs clearing
+    HIPCHK(hipMemcpy(d_normal, normal, 3 * nf * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_area, area, nf * 8, hipMemcpyHostToDevice));
+    HIPCHK(launch_flux_mc(d_normal, d_area, nf, F, seed, rng_species, n_iter, !cn, !cm, d_num, d_max, nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    if (!cn) HIPCHK(hipMemcpy(num, d_num, nf * 8, hipMemcpyDeviceToHost));
+    if (!cm) HIPCHK(hipMemcpy(pdf_max, d_max, nf * 8, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
