@@ -357,6 +357,15 @@ int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbed, int64_t*
  * evaluation order could place differently.  Reporting only; the relocation
  * itself is unchanged. */
 int punc_gpu_near_face(punc_pop_t* p, int64_t* n_near);
+/* Order-independent digest of the population, per species (nspecies entries
+ * each): digest[s] the wrapping 64-bit sum of a hash of every species-s
+ * particle's x, y, z, vx, vy, vz bit patterns and its cell id (relative to
+ * the species' instance cell range, punc_gpu_species_instance), counts[s]
+ * (may be NULL) the number of such particles.  Equal particle sets give equal
+ * digests whatever the order, so a batched instance can be compared with its
+ * single-instance run without downloading either.  No counterpart in the
+ * reference. */
+int punc_gpu_pop_digest(punc_pop_t* p, uint64_t* digest, int64_t* counts);
 /* rho (ndof) from the last deposit. */
 int punc_gpu_get_rho(punc_pop_t* p, double* rho);
 /* The potential phi (ndof) at the particles' positions at the start of each

@@ -16,6 +16,7 @@ _lib = None
 d_p = C.POINTER(C.c_double)
 i32_p = C.POINTER(C.c_int32)
 i64_p = C.POINTER(C.c_int64)
+u64_p = C.POINTER(C.c_uint64)
 vp = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -97,6 +98,7 @@ _SIGS = {
                                 C.c_uint64]),
     "punc_gpu_step_results": (C.c_int, [vp, d_p, i64_p, i64_p]),
     "punc_gpu_near_face": (C.c_int, [vp, i64_p]),
+    "punc_gpu_pop_digest": (C.c_int, [vp, u64_p, i64_p]),
     "punc_gpu_vdf_flux_facets": (C.c_int, [C.c_int64, d_p, d_p, C.c_int, C.c_double, d_p, C.c_double, C.c_double,
                                            C.c_double, C.c_uint64, C.c_int, C.c_int64, d_p, d_p]),
     "punc_gpu_ema": (C.c_int, [C.c_int64, d_p, d_p, C.c_double, C.c_double]),

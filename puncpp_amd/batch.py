@@ -101,3 +101,82 @@ class BatchedPopulation:
 
     def split_absorbed(self, ab):
         return [ab[i * self.ns:(i + 1) * self.ns] for i in range(self.table.k)]
+
+
+class BiasSweep:
+    """The probe-bias sweep workload: n_inst independent sphere-probe
+    instances on [-10, 10]^3 with a sphere of radius 1 as the object, meshes
+    of n^3 cubes x 6 tetrahedra with n drawn uniformly from {16, 20, 24, 28,
+    32}, probe bias linspace(-5, 5) (in T_e/e) shaping each instance's frozen
+    field E = 0.05 bias x / max(|x|, r)^3, density scale drawn uniformly from
+    [0.5, 2] (scale * base_particles particles per instance, half electrons
+    and half ions), Maxwellian velocities with drift (0, 0, 0.5 vth_e) and
+    boundary injection, Boris push with B = (0, 0, 1), dt = 0.02.  Instance i
+    loads and injects with seed seed_base + i."""
+
+    def __init__(self, n_inst=64, seed=1, base_particles=2_000_000, seed_base=1000):
+        from .punc import Maxwellian, Mesh, ParticleAmountType, Species, UniformPosition, box_mesh
+        rng = np.random.default_rng(seed)
+        self.n_inst = n_inst
+        self.ns = rng.choice([16, 20, 24, 28, 32], size=n_inst)
+        self.bias = np.linspace(-5.0, 5.0, n_inst)
+        self.scale = rng.uniform(0.5, 2.0, size=n_inst)
+        self.L, self.r, self.dt = 10.0, 1.0, 0.02
+        self.vd = [0.0, 0.0, 0.5]
+        self.B = np.array([0.0, 0.0, 1.0])
+        self.seeds = [seed_base + i for i in range(n_inst)]
+        self.host_meshes, self.meshes, self.species, self.fields = [], [], [], []
+        for i in range(n_inst):
+            hm = box_mesh(int(self.ns[i]), [-self.L] * 3, [self.L] * 3, sphere_r=self.r)
+            mesh = Mesh(hm)
+            dens = self.scale[i] * base_particles / 2 / mesh.volume()
+            sp = [Species(q, m, dens, 1.0, ParticleAmountType.phys_per_sim, mesh, UniformPosition(mesh),
+                          Maxwellian(v, self.vd))
+                  for (q, m), v in zip([(-1.0, 1.0), (1.0, 1836.0)], [1.0, 1.0 / np.sqrt(1836.0)])]
+            xyz = np.zeros((hm.ndof, 3))
+            xyz[hm.vdof] = hm.xyz
+            r2 = np.maximum((xyz ** 2).sum(1), self.r * self.r)
+            self.fields.append(np.ascontiguousarray(0.05 * self.bias[i] * xyz / r2[:, None] ** 1.5))
+            self.host_meshes.append(hm)
+            self.meshes.append(mesh)
+            self.species.append(sp)
+
+    def particles(self, i):
+        return sum(int(s.num) for s in self.species[i])
+
+    def total_particles(self):
+        return sum(self.particles(i) for i in range(self.n_inst))
+
+    def plain_population(self, i, sort=True):
+        """Instance i as a population of its own: loaded, (sorted,) injectors
+        registered, field set, deposited once."""
+        from .punc import Population, _register_injectors, element_volume
+        L = N.lib()
+        pop = Population(self.meshes[i], [(-1.0, 1.0), (1.0, 1836.0)],
+                         capacity=int(self.particles(i) * 1.1) + 100000)
+        for s, S in enumerate(self.species[i]):
+            check(L.punc_gpu_pop_load(pop._h, s, int(S.num), self.seeds[i], 0.0, 0.0, 0, S.vdf.vth(),
+                                      dptr(S.vdf.vd())))
+        if sort:
+            pop.sort()
+        _register_injectors(pop, self.species[i])
+        check(L.punc_gpu_set_field(pop._h, dptr(self.fields[i])))
+        pop.dv_inv = np.ascontiguousarray(element_volume(self.meshes[i]))
+        check(L.punc_gpu_distribute(pop._h, dptr(pop.dv_inv), None))
+        return pop
+
+    def batched_population(self, sort=True):
+        """All instances in one population: loaded, (sorted,) injectors
+        registered, field set, deposited once."""
+        from .punc import element_volume
+        L = N.lib()
+        bp = BatchedPopulation(self.host_meshes, self.species, self.seeds,
+                               capacity=int(self.total_particles() * 1.1) + 1_000_000)
+        bp.load()
+        if sort:
+            bp.pop.sort()
+        bp.register_injectors()
+        check(L.punc_gpu_set_field(bp.pop._h, dptr(np.concatenate(self.fields))))
+        bp.dv_inv = np.ascontiguousarray(element_volume(bp.mesh))
+        check(L.punc_gpu_distribute(bp.pop._h, dptr(bp.dv_inv), None))
+        return bp

@@ -152,6 +152,9 @@ struct punc_pop_s {
     int32_t* d_ccs = nullptr;
     int64_t* d_keep = nullptr;
     int64_t* d_pos = nullptr;
+    // population digest (allocated on first use): 2 * MAX_SPECIES sums and
+    // counts, then MAX_SPECIES int32 cell offsets
+    unsigned long long* d_digest = nullptr;
     // diagnostics (allocated on first use)
     double* d_diag = nullptr;      // per-block partials (6 per block) + results
     double* d_phi = nullptr;       // ndof
@@ -565,6 +568,7 @@ static void invalidate_graph(punc_pop_s* P) {
 
 static void pop_free(punc_pop_s* P) {
     graph_free(P);
+    hipFree(P->d_digest);
     hipFree(P->d_pair_sp);
     hipFree(P->d_pair_f);
     if (P->d_state) hipFree(P->d_state);
@@ -2183,6 +2187,30 @@ extern "C" int punc_gpu_step_results(punc_pop_t* p, double* KE, int64_t* absorbe
         r = read_state(p, s);
         if (r) return r;
         *n_injected = s.inj_kept;
+    }
+    return 0;
+}
+
+extern "C" int punc_gpu_pop_digest(punc_pop_t* p, uint64_t* digest, int64_t* counts) {
+    if (!p || !digest) return fail("pop_digest: null argument");
+    HIPCHK(hipSetDevice(p->device));
+    int r = ensure_order(p);
+    if (r) return r;
+    if (!p->d_digest) HIPCHK(dalloc(&p->d_digest, 2 * MAX_SPECIES + MAX_SPECIES / 2));
+    const int nsp = p->nsp;
+    if (nsp == 0) return 0;
+    std::vector<int32_t> c0((size_t)nsp);
+    for (int s = 0; s < nsp; ++s) c0[s] = p->inst[s].bound ? (int32_t)p->inst[s].c0 : 0;
+    int32_t* d_c0 = (int32_t*)(p->d_digest + 2 * MAX_SPECIES);
+    HIPCHK(hipMemcpyAsync(d_c0, c0.data(), nsp * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(launch_fill_u64(p->d_digest, 2 * nsp, p->stream));
+    HIPCHK(launch_pop_digest(p->d_state, p->cell_bits, d_c0, nsp, p->d_digest, p->num_cu * 8, p->stream));
+    std::vector<unsigned long long> h((size_t)2 * nsp);
+    HIPCHK(hipMemcpyAsync(h.data(), p->d_digest, 2 * nsp * 8, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (int s = 0; s < nsp; ++s) {
+        digest[s] = h[s];
+        if (counts) counts[s] = (int64_t)h[nsp + s];
     }
     return 0;
 }

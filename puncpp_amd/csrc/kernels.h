@@ -141,6 +141,10 @@ hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale
 hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, double* part, double* out, int grid,
                              hipStream_t st);
 hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream_t st);
+// Per species (nsp <= MAX_SPECIES): out[s] += sum of particle hashes (cell ids
+// relative to c0[s]), out[nsp + s] += count; out must be zeroed first.
+hipError_t launch_pop_digest(DevState* S, int cell_bits, const int32_t* c0, int nsp, unsigned long long* out,
+                             int grid, hipStream_t st);
 
 // Population files: pack particles [first, first+count) as 8-double records
 // (x, y, z, vx, vy, vz, q, m); unpack `count` records, locate them and append

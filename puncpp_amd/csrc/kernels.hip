@@ -1739,6 +1739,47 @@ __global__ void k_ema(const double* __restrict__ f, double* __restrict__ g, doub
 }
 
 // ---------------------------------------------------------------------------
+// Order-independent digest of a population: per species the wrapping sum of a
+// 64-bit hash of each particle's bit patterns (x, y, z, vx, vy, vz and the
+// cell id relative to c0[species]) and the particle count.  Two populations
+// hold the same particles (as sets, bit for bit) exactly when their
+// digests agree, up to hash collisions.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t mix64(uint64_t z) {  // the splitmix64 finalizer
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256)
+k_pop_digest(DevState* __restrict__ S, int cell_bits, const int32_t* __restrict__ c0, int nsp,
+             unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sh[2 * MAX_SPECIES];
+    for (int k = threadIdx.x; k < 2 * nsp; k += blockDim.x) sh[k] = 0ull;
+    __syncthreads();
+    const GArrays A = gview(S->cur);
+    const int64_t n = S->n;
+    const int32_t cmask = (1 << cell_bits) - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t cs = A.cs[i];
+        const int sp = cs >> cell_bits;
+        const uint32_t cell = (uint32_t)((cs & cmask) - c0[sp]);
+        uint64_t h = mix64((uint64_t)__double_as_longlong(A.x[i]) ^ 0x9e3779b97f4a7c15ull);
+        h = mix64(h ^ (uint64_t)__double_as_longlong(A.y[i]));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(A.z[i]));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vx[i]));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vy[i]));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vz[i]));
+        h = mix64(h ^ (uint64_t)cell);
+        atomicAdd(&sh[sp], (unsigned long long)h);
+        atomicAdd(&sh[nsp + sp], 1ull);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * nsp; k += blockDim.x)
+        if (sh[k]) atomicAdd(&out[k], sh[k]);
+}
+
+// ---------------------------------------------------------------------------
 // Flux tables by Monte Carlo (create_flux, injector.cpp:127-168) for the
 // distributions without a closed form: per exterior facet, n_iter uniform
 // velocities in the box vd +- range*vth give the flux integral
@@ -2239,6 +2280,13 @@ hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream
 }  // namespace punc_gpu
 
 namespace punc_gpu {
+hipError_t launch_pop_digest(DevState* S, int cell_bits, const int32_t* c0, int nsp, unsigned long long* out,
+                             int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_pop_digest, dim3(grid), dim3(256), 0, st, S, cell_bits, c0, nsp, out);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
 hipError_t launch_flux_mc(const double* fnormal, const double* farea, int64_t nf, const VdfParams& F, uint64_t seed,
                           int rng_sp, int64_t n_iter, bool do_num, bool do_max, double* num, double* pmax,
                           hipStream_t st) {

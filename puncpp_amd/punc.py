@@ -463,6 +463,15 @@ class Population:
     def sort(self):
         check(N.lib().punc_gpu_sort(self._h))
 
+    def digest(self):
+        """(digest, counts) per species: an order-independent 64-bit hash sum
+        of the particles' bit patterns and cells (instance-local for bound
+        species), and the particle counts."""
+        d = np.zeros(self.nspecies, dtype=np.uint64)
+        c = np.zeros(self.nspecies, dtype=np.int64)
+        check(N.lib().punc_gpu_pop_digest(self._h, d.ctypes.data_as(N.u64_p), i64ptr(c)))
+        return d, c
+
     def near_face_count(self):
         """Particles relocated within 1e-12 of a facet plane of their cell
         (the walk's proj, population.h:547-565) since the last call."""

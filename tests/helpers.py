@@ -101,3 +101,30 @@ def first_positive_meshes():
                     np.array([[2, 0, 1, 1], [1, 1, 1, 0]], dtype=np.int32))
     p = np.array([-0.05, 0.6, 0.6])
     return (m0, 1), (m1, -2), p
+
+
+def digest_np(st, nspecies, c0=None):
+    """The population digest (punc_gpu_pop_digest) of a particle state in
+    numpy: per species the wrapping sum of the splitmix64-chained hash of the
+    x, y, z, vx, vy, vz bit patterns and the cell id (minus c0[species])."""
+    def mix(z):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+        return z ^ (z >> np.uint64(31))
+
+    sp = st["species"].astype(np.int64)
+    cell = st["cell"].astype(np.int64)
+    if c0 is not None:
+        cell = cell - np.asarray(c0, dtype=np.int64)[sp]
+    with np.errstate(over="ignore"):
+        x = np.ascontiguousarray(st["x"]).view(np.uint64)
+        v = np.ascontiguousarray(st["v"]).view(np.uint64)
+        h = mix(x[:, 0] ^ np.uint64(0x9e3779b97f4a7c15))
+        for col in (x[:, 1], x[:, 2], v[:, 0], v[:, 1], v[:, 2]):
+            h = mix(h ^ col)
+        h = mix(h ^ (cell.astype(np.uint32).astype(np.uint64)))
+        out = np.zeros(nspecies, dtype=np.uint64)
+        for s in range(nspecies):
+            out[s] = np.sum(h[sp == s], dtype=np.uint64)
+    counts = np.bincount(sp, minlength=nspecies).astype(np.int64)
+    return out, counts
