@@ -276,6 +276,12 @@ int punc_gpu_poisson_solve(punc_field_t* f, const double* rho, const double* ext
                            const double* object_potentials, double* phi, int* iterations);
 /* phi (ndof) -> E (3*ndof, dof-major x, y, z). */
 int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double* E, int* iterations);
+/* Reverse of punc_gpu_efield_solve: given gE = dJ/dE (3*ndof, dof-major),
+ * gphi = dJ/dphi (ndof) = B^T M^-1 gE, with M the vector-CG1 mass matrix and
+ * B the gradient form of the projection (EField3D.ufl, efield.cpp:70-75;
+ * E = M^-1 B phi, M symmetric).  One mass-matrix CG to the solver tolerance.
+ * No counterpart in the reference (used by the adjoint). */
+int punc_gpu_efield_adjoint(punc_field_t* f, const double* gE, double* gphi, int* iterations);
 /* Charge on each object of the potential phi (ndof): the reference's Charge
  * form eps0 grad(phi).n ds over the object's facets (Charge.ufl), n_objects
  * values. */
@@ -400,6 +406,22 @@ int punc_gpu_adjoint_forward(punc_adj_t* a, const double* x0, const int64_t* cel
 /* Given g = dJ/dw (ndof), returns dJ/dE (ndof*3), dJ/dv0 and dJ/dx0 (n*3);
  * any output may be NULL. */
 int punc_gpu_adjoint_backward(punc_adj_t* a, const double* g, double* dJdE, double* dJdv0, double* dJdx0);
+/* The collected charge of the last forward: per particle, weight the weight
+ * (K - n - s)/K of its charge in Q_b, the time average over the K steps of
+ * the charge boundary b has collected (n: the step in which it left, s in
+ * [0, 1): the fraction of that step at which its path crosses the plane of
+ * the facet the walk left through); bnd the boundary it left through and
+ * step the step (0 and -1 while in the mesh).  Q_b = sum_p q_p weight_p over
+ * bnd_p == b is continuous in the inputs along a fixed crossing path, unlike
+ * the collected charge itself.  irregular: exits where the walk left through
+ * a plane the step's segment does not cross (weight uses s clamped to 0 or
+ * 1; no sensitivity).  Any pointer may be NULL. */
+int punc_gpu_adjoint_collected(punc_adj_t* a, double* weight, int32_t* bnd, int32_t* step, int64_t* irregular);
+/* As punc_gpu_adjoint_backward with also gq[b] = dJ/dQ_b (64 values, index =
+ * boundary id; NULL: 0), the derivative of J with respect to the collected
+ * charges above. */
+int punc_gpu_adjoint_backward_q(punc_adj_t* a, const double* g, const double* gq, double* dJdE, double* dJdv0,
+                                double* dJdx0);
 
 /* ---- launch control, timing -------------------------------------------- */
 /* Launch shape of the particle kernels: grid size (each block takes one

@@ -117,6 +117,9 @@ _SIGS = {
     "punc_gpu_synchronize": (C.c_int, [vp]),
     "punc_gpu_copy_probe": (C.c_int, [C.c_int64, C.c_int, d_p]),
     "punc_gpu_adjoint_last_error": (C.c_char_p, []),
+    "punc_gpu_adjoint_collected": (C.c_int, [vp, d_p, i32_p, i32_p, i64_p]),
+    "punc_gpu_adjoint_backward_q": (C.c_int, [vp, d_p, d_p, d_p, d_p, d_p]),
+    "punc_gpu_efield_adjoint": (C.c_int, [vp, d_p, d_p, i32_p]),
     "punc_gpu_adjoint_create": (C.c_int, [vp, C.c_int, d_p, d_p, C.c_int64, i32_p, C.POINTER(vp)]),
     "punc_gpu_adjoint_destroy": (None, [vp]),
     "punc_gpu_adjoint_forward": (C.c_int, [vp, d_p, i64_p, d_p, d_p, d_p, C.c_double, d_p, C.c_int, C.c_int,

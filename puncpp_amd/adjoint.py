@@ -2,9 +2,14 @@
 
 ``PushDeposit`` runs K steps of boris_cg1 + move (+ relocation) in a fixed CG1
 field and deposits the final positions; ``push_deposit(v0, E, runner)`` is the
-differentiable map (v0, E) -> w, w_j = sum_p q_p lambda_j(x_K), whose
-backward runs the hand-written reverse kernels.  Objectives are ordinary torch
-expressions of w, e.g. ||rho||^2 = sum_j (dv_inv_j w_j)^2.
+differentiable map (v0, E) -> w, w_j = sum_p q_p lambda_j(x_K), and
+``push_deposit_collect(v0, E, runner)`` the map (v0, E) -> (w, Q), Q_b the
+time average over the K steps of the charge boundary b has collected (the
+collected charge's continuous form, punc_gpu_adjoint_collected).  Their
+backward runs the hand-written reverse kernels.  ``efield_projection(phi,
+field)`` is the device E projection E = M^-1 B phi with its reverse, so
+objectives differentiate down to the vertex potentials.  Objectives are
+ordinary torch expressions, e.g. ||rho||^2 = sum_j (dv_inv_j w_j)^2 or Q[2].
 """
 import numpy as np
 import torch
@@ -59,12 +64,36 @@ class PushDeposit:
             return w, xK, vK, cK
         return w
 
-    def backward(self, g):
+    def collected(self):
+        """Of the last forward: per particle the weight (K - n - s)/K of its
+        charge in Q, the boundary it left through and the step (0 and -1 while
+        in the mesh), and the number of irregular exits."""
+        w = np.empty(self.n)
+        b = np.empty(self.n, dtype=np.int32)
+        st = np.empty(self.n, dtype=np.int32)
+        irr = N.C.c_int64()
+        self._check(N.lib().punc_gpu_adjoint_collected(self._h, dptr(w), i32ptr(b), i32ptr(st), N.C.byref(irr)))
+        return w, b, st, irr.value
+
+    def collected_charge(self):
+        """Q (64,): Q_b = sum_p q_p weight_p over the particles that left
+        through boundary b, summed in particle order."""
+        w, b, _, _ = self.collected()
+        return np.bincount(b, weights=self.q[self.species] * w, minlength=64)[:64]
+
+    def backward(self, g, gq=None):
         g = np.ascontiguousarray(g, dtype=np.float64)
         dE = np.empty((self.mesh.ndof, 3))
         dv = np.empty((self.n, 3))
         dx = np.empty((self.n, 3))
-        self._check(N.lib().punc_gpu_adjoint_backward(self._h, dptr(g), dptr(dE), dptr(dv), dptr(dx)))
+        if gq is None:
+            self._check(N.lib().punc_gpu_adjoint_backward(self._h, dptr(g), dptr(dE), dptr(dv), dptr(dx)))
+        else:
+            gq = np.ascontiguousarray(gq, dtype=np.float64)
+            if gq.shape != (64,):
+                raise ValueError("gq must hold 64 values (one per boundary id)")
+            self._check(N.lib().punc_gpu_adjoint_backward_q(self._h, dptr(g), dptr(gq), dptr(dE), dptr(dv),
+                                                            dptr(dx)))
         return dE, dv, dx
 
 
@@ -85,6 +114,51 @@ class _PushDepositFn(torch.autograd.Function):
         ctx.runner.forward(*ctx.inputs)
         dE, dv, _ = ctx.runner.backward(gw.detach().cpu().numpy())
         return torch.from_numpy(dv), torch.from_numpy(dE), None
+
+
+class _PushDepositCollectFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, v0, E, runner):
+        v = v0.detach().cpu().numpy()
+        e = E.detach().cpu().numpy()
+        w = runner.forward(v, e)
+        Q = runner.collected_charge()
+        ctx.runner = runner
+        ctx.inputs = (v, e)
+        return torch.from_numpy(w), torch.from_numpy(Q)
+
+    @staticmethod
+    def backward(ctx, gw, gQ):
+        ctx.runner.forward(*ctx.inputs)
+        g = np.zeros(ctx.runner.mesh.ndof) if gw is None else gw.detach().cpu().numpy()
+        gq = np.zeros(64) if gQ is None else gQ.detach().cpu().numpy()
+        dE, dv, _ = ctx.runner.backward(g, gq)
+        return torch.from_numpy(dv), torch.from_numpy(dE), None
+
+
+class _EProjectionFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi, field):
+        ctx.field = field
+        return torch.from_numpy(field.efield(phi.detach().cpu().numpy()))
+
+    @staticmethod
+    def backward(ctx, gE):
+        return torch.from_numpy(ctx.field.efield_adjoint(gE.detach().cpu().numpy())), None
+
+
+def efield_projection(phi, field):
+    """phi [ndof] -> E [ndof, 3], the device's vector-CG1 projection
+    (ESolver, efield.cpp:70-75), differentiable in phi through
+    dJ/dphi = B^T M^-1 dJ/dE.  field: a puncpp_amd.FieldSolver."""
+    return _EProjectionFn.apply(phi, field)
+
+
+def push_deposit_collect(v0, E, runner):
+    """(v0 [n,3], E [ndof,3]) -> (w [ndof], Q [64]), differentiable in v0 and
+    E: w the deposit of the final positions, Q the time-averaged collected
+    charge per boundary id (PushDeposit.collected_charge)."""
+    return _PushDepositCollectFn.apply(v0, E, runner)
 
 
 def gradient_operator(mesh):

@@ -19,7 +19,22 @@
 // The position -> cell map is piecewise constant and both the CG1 field and
 // the CG1 deposit are continuous across faces, so the path-wise derivative is
 // the derivative of J wherever no particle crosses the mesh boundary.
-// Particles absorbed before step K carry no sensitivity.
+//
+// Collected charge.  A particle that leaves through boundary b during step n
+// (walk exit through facet f of cell c, plane pi(x) = c0 + n.x, negative
+// inside) crosses pi = 0 at the fraction s = pi(x_n) / (pi(x_n) - pi(x_{n+1}))
+// of the step.  The objective input Q_b = sum_p q_p (K - n_p - s_p) / K over
+// the particles that left through b is the time average over the K steps of
+// the charge b has collected: unlike the collected charge itself (piecewise
+// constant along a fixed crossing path) it is continuous, also when a
+// crossing moves from the end of one step to the start of the next, and its
+// derivative flows through s.  Given gq_b = dJ/dQ_b, the reverse sweep seeds
+// such a particle at its exit step with
+//   xbar_{n+1} = gq_b q (-1/K) ds/dpi_{n+1} n,   vbar_{n+1} = 0,
+//   xbar_n    += gq_b q (-1/K) ds/dpi_n n,
+//   ds/dpi_n = -pi_{n+1} / D^2,  ds/dpi_{n+1} = pi_n / D^2,  D = pi_n - pi_{n+1},
+// and propagates it through the steps before as any other particle.
+// A particle absorbed without such a seed carries no sensitivity.
 //
 // States of one checkpoint segment are recomputed forward from the
 // segment's checkpoint and held on the device, so memory is
@@ -62,7 +77,10 @@ __device__ inline void adj_bary(const double* A, double x, double y, double z, d
     l[3] = ((1 - l[0]) - l[1]) - l[2];
 }
 
-__device__ inline int32_t adj_walk(const DevMesh& M, int32_t c, double x, double y, double z) {
+// relocate_fast's walk (population.h:547-565).  Returns the cell, -bnd when
+// the walk leaves the mesh (exit: cell * 4 + facet of the exit), or
+// WALK_FAILED after the hop cap.
+__device__ inline int32_t adj_walk(const DevMesh& M, int32_t c, double x, double y, double z, int32_t& exit) {
     for (int h = 0; h < WALK_MAX_HOPS; ++h) {
         const double* p = M.planes[c].p;
         const int f = plane_proj(p, x, y, z) > 0 ? 0
@@ -71,16 +89,29 @@ __device__ inline int32_t adj_walk(const DevMesh& M, int32_t c, double x, double
                       : plane_proj(p + 12, x, y, z) > 0 ? 3 : -1;
         if (f < 0) return c;
         const int32_t nb = M.rec[c].nbr[f];
-        if (nb < 0) return -1;
+        if (nb < 0) {
+            exit = c * 4 + f;
+            return nb;
+        }
         c = nb;
     }
-    return -2;
+    return WALK_FAILED;
 }
+
+// Absorption record of the main forward sweep: abs_info = bnd | step << 8
+// (0: not absorbed), abs_exit = exit cell * 4 + facet, abs_w = the particle's
+// weight (K - n - s) / K in Q_bnd.
+struct AdjAbsorb {
+    int32_t* info;
+    int32_t* exit;
+    double* w;
+};
 
 __global__ void __launch_bounds__(256)
 k_adj_step(const DevMesh M, AdjState in, AdjState out, const int32_t* __restrict__ sp,
            const AdjCoef* __restrict__ coef, const double4* __restrict__ E, double dt, int periodic,
-           double Lx, double Ly, double Lz, int64_t n, unsigned long long* __restrict__ lost) {
+           double Lx, double Ly, double Lz, int64_t n, unsigned long long* __restrict__ lost, int record,
+           int step, int Ksteps, AdjAbsorb ab, unsigned long long* __restrict__ irregular) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = in.c[i];
         double x = in.x[i], y = in.y[i], z = in.z[i];
@@ -115,9 +146,27 @@ k_adj_step(const DevMesh M, AdjState in, AdjState out, const int32_t* __restrict
                 y -= Ly * floor(y / Ly);
                 z -= Lz * floor(z / Lz);
             }
-            c2 = adj_walk(M, c, x, y, z);
-            if (c2 == -2) {
+            int32_t exit = -1;
+            c2 = adj_walk(M, c, x, y, z, exit);
+            if (c2 == WALK_FAILED) {
                 atomicAdd(lost, 1ull);
+                c2 = -1;
+            } else if (c2 < 0) {
+                if (record) {
+                    const double* p = M.planes[exit >> 2].p + 4 * (exit & 3);
+                    const double pa = plane_proj(p, in.x[i], in.y[i], in.z[i]);
+                    const double pb = plane_proj(p, x, y, z);
+                    double sc = 0.0;
+                    if (pa <= 0.0 && pb > 0.0) {
+                        sc = pa / (pa - pb);
+                    } else {  // the visibility walk left through a plane the segment does not cross
+                        atomicAdd(irregular, 1ull);
+                        sc = pa > 0.0 ? 0.0 : 1.0;
+                    }
+                    ab.info[i] = (-c2) | (step << 8);
+                    ab.exit[i] = exit;
+                    ab.w[i] = ((double)(Ksteps - step) - sc) / (double)Ksteps;
+                }
                 c2 = -1;
             }
         }
@@ -172,18 +221,47 @@ k_adj_seed(const DevMesh M, AdjState st, const int32_t* __restrict__ sp, const A
 }
 
 __global__ void __launch_bounds__(256)
-k_adj_reverse(const DevMesh M, AdjState st, const int32_t* __restrict__ c_next, const int32_t* __restrict__ sp,
+k_adj_reverse(const DevMesh M, AdjState st, AdjState nx, const int32_t* __restrict__ sp,
               const AdjCoef* __restrict__ coef, const double4* __restrict__ E, double dt, int64_t n,
-              double* __restrict__ xb, double* __restrict__ vb, double* __restrict__ gE) {
+              double* __restrict__ xb, double* __restrict__ vb, double* __restrict__ gE, AdjAbsorb ab,
+              const double* __restrict__ gq, int Ksteps) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = st.c[i];
-        if (c < 0 || c_next[i] < 0) continue;  // absorbed: no sensitivity
+        if (c < 0) continue;  // absorbed earlier
+        const AdjCoef K = coef[sp[i]];
+        double xbx, xby, xbz;
+        double sx0 = 0.0, sy0 = 0.0, sz0 = 0.0;  // seed of xbar_n at an exit step
+        if (nx.c[i] < 0) {
+            // left the mesh during this step: only a collected-charge seed
+            // gives it sensitivity
+            const double G = gq ? gq[ab.info[i] & 0xff] * K.q : 0.0;
+            if (G == 0.0) continue;
+            const int32_t ex = ab.exit[i];
+            const double* p = M.planes[ex >> 2].p + 4 * (ex & 3);
+            const double pa = plane_proj(p, st.x[i], st.y[i], st.z[i]);
+            const double pb = plane_proj(p, nx.x[i], nx.y[i], nx.z[i]);
+            if (!(pa <= 0.0 && pb > 0.0)) continue;  // irregular exit (counted by the forward): no seed
+            const double D = pa - pb;
+            const double dJds = -G / (double)Ksteps;
+            const double ga = dJds * (-pb / (D * D)), gb = dJds * (pa / (D * D));
+            xbx = gb * p[1];
+            xby = gb * p[2];
+            xbz = gb * p[3];
+            sx0 = ga * p[1];
+            sy0 = ga * p[2];
+            sz0 = ga * p[3];
+            vb[i] = 0.0;
+            vb[n + i] = 0.0;
+            vb[2 * n + i] = 0.0;
+        } else {
+            xbx = xb[i];
+            xby = xb[n + i];
+            xbz = xb[2 * n + i];
+        }
         const CellRec& R = M.rec[c];
         double l[4];
         adj_bary(R.bary, st.x[i], st.y[i], st.z[i], l);
         const double4 e[4] = {E[R.dof[0]], E[R.dof[1]], E[R.dof[2]], E[R.dof[3]]};
-        const AdjCoef K = coef[sp[i]];
-        double xbx = xb[i], xby = xb[n + i], xbz = xb[2 * n + i];
         // vbar_{n+1} += dt xbar_{n+1}
         const double wx = vb[i] + dt * xbx, wy = vb[n + i] + dt * xby, wz = vb[2 * n + i] + dt * xbz;
         // v'bar = s x vbar_{n+1}
@@ -202,9 +280,9 @@ k_adj_reverse(const DevMesh M, AdjState st, const int32_t* __restrict__ c_next, 
         double d[4];
         for (int k = 0; k < 4; ++k) d[k] = Ebx * e[k].x + Eby * e[k].y + Ebz * e[k].z;
         const double* A = R.bary;
-        xbx += (d[0] - d[3]) * A[1] + (d[1] - d[3]) * A[5] + (d[2] - d[3]) * A[9];
-        xby += (d[0] - d[3]) * A[2] + (d[1] - d[3]) * A[6] + (d[2] - d[3]) * A[10];
-        xbz += (d[0] - d[3]) * A[3] + (d[1] - d[3]) * A[7] + (d[2] - d[3]) * A[11];
+        xbx += (d[0] - d[3]) * A[1] + (d[1] - d[3]) * A[5] + (d[2] - d[3]) * A[9] + sx0;
+        xby += (d[0] - d[3]) * A[2] + (d[1] - d[3]) * A[6] + (d[2] - d[3]) * A[10] + sy0;
+        xbz += (d[0] - d[3]) * A[3] + (d[1] - d[3]) * A[7] + (d[2] - d[3]) * A[11] + sz0;
         xb[i] = xbx;
         xb[n + i] = xby;
         xb[2 * n + i] = xbz;
@@ -241,7 +319,9 @@ struct punc_adj_s {
     double* d_g = nullptr;
     double* d_xb = nullptr;
     double* d_vb = nullptr;
-    unsigned long long* d_lost = nullptr;
+    unsigned long long* d_lost = nullptr;  // [0] walks over the hop cap, [1] irregular exits
+    double* d_gq = nullptr;                // 64: dJ/dQ_b
+    AdjAbsorb ab{};                        // absorption record of the forward sweep
     // checkpoints (K/every + 1 states) and one segment (every + 1 states)
     std::vector<AdjState> ckpt, seg;
     std::vector<void*> allocs;
@@ -297,6 +377,10 @@ extern "C" void punc_gpu_adjoint_destroy(punc_adj_t* A) {
     hipFree(A->d_xb);
     hipFree(A->d_vb);
     hipFree(A->d_lost);
+    hipFree(A->d_gq);
+    hipFree(A->ab.info);
+    hipFree(A->ab.exit);
+    hipFree(A->ab.w);
     if (A->stream) hipStreamDestroy(A->stream);
     delete A;
 }
@@ -341,7 +425,11 @@ extern "C" int punc_gpu_adjoint_create(punc_mesh_t* m, int nspecies, const doubl
     ACHK(hipMalloc((void**)&A->d_gE, A->ndof * 3 * 8));
     ACHK(hipMalloc((void**)&A->d_xb, n * 3 * 8));
     ACHK(hipMalloc((void**)&A->d_vb, n * 3 * 8));
-    ACHK(hipMalloc((void**)&A->d_lost, 8));
+    ACHK(hipMalloc((void**)&A->d_lost, 16));
+    ACHK(hipMalloc((void**)&A->d_gq, 64 * 8));
+    ACHK(hipMalloc((void**)&A->ab.info, n * 4));
+    ACHK(hipMalloc((void**)&A->ab.exit, n * 4));
+    ACHK(hipMalloc((void**)&A->ab.w, n * 8));
 #undef ACHK
     *out = A;
     return 0;
@@ -372,9 +460,12 @@ static int copy_state(punc_adj_s* A, const AdjState& a, AdjState& b) {
     return 0;
 }
 
-static int adj_step(punc_adj_s* A, const AdjState& in, AdjState& out) {
+// record: the main forward sweep (writes the absorption record); step: the
+// step's index in 0..K-1.
+static int adj_step(punc_adj_s* A, const AdjState& in, AdjState& out, bool record, int step) {
     hipLaunchKernelGGL(k_adj_step, dim3(A->grid), dim3(256), 0, A->stream, A->M, in, out, A->d_sp, A->d_coef, A->d_E,
-                       A->dt, A->periodic, A->Ld[0], A->Ld[1], A->Ld[2], A->n, A->d_lost);
+                       A->dt, A->periodic, A->Ld[0], A->Ld[1], A->Ld[2], A->n, A->d_lost, record ? 1 : 0, step,
+                       A->K, A->ab, A->d_lost + 1);
     AHIP(hipGetLastError());
     return 0;
 }
@@ -428,7 +519,10 @@ extern "C" int punc_gpu_adjoint_forward(punc_adj_t* A, const double* x0, const i
     AHIP(hipMemcpy(A->d_gE, E, A->ndof * 3 * sizeof(double), hipMemcpyHostToDevice));
     AHIP(launch_expand_field(A->d_gE, A->d_E, A->ndof, A->stream));
     AHIP(hipStreamSynchronize(A->stream));
-    AHIP(hipMemset(A->d_lost, 0, 8));
+    AHIP(hipMemset(A->d_lost, 0, 16));
+    AHIP(hipMemset(A->ab.info, 0, A->n * 4));
+    AHIP(hipMemset(A->ab.exit, 0xff, A->n * 4));
+    AHIP(hipMemset(A->ab.w, 0, A->n * 8));
     const int nck = K / every + 1;
     A->ckpt.resize(nck);
     for (auto& s : A->ckpt)
@@ -443,7 +537,7 @@ extern "C" int punc_gpu_adjoint_forward(punc_adj_t* A, const double* x0, const i
     int cur = 0;
     for (int nstep = 0; nstep < K; ++nstep) {
         const int nxt = 1 - cur;
-        if (adj_step(A, A->seg[cur], A->seg[nxt])) return -1;
+        if (adj_step(A, A->seg[cur], A->seg[nxt], true, nstep)) return -1;
         cur = nxt;
         if ((nstep + 1) % every == 0 && (nstep + 1) / every < nck)
             if (copy_state(A, A->seg[cur], A->ckpt[(nstep + 1) / every])) return -1;
@@ -463,11 +557,38 @@ extern "C" int punc_gpu_adjoint_forward(punc_adj_t* A, const double* x0, const i
     return 0;
 }
 
+extern "C" int punc_gpu_adjoint_collected(punc_adj_t* A, double* weight, int32_t* bnd, int32_t* step,
+                                          int64_t* irregular) {
+    if (!A) return adj_fail("adjoint_collected: null argument");
+    if (!A->have_forward) return adj_fail("adjoint_collected: run adjoint_forward first");
+    if (weight) AHIP(hipMemcpy(weight, A->ab.w, A->n * 8, hipMemcpyDeviceToHost));
+    if (bnd || step) {
+        std::vector<int32_t> info(A->n);
+        AHIP(hipMemcpy(info.data(), A->ab.info, A->n * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < A->n; ++i) {
+            if (bnd) bnd[i] = info[i] & 0xff;
+            if (step) step[i] = info[i] ? (info[i] >> 8) : -1;
+        }
+    }
+    if (irregular) {
+        unsigned long long c[2];
+        AHIP(hipMemcpy(c, A->d_lost, 16, hipMemcpyDeviceToHost));
+        *irregular = (int64_t)c[1];
+    }
+    return 0;
+}
+
 extern "C" int punc_gpu_adjoint_backward(punc_adj_t* A, const double* g, double* dJdE, double* dJdv0, double* dJdx0) {
+    return punc_gpu_adjoint_backward_q(A, g, nullptr, dJdE, dJdv0, dJdx0);
+}
+
+extern "C" int punc_gpu_adjoint_backward_q(punc_adj_t* A, const double* g, const double* gq, double* dJdE,
+                                           double* dJdv0, double* dJdx0) {
     if (!A || !g) return adj_fail("adjoint_backward: null argument");
     if (!A->have_forward) return adj_fail("adjoint_backward: run adjoint_forward first");
     const int K = A->K, every = A->every;
     AHIP(hipMemcpy(A->d_g, g, A->ndof * 8, hipMemcpyHostToDevice));
+    if (gq) AHIP(hipMemcpy(A->d_gq, gq, 64 * 8, hipMemcpyHostToDevice));
     AHIP(hipMemsetAsync(A->d_gE, 0, A->ndof * 3 * 8, A->stream));
     // Segments [s0, s1) from the last to the first.
     bool seeded = false;
@@ -475,7 +596,7 @@ extern "C" int punc_gpu_adjoint_backward(punc_adj_t* A, const double* g, double*
         const int s1 = std::min(K, s0 + every);
         if (copy_state(A, A->ckpt[s0 / every], A->seg[0])) return -1;
         for (int k = s0; k < s1; ++k)
-            if (adj_step(A, A->seg[k - s0], A->seg[k - s0 + 1])) return -1;
+            if (adj_step(A, A->seg[k - s0], A->seg[k - s0 + 1], false, k)) return -1;
         if (!seeded) {
             hipLaunchKernelGGL(k_adj_seed, dim3(A->grid), dim3(256), 0, A->stream, A->M, A->seg[s1 - s0], A->d_sp,
                                A->d_coef, A->d_g, A->n, A->d_xb, A->d_vb);
@@ -484,8 +605,8 @@ extern "C" int punc_gpu_adjoint_backward(punc_adj_t* A, const double* g, double*
         }
         for (int k = s1 - 1; k >= s0; --k) {
             hipLaunchKernelGGL(k_adj_reverse, dim3(A->grid), dim3(256), 0, A->stream, A->M, A->seg[k - s0],
-                               A->seg[k - s0 + 1].c, A->d_sp, A->d_coef, A->d_E, A->dt, A->n, A->d_xb, A->d_vb,
-                               A->d_gE);
+                               A->seg[k - s0 + 1], A->d_sp, A->d_coef, A->d_E, A->dt, A->n, A->d_xb, A->d_vb,
+                               A->d_gE, A->ab, gq ? (const double*)A->d_gq : nullptr, K);
             AHIP(hipGetLastError());
         }
     }

@@ -1239,7 +1239,7 @@ extern "C" int punc_gpu_kinetic_energy(punc_pop_t* p, double* KE) {
 struct punc_field_s {
     punc_mesh_s* mesh = nullptr;
     FieldSolver* F = nullptr;
-    double* d_in = nullptr;  // host-given rho or phi staged on the device
+    double* d_in = nullptr;  // host-given rho, phi (ndof) or dJ/dE (3 ndof) + dJ/dphi (ndof) staged on the device
 };
 
 extern "C" int punc_gpu_field_create(punc_mesh_t* m, double eps0, int dirichlet_exterior, int n_objects,
@@ -1253,7 +1253,7 @@ extern "C" int punc_gpu_field_create(punc_mesh_t* m, double eps0, int dirichlet_
     punc_field_s* h = new punc_field_s();
     h->mesh = m;
     h->F = F;
-    hipError_t e = dalloc(&h->d_in, (size_t)m->topo.ndof);
+    hipError_t e = dalloc(&h->d_in, 4 * (size_t)m->topo.ndof);
     if (e != hipSuccess) {
         field_destroy(F);
         delete h;
@@ -1350,6 +1350,17 @@ extern "C" int punc_gpu_efield_solve(punc_field_t* f, const double* phi, double*
     std::string err;
     if (!field_efield(f->F, f->d_in, iterations, err)) return fail("efield_solve: " + err);
     HIPCHK(hipMemcpy(E, field_E(f->F), n * 24, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int punc_gpu_efield_adjoint(punc_field_t* f, const double* gE, double* gphi, int* iterations) {
+    if (!f || !gE || !gphi) return fail("efield_adjoint: null argument");
+    HIPCHK(hipSetDevice(f->mesh->device));
+    const int64_t n = f->mesh->topo.ndof;
+    HIPCHK(hipMemcpy(f->d_in, gE, n * 24, hipMemcpyHostToDevice));
+    std::string err;
+    if (!field_efield_adjoint(f->F, f->d_in, f->d_in + 3 * n, iterations, err)) return fail("efield_adjoint: " + err);
+    HIPCHK(hipMemcpy(gphi, f->d_in + 3 * n, n * 8, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -42,6 +42,9 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
                    int* iterations, std::string& err);
 // E projection of d_phi (device, ndof) into field_E.
 bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::string& err);
+// Reverse of the E projection: d_gphi (device, ndof) = B^T M^-1 d_gE (device,
+// 3*ndof dof-major), the derivative of J(E(phi)) in phi given dJ/dE.
+bool field_efield_adjoint(FieldSolver* F, const double* d_gE, double* d_gphi, int* iterations, std::string& err);
 // Charges on the objects (the Charge form, Charge.ufl) of the potential d_phi
 // (device, ndof): out[j] for j < n_objects (host).
 bool field_object_charges(FieldSolver* F, const double* d_phi, double* out, std::string& err);

@@ -167,6 +167,20 @@ __global__ void __launch_bounds__(FB) k_grad3(const int32_t* __restrict__ col, i
     }
 }
 
+// out_j = sum_i sum_c (B_c)_ij y[3i + c]: the transpose of k_grad3's product
+// (out zeroed first; FP64 atomics, the sum order varies).
+__global__ void __launch_bounds__(FB) k_grad3_t(const int32_t* __restrict__ col, int w, const double* __restrict__ g0,
+                                                const double* __restrict__ g1, const double* __restrict__ g2,
+                                                const double* __restrict__ y, double* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double y0 = y[3 * i], y1 = y[3 * i + 1], y2 = y[3 * i + 2];
+        for (int k = 0; k < w; ++k) {
+            const double v = (g0[k * n + i] * y0 + g1[k * n + i] * y1) + g2[k * n + i] * y2;
+            if (v != 0.0) unsafeAtomicAdd(&out[col[k * n + i]], v);
+        }
+    }
+}
+
 // Per-block sums of v, then the mean (one block).
 __global__ void __launch_bounds__(FB) k_partial_sum(const double* __restrict__ v, int64_t n, double* part) {
     __shared__ double sh[8];
@@ -1157,6 +1171,26 @@ bool field_efield(FieldSolver* F, const double* d_phi, int* iterations, std::str
     int it = 0;
     if (!run_cg(F, CgOp{nullptr, &F->mell, 1}, F->dinv_m, F->rhs, &it, err, 3)) return false;
     FCHK(hipMemcpyAsync(F->E, F->x, n * 24, hipMemcpyDeviceToDevice, F->st));
+    FCHK(hipStreamSynchronize(F->st));
+    if (iterations) *iterations = it;
+    return true;
+}
+
+// Reverse of the E projection: E = M^-1 B phi, M symmetric, so
+// dJ/dphi = B^T M^-1 dJ/dE: one CG on the vector mass matrix (right-hand side
+// d_gE, dof-major like E), then the transposed gradient product into d_gphi.
+bool field_efield_adjoint(FieldSolver* F, const double* d_gE, double* d_gphi, int* iterations, std::string& err) {
+    FCHK(hipSetDevice(F->device));
+    const int64_t n = F->n;
+    const dim3 g(F->grid), blk(FB);
+    FCHK(hipMemcpyAsync(F->rhs, d_gE, n * 24, hipMemcpyDeviceToDevice, F->st));
+    FCHK(hipMemsetAsync(F->x, 0, n * 24, F->st));
+    int it = 0;
+    if (!run_cg(F, CgOp{nullptr, &F->mell, 1}, F->dinv_m, F->rhs, &it, err, 3)) return false;
+    FCHK(hipMemsetAsync(d_gphi, 0, n * 8, F->st));
+    hipLaunchKernelGGL(k_grad3_t, g, blk, 0, F->st, F->col, F->w, F->grad[0], F->grad[1], F->grad[2], F->x, d_gphi,
+                       n);
+    FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(F->st));
     if (iterations) *iterations = it;
     return true;

@@ -667,6 +667,16 @@ class FieldSolver:
         self.iterations = (0, it.value)
         return E
 
+    def efield_adjoint(self, gE):
+        """Reverse of efield: dJ/dphi (ndof) = B^T M^-1 dJ/dE for gE = dJ/dE
+        (ndof x 3), M the vector-CG1 mass matrix and B the projection's
+        gradient form (E = M^-1 B phi)."""
+        gE = np.ascontiguousarray(gE, dtype=np.float64).reshape(self.mesh.ndof, 3)
+        gphi = np.empty(self.mesh.ndof)
+        it = N.C.c_int32()
+        check(N.lib().punc_gpu_efield_adjoint(self._h, dptr(gE), dptr(gphi), N.C.byref(it)))
+        return gphi
+
     def object_charges(self, phi):
         """Charge on each object of the potential phi (Charge.ufl)."""
         phi = np.ascontiguousarray(phi, dtype=np.float64)
