@@ -522,6 +522,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the diagnostics, flux Monte Carlo and field-solve timings (profiling runs)")
     ap.add_argument("--launch", default=None,
                     help="GRID,BLOCK,MODE: fixed launch shape instead of the autotuner (profiling runs)")
     ap.add_argument("--batched-instances", type=int, default=64)
@@ -543,7 +545,7 @@ def main():
         sys.exit(2)
     launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     res = run_gpu(args.config, args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune,
-                  launch=launch)
+                  launch=launch, extras=not args.no_extras)
     batched = None if args.no_batched else run_batched(args.batched_instances, args.batched_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
     out = {
