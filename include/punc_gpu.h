@@ -177,7 +177,9 @@ int punc_gpu_pop_save_file(punc_pop_t* p, const char* fname, int binary);
  * file in that format (text: the numbers of each line fill x, v, q, m in
  * turn; a line that gives fewer keeps the previous line's values), locating
  * each point and skipping those outside the mesh.  Every new (q, m) pair
- * becomes a species (punc_gpu_pop_add_species).  n_added may be NULL. */
+ * becomes a species (punc_gpu_pop_add_species).  n_added may be NULL.  On
+ * an error (species limit, capacity, HIP failure) the population keeps its
+ * particles and species from before the call and *n_added is not set. */
 int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int binary, int64_t* n_added);
 
 /* ---- the particle operators -------------------------------------------- */

@@ -1507,6 +1507,36 @@ extern "C" int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int bina
     if (r) return r;
     FILE* f = fopen(fname, binary ? "rb" : "r");
     if (!f) return fail(std::string("pop_load_file: cannot open ") + fname);
+    // A failed load leaves the population as it was: the count, the species
+    // list and the overflow counter from here are restored on every error.
+    DevState s0;
+    r = read_state(p, s0);
+    if (r) {
+        fclose(f);
+        return r;
+    }
+    const int nsp0 = p->nsp;
+    auto rollback = [&]() {
+        if (p->nsp > nsp0) {
+            invalidate_graph(p);
+            const size_t nf = p->mesh->topo.ext.size();
+            p->q.resize(nsp0);
+            p->m.resize(nsp0);
+            p->inj_on.resize(nsp0);
+            p->inst.resize(nsp0);
+            p->spi.resize(nsp0);
+            p->flux_num.resize((size_t)nsp0 * nf);
+            p->flux_max.resize((size_t)nsp0 * nf);
+            for (auto& c : p->coef_cache) hipFree(c.d);
+            p->coef_cache.clear();
+            p->inj_dt = -1.0;
+            p->nsp = nsp0;
+        }
+        const int64_t keep[2] = {s0.n, s0.overflow};
+        hipStreamSynchronize(p->stream);
+        hipMemcpy((char*)p->d_state + offsetof(DevState, n), &keep[0], 8, hipMemcpyHostToDevice);
+        hipMemcpy((char*)p->d_state + offsetof(DevState, overflow), &keep[1], 8, hipMemcpyHostToDevice);
+    };
     const int64_t chunk = FILE_CHUNK;
     double *d_rec = nullptr, *d_cx = nullptr, *d_sp = nullptr;
     int32_t* d_ccs = nullptr;
@@ -1571,6 +1601,7 @@ extern "C" int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int bina
                 r = punc_gpu_pop_add_species(p, q, m, &idx);
                 if (r) {
                     cleanup();
+                    rollback();
                     return r;
                 }
             }
@@ -1590,11 +1621,22 @@ extern "C" int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int bina
         if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
         if (e != hipSuccess) {
             cleanup();
+            rollback();
             return hipfail(e, "pop_load_file");
         }
         added += kept;
     }
     cleanup();
+    DevState s1;
+    r = read_state(p, s1);
+    if (r) return r;
+    if (s1.overflow > s0.overflow) {
+        const int64_t dropped = s1.overflow - s0.overflow;
+        rollback();
+        if (n_added) *n_added = 0;
+        return fail("pop_load_file: population capacity exceeded (" + std::to_string(dropped) +
+                    " record(s) did not fit); nothing was loaded");
+    }
     if (n_added) *n_added = added;
     return check_errors(p, "pop_load_file");
 }

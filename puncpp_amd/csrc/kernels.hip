@@ -717,6 +717,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         load_particle(A, permuted ? (int64_t)perm[j0] : j0, nxt);
         if (permuted) src_next = perm[clampi(s + threadIdx.x + blockDim.x)];
     }
+    int grp = 0;  // groups of this chunk done
     for (int64_t base = s; base < e; base += blockDim.x) {
         const int64_t i = base + threadIdx.x;
         const bool active = i < e;
@@ -828,6 +829,13 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             }
         }
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, K.q, true, dl, ddof);
+        // Table occupancy, every 8 groups before the chunk's last (all
+        // block-uniform): a long chunk flushes a table that is 5/8 full
+        // instead of sending its further adds to HBM after 16 failed probes.
+        // A chunk of 8 groups or fewer never takes this branch.
+        if (use_lds && (++grp & 7) == 0 && base + blockDim.x < e &&
+            __syncthreads_or(threadIdx.x == 0 && *lds_table().count > LDS_SLOTS * 5 / 8))
+            lds_flush(lds_table(), rho_raw);
     }
     const double t = block_sum(ke, sh);  // synchronizes: wcnt and the segment are complete
     if (threadIdx.x == 0) ke_part[blockIdx.x] = t;

@@ -108,3 +108,41 @@ def test_round_trip_large(gpu, tmp_path):
     for k in ("x", "v", "species", "cell"):
         np.testing.assert_array_equal(a[k], b[k])
     assert np.all(np.diff(b["cell"]) >= 0)
+
+
+def test_failed_load_leaves_population_unchanged(gpu, tmp_path):
+    # An error part way through leaves the particles and species as they were,
+    # and the population keeps working: a later load that fits succeeds.
+    _print_case("failed_load_rollback")
+    hm = P.box_mesh(2, [0.0] * 3, [1.0] * 3)
+    mesh = P.Mesh(hm)
+    pop = P.Population(mesh, [(-1.0, 1.0)], capacity=64)
+    rng = np.random.default_rng(5)
+    x0 = rng.uniform(0.05, 0.95, (10, 3))
+    assert pop.add_particles(x0, rng.normal(size=(10, 3)), -1.0, 1.0) == 10
+    before = pop.get_particles()
+
+    def recs(n, q, m):
+        r = np.zeros((n, 8))
+        r[:, :3] = rng.uniform(0.05, 0.95, (n, 3))
+        r[:, 3:6] = rng.normal(size=(n, 3))
+        r[:, 6], r[:, 7] = q, m
+        return r
+
+    # 1) capacity: 100 records into 54 free slots, with a new species among them
+    f = tmp_path / "big.bin"
+    np.concatenate([recs(50, -1.0, 1.0), recs(50, 2.0, 3.0)]).tofile(f)
+    with pytest.raises(N.PuncGpuError, match="capacity"):
+        pop.load_file(f, binary=True)
+    # 2) species limit: more distinct (q, m) pairs than the packed field holds
+    f2 = tmp_path / "many.bin"
+    np.concatenate([recs(1, 1.0 + k, 1.0) for k in range(200)]).tofile(f2)
+    with pytest.raises(N.PuncGpuError, match="species limit"):
+        pop.load_file(f2, binary=True)
+    assert pop.size() == 10 and list(zip(pop.q, pop.m)) == [(-1.0, 1.0)]
+    assert_same_sets(pop.get_particles(), before, rel=0.0, what="after failed loads")
+    f3 = tmp_path / "fits.bin"
+    extra = np.concatenate([recs(20, -1.0, 1.0), recs(20, 2.0, 3.0)])
+    extra.tofile(f3)
+    assert pop.load_file(f3, binary=True) == 40
+    assert pop.size() == 50 and sorted(zip(pop.q, pop.m)) == [(-1.0, 1.0), (2.0, 3.0)]

@@ -263,11 +263,12 @@ def test_fused_step_matches_sequence(gpu, pusher, mode, grid):
         assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * scale)
 
 
-@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("mode", [2, 3, 4])
 def test_fused_step_table_overflow(gpu, mode):
-    """One block processes the whole population on a mesh with more dofs than
-    the LDS deposit table has slots, so the table fills up and further
-    contributions take the direct HBM path; results as the oracle's."""
+    """One block processes the whole population (70 groups) on a mesh with
+    more dofs than the LDS deposit table has slots: the table is flushed
+    whenever it is 5/8 full at an 8-group boundary, and adds that find it
+    full in between take the direct HBM path; results as the oracle's."""
     _print_case("fused_step_overflow", mode=mode)
     hm = sphere_mesh(n=14, L=4.0, r=1.0)
     assert hm.ndof > 2048
