@@ -6,13 +6,15 @@ of 64^3 cubes x 6 tetrahedra with the r=2 sphere object removed (1,572,864
 tets before removal), 64M particles (32M electrons q=-1 m=1 vth=1, 32M ions
 q=+1 m=1836 vth=1/sqrt(1836), both drifting (0,0,0.5)), Boris push with
 B=(0,0,1), dt=0.02, Maxwellian flux injection at the exterior boundary, CG1
-deposit each step, cell re-sort every 12 steps.  The field is a frozen
-synthetic CG1 field resident on the GPU (the host field solve is outside this
-path).  One step = push + move + relocation walk with absorption + removal +
-injection + deposit, all on the device.
+deposit each step, cell re-sort every 20 steps (also timed at 12).  The
+field is a frozen synthetic CG1 field resident on the GPU (the host field
+solve is outside this path).  One step = push + move + relocation walk with
+absorption + removal + injection + deposit, all on the device.
 
 Prints one JSON line (metric, value in particle-steps/s, roofline of the fused
-particle kernel, CPU baseline of the oracle on a bounded sample).
+particle kernel, the same for the sphere-probe configuration (4M particles),
+CPU baseline of the oracle on a bounded sample, per-phase times of the
+drop-in calls and of the oracle under the reference's Timer task names).
 """
 import argparse
 import json
@@ -70,7 +72,7 @@ def synthetic_field(hm, seed=1):
     return np.ascontiguousarray(E)
 
 
-def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, extras=True):
+def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, extras=True, alt_sort_every=0):
     import puncpp_amd as P
     from puncpp_amd import _native as N
 
@@ -119,22 +121,37 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
     N.check(lib.punc_gpu_step_graph(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, 1))
     k_done = [1]
 
-    def run(nsteps):
+    def run(nsteps, every):
         for _ in range(nsteps):
             N.check(lib.punc_gpu_graph_launch(pop._h, 1))
             k_done[0] += 1
-            if sort_every and k_done[0] % sort_every == 0:
+            if every and k_done[0] % every == 0:
                 N.check(lib.punc_gpu_sort(pop._h))
 
-    run(max(warmup - 1, 0))
+    run(max(warmup - 1, 0), sort_every)
     N.check(lib.punc_gpu_synchronize(pop._h))
     n_start = pop.size()
     log(f"[bench] warmup done, {n_start} particles")
     t1 = time.perf_counter()
-    run(steps)
+    run(steps, sort_every)
     N.check(lib.punc_gpu_synchronize(pop._h))
     t2 = time.perf_counter()
     n_end = pop.size()
+    alt = None
+    if alt_sort_every and alt_sort_every != sort_every:
+        # the same steps with another re-sort interval, from a fresh sort
+        N.check(lib.punc_gpu_sort(pop._h))
+        k_done[0] += -k_done[0] % alt_sort_every
+        na = pop.size()
+        ta = time.perf_counter()
+        run(steps, alt_sort_every)
+        N.check(lib.punc_gpu_synchronize(pop._h))
+        tb = time.perf_counter()
+        nb = pop.size()
+        alt = {"sort_every": alt_sort_every, "value": 0.5 * (na + nb) * steps / (tb - ta),
+               "ms_per_step": 1e3 * (tb - ta) / steps}
+        log(f"[bench] re-sort every {alt_sort_every}: {alt['value']:.4g} particle-steps/s, "
+            f"{alt['ms_per_step']:.3f} ms per step")
     # Kernel timing pass: one full re-sort cycle of eager steps with HIP events
     # around the fused particle kernel on the population's stream, so that the
     # average covers freshly sorted and aged particle orders alike.
@@ -160,8 +177,24 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
     for _ in range(5):
         N.check(lib.punc_gpu_get_rho(pop._h, N.dptr(rho)))
     t_out = time.perf_counter()
+    # the same from page-locked host memory (the drop-in layer's Function values)
+    pin_E, pin_rho = N.PinnedArray(E.shape), N.PinnedArray((mesh.ndof,))
+    pin_E.array[:] = E
+    N.check(lib.punc_gpu_set_field(pop._h, N.dptr(pin_E.array)))
+    t_pin = time.perf_counter()
+    for _ in range(5):
+        N.check(lib.punc_gpu_set_field(pop._h, N.dptr(pin_E.array)))
+    t_pmid = time.perf_counter()
+    for _ in range(5):
+        N.check(lib.punc_gpu_get_rho(pop._h, N.dptr(pin_rho.array)))
+    t_pout = time.perf_counter()
+    assert np.array_equal(pin_rho.array, rho)
     xfer = {"field_in_ms": 1e3 * (t_mid - t_in) / 5, "rho_out_ms": 1e3 * (t_out - t_mid) / 5,
-            "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof}
+            "field_in_ms_pinned": 1e3 * (t_pmid - t_pin) / 5, "rho_out_ms_pinned": 1e3 * (t_pout - t_pmid) / 5,
+            "field_bytes": 24 * mesh.ndof, "rho_bytes": 8 * mesh.ndof,
+            "note": "field_in includes the per-cell field copy kernel; *_pinned: from/to punc_gpu_host_alloc memory"}
+    del pin_E, pin_rho
+    phases = run_dropin_phases(pop, mesh, dv, dt, B, seed, k_done[0] + max(sort_every, 1)) if extras else None
     diag = run_diagnostics(pop, mesh, dv) if extras else None
     flux_mc = run_flux_mc(mesh) if extras else None
     field = run_field_solve(pop, mesh) if extras else None
@@ -177,7 +210,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
         "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
-        "diag": diag, "flux_mc": flux_mc, "field": field,
+        "diag": diag, "flux_mc": flux_mc, "field": field, "phases": phases, "alt": alt, "cfg": cfg,
     }
 
 
@@ -277,6 +310,44 @@ def run_field_solve(pop, mesh, reps=3):
             "step_kernel_ms_with_potential_energy": kern["pe_on"], "step_kernel_ms_without": kern["pe_off"]}
 
 
+# The reference loop's Timer tasks on the particle path (interaction.cpp:
+# 324-334), in loop order.
+PHASES = ("distributor", "counting particles", "accelerator", "move", "update", "injector")
+
+
+def run_dropin_phases(pop, mesh, dv, dt, B, seed, step0, steps=3):
+    """Wall ms per call of the drop-in calls of the reference's time loop
+    (each returns its host result, so each ends with a synchronize), keyed
+    by the reference's Timer task names; the fused, graph-replayed step that
+    `value` times does the same work in one launch sequence."""
+    from puncpp_amd import _native as N
+    lib = N.lib()
+    h = pop._h
+    rho = np.empty(mesh.ndof)
+    counts = np.zeros(2, dtype=np.int64)
+    ke = N.C.c_double()
+    ab = np.zeros((2, 64), dtype=np.int64)
+    ninj = N.C.c_int64()
+    tot = dict.fromkeys(PHASES, 0.0)
+    for k in range(steps + 1):  # the first pass is untimed
+        calls = {
+            "distributor": lambda: lib.punc_gpu_distribute(h, N.dptr(dv), N.dptr(rho)),
+            "counting particles": lambda: lib.punc_gpu_pop_counts(h, N.i64ptr(counts)),
+            "accelerator": lambda: lib.punc_gpu_boris(h, N.dptr(B), dt, N.C.byref(ke)),
+            "move": lambda: lib.punc_gpu_move(h, dt),
+            "update": lambda: lib.punc_gpu_update(h, N.i64ptr(ab)),
+            "injector": lambda: lib.punc_gpu_inject(h, dt, seed, step0 + k, N.C.byref(ninj)),
+        }
+        for name in PHASES:
+            t = time.perf_counter()
+            N.check(calls[name]())
+            if k:
+                tot[name] += 1e3 * (time.perf_counter() - t)
+    out = {name: tot[name] / steps for name in PHASES}
+    log("[bench] drop-in phases (ms per call): " + ", ".join(f"{k} {v:.3f}" for k, v in out.items()))
+    return out
+
+
 def run_diagnostics(pop, mesh, dv, reps=5):
     """Wall time per call of the particle diagnostics over the whole
     population (each call ends with a stream synchronize; results stay on the
@@ -317,6 +388,18 @@ def run_diagnostics(pop, mesh, dv, reps=5):
     return out_d
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     """Oracle (serial, 1 core) on a bounded sample: the same mesh and species,
     1M particles per species, `steps` full particle steps."""
@@ -335,16 +418,26 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     log(f"[bench] oracle set up ({time.time() - t0:.1f}s)")
     t1 = time.perf_counter()
     ps = 0
+    ph = dict.fromkeys(PHASES, 0.0)
+
+    def timed(name, f):
+        t = time.perf_counter()
+        f()
+        ph[name] += time.perf_counter() - t
+
     for k in range(steps):
         ps += o.size()
-        o.boris(E, B, dt)
-        o.move(dt)
-        o.update()
-        o.inject(dt, seed, k)
-        o.distribute(dv)
+        timed("distributor", lambda: o.distribute(dv))
+        timed("counting particles", lambda: (o.num_of_negatives(), o.num_of_positives(), o.size()))
+        timed("accelerator", lambda: o.boris(E, B, dt))
+        timed("move", lambda: o.move(dt))
+        timed("update", lambda: o.update())
+        timed("injector", lambda: o.inject(dt, seed, k))
     t2 = time.perf_counter()
     v = ps / (t2 - t1)
-    log(f"[bench] oracle {ps} particle-steps in {t2 - t1:.2f}s -> {v:.4g}/s")
+    phases_ms = {name: 1e3 * ph[name] / steps for name in PHASES}
+    log(f"[bench] oracle {ps} particle-steps in {t2 - t1:.2f}s -> {v:.4g}/s; per step (ms): " +
+        ", ".join(f"{k} {x:.1f}" for k, x in phases_ms.items()))
     # field solve: the oracle's assembly + sparse direct solves (Poisson and
     # the E projection) on a 16^3-cube sphere mesh (direct factorisations of
     # the larger meshes take minutes)
@@ -368,11 +461,13 @@ def run_cpu_baseline(res, sample_per_species=1_000_000, steps=3, seed=0):
     flux_rate = nf * 2 * 100000 / (t4 - t3)
     log(f"[bench] oracle flux Monte Carlo {flux_rate:.4g} samples/s")
     return {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "phases_ms_per_step": phases_ms,
+            "phases_particles": ps / steps,
             "flux_mc_samples_per_s": flux_rate,
             "field_solve_s_16cube_mesh": field_s, "field_solve_dofs": hmf.ndof,
             "sample": f"oracle (serial C++ restatement, g++ -O2) on the same mesh with "
                       f"{2 * sample_per_species} particles, {steps} steps of boris+move+update+inject+distribute; "
-                      f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cores visible"}
+                      f"host {cpu_model()}, {os.cpu_count()} cores visible, 1 used"}
 
 
 def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1):
@@ -495,21 +590,37 @@ def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1)
     batched_rate = 0.5 * (m0 + m1) * steps / (t6 - t5)
     N.check(lib.punc_gpu_graph_launch(bp.pop._h, steps))
     N.check(lib.punc_gpu_synchronize(bp.pop._h))
-    # identical evolution: per-instance particle counts after the same steps
-    counts_b = np.bincount(bp.pop.get_particles()["species"] // 2, minlength=n_inst)
-    counts_p = np.array([p.size() for p in plain])
-    same = bool(np.array_equal(counts_b, counts_p))
+    # identical evolution: per instance and species, the order-independent
+    # digest of the particles (bit patterns and instance-local cells) and the
+    # count, batched against the instance's own population after the same steps
+    dig_b, cnt_b = bp.pop.digest()
+    dig_p = np.concatenate([p.digest()[0] for p in plain])
+    cnt_p = np.concatenate([p.digest()[1] for p in plain])
+    same = bool(np.array_equal(cnt_b, cnt_p))
+    same_dig = bool(np.array_equal(dig_b, dig_p))
     log(f"[bench] batched {batched_rate:.4g} vs drop-in loop {dropin_rate:.4g} ({batched_rate / dropin_rate:.2f}x) "
         f"vs fused loop {fused_rate:.4g} ({batched_rate / fused_rate:.2f}x) particle-steps/s; "
-        f"per-instance counts identical: {same}")
+        f"per-instance counts identical: {same}, particle digests identical: {same_dig}")
     return {"instances": n_inst, "particles": int(0.5 * (m0 + m1)), "steps": steps,
             "value": batched_rate, "unit": "particle-steps/s",
             "plain_loop_value": dropin_rate, "speedup": batched_rate / dropin_rate,
             "plain_loop": "per instance: boris, move, update, inject, distribute (drop-in calls, host results)",
             "fused_loop_value": fused_rate, "speedup_vs_fused_loop": batched_rate / fused_rate,
-            "per_instance_counts_identical": same, "launch": tune,
+            "per_instance_counts_identical": same, "per_instance_digests_identical": same_dig,
+            "launch": tune,
             "ms_per_step_batched": 1e3 * (t6 - t5) / steps, "ms_per_step_plain": 1e3 * (t2 - t1) / steps,
             "ms_per_step_fused_loop": 1e3 * (t4 - t3) / steps}
+
+
+def roofline(res):
+    """Roofline of the fused particle kernel: algorithmic bytes per launch
+    (104 B per particle, 128 B per cell record, 32 B per dof) over its
+    average launch time from HIP events on the population's stream."""
+    return {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK,
+            "traffic": measured_traffic(res["cfg"], res["tune"]),
+            "kernel": "k_step_main<boris,move> (push, move, relocation, deposit)",
+            "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]}
 
 
 def main():
@@ -518,7 +629,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="large", choices=sorted(CONFIGS))
-    ap.add_argument("--sort-every", type=int, default=12)
+    ap.add_argument("--sort-every", type=int, default=20)
+    ap.add_argument("--alt-sort-every", type=int, default=12,
+                    help="also time the same steps with this re-sort interval (0: skip)")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the sphere-probe configuration's line (4M particles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -545,7 +660,15 @@ def main():
         sys.exit(2)
     launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     res = run_gpu(args.config, args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune,
-                  launch=launch, extras=not args.no_extras)
+                  launch=launch, extras=not args.no_extras, alt_sort_every=args.alt_sort_every)
+    probe = None
+    if not args.no_probe and args.config != "probe":
+        pr = run_gpu("probe", args.steps, args.warmup, args.sort_every, autotune=not args.no_autotune,
+                      extras=False)
+        probe = {"workload": f"probe: {pr['desc']}", "value": pr["value"], "unit": "particle-steps/s",
+                 "ms_per_step": pr["ms_per_step"], "particles": int(pr["n_avg"]), "cells": pr["cells"],
+                 "dofs": pr["ndof"], "sort_every": args.sort_every, "launch": pr["tune"],
+                 "roofline": roofline(pr)}
     batched = None if args.no_batched else run_batched(args.batched_instances, args.batched_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
     out = {
@@ -564,13 +687,12 @@ def main():
         "config": {"workload": f"{args.config}: {res['desc']}", "cells": res["cells"], "dofs": res["ndof"],
                    "particles": int(res["n_avg"]), "sort_every": args.sort_every, "launch": res["tune"],
                    "field": "frozen synthetic CG1 field on device"},
-        "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK,
-                     "traffic": measured_traffic(args.config, res["tune"]),
-                     "kernel": "k_step_main<boris,move> (push, move, relocation, deposit)",
-                     "bytes_per_launch": res["b_launch"], "avg_launch_ms": res["kern_ms"]},
+        "roofline": roofline(res),
+        "resort_interval_alt": res["alt"],
+        "probe": probe,
         "cpu_baseline": cpu,
         "host_transfer": res["xfer"],
+        "dropin_phases_ms": res["phases"],
         "diagnostics": res["diag"],
         "flux_monte_carlo": res["flux_mc"],
         "field_solve": res["field"],

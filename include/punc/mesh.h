@@ -12,7 +12,9 @@
 #include <punc_gpu.h>
 
 #include <array>
+#include <cstdlib>
 #include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -149,6 +151,39 @@ inline FunctionSpace CG1_vector_space(const Mesh& mesh) { return FunctionSpace(m
 inline FunctionSpace DG0_space(const Mesh& mesh) { return FunctionSpace(mesh, 1, true); }
 inline FunctionSpace DG0_vector_space(const Mesh& mesh) { return FunctionSpace(mesh, 3, true); }
 
+//! Allocator of page-locked host memory (punc_gpu_host_alloc), so that a
+//! Function's values go to and from the device as one DMA transfer.  Where
+//! page-locked memory cannot be had (no device), ordinary heap memory is used;
+//! a 64-byte header records which, for deallocate.
+template <class T>
+struct PinnedAllocator {
+    using value_type = T;
+    PinnedAllocator() = default;
+    template <class U>
+    PinnedAllocator(const PinnedAllocator<U>&) {}
+    T* allocate(std::size_t n) {
+        const std::size_t bytes = n * sizeof(T) + 64;
+        void* p = nullptr;
+        bool pinned = punc_gpu_host_alloc((int64_t)bytes, &p) == 0 && p;
+        if (!pinned) p = std::malloc(bytes);
+        if (!p) throw std::bad_alloc();
+        *static_cast<int*>(p) = pinned ? 1 : 0;
+        return reinterpret_cast<T*>(static_cast<char*>(p) + 64);
+    }
+    void deallocate(T* q, std::size_t) {
+        if (!q) return;
+        char* p = reinterpret_cast<char*>(q) - 64;
+        if (*reinterpret_cast<int*>(p))
+            punc_gpu_host_free(p);
+        else
+            std::free(p);
+    }
+    template <class U>
+    bool operator==(const PinnedAllocator<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+
 //! A CG1 function on a mesh: ndof values per component (the stand-in for
 //! df::Function on the particle path; E has 3 components, rho has 1).
 class Function {
@@ -161,7 +196,7 @@ public:
     explicit Function(const std::shared_ptr<const FunctionSpace>& V) : Function(*V) {}
     int value_dim;
     bool dg0;
-    std::vector<double> values;  ///< dof-major: values[dof*value_dim + component]
+    std::vector<double, PinnedAllocator<double>> values;  ///< dof-major: values[dof*value_dim + component]
 };
 
 }  // namespace punc

@@ -183,8 +183,16 @@ int punc_gpu_pop_save_file(punc_pop_t* p, const char* fname, int binary);
 int punc_gpu_pop_load_file(punc_pop_t* p, const char* fname, int binary, int64_t* n_added);
 
 /* ---- the particle operators -------------------------------------------- */
-/* Upload the CG1 electric field, ndof*3 doubles (E_x,E_y,E_z per dof). */
+/* Upload the CG1 electric field, ndof*3 doubles (E_x,E_y,E_z per dof).
+ * From memory of punc_gpu_host_alloc the copy is one DMA transfer; from
+ * pageable memory the runtime stages it. */
 int punc_gpu_set_field(punc_pop_t* p, const double* E);
+/* Page-locked host memory (hipHostMalloc) for arrays that cross the host
+ * boundary every step: the drop-in layer's Function values live in it.  No
+ * reference counterpart (the reference's field dofs never leave the host).
+ * bytes 0 allocates a minimal block; free with punc_gpu_host_free. */
+int punc_gpu_host_alloc(int64_t bytes, void** ptr);
+int punc_gpu_host_free(void* ptr);
 /* accel_cg1(pop, E, dt) (pusher.h:154-201); returns the mid-step KE. */
 int punc_gpu_accel(punc_pop_t* p, double dt, double* KE);
 /* boris_cg1(pop, E, B, dt) (pusher.h:282-372); returns 0.5*m*|v-|^2 summed. */

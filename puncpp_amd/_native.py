@@ -59,6 +59,8 @@ _SIGS = {
     "punc_gpu_pop_species": (C.c_int, [vp, i32_p, d_p, d_p]),
     "punc_gpu_pop_load_file": (C.c_int, [vp, C.c_char_p, C.c_int, i64_p]),
     "punc_gpu_set_field": (C.c_int, [vp, d_p]),
+    "punc_gpu_host_alloc": (C.c_int, [C.c_int64, C.POINTER(vp)]),
+    "punc_gpu_host_free": (C.c_int, [vp]),
     "punc_gpu_accel": (C.c_int, [vp, C.c_double, d_p]),
     "punc_gpu_boris": (C.c_int, [vp, d_p, C.c_double, d_p]),
     "punc_gpu_move": (C.c_int, [vp, C.c_double]),
@@ -176,3 +178,23 @@ def i64ptr(a):
 
 def vec3(v):
     return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+class PinnedArray:
+    """A float64 numpy array in page-locked host memory (punc_gpu_host_alloc):
+    set_field / get_rho with it copy by DMA.  Keep this object alive while
+    ``.array`` is in use; the memory is freed with it."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        self._ptr = vp()
+        check(lib().punc_gpu_host_alloc(8 * max(n, 1), C.byref(self._ptr)))
+        buf = (C.c_double * max(n, 1)).from_address(self._ptr.value)
+        self.array = np.ctypeslib.as_array(buf)[:n].reshape(shape)
+
+    def __del__(self):
+        p = getattr(self, "_ptr", None)
+        if p is not None and p.value:
+            self.array = None
+            lib().punc_gpu_host_free(p)
+            self._ptr = None

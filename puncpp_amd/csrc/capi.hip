@@ -1101,6 +1101,21 @@ static int push_common(punc_pop_s* P, int kind, double dt, const double* B, doub
     return 0;
 }
 
+// Page-locked host memory for the arrays that cross the host boundary each
+// step (field dofs in, charge dofs out): copies from and to it are DMA
+// transfers at the link's rate, with no staging through pageable memory.
+extern "C" int punc_gpu_host_alloc(int64_t bytes, void** ptr) {
+    if (!ptr || bytes < 0) return fail("host_alloc: null pointer or negative size");
+    *ptr = nullptr;
+    HIPCHK(hipHostMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault));
+    return 0;
+}
+
+extern "C" int punc_gpu_host_free(void* ptr) {
+    if (ptr) HIPCHK(hipHostFree(ptr));
+    return 0;
+}
+
 extern "C" int punc_gpu_set_field(punc_pop_t* p, const double* E) {
     if (!p || !E) return fail("set_field: null argument");
     HIPCHK(hipSetDevice(p->device));

@@ -183,3 +183,33 @@ def test_graph_dropped_when_captured_state_changes(gpu):
         N.check(L.punc_gpu_step(g2._h, 2, 0.05, N.dptr(B), 0.05, None, 1, 7, step))
     N.check(L.punc_gpu_synchronize(pop._h))
     assert_same_sets(pop.get_particles(), g2.get_particles(), rel=0.0, what="replay after re-capture")
+
+
+def test_pinned_host_transfers(gpu):
+    # The field from page-locked memory (punc_gpu_host_alloc) pushes exactly
+    # as from pageable memory, and the deposit reads back the same either way.
+    _print_case("pinned_host_transfers")
+    from helpers import loaded_pair, synthetic_field
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (2000, 2000))
+    st = o.get_particles()
+    pop2 = P.Population(mesh, [(QE, ME), (QI, MI)], capacity=20000)
+    pop2.add_located(st["x"], st["v"], st["species"], st["cell"])
+    E = synthetic_field(hm, scale=0.2)
+    pin = N.PinnedArray(E.shape)
+    pin.array[:] = E
+    L = N.lib()
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    ke = [N.C.c_double(), N.C.c_double()]
+    for k, (pp, src) in enumerate(((pop, E), (pop2, pin.array))):
+        N.check(L.punc_gpu_set_field(pp._h, N.dptr(src)))
+        N.check(L.punc_gpu_boris(pp._h, N.dptr(B), 0.05, N.C.byref(ke[k])))
+        N.check(L.punc_gpu_distribute(pp._h, N.dptr(dv), None))
+    assert ke[0].value == ke[1].value
+    assert_same_sets(pop.get_particles(), pop2.get_particles(), rel=0.0, what="pinned field")
+    rho = np.empty(hm.ndof)
+    rho_pin = N.PinnedArray((hm.ndof,))
+    N.check(L.punc_gpu_get_rho(pop2._h, N.dptr(rho)))
+    N.check(L.punc_gpu_get_rho(pop2._h, N.dptr(rho_pin.array)))
+    assert np.array_equal(rho, rho_pin.array) and np.any(rho != 0)
