@@ -103,8 +103,8 @@ struct punc_pop_s {
     DevState* d_state = nullptr;
     void* bufs[2][7] = {};
     int32_t* d_holes = nullptr;
-    int32_t* d_H = nullptr;
-    int32_t* d_S = nullptr;
+    int32_t* d_slot = nullptr;  // compaction meeting slots (capacity), -1 between calls
+    int32_t* d_S = nullptr;     // the re-sort's resulting order (capacity)
     unsigned long long* d_absorbed = nullptr;
     unsigned long long* d_cnt = nullptr;
     unsigned long long* d_errs = nullptr;  // [0] load failures, [1] injection failures
@@ -125,7 +125,7 @@ struct punc_pop_s {
     int64_t* d_soffset = nullptr;
     int64_t* d_sbsum = nullptr;
     int64_t* d_bsum = nullptr;
-    int64_t* d_total = nullptr;  // [0] scan total, [1] injection candidates, [2] kept
+    int64_t* d_total = nullptr;  // [0] scan total, [2] injected particles kept
     int64_t bsum_len = 0;
     // captured step
     hipGraph_t graph = nullptr;
@@ -145,13 +145,13 @@ struct punc_pop_s {
     double* d_num = nullptr;
     double* d_pmax = nullptr;
     double inj_dt = -1.0;
-    int64_t cand_cap = 0;
-    int64_t* d_fcount = nullptr;
-    int64_t* d_foff = nullptr;
+    int64_t cand_cap = 0;           // candidate slots: floor(num)+1 per pair
+    int64_t cand_alloc = 0;         // slots the candidate buffers hold
+    int64_t* d_soff = nullptr;      // npairs: first candidate slot of each pair
     double* d_cx = nullptr;
     int32_t* d_ccs = nullptr;
-    int64_t* d_keep = nullptr;
-    int64_t* d_pos = nullptr;
+    int32_t* d_kc = nullptr;        // kept per group of slots
+    int32_t* d_spair = nullptr;     // cand_cap: the pair of each candidate slot
     // population digest (allocated on first use): 2 * MAX_SPECIES sums and
     // counts, then MAX_SPECIES int32 cell offsets
     unsigned long long* d_digest = nullptr;
@@ -575,7 +575,7 @@ static void pop_free(punc_pop_s* P) {
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 7; ++k) hipFree(P->bufs[b][k]);
     hipFree(P->d_holes);
-    hipFree(P->d_H);
+    hipFree(P->d_slot);
     hipFree(P->d_S);
     hipFree(P->d_absorbed);
     hipFree(P->d_cnt);
@@ -607,12 +607,11 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_spi);
     hipFree(P->d_num);
     hipFree(P->d_pmax);
-    hipFree(P->d_fcount);
-    hipFree(P->d_foff);
+    hipFree(P->d_soff);
     hipFree(P->d_cx);
     hipFree(P->d_ccs);
-    hipFree(P->d_keep);
-    hipFree(P->d_pos);
+    hipFree(P->d_kc);
+    hipFree(P->d_spair);
     for (auto& e : P->ev) {
         hipEventDestroy(e.first);
         hipEventDestroy(e.second);
@@ -679,7 +678,7 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(dalloc(&P->d_state, 1));
     PCHK(dalloc(&P->d_holes, capacity));
     PCHK(dalloc(&P->d_walk, capacity));
-    PCHK(dalloc(&P->d_H, capacity));
+    PCHK(dalloc(&P->d_slot, capacity));
     PCHK(dalloc(&P->d_S, capacity));
     PCHK(dalloc(&P->d_absorbed, (size_t)MAX_SPECIES * MAX_BND));
     PCHK(dalloc(&P->d_cnt, MAX_SPECIES));
@@ -720,7 +719,8 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipMemset(P->d_absorbed, 0, (size_t)MAX_SPECIES * MAX_BND * 8));
     PCHK(hipMemset(P->d_errs, 0, 4 * 8));
     PCHK(hipMemset(P->d_E, 0, 3 * ndof * sizeof(double)));
-    PCHK(hipMemset(P->d_rho_raw, 0, ndof * 8));
+    PCHK(hipMemset(P->d_rho_raw, 0, ndof * 8));  // zero outside a deposit (the scale re-zeroes it)
+    PCHK(hipMemset(P->d_slot, 0xff, (size_t)capacity * 4));  // compaction slots: -1 between calls
     PCHK(hipMemset(P->d_rho, 0, ndof * 8));
     PCHK(hipMemset(P->d_ke, 0, 4 * 8));
     PCHK(hipMemset(P->d_total, 0, 8 * 8));
@@ -1191,7 +1191,7 @@ extern "C" int punc_gpu_update(punc_pop_t* p, int64_t* absorbed) {
     L.coef = coef;
     L.locate = true;
     HIPCHK(timed_launch(p, L));
-    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_H, p->d_S, p->grid, p->stream));
+    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_slot, p->grid, false, p->stream));
     r = read_absorbed(p, absorbed);
     if (r) return r;
     return check_errors(p, "update");
@@ -1925,6 +1925,8 @@ static int inject_prepare(punc_pop_s* P, double dt) {
     const int64_t nf = (int64_t)P->mesh->topo.ext.size();
     std::vector<int32_t> psp, pf;
     std::vector<double> num, pmax;
+    std::vector<int64_t> soff;  // first candidate slot of each pair
+    std::vector<int32_t> spair; // pair of each candidate slot
     int64_t cap = 0;
     for (int s = 0; s < P->nsp; ++s) {
         if (!P->inj_on[s]) continue;
@@ -1943,11 +1945,14 @@ static int inject_prepare(punc_pop_s* P, double dt) {
             pf.push_back((int32_t)j);
             num.push_back(v);
             pmax.push_back(P->flux_max[s * nf + j]);
-            cap += (int64_t)v + 1;
+            soff.push_back(cap);
+            const int64_t mx = (int64_t)v + 1;  // the most candidates a step can draw: floor(num) + 1
+            spair.insert(spair.end(), (size_t)mx, (int32_t)(psp.size() - 1));
+            cap += mx;
         }
     }
     const int64_t np = (int64_t)psp.size();
-    cap = std::max<int64_t>(cap, 1024);
+    cap = std::max<int64_t>(cap, 1);
     HIPCHK(hipStreamSynchronize(P->stream));
     invalidate_graph(P);  // the tables below are freed and rebuilt
     hipFree(P->d_num);
@@ -1955,17 +1960,17 @@ static int inject_prepare(punc_pop_s* P, double dt) {
     hipFree(P->d_spi);
     hipFree(P->d_pair_sp);
     hipFree(P->d_pair_f);
-    hipFree(P->d_fcount);
-    hipFree(P->d_foff);
+    hipFree(P->d_soff);
     P->d_num = P->d_pmax = nullptr;
+    P->d_soff = nullptr;
     HIPCHK(dalloc(&P->d_num, np));
     HIPCHK(dalloc(&P->d_pmax, np));
     HIPCHK(dalloc(&P->d_pair_sp, np));
     HIPCHK(dalloc(&P->d_pair_f, np));
-    HIPCHK(dalloc(&P->d_fcount, np));
-    HIPCHK(dalloc(&P->d_foff, np));
+    HIPCHK(dalloc(&P->d_soff, np));
     HIPCHK(dalloc(&P->d_spi, P->nsp));
     if (np) {
+        HIPCHK(hipMemcpy(P->d_soff, soff.data(), np * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(P->d_num, num.data(), np * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(P->d_pmax, pmax.data(), np * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(P->d_pair_sp, psp.data(), np * 4, hipMemcpyHostToDevice));
@@ -1973,23 +1978,23 @@ static int inject_prepare(punc_pop_s* P, double dt) {
     }
     HIPCHK(hipMemcpy(P->d_spi, P->spi.data(), P->nsp * sizeof(SpInject), hipMemcpyHostToDevice));
     P->npairs = np;
-    if (cap > P->cand_cap) {
+    // The slot layout is fixed by the tables: cand_cap is exactly the sum,
+    // the buffers grow when it does.
+    if (cap > P->cand_alloc) {
         hipFree(P->d_cx);
         hipFree(P->d_ccs);
-        hipFree(P->d_keep);
-        hipFree(P->d_pos);
-        P->cand_cap = cap;
+        hipFree(P->d_kc);
+        hipFree(P->d_spair);
+        P->d_cx = nullptr;
+        P->d_ccs = P->d_kc = P->d_spair = nullptr;
+        P->cand_alloc = cap;
         HIPCHK(dalloc(&P->d_cx, (size_t)cap * 6));
         HIPCHK(dalloc(&P->d_ccs, cap));
-        HIPCHK(dalloc(&P->d_keep, cap));
-        HIPCHK(dalloc(&P->d_pos, cap));
+        HIPCHK(dalloc(&P->d_kc, inject_group_count(cap)));
+        HIPCHK(dalloc(&P->d_spair, cap));
     }
-    const int64_t need = std::max<int64_t>(np / 1024, P->cand_cap / 1024) + 2;
-    if (need > P->bsum_len) {
-        hipFree(P->d_bsum);
-        P->bsum_len = need;
-        HIPCHK(dalloc(&P->d_bsum, P->bsum_len));
-    }
+    if (!spair.empty()) HIPCHK(hipMemcpy(P->d_spair, spair.data(), spair.size() * 4, hipMemcpyHostToDevice));
+    P->cand_cap = cap;
     P->inj_dt = dt;
     return 0;
 }
@@ -2017,15 +2022,12 @@ static int inject_launch_args(punc_pop_s* P, double dt, uint64_t seed, double* r
     L.dt = dt;
     L.cell_bits = P->cell_bits;
     L.cand_cap = P->cand_cap;
-    L.fcount = P->d_fcount;
-    L.foff = P->d_foff;
-    L.bsum = P->d_bsum;
-    L.ntot = P->d_total + 1;
+    L.soff = P->d_soff;
+    L.spair = P->d_spair;
     L.nkept = P->d_total + 2;
     L.cx = P->d_cx;
     L.ccs = P->d_ccs;
-    L.keep = P->d_keep;
-    L.pos = P->d_pos;
+    L.kc = P->d_kc;
     L.rho_raw = rho_raw;
     L.errs = P->d_errs + 1;
     return 0;
@@ -2119,11 +2121,8 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
         int r = inject_launch_args(P, a.dt, a.seed, P->d_rho_raw, IL);
         if (r) return r;
         HIPCHK(hipEventRecord(P->ev_fork, P->stream));
-        HIPCHK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
-        HIPCHK(launch_inject_gen(IL, P->side));
-        HIPCHK(hipEventRecord(P->ev_join, P->side));
     }
-    HIPCHK(launch_step_begin(P->d_rho_raw, ndof, P->stream));
+    // rho_raw is zero here: every deposit ends with a scale that re-zeroes it.
     ParticleLaunch L = base_launch(P);
     L.coef = coef;
     L.push = a.pusher;
@@ -2133,18 +2132,35 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
     L.locate = true;
     L.deposit = true;
     if (P->pe_on) L.pe_part = P->d_pe_part;
+    // The fused kernel is enqueued before the injection branch: enqueued
+    // first, the injection's blocks take compute units from the particle
+    // kernel and the step is slower (measured).
     HIPCHK(timed_launch(P, L, true));
-    HIPCHK(launch_step_end(P->d_state, P->stream));
-    HIPCHK(launch_sum_partials(P->d_ke_part, L.grid, P->d_ke, P->stream));
-    if (P->pe_on) HIPCHK(launch_sum_partials(P->d_pe_part, L.grid, P->d_pe, P->stream));
-    else HIPCHK(launch_fill_f64(P->d_pe, 1, NAN, P->stream));
-    HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_H, P->d_S, P->grid, P->stream));
+    // Injection generation depends only on the mesh, the injection tables and
+    // the step counter: it runs on the side stream beside the particle kernel
+    // and is joined back before its particles are appended.
+    if (inj) {
+        HIPCHK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+        HIPCHK(launch_inject_gen(IL, P->side));
+        HIPCHK(hipEventRecord(P->ev_join, P->side));
+    }
+    // Three kernels close the step: removal, append of the injected
+    // particles, and scale + bookkeeping (buffer swap, KE/PE sums, count,
+    // step counter) - no single-block kernels in between.
+    HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_slot, P->grid, true, P->stream));
     if (inj) {
         HIPCHK(hipStreamWaitEvent(P->stream, P->ev_join, 0));
-        HIPCHK(launch_inject_append(IL, P->stream));
+        HIPCHK(launch_inject_append(IL, P->stream, false));
     }
-    HIPCHK(launch_scale_rho(P->d_rho_raw, P->d_dv_inv, P->d_rho, ndof, P->stream));
-    if (advance_step) HIPCHK(launch_next_step(P->d_state, P->stream));
+    StepClose C;
+    C.ke_part = P->d_ke_part;
+    C.pe_part = P->pe_on ? P->d_pe_part : nullptr;
+    C.np = L.grid;
+    C.ke_out = P->d_ke;
+    C.pe_out = P->d_pe;
+    C.nkept = inj ? IL.nkept : nullptr;
+    C.advance = advance_step ? 1 : 0;
+    HIPCHK(launch_scale_finish(P->d_rho_raw, P->d_dv_inv, P->d_rho, ndof, P->d_state, C, P->stream));
     return 0;
 }
 

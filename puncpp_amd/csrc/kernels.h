@@ -39,16 +39,13 @@ struct InjectLaunch {
     uint64_t seed;
     double dt;
     int cell_bits;
-    int64_t cand_cap;
-    int64_t* fcount;
-    int64_t* foff;
-    int64_t* bsum;
-    int64_t* ntot;
-    int64_t* nkept;
-    double* cx;
-    int32_t* ccs;
-    int64_t* keep;
-    int64_t* pos;
+    int64_t cand_cap;     // candidate slots: floor(num)+1 per (species, facet) pair
+    const int64_t* soff;  // npairs: first slot of each pair (prefix sums of floor(num)+1)
+    const int32_t* spair; // cand_cap: the pair of each slot
+    int64_t* nkept;       // kept candidates this call (written by the append)
+    double* cx;           // 6 * cand_cap: candidate positions and velocities
+    int32_t* ccs;         // cand_cap: packed cell of a kept candidate, else -1
+    int32_t* kc;          // inject_group_count(cand_cap): kept per group of slots
     double* rho_raw;  // nullable: deposit the kept particles
     unsigned long long* errs;
 };
@@ -58,8 +55,6 @@ hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st);
 // relocation walk + deposit for the particles that left their cell).
 // Requires L.push != 0, L.move != 0, L.locate and L.deposit.
 hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st);
-// Zero rho_raw before a fused step.
-hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st);
 // Per-cell field copy: cellE[c*16 + 3k + j] = E[dof_k(c)].j.
 // E3: 3 FP64 components per dof, packed.
 hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st);
@@ -73,10 +68,31 @@ hipError_t launch_sum_partials(const double* part, int np, double* out, hipStrea
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
-hipError_t launch_scale_rho(const double* raw, const double* dv_inv, double* rho, int64_t n,
+// rho = raw * dv_inv, then raw = 0 (raw is all zero outside a deposit).
+hipError_t launch_scale_rho(double* raw, const double* dv_inv, double* rho, int64_t n,
                             hipStream_t st);
-hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* H, int32_t* Sv, int grid,
+// Removal of the particles recorded in holes (DevState::n_holes of them):
+// survivors from above the new count fill the holes below it.  slot:
+// capacity int32 entries, all -1 (left so).  after_step: the fused step's
+// arrays, count update left to launch_scale_finish; otherwise the count is
+// updated here.
+hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* slot, int grid, bool after_step,
                           hipStream_t st);
+// Closing bookkeeping of a fused step, done by block 0 of its last kernel.
+struct StepClose {
+    const double* ke_part;  // np per-block partials of the fused kernel
+    const double* pe_part;  // nullptr: no potential energy (pe_out = NaN)
+    int np;
+    double* ke_out;
+    double* pe_out;
+    const int64_t* nkept;   // injected particles kept (nullptr: no injection)
+    int advance;            // step counter += 1 (a captured step)
+};
+// Last kernel of a fused step: rho = raw * dv_inv, raw = 0, and the closing
+// bookkeeping (buffer swap, KE/PE sums, count after removal and injection,
+// step counter).
+hipError_t launch_scale_finish(double* raw, const double* dv_inv, double* rho, int64_t ndof, DevState* S,
+                               const StepClose& C, hipStream_t st);
 hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned long long* cnt,
                                 int grid, hipStream_t st);
 hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
@@ -121,10 +137,13 @@ hipError_t launch_flux_mc(const double* fnormal, const double* farea, int64_t nf
 // the step counter, so it may run beside a particle kernel) and the append of
 // the kept candidates at the end of the population.  launch_inject = both.
 hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st);
-hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st);
+// finish = false inside a fused step: the count update is left to its
+// launch_scale_finish.
+hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st, bool finish = true);
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st);
+// Entries of InjectLaunch::kc for cand_cap slots.
+int64_t inject_group_count(int64_t cand_cap);
 hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st);
-hipError_t launch_next_step(DevState* S, hipStream_t st);
 hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStream_t st);
 
 // Diagnostics.  potential energy: per-cell copy of phi into cellPhi (nc),

@@ -897,11 +897,6 @@ __global__ void k_set_perm(DevState* __restrict__ S, const uint32_t* __restrict_
     }
 }
 
-__global__ void k_step_begin(double* __restrict__ rho_raw, int64_t ndof) {
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ndof; k += (int64_t)gridDim.x * blockDim.x)
-        rho_raw[k] = 0.0;
-}
-
 // E3: the CG1 field as the caller hands it over, 3 components per dof.
 __global__ void k_cell_field(const DevMesh M, const double* __restrict__ E3, double* __restrict__ cellE) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < M.nc; c += (int64_t)gridDim.x * blockDim.x) {
@@ -954,27 +949,77 @@ k_kinetic(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bi
     if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
 }
 
-// Deterministic final sum of the per-block partials (one block of 1024;
-// eight independent loads in flight per thread).
-__global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ part, int np,
-                                                       double* __restrict__ out) {
-    __shared__ double sh[16];
+// Deterministic sum of np per-block partials by one block of 1024 threads
+// (eight independent loads in flight per thread); the total in thread 0.
+__device__ inline double sum_partials_1024(const double* __restrict__ part, int np, double* sh) {
     double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int k = threadIdx.x;
     for (; k + 7 * 1024 < np; k += 8 * 1024)
         for (int u = 0; u < 8; ++u) a[u] += part[k + u * 1024];
     for (; k < np; k += 1024) a[0] += part[k];
     const double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    const double t = block_sum(v, sh);
+    return block_sum(v, sh);
+}
+
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ part, int np,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[16];
+    const double t = sum_partials_1024(part, np, sh);
     if (threadIdx.x == 0) *out = t;
 }
 
-// rho_j = dv_inv_j * raw_j (distributor.h:205-208).
-__global__ void k_scale_rho(const double* __restrict__ raw, const double* __restrict__ dv_inv,
+// rho_j = dv_inv_j * raw_j (distributor.h:205-208); raw is left zeroed for
+// the next deposit.
+__global__ void k_scale_rho(double* __restrict__ raw, const double* __restrict__ dv_inv,
                             double* __restrict__ rho, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
+         i += (int64_t)gridDim.x * blockDim.x) {
         rho[i] = raw[i] * dv_inv[i];
+        raw[i] = 0.0;
+    }
+}
+
+// Last kernel of a fused step: every block scales its share of the deposit
+// (and zeroes raw for the next step); block 0 also does the step's
+// bookkeeping, which no other block reads: the swap to the buffers a
+// permuted step wrote, the KE and PE sums of the per-block partials (the
+// order of k_sum_partials), the count after removal and injection, and the
+// step counter.  Blocks of 1024 threads.
+__global__ void __launch_bounds__(1024)
+k_scale_finish(double* __restrict__ raw, const double* __restrict__ dv_inv, double* __restrict__ rho, int64_t ndof,
+               DevState* __restrict__ S, const StepClose C) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ndof;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        rho[i] = raw[i] * dv_inv[i];
+        raw[i] = 0.0;
+    }
+    if (blockIdx.x != 0) return;
+    __shared__ double sh[16];
+    const double ke = sum_partials_1024(C.ke_part, C.np, sh);
+    const double pe = C.pe_part ? sum_partials_1024(C.pe_part, C.np, sh) : NAN;
+    if (threadIdx.x == 0) {
+        *C.ke_out = ke;
+        *C.pe_out = pe;
+        if (S->perm_pending) {
+            const PopArrays t = S->cur;
+            S->cur = S->alt;
+            S->alt = t;
+            S->perm_pending = 0;
+        }
+        int64_t n = S->n - S->n_holes;
+        S->n_holes = 0;
+        S->nH = 0;
+        S->nS = 0;
+        if (C.nkept) {  // injection (k_inject_finish's bookkeeping)
+            int64_t kept = *C.nkept;
+            const int64_t room = S->capacity - n;
+            if (kept > room) kept = room;
+            S->inj_kept = kept;
+            n += kept;
+        }
+        S->n = n;
+        if (C.advance) S->step += 1;
+    }
 }
 
 __global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
@@ -999,27 +1044,39 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
 // surviving particle from [n', n).  Mirrors the effect of the reference's
 // swap-with-back deletion (population.h:686-692) on the particle set.
 // ---------------------------------------------------------------------------
-__global__ void k_compact_collect(DevState* __restrict__ S, const int32_t* __restrict__ holes,
-                                  int32_t* __restrict__ H, int32_t* __restrict__ Sv) {
-    const int64_t n = S->n, L = S->n_holes, nn = n - L;
-    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    for (int64_t k = t0; k < L; k += stride) {
-        const int32_t h = holes[k];
-        if (h < nn) H[atomicAdd((unsigned long long*)&S->nH, 1ull)] = h;
-    }
-    for (int64_t j = nn + t0; j < n; j += stride)
-        if (cs[j] >= 0) Sv[atomicAdd((unsigned long long*)&S->nS, 1ull)] = (int32_t)j;
-}
+// The particle arrays a step's later kernels work on: the alternate buffers
+// when the fused step read through a pending cell order (the swap itself is
+// the step's last bookkeeping, k_scale_finish).
+__device__ inline PopArrays step_arrays(const DevState* S) { return S->perm_pending ? S->alt : S->cur; }
 
-__global__ void k_compact_move(DevState* __restrict__ S, const int32_t* __restrict__ H,
-                               const int32_t* __restrict__ Sv) {
-    const GArrays A = gview(S->cur);
-    const int64_t m = S->nH;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t d = H[k], s = Sv[k];
+// One pass: holes below n' and survivors in [n', n) each take a ticket from
+// their own counter (the counts are equal); ticket k of both kinds meets in
+// slot[k] by atomic exchange, and the second to arrive finds its partner's
+// index there, moves the survivor into the hole and empties the slot (slots
+// hold -1 between calls).  No block waits for another.  step: the arrays
+// after a fused step (before its swap); the count is updated afterwards
+// (k_compact_finish, or the step's k_scale_finish).
+__global__ void k_compact(DevState* __restrict__ S, const int32_t* __restrict__ holes,
+                          int32_t* __restrict__ slot, int step) {
+    const int64_t n = S->n, L = S->n_holes, nn = n - L;
+    const GArrays A = gview(step ? step_arrays(S) : S->cur);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t w0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63);  // this wavefront's first index
+    // Tickets for the lanes that want one: one atomic per wavefront.  Called
+    // by all lanes of the wavefront (the loops below are wavefront-uniform).
+    auto meet = [&](unsigned long long* ctr, bool want, int32_t idx, bool is_hole) {
+        const unsigned long long m = __ballot(want);
+        if (m == 0ull) return;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (!want) return;
+        const int64_t k = (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+        const int32_t other = atomicExch(&slot[k], idx);
+        if (other < 0) return;  // first to arrive
+        const int64_t d = is_hole ? idx : other, s = is_hole ? other : idx;
         A.x[d] = A.x[s];
         A.y[d] = A.y[s];
         A.z[d] = A.z[s];
@@ -1027,6 +1084,16 @@ __global__ void k_compact_move(DevState* __restrict__ S, const int32_t* __restri
         A.vy[d] = A.vy[s];
         A.vz[d] = A.vz[s];
         A.cs[d] = A.cs[s];
+        slot[k] = -1;
+    };
+    for (int64_t b = w0; b < L; b += stride) {
+        const int64_t k = b + lane;
+        const int32_t h = k < L ? holes[k] : INT32_MAX;
+        meet((unsigned long long*)&S->nH, h < nn, h, true);
+    }
+    for (int64_t b = nn + w0; b < n; b += stride) {
+        const int64_t j = b + lane;
+        meet((unsigned long long*)&S->nS, j < n && A.cs[j] >= 0, (int32_t)j, false);
     }
 }
 
@@ -1304,25 +1371,6 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
 // (injector.h:168-205).  The step index comes from the device state so that a
 // captured graph advances the stream on every replay.
 //
-// Per pair: N = floor(num) + [U < frac] (injector.h:186-191).
-__global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S,
-                               const SpInject* __restrict__ spi, uint64_t seed,
-                               int64_t* __restrict__ fcount) {
-    const uint64_t step = S->step;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < I.npairs;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = I.pair_sp[p], j = I.pair_f[p];
-        const SpInject& P = spi[s];
-        const uint64_t key = P.use_seed ? P.seed : seed;
-        const double num = I.num[p];
-        int64_t N = (int64_t)num;
-        U64x4 ctr = {{step, (uint64_t)(j - P.facet_base), (uint64_t)P.rng_sp, 0ull}};
-        const U64x4 r = philox4x64_10(ctr, key, TAG_INJECT_COUNT);
-        if (u01(r.v[0]) < (num - N)) N += 1;
-        fcount[p] = N;
-    }
-}
-
 // Candidates are processed INJECT_PER_WAVE per wavefront (lanes 0.. own one
 // each).  The velocity rejection sampler (injector.cpp:51-76 against the flux
 // pdf, distributions.h:93-102) runs cooperatively: for one candidate at a
@@ -1336,46 +1384,53 @@ __global__ void k_inject_count(const DevInject I, const DevState* __restrict__ S
 // (injector.cpp:78-102), push by U*dt*v (injector.h:212-216), location by a
 // walk from the facet's cell.
 constexpr uint64_t INJECT_MAX_ATTEMPTS = 1000000ull;
-constexpr int INJECT_PER_WAVE = 8;
+constexpr int INJECT_PER_WAVE = 16;
 
 __global__ void __launch_bounds__(256)
 k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
-             const SpInject* __restrict__ spi, const int64_t* __restrict__ foff,
-             const int64_t* __restrict__ ntot, uint64_t seed, double dt, int cell_bits,
+             const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
+             const int32_t* __restrict__ spair, uint64_t seed, double dt, int cell_bits,
              int64_t cand_cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
-             int64_t* __restrict__ keep, unsigned long long* __restrict__ errs) {
+             int32_t* __restrict__ kc, unsigned long long* __restrict__ errs) {
     const uint64_t step = S->step;
-    const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t g0 = wave * INJECT_PER_WAVE; g0 < cand_cap; g0 += nwaves * INJECT_PER_WAVE) {
         const int64_t k = g0 + lane;
         const bool owner = lane < INJECT_PER_WAVE;
-        const bool mine = owner && k < total;
-        // This lane's candidate: its pair (last p with foff[p] <= k; count > 0)
+        const bool in_cap = owner && k < cand_cap;
+        // This lane's slot: its pair p (a table lookup) and its index within
+        // the pair's floor(num)+1 slots.
         int64_t p = 0, s = 0, j = 0, slot = 0;
-        if (mine) {
-            int64_t lo = 0, hi = I.npairs - 1;
-            while (lo < hi) {
-                const int64_t mid = (lo + hi + 1) >> 1;
-                if (foff[mid] <= k) lo = mid; else hi = mid - 1;
-            }
-            p = lo;
+        if (in_cap) {
+            p = spair[k];
             s = I.pair_sp[p];
             j = I.pair_f[p];
-            slot = k - foff[p];
+            slot = k - soff[p];
         }
         const SpInject P = spi[s];
         const uint64_t key = P.use_seed ? P.seed : seed;
+        // The pair's candidate count this step, N = floor(num) + [U < frac]
+        // (injector.h:186-191); the slot holds a candidate when slot < N.
+        bool mine = false;
+        if (in_cap) {
+            const double num = I.num[p];
+            int64_t N = (int64_t)num;
+            U64x4 cc = {{step, (uint64_t)(j - P.facet_base), (uint64_t)P.rng_sp, 0ull}};
+            const U64x4 rc = philox4x64_10(cc, key, TAG_INJECT_COUNT);
+            if (u01(rc.v[0]) < (num - N)) N += 1;
+            mine = slot < N;
+        }
         const uint64_t sj = ((uint64_t)P.rng_sp << 32) | (uint64_t)(j - P.facet_base);
         const double* nrm = I.fnormal + 3 * j;
         const double n0 = nrm[0], n1 = nrm[1], n2 = nrm[2];
         const double pmax = I.pmax[p];
         double v[3] = {0.0, 0.0, 0.0};
         bool ok = false;
-        const int64_t ng = total - g0 < INJECT_PER_WAVE ? total - g0 : INJECT_PER_WAVE;  // wavefront-uniform
+        const int64_t ng = cand_cap - g0 < INJECT_PER_WAVE ? cand_cap - g0 : INJECT_PER_WAVE;  // wavefront-uniform
         for (int c = 0; c < ng; ++c) {
+            if (!__shfl((int)mine, c, 64)) continue;  // wavefront-uniform: slot c holds no candidate
             // broadcast candidate c's parameters
             const uint64_t key_c = __shfl(key, c, 64), sj_c = __shfl(sj, c, 64);
             const uint64_t slot_c = (uint64_t)__shfl(slot, c, 64);
@@ -1417,83 +1472,121 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
                 }
             }
         }
-        if (!owner || k >= cand_cap) continue;
-        if (!mine) {
-            keep[k] = 0;
-            continue;
+        bool kept = false;
+        if (mine) {
+            if (!ok) atomicAdd(errs, 1ull);
+            const double* V = I.fverts + 9 * j;
+            U64x4 c0 = {{step, sj, (uint64_t)slot, 0ull}};
+            const U64x4 r0 = philox4x64_10(c0, key, TAG_INJECT);
+            double x[3];
+            double r = u01(r0.v[0]);
+            for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
+            r = sqrt(u01(r0.v[1]));
+            for (int d = 0; d < 3; ++d) x[d] = r * x[d] + (1.0 - r) * V[6 + d];
+            const double rp = u01(r0.v[2]);
+            for (int d = 0; d < 3; ++d) x[d] = x[d] + rp * dt * v[d];
+            const int32_t c = ok ? walk(M, I.fcell[j], x[0], x[1], x[2]) : -1;
+            kept = c >= 0;
+            cx[k] = x[0];
+            cx[cand_cap + k] = x[1];
+            cx[2 * cand_cap + k] = x[2];
+            cx[3 * cand_cap + k] = v[0];
+            cx[4 * cand_cap + k] = v[1];
+            cx[5 * cand_cap + k] = v[2];
+            ccs[k] = kept ? (int32_t)((s << cell_bits) | c) : -1;
+        } else if (in_cap) {
+            ccs[k] = -1;
         }
-        if (!ok) atomicAdd(errs, 1ull);
-        const double* V = I.fverts + 9 * j;
-        U64x4 c0 = {{step, sj, (uint64_t)slot, 0ull}};
-        const U64x4 r0 = philox4x64_10(c0, key, TAG_INJECT);
-        double x[3];
-        double r = u01(r0.v[0]);
-        for (int d = 0; d < 3; ++d) x[d] = (1.0 - r) * V[d] + r * V[3 + d];
-        r = sqrt(u01(r0.v[1]));
-        for (int d = 0; d < 3; ++d) x[d] = r * x[d] + (1.0 - r) * V[6 + d];
-        const double rp = u01(r0.v[2]);
-        for (int d = 0; d < 3; ++d) x[d] = x[d] + rp * dt * v[d];
-        const int32_t c = ok ? walk(M, I.fcell[j], x[0], x[1], x[2]) : -1;
-        const bool kept = c >= 0;
-        keep[k] = kept ? 1 : 0;
-        cx[k] = x[0];
-        cx[cand_cap + k] = x[1];
-        cx[2 * cand_cap + k] = x[2];
-        cx[3 * cand_cap + k] = v[0];
-        cx[4 * cand_cap + k] = v[1];
-        cx[5 * cand_cap + k] = v[2];
-        ccs[k] = kept ? (int32_t)((s << cell_bits) | c) : -1;
+        // kept candidates of this group of INJECT_PER_WAVE slots (the append's
+        // prefix sums read these instead of a scan)
+        const unsigned long long km = __ballot(kept);
+        if (lane == 0) kc[g0 / INJECT_PER_WAVE] = __popcll(km);
     }
 }
 
-// Append kept candidates at the end of the population, in candidate order,
-// and deposit their charge when rho_raw is given.
-__global__ void __launch_bounds__(256)
+// Append kept candidates at the end of the population, in candidate (slot)
+// order, and deposit their charge when rho_raw is given.  Block b (256
+// threads) takes slots [256b, 256b + 256): its output base is the sum of the
+// per-group kept counts before it (32 groups of INJECT_PER_WAVE per block),
+// each slot's rank within the block comes from wavefront ballots.  The last
+// block writes the total kept to *nkept.  Grid: ceil(cand_cap / 256).
+constexpr int APPEND_BLOCK = 256;
+static_assert(APPEND_BLOCK % INJECT_PER_WAVE == 0, "append blocks cover whole slot groups");
+
+__global__ void __launch_bounds__(APPEND_BLOCK)
 k_inject_append(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef,
-                int64_t cand_cap, const int64_t* __restrict__ ntot, const double* __restrict__ cx,
-                const int32_t* __restrict__ ccs, const int64_t* __restrict__ pos, int cell_bits,
+                int64_t cand_cap, const double* __restrict__ cx, const int32_t* __restrict__ ccs,
+                const int32_t* __restrict__ kc, int64_t* __restrict__ nkept, int cell_bits,
                 double* __restrict__ rho_raw) {
-    const GArrays A = gview(S->cur);
-    const int64_t n0 = S->n, cap = S->capacity;
-    const int64_t total = *ntot < cand_cap ? *ntot : cand_cap;
+    __shared__ int64_t red[APPEND_BLOCK / 64];
+    __shared__ int32_t wcnt[APPEND_BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // kept candidates in the slots before this block
+    const int64_t g_end = (int64_t)blockIdx.x * (APPEND_BLOCK / INJECT_PER_WAVE);
+    int64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int64_t g = threadIdx.x;
+    for (; g + 3 * APPEND_BLOCK < g_end; g += 4 * APPEND_BLOCK) {
+        a0 += kc[g];
+        a1 += kc[g + APPEND_BLOCK];
+        a2 += kc[g + 2 * APPEND_BLOCK];
+        a3 += kc[g + 3 * APPEND_BLOCK];
+    }
+    for (; g < g_end; g += APPEND_BLOCK) a0 += kc[g];
+    int64_t part = (a0 + a1) + (a2 + a3);
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+    const int64_t k = (int64_t)blockIdx.x * APPEND_BLOCK + threadIdx.x;
+    const int32_t cs = k < cand_cap ? ccs[k] : -1;
+    const bool kept = cs >= 0;
+    const unsigned long long km = __ballot(kept);
+    if (lane == 0) {
+        red[w] = part;
+        wcnt[w] = __popcll(km);
+    }
+    __syncthreads();
+    int64_t base = 0, before = 0, in_block = 0;
+    for (int v = 0; v < APPEND_BLOCK / 64; ++v) {
+        base += red[v];
+        if (v < w) before += wcnt[v];
+        in_block += wcnt[v];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && nkept) *nkept = base + in_block;
+    if (!kept) return;
+    // After a fused step the removal's count update and the buffer swap are
+    // still pending (k_scale_finish): the base is n - n_holes, the arrays
+    // those the step wrote.  Otherwise n_holes is 0 and no order is pending.
+    const GArrays A = gview(step_arrays(S));
+    const int64_t n0 = S->n - S->n_holes, cap = S->capacity;
     const int32_t cmask = (1 << cell_bits) - 1;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t cs = ccs[k];
-        if (cs < 0) continue;
-        const int64_t d = n0 + pos[k];
-        if (d >= cap) {
-            atomicAdd((unsigned long long*)&S->overflow, 1ull);
-            continue;
-        }
-        const double x = cx[k], y = cx[cand_cap + k], z = cx[2 * cand_cap + k];
-        A.x[d] = x;
-        A.y[d] = y;
-        A.z[d] = z;
-        A.vx[d] = cx[3 * cand_cap + k];
-        A.vy[d] = cx[4 * cand_cap + k];
-        A.vz[d] = cx[5 * cand_cap + k];
-        A.cs[d] = cs;
-        if (rho_raw) {
-            const CellRec& R = M.rec[cs & cmask];
-            double l[4];
-            bary4(R.bary, x, y, z, l);
-            const double q = coef[cs >> cell_bits].q;
-            unsafeAtomicAdd(&rho_raw[R.dof[0]], q * l[0]);
-            unsafeAtomicAdd(&rho_raw[R.dof[1]], q * l[1]);
-            unsafeAtomicAdd(&rho_raw[R.dof[2]], q * l[2]);
-            unsafeAtomicAdd(&rho_raw[R.dof[3]], q * l[3]);
-        }
+    const int64_t d = n0 + base + before + __popcll(km & ((1ull << lane) - 1ull));
+    if (d >= cap) {
+        atomicAdd((unsigned long long*)&S->overflow, 1ull);
+        return;
+    }
+    const double x = cx[k], y = cx[cand_cap + k], z = cx[2 * cand_cap + k];
+    A.x[d] = x;
+    A.y[d] = y;
+    A.z[d] = z;
+    A.vx[d] = cx[3 * cand_cap + k];
+    A.vy[d] = cx[4 * cand_cap + k];
+    A.vz[d] = cx[5 * cand_cap + k];
+    A.cs[d] = cs;
+    if (rho_raw) {
+        const CellRec& R = M.rec[cs & cmask];
+        double l[4];
+        bary4(R.bary, x, y, z, l);
+        const double q = coef[cs >> cell_bits].q;
+        unsafeAtomicAdd(&rho_raw[R.dof[0]], q * l[0]);
+        unsafeAtomicAdd(&rho_raw[R.dof[1]], q * l[1]);
+        unsafeAtomicAdd(&rho_raw[R.dof[2]], q * l[2]);
+        unsafeAtomicAdd(&rho_raw[R.dof[3]], q * l[3]);
     }
 }
 
-__global__ void k_inject_finish(DevState* __restrict__ S, const int64_t* __restrict__ ntot,
-                                const int64_t* __restrict__ nkept, int64_t cand_cap) {
+// Count update after an injection outside a fused step (room: particles
+// past the capacity were dropped and counted as overflow by the append).
+__global__ void k_inject_finish(DevState* __restrict__ S, const int64_t* __restrict__ nkept) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const int64_t total = *ntot;
         int64_t kept = *nkept;
-        S->inj_cand = total;
-        if (total > cand_cap) S->overflow += total - cand_cap;
         const int64_t room = S->capacity - S->n;
         if (kept > room) kept = room;
         S->inj_kept = kept;
@@ -1503,9 +1596,6 @@ __global__ void k_inject_finish(DevState* __restrict__ S, const int64_t* __restr
 
 __global__ void k_set_step(DevState* __restrict__ S, uint64_t step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) S->step = step;
-}
-__global__ void k_next_step(DevState* __restrict__ S) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) S->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1997,12 +2087,6 @@ hipError_t launch_step_passes(const ParticleLaunch& L, hipStream_t st) {
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_step_begin(double* rho_raw, int64_t ndof, hipStream_t st) {
-    hipLaunchKernelGGL(k_step_begin, dim3(fill_grid(ndof)), dim3(256), 0, st, rho_raw, ndof);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
-
 hipError_t launch_cell_field(const DevMesh& M, const double* E3, double* cellE, hipStream_t st) {
     hipLaunchKernelGGL(k_cell_field, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, E3, cellE);
     LAUNCH_CHECK();
@@ -2076,7 +2160,7 @@ hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
     return hipSuccess;
 }
 
-hipError_t launch_scale_rho(const double* raw, const double* dv_inv, double* rho, int64_t n,
+hipError_t launch_scale_rho(double* raw, const double* dv_inv, double* rho, int64_t n,
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scale_rho, dim3(fill_grid(n)), dim3(256), 0, st, raw, dv_inv, rho, n);
@@ -2084,13 +2168,22 @@ hipError_t launch_scale_rho(const double* raw, const double* dv_inv, double* rho
     return hipSuccess;
 }
 
-hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* H, int32_t* Sv, int grid,
+hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* slot, int grid, bool after_step,
                           hipStream_t st) {
-    hipLaunchKernelGGL(k_compact_collect, dim3(grid), dim3(256), 0, st, S, holes, H, Sv);
+    hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, st, S, holes, slot, after_step ? 1 : 0);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_compact_move, dim3(grid), dim3(256), 0, st, S, H, Sv);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(64), 0, st, S);
+    if (!after_step) {
+        hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(64), 0, st, S);
+        LAUNCH_CHECK();
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_scale_finish(double* raw, const double* dv_inv, double* rho, int64_t ndof, DevState* S,
+                               const StepClose& C, hipStream_t st) {
+    int64_t g = (ndof + 1023) / 1024;
+    g = g < 1 ? 1 : (g > 1024 ? 1024 : g);
+    hipLaunchKernelGGL(k_scale_finish, dim3((unsigned)g), dim3(1024), 0, st, raw, dv_inv, rho, ndof, S, C);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -2177,29 +2270,28 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
 }
 
 hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st) {
-    const int64_t np = L.inj.npairs;
-    hipLaunchKernelGGL(k_inject_count, dim3(fill_grid(np)), dim3(256), 0, st, L.inj, L.state, L.spi,
-                       L.seed, L.fcount);
-    LAUNCH_CHECK();
-    hipError_t e = launch_scan(L.fcount, L.foff, np, L.bsum, L.ntot, st);
-    if (e != hipSuccess) return e;
-    // INJECT_PER_WAVE candidates per wavefront, 4 wavefronts per block.
+    // INJECT_PER_WAVE slots per wavefront, 4 wavefronts per block.
     const int64_t gen_blocks = std::min<int64_t>((L.cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE), 16384);
     hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st, L.mesh, L.inj,
-                       L.state, L.spi, L.foff, L.ntot, L.seed, L.dt, L.cell_bits, L.cand_cap,
-                       L.cx, L.ccs, L.keep, L.errs);
-    LAUNCH_CHECK();
-    return launch_scan(L.keep, L.pos, L.cand_cap, L.bsum, L.nkept, st);
-}
-
-hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st) {
-    hipLaunchKernelGGL(k_inject_append, dim3(fill_grid(L.cand_cap)), dim3(256), 0, st, L.mesh, L.state,
-                       L.coef, L.cand_cap, L.ntot, L.cx, L.ccs, L.pos, L.cell_bits, L.rho_raw);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_inject_finish, dim3(1), dim3(64), 0, st, L.state, L.ntot, L.nkept, L.cand_cap);
+                       L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx, L.ccs, L.kc,
+                       L.errs);
     LAUNCH_CHECK();
     return hipSuccess;
 }
+
+hipError_t launch_inject_append(const InjectLaunch& L, hipStream_t st, bool finish) {
+    const int64_t blocks = std::max<int64_t>((L.cand_cap + APPEND_BLOCK - 1) / APPEND_BLOCK, 1);
+    hipLaunchKernelGGL(k_inject_append, dim3((unsigned)blocks), dim3(APPEND_BLOCK), 0, st, L.mesh, L.state,
+                       L.coef, L.cand_cap, L.cx, L.ccs, L.kc, L.nkept, L.cell_bits, L.rho_raw);
+    LAUNCH_CHECK();
+    if (finish) {
+        hipLaunchKernelGGL(k_inject_finish, dim3(1), dim3(64), 0, st, L.state, L.nkept);
+        LAUNCH_CHECK();
+    }
+    return hipSuccess;
+}
+
+int64_t inject_group_count(int64_t cand_cap) { return (cand_cap + INJECT_PER_WAVE - 1) / INJECT_PER_WAVE; }
 
 hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
     hipError_t e = launch_inject_gen(L, st);
@@ -2212,11 +2304,6 @@ hipError_t launch_inject(const InjectLaunch& L, hipStream_t st) {
 namespace punc_gpu {
 hipError_t launch_set_step(DevState* S, uint64_t step, hipStream_t st) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, S, step);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
-hipError_t launch_next_step(DevState* S, hipStream_t st) {
-    hipLaunchKernelGGL(k_next_step, dim3(1), dim3(64), 0, st, S);
     LAUNCH_CHECK();
     return hipSuccess;
 }
