@@ -1718,7 +1718,11 @@ static int density_common(punc_pop_s* P, bool dg0, const double* dv_inv, double 
         scale = P->d_dv_inv;
     }
     HIPCHK(launch_fill_f64(P->d_draw, 2 * len, 0.0, P->stream));
-    HIPCHK(launch_density(P->mesh->dm, P->d_state, coef, P->nsp, P->cell_bits, dg0, P->d_draw, diag_grid(P),
+    // CG1 runs at the fused step's launch grid (the same pipeline and deposit);
+    // DG0 at fewer, longer chunks: every block flushes its table once, and the
+    // flush's global atomics are what that pass costs.
+    HIPCHK(launch_density(P->mesh->dm, P->d_state, coef, P->nsp, P->cell_bits, dg0, P->d_draw,
+                          dg0 ? diag_grid(P) : P->grid,
                           P->stream));
     HIPCHK(launch_density_scale(dg0, P->d_draw, scale, we, wi, P->d_dens, P->d_dens + len, len, P->stream));
     P->dens_len = len;

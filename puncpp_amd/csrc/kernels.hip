@@ -1658,22 +1658,23 @@ k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __re
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Number densities (density_cg1, diagnostics.h:492-555; density_dg0,
-// diagnostics.h:432-465) summed into raw with 2 entries per dof or cell:
-// raw[2 * index + class].  CG1: class 0 for q < 0, else 1; the barycentric
-// coordinates go to the cell's four dofs.  DG0: class 1 for q > 0, else 0; one
-// count per particle to its cell.  Per wavefront, runs of consecutive lanes
-// with the same (cell, class) are summed first (cell-sorted particles give
-// long runs); the run's last lane adds the sums into the block's LDS table,
-// flushed to raw with one global atomic per occupied slot.
-template <bool DG0>
+// DG0 counts (density_dg0, diagnostics.h:432-465) into raw[2 * cell + class]
+// (class 1 for q > 0).  The pass reads 4 B per particle, so what limits it is
+// how many bytes each wavefront keeps in flight: a block iteration covers
+// DG0_GROUPS groups of blockDim particles, whose cell ids (one per lane and
+// group, coalesced) are loaded one iteration ahead.  Per group, runs of
+// consecutive lanes with the same (cell, class) are counted with a ballot
+// and the run's last lane adds the count to the block's LDS table (one LDS
+// add per run: cell-sorted particles give long runs); the counts are
+// integers, so the sums are exact in any order.
+constexpr int DG0_GROUPS = 8;
+
 __global__ void __launch_bounds__(256)
-k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
-          double* __restrict__ raw) {
-    __shared__ int32_t cls_of[MAX_SPECIES];  // class per species: DG0 1 for q > 0, CG1 0 for q < 0
-    for (int k = threadIdx.x; k < nsp; k += blockDim.x)
-        cls_of[k] = DG0 ? (coef[k].q > 0 ? 1 : 0) : (coef[k].q < 0 ? 0 : 1);
-    const GArrays A = gview(S->cur);
+k_density_dg0(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
+              double* __restrict__ raw) {
+    __shared__ int32_t cls_of[MAX_SPECIES];
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x) cls_of[k] = coef[k].q > 0 ? 1 : 0;
+    const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
     int64_t s, e;
     chunk_range(S->n, s, e);
     if (s >= e) return;  // block-uniform
@@ -1682,27 +1683,23 @@ k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ 
     lds_clear(t);
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    // the next group's cell ids are loaded one group ahead
-    int32_t cs_next = s + threadIdx.x < e ? A.cs[s + threadIdx.x] : 0;
+    const int64_t per_iter = (int64_t)blockDim.x * DG0_GROUPS;
+    int32_t nxt[DG0_GROUPS];
+    for (int g = 0; g < DG0_GROUPS; ++g) {
+        const int64_t i = s + (int64_t)g * blockDim.x + threadIdx.x;
+        nxt[g] = i < e ? cs[i] : -1;
+    }
     int iter = 0;
-    for (int64_t b = s; b < e; b += blockDim.x) {  // block-uniform trip count
-        const int64_t i = b + threadIdx.x;
-        const int32_t cs = cs_next;
-        if (i + blockDim.x < e) cs_next = A.cs[i + blockDim.x];
-        int32_t key = -1;
-        double d[4] = {0.0, 0.0, 0.0, 0.0};
-        int32_t dof[4] = {0, 0, 0, 0};
-        if (i < e) {
-            const int32_t c = cs & cmask;
-            const int cls = cls_of[cs >> cell_bits];
-            key = 2 * c + cls;
-            if (!DG0) {
-                const CellRec& R = M.rec[c];
-                bary4(R.bary, A.x[i], A.y[i], A.z[i], d);
-                for (int k = 0; k < 4; ++k) dof[k] = 2 * R.dof[k] + cls;
-            }
+    for (int64_t b = s; b < e; b += per_iter) {  // block-uniform trip count
+        int32_t cur[DG0_GROUPS];
+        for (int g = 0; g < DG0_GROUPS; ++g) cur[g] = nxt[g];
+        for (int g = 0; g < DG0_GROUPS; ++g) {
+            const int64_t i = b + per_iter + (int64_t)g * blockDim.x + threadIdx.x;
+            nxt[g] = i < e ? cs[i] : -1;
         }
-        if (DG0) {
+        for (int g = 0; g < DG0_GROUPS; ++g) {
+            const int32_t x = cur[g];
+            const int32_t key = x >= 0 ? 2 * (x & cmask) + cls_of[x >> cell_bits] : -1;
             const int32_t kprev = __shfl_up(key, 1, 64);
             const bool head = lane == 0 || kprev != key;
             const unsigned long long heads = __ballot(head);
@@ -1710,15 +1707,81 @@ k_density(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ 
             const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
             const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
             if (key >= 0 && tail) lds_add(t, raw, key, (double)(lane - hl + 1));
-        } else if (key >= 0) {
-            for (int k = 0; k < 4; ++k) lds_add(t, raw, dof[k], d[k]);
         }
-        // every 8 groups, one decision for the whole block (the count read is
-        // a heuristic; a full table region sends adds straight to HBM)
-        if ((++iter & 7) == 0 && __syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8))
+        // every 2 iterations (16 groups), one decision for the whole block
+        if ((++iter & 1) == 0 && __syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8))
             lds_flush(t, raw);
     }
     lds_flush(t, raw);
+}
+
+// CG1 number densities (density_cg1, diagnostics.h:492-555) into
+// raw[2 * dof + class] (class 0 for q < 0, else 1), with the fused step's
+// pipeline and deposit: a group's cell records are requested first, then the
+// next group's x, y, z and cell ids, and only then does the group wait for
+// its cell data; the lanes of a quad with the same (cell, class) are summed
+// with DPP before the block's LDS table takes their lambda_k (deposit mode 4,
+// q = 1: the added values are lambda_k exactly).
+__global__ void __launch_bounds__(256)
+k_density_cg1(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
+              double* __restrict__ raw) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    typedef int32_t v4i __attribute__((ext_vector_type(4)));
+    __shared__ int32_t cls_of[MAX_SPECIES];
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x) cls_of[k] = coef[k].q < 0 ? 0 : 1;
+    const GArrays A = gview(S->cur);
+    int64_t s, e;
+    chunk_range(S->n, s, e);
+    if (s >= e) return;  // block-uniform
+    const int32_t cmask = (1 << cell_bits) - 1;
+    lds_clear(lds_table());
+    __syncthreads();
+    const int64_t last = e - 1;
+    auto clampi = [&](int64_t j) { return j < e ? j : last; };
+    double nx, ny, nz;
+    int32_t ncs;
+    {
+        const int64_t j0 = clampi(s + threadIdx.x);
+        nx = A.x[j0];
+        ny = A.y[j0];
+        nz = A.z[j0];
+        ncs = A.cs[j0];
+    }
+    int grp = 0;
+    for (int64_t base = s; base < e; base += blockDim.x) {  // block-uniform trip count
+        const int64_t i = base + threadIdx.x;
+        const bool active = i < e;
+        const double x = nx, y = ny, z = nz;
+        const int32_t cs = ncs;
+        const int32_t c = cs & cmask;
+        const GLOBAL v2d* rp = (const GLOBAL v2d*)(M.rec + c);
+        double rb[12];
+        for (int m = 0; m < 6; ++m) {
+            const v2d t = rp[m];
+            rb[2 * m] = t.x;
+            rb[2 * m + 1] = t.y;
+        }
+        const v4i dv = *(const GLOBAL v4i*)(rp + 6);
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            const int64_t j = clampi(i + blockDim.x);
+            nx = A.x[j];
+            ny = A.y[j];
+            nz = A.z[j];
+            ncs = A.cs[j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int cls = cls_of[cs >> cell_bits];
+        double l[4];
+        bary4(rb, x, y, z, l);
+        const int32_t ddof[4] = {2 * dv.x + cls, 2 * dv.y + cls, 2 * dv.z + cls, 2 * dv.w + cls};
+        deposit_cg1<4>(M, raw, active ? 2 * c + cls : -1, 0.0, 0.0, 0.0, 1.0, true, l, ddof);
+        // every 8 groups, one decision for the whole block
+        if ((++grp & 7) == 0 && base + blockDim.x < e &&
+            __syncthreads_or(threadIdx.x == 0 && *lds_table().count > LDS_SLOTS * 5 / 8))
+            lds_flush(lds_table(), raw);
+    }
+    lds_flush(lds_table(), raw);
 }
 
 // CG1: n_j = raw_j * (dv_inv_j * w) (diagnostics.h:549-553).  DG0: n_k =
@@ -2339,9 +2402,9 @@ hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int
                           double* raw, int grid, hipStream_t st) {
     if (nsp < 1 || nsp > MAX_SPECIES) return hipErrorInvalidValue;
     if (dg0)
-        hipLaunchKernelGGL(k_density<true>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, nsp, cell_bits, raw);
+        hipLaunchKernelGGL(k_density_dg0, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, S, coef, nsp, cell_bits, raw);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, nsp, cell_bits, raw);
+        hipLaunchKernelGGL(k_density_cg1, dim3(grid), dim3(256), LDS_TABLE_BYTES, st, M, S, coef, nsp, cell_bits, raw);
     LAUNCH_CHECK();
     return hipSuccess;
 }
