@@ -1695,10 +1695,10 @@ extern "C" int punc_gpu_potential_energy(punc_pop_t* p, const double* phi, doubl
     const SpCoef* coef;
     int r = diag_begin(p, &coef);
     if (r) return r;
-    const int g = diag_grid(p);
+    const int g = p->grid;  // the fused step's launch grid: the same pipeline (at most ke_np partials)
     HIPCHK(hipMemcpyAsync(p->d_phi, phi, p->mesh->topo.ndof * 8, hipMemcpyHostToDevice, p->stream));
-    HIPCHK(launch_potential_energy(p->mesh->dm, p->d_state, coef, p->cell_bits, p->d_phi, p->d_cellphi, p->d_diag, g,
-                                   p->stream));
+    HIPCHK(launch_potential_energy(p->mesh->dm, p->d_state, coef, p->nsp, p->cell_bits, p->d_phi, p->d_cellphi,
+                                   p->d_diag, g, p->stream));
     HIPCHK(launch_sum_partials(p->d_diag, g, p->d_diag + 6 * (size_t)p->ke_np, p->stream));
     HIPCHK(hipMemcpyAsync(PE, p->d_diag + 6 * (size_t)p->ke_np, 8, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));

@@ -151,7 +151,7 @@ hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStrea
 // raw, which must be zeroed first; density_scale: CG1 n = raw * (dv_inv * w),
 // DG0 n = (raw * w) / volume; statistics: per-block (n, mean, M2) per class
 // into part (6 * grid), merged into out (6).
-hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
+hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int nsp, int cell_bits,
                                    const double* phi, double4* cellPhi, double* part, int grid, hipStream_t st);
 hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int nsp, int cell_bits, bool dg0,
                           double* raw, int grid, hipStream_t st);

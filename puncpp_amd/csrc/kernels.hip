@@ -1613,45 +1613,68 @@ __global__ void k_cell_scalar(const DevMesh M, const double* __restrict__ phi, d
 
 // particle_potential_energy_cg1 (diagnostics.h:379-412): per block the sum of
 // 0.5 q phi(x) over its chunk, phi_x = sum_k lambda_k phi_k in the
-// reference's order.  Each thread takes DIAG_UNROLL particles per iteration
-// (independent loads in flight; the cell gathers are the latency).
-constexpr int DIAG_UNROLL = 4;
+// reference's order.
 
 __global__ void __launch_bounds__(256)
-k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits,
-                   const double4* __restrict__ cellPhi, double* __restrict__ part) {
+k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp,
+                   int cell_bits, const double4* __restrict__ cellPhi, double* __restrict__ part) {
+    // One particle per lane and group, pipelined as the fused step: the
+    // group's cell rows (barycentric rows, phi at the four dofs) are
+    // requested first, then the next group's x, y, z and cell ids, and only
+    // then does the group wait for its cell data.  q per species from LDS.
+    typedef double v2d __attribute__((ext_vector_type(2)));
     __shared__ double sh[8];
+    __shared__ double q_of[MAX_SPECIES];
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x) q_of[k] = coef[k].q;
+    __syncthreads();
     const GArrays A = gview(S->cur);
     int64_t s, e;
     chunk_range(S->n, s, e);
     const int32_t cmask = (1 << cell_bits) - 1;
-    const int64_t bd = blockDim.x;
     double pe = 0.0;
-    for (int64_t b = s + threadIdx.x; b < e; b += DIAG_UNROLL * bd) {
-        int32_t cs[DIAG_UNROLL];
-        double x[DIAG_UNROLL], y[DIAG_UNROLL], z[DIAG_UNROLL];
-#pragma unroll
-        for (int u = 0; u < DIAG_UNROLL; ++u) {
-            const int64_t i = b + u * bd;
-            const bool ok = i < e;
-            cs[u] = ok ? A.cs[i] : -1;
-            x[u] = ok ? A.x[i] : 0.0;
-            y[u] = ok ? A.y[i] : 0.0;
-            z[u] = ok ? A.z[i] : 0.0;
+    if (s < e) {  // block-uniform
+        const int64_t last = e - 1;
+        auto clampi = [&](int64_t j) { return j < e ? j : last; };
+        double nx, ny, nz;
+        int32_t ncs;
+        {
+            const int64_t j0 = clampi(s + threadIdx.x);
+            nx = A.x[j0];
+            ny = A.y[j0];
+            nz = A.z[j0];
+            ncs = A.cs[j0];
         }
-#pragma unroll
-        for (int u = 0; u < DIAG_UNROLL; ++u) {
-            if (cs[u] < 0) continue;
-            const int32_t c = cs[u] & cmask;
+        for (int64_t base = s; base < e; base += blockDim.x) {  // block-uniform trip count
+            const bool active = base + threadIdx.x < e;
+            const double x = nx, y = ny, z = nz;
+            const int32_t cs = ncs;
+            const int32_t c = cs & cmask;
+            const GLOBAL v2d* rp = (const GLOBAL v2d*)(M.rec + c);
+            const GLOBAL v2d* fp = (const GLOBAL v2d*)(cellPhi + c);
+            double rb[12];
+            for (int m = 0; m < 6; ++m) {
+                const v2d t = rp[m];
+                rb[2 * m] = t.x;
+                rb[2 * m + 1] = t.y;
+            }
+            const v2d f01 = fp[0], f23 = fp[1];
+            __builtin_amdgcn_sched_barrier(0);
+            {
+                const int64_t j = clampi(base + threadIdx.x + blockDim.x);
+                nx = A.x[j];
+                ny = A.y[j];
+                nz = A.z[j];
+                ncs = A.cs[j];
+            }
+            __builtin_amdgcn_sched_barrier(0);
             double l[4];
-            bary4(M.rec[c].bary, x[u], y[u], z[u], l);
-            const double4 f = cellPhi[c];
+            bary4(rb, x, y, z, l);
             double phi_x = 0.0;
-            phi_x += l[0] * f.x;
-            phi_x += l[1] * f.y;
-            phi_x += l[2] * f.z;
-            phi_x += l[3] * f.w;
-            pe += 0.5 * coef[cs[u] >> cell_bits].q * phi_x;
+            phi_x += l[0] * f01.x;
+            phi_x += l[1] * f01.y;
+            phi_x += l[2] * f23.x;
+            phi_x += l[3] * f23.y;
+            if (active) pe += 0.5 * q_of[cs >> cell_bits] * phi_x;
         }
     }
     const double t = block_sum(pe, sh);
@@ -2389,11 +2412,11 @@ hipError_t launch_copy16(const void* a, void* b, int64_t n16, int grid, hipStrea
 }  // namespace punc_gpu
 
 namespace punc_gpu {
-hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int cell_bits,
+hipError_t launch_potential_energy(const DevMesh& M, DevState* S, const SpCoef* coef, int nsp, int cell_bits,
                                    const double* phi, double4* cellPhi, double* part, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_cell_scalar, dim3(fill_grid(M.nc)), dim3(256), 0, st, M, phi, cellPhi);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_potential_energy, dim3(grid), dim3(256), 0, st, M, S, coef, cell_bits, cellPhi, part);
+    hipLaunchKernelGGL(k_potential_energy, dim3(grid), dim3(256), 0, st, M, S, coef, nsp, cell_bits, cellPhi, part);
     LAUNCH_CHECK();
     return hipSuccess;
 }
