@@ -42,19 +42,19 @@ CONFIGS = {
 }
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_main_large_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_main_traffic.json")
 
 
 def measured_traffic(cfg, tune):
     """HBM bytes per launch of the fused kernel from the committed counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel factor), used
     only when the run has the workload and launch shape those runs measured."""
-    if cfg != "large" or not tune or not os.path.exists(TRAFFIC_FILE):
+    if not tune or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     shape = f"{tune['grid']},{tune['block']},{tune['deposit_mode']}"
-    rec = t.get("launches", {}).get(shape)
+    rec = t.get("runs", {}).get(f"{cfg}:{shape}")
     return rec["traffic_bytes"] if rec else None
 
 
