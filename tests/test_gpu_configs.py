@@ -8,8 +8,9 @@
 * Sphere probe (32^3 cubes x 6 tets minus the r = 1 sphere, Boris with
   B = (0, 0, 1), drifting Maxwellian flux injection on both species): the
   same comparison with absorption and injection, at 300k particles.
-* Large box (64^3 cubes x 6 tets minus the r = 2 sphere, 64M particles): no
-  oracle at this size; the properties that hold whatever the size - the
+* Large box (64^3 cubes x 6 tets minus the r = 2 sphere, 64M particles, 40
+  graph-replayed steps with a re-sort every 20): no oracle at this size; the
+  properties that hold whatever the size - the
   count balance n' = n - absorbed + injected every step, charge conservation
   of the deposit, every particle inside its cell (checked with the oracle's
   point location on a sample), non-decreasing cells right after a re-sort.
@@ -155,8 +156,8 @@ def test_large_config_properties(gpu):
     B = np.array([0.0, 0.0, 1.0])
     dt = 0.02
     N.check(L.punc_gpu_step_graph(pop._h, 2, dt, N.dptr(B), dt, None, 1, 0, 0))
-    for n in range(6):
-        if n == 3:
+    for n in range(40):  # the configuration's re-sort interval: every 20 steps
+        if n % 20 == 19:
             pop.sort()
         n_before = pop.size()
         N.check(L.punc_gpu_graph_launch(pop._h, 1))
