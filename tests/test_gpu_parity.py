@@ -388,3 +388,60 @@ def test_fused_step_after_sort(gpu, mode, graph):
     got = pop.get_particles()
     assert np.all(np.diff(got["cell"]) >= 0)
     assert_same_sets(got, o.get_particles(), rel=0.0, what="sorted read")
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_steps_with_injection_and_resort_match_oracle(gpu, graph):
+    """Eight fused steps back to back with no download in between (eager, or
+    replays of the captured step), each removing the particles that leave and
+    appending the injected ones, with a re-sort after the fifth that the next
+    step reads through.  Per step absorbed and injected counts exact and the
+    count balanced; at the end the particle set and rho equal the oracle's."""
+    _print_case("steps_inject_resort", graph=graph)
+    hm = sphere_mesh(n=6, L=3.0, r=1.0)
+    o, mesh, pop = loaded_pair(hm, (4000, 4000))
+    E = synthetic_field(hm, scale=0.2)
+    B = np.array([0.0, 0.0, 1.0])
+    dv = P.element_volume(mesh)
+    vth = [1.0, 1.0 / np.sqrt(1836.0)]
+    vd = [0.0, 0.0, 0.5]
+    species = [P.Species(q, m, 30.0, 1.0, P.ParticleAmountType.phys_per_sim, mesh,
+                         P.UniformPosition(mesh), P.Maxwellian(vth[s], vd))
+               for s, (q, m) in enumerate([(QE, ME), (QI, MI)])]
+    P.punc._register_injectors(pop, species)
+    for s in range(2):
+        o.set_injector(s, vth[s], vd, species[s].n, 5.0)
+    L = N
+    L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
+    L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
+    dt = 0.05
+    if graph:
+        L.check(L.lib().punc_gpu_step_graph(pop._h, 2, dt, L.dptr(B), dt, None, 1, 5, 0))
+    total_abs = 0
+    for n in range(8):
+        if n == 5:
+            pop.sort()  # the next step reads the new order
+            if graph:
+                L.check(L.lib().punc_gpu_step_graph(pop._h, 2, dt, L.dptr(B), dt, None, 1, 5, n))
+        n_before = pop.size()
+        if graph:
+            L.check(L.lib().punc_gpu_graph_launch(pop._h, 1))
+        else:
+            L.check(L.lib().punc_gpu_step(pop._h, 2, dt, L.dptr(B), dt, None, 1, 5, n))
+        ab = np.zeros((2, 64), dtype=np.int64)
+        kin = L.C.c_int64()
+        L.check(L.lib().punc_gpu_step_results(pop._h, None, L.i64ptr(ab), L.C.byref(kin)))
+        o.boris(E, B, dt)
+        o.move(dt)
+        ab_o, _, lost = o.update()
+        k_o = o.inject(dt, 5, n)
+        assert lost == 0
+        np.testing.assert_array_equal(ab, ab_o)
+        assert kin.value == k_o
+        assert pop.size() == n_before - ab.sum() + kin.value == o.size()
+        total_abs += ab.sum()
+    assert total_abs > 0
+    assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what="after 8 steps with a re-sort")
+    rho_g = np.empty(hm.ndof)
+    L.check(L.lib().punc_gpu_get_rho(pop._h, L.dptr(rho_g)))
+    assert np.all(np.abs(rho_g - o.distribute(dv)) <= 1e-11 * rho_scale(o, hm, dv))
