@@ -1394,8 +1394,11 @@ static int field_solve_pop_common(punc_field_t* f, punc_pop_t* p, const double* 
                             : field_poisson(f->F, p->d_rho, exterior_values, object_potentials, &it_phi, err);
     if (!ok) return fail("field_solve_pop: " + err);
     if (!field_efield(f->F, field_phi(f->F), &it_e, err)) return fail("field_solve_pop: " + err);
-    // the field the next push reads: packed dofs and the per-cell copy
-    HIPCHK(hipMemcpy(p->d_E, field_E(f->F), n * 24, hipMemcpyDeviceToDevice));
+    // the field the next push reads: packed dofs and the per-cell copy, on
+    // the population's stream (a device-to-device hipMemcpy may return before
+    // the copy is done, and the null stream does not order the population's
+    // non-blocking stream behind it)
+    HIPCHK(hipMemcpyAsync(p->d_E, field_E(f->F), n * 24, hipMemcpyDeviceToDevice, p->stream));
     HIPCHK(launch_cell_field(p->mesh->dm, p->d_E, p->d_cellE, p->stream));
     // the potential too: the next fused step reports the potential energy
     HIPCHK(launch_cell_potential(p->mesh->dm, field_phi(f->F), p->d_cellE, p->stream));
