@@ -724,6 +724,9 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     PCHK(hipMemset(P->d_rho, 0, ndof * 8));
     PCHK(hipMemset(P->d_ke, 0, 4 * 8));
     PCHK(hipMemset(P->d_total, 0, 8 * 8));
+    // the fills above run on the null stream, which does not order the
+    // population's non-blocking stream behind them: finish them here
+    PCHK(hipDeviceSynchronize());
     P->inj_on.assign(nspecies, 0);
     P->inst.assign(nspecies, SpInstance{});
     P->spi.assign(nspecies, SpInject{});

@@ -49,10 +49,15 @@ __device__ inline GArrays gview(const PopArrays& a) {
     return g;
 }
 
+// Chunk of the population for this block: XCD x = b % 8 (blocks are
+// dispatched round-robin over the 8 XCDs) owns the contiguous range
+// [x G/8, (x+1) G/8) and takes it from the top down, so the population's
+// tail - a step's freshly injected particles, one boundary cell per lane and
+// the slowest chunks of the pass - is dispatched first instead of last.
 __device__ inline int64_t logical_block() {
     const int64_t G = gridDim.x, b = blockIdx.x;
-    if (G % 8 == 0) return (b % 8) * (G / 8) + (b / 8);
-    return b;
+    if (G % 8 == 0) return (b % 8) * (G / 8) + (G / 8 - 1 - b / 8);
+    return G - 1 - b;
 }
 
 __device__ inline void chunk_range(int64_t n, int64_t& s, int64_t& e) {
