@@ -1219,6 +1219,11 @@ __device__ inline uint32_t radix_key(const GLOBAL int32_t* cs, const uint32_t* k
     return cs ? (uint32_t)(cs[i] & cmask) : keys[i];
 }
 
+// Keys (and values) are loaded RADIX_UNROLL wavefront steps at a time, all
+// loads in flight together, then ranked step by step in order.
+constexpr int RADIX_UNROLL = 8;
+static_assert(RADIX_TILE % (64 * RADIX_UNROLL) == 0, "tiles hold whole unrolled groups");
+
 // Digit counts of each tile: counts[digit * ntiles + tile].
 __global__ void __launch_bounds__(256)
 k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __restrict__ keys, int shift,
@@ -1233,12 +1238,17 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __r
     if (tile < ntiles) {
         const int64_t t0 = tile * RADIX_TILE;
         const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
-        for (int64_t base = t0; base < t1; base += 64) {  // wavefront-uniform
-            const int64_t i = base + lane;
-            const bool act = i < t1;
-            const uint32_t dg = act ? (radix_key(cs, keys, cmask, i) >> shift) & (RADIX - 1) : RADIX;
-            const unsigned long long m = match_digit(dg) & __ballot(act);
-            if (act && lane == __ffsll((long long)m) - 1) cnt[w][dg] += (uint32_t)__popcll(m);
+        for (int64_t base = t0; base < t1; base += 64 * RADIX_UNROLL) {  // wavefront-uniform
+            uint32_t dg[RADIX_UNROLL];
+            for (int u = 0; u < RADIX_UNROLL; ++u) {
+                const int64_t i = base + 64 * u + lane;
+                dg[u] = i < t1 ? (radix_key(cs, keys, cmask, i) >> shift) & (RADIX - 1) : RADIX;
+            }
+            for (int u = 0; u < RADIX_UNROLL; ++u) {
+                const bool act = dg[u] < RADIX;
+                const unsigned long long m = match_digit(dg[u]) & __ballot(act);
+                if (act && lane == __ffsll((long long)m) - 1) cnt[w][dg[u]] += (uint32_t)__popcll(m);
+            }
         }
         for (int d = lane; d < RADIX; d += 64) counts[(int64_t)d * ntiles + tile] = cnt[w][d];
     }
@@ -1263,21 +1273,27 @@ k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint32_t* _
     const int64_t t0 = tile * RADIX_TILE;
     const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int64_t base = t0; base < t1; base += 64) {  // wavefront-uniform
-        const int64_t i = base + lane;
-        const bool act = i < t1;
-        const uint32_t key = act ? radix_key(cs, keys, cmask, i) : 0u;
-        const uint32_t val = act ? (vals ? vals[i] : (uint32_t)i) : 0u;
-        const uint32_t dg = act ? (key >> shift) & (RADIX - 1) : RADIX;
-        const unsigned long long m = match_digit(dg) & __ballot(act);
-        uint32_t dst = 0;
-        if (act) dst = run[w][dg] + (uint32_t)__popcll(m & below);
-        // Every lane has read its group's counter before the group's lowest
-        // lane advances it (one wavefront: LDS operations complete in order).
-        if (act && lane == __ffsll((long long)m) - 1) run[w][dg] += (uint32_t)__popcll(m);
-        if (act) {
-            if (keys_out) keys_out[dst] = key;
-            vals_out[dst] = val;
+    for (int64_t base = t0; base < t1; base += 64 * RADIX_UNROLL) {  // wavefront-uniform
+        uint32_t kk[RADIX_UNROLL], vv[RADIX_UNROLL];
+        for (int u = 0; u < RADIX_UNROLL; ++u) {
+            const int64_t i = base + 64 * u + lane;
+            const bool act = i < t1;
+            kk[u] = act ? radix_key(cs, keys, cmask, i) : 0u;
+            vv[u] = act ? (vals ? vals[i] : (uint32_t)i) : 0u;
+        }
+        for (int u = 0; u < RADIX_UNROLL; ++u) {
+            const bool act = base + 64 * u + lane < t1;
+            const uint32_t dg = act ? (kk[u] >> shift) & (RADIX - 1) : RADIX;
+            const unsigned long long m = match_digit(dg) & __ballot(act);
+            uint32_t dst = 0;
+            if (act) dst = run[w][dg] + (uint32_t)__popcll(m & below);
+            // Every lane has read its group's counter before the group's lowest
+            // lane advances it (one wavefront: LDS operations complete in order).
+            if (act && lane == __ffsll((long long)m) - 1) run[w][dg] += (uint32_t)__popcll(m);
+            if (act) {
+                if (keys_out) keys_out[dst] = kk[u];
+                vals_out[dst] = vv[u];
+            }
         }
     }
 }
@@ -1687,26 +1703,34 @@ k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __re
 }
 
 // DG0 counts (density_dg0, diagnostics.h:432-465) into raw[2 * cell + class]
-// (class 1 for q > 0).  The pass reads 4 B per particle, so what limits it is
-// how many bytes each wavefront keeps in flight: a block iteration covers
-// DG0_GROUPS groups of blockDim particles, whose cell ids (one per lane and
-// group, coalesced) are loaded one iteration ahead.  Per group, runs of
-// consecutive lanes with the same (cell, class) are counted with a ballot
-// and the run's last lane adds the count to the block's LDS table (one LDS
-// add per run: cell-sorted particles give long runs); the counts are
-// integers, so the sums are exact in any order.
+// (class 1 for q > 0).  The pass reads 4 B per particle; a block iteration
+// covers DG0_GROUPS groups of blockDim particles, whose cell ids (one per
+// lane and group, coalesced) are loaded one iteration ahead.  Per group,
+// runs of consecutive lanes with the same (cell, class) are counted with a
+// ballot and the run's last lane adds the count.  A block's chunk of the
+// cell-sorted population covers a compact range of cells, so the counts go
+// to a direct-mapped window of DG0_WINDOW cells in LDS (an integer LDS add,
+// no hashing) that starts a little below the chunk's first cell; only
+// particles outside it (moved far, or placed by compaction) use the block's
+// hash table.  The counts are integers, so the sums are exact in any order.
 constexpr int DG0_GROUPS = 8;
+constexpr int DG0_WINDOW = 1024;  // cells; 2 int32 counters each
+constexpr int DG0_SLACK = 128;    // window cells below the chunk's first cell
 
 __global__ void __launch_bounds__(256)
 k_density_dg0(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
               double* __restrict__ raw) {
     __shared__ int32_t cls_of[MAX_SPECIES];
+    __shared__ uint32_t win[2 * DG0_WINDOW];
     for (int k = threadIdx.x; k < nsp; k += blockDim.x) cls_of[k] = coef[k].q > 0 ? 1 : 0;
+    for (int k = threadIdx.x; k < 2 * DG0_WINDOW; k += blockDim.x) win[k] = 0u;
     const GLOBAL int32_t* cs = (const GLOBAL int32_t*)S->cur.cs;
     int64_t s, e;
     chunk_range(S->n, s, e);
     if (s >= e) return;  // block-uniform
     const int32_t cmask = (1 << cell_bits) - 1;
+    const int32_t c0 = cs[s] & cmask;  // the same address for every lane: one broadcast load
+    const int32_t wbase = c0 > DG0_SLACK ? c0 - DG0_SLACK : 0;
     const LdsTable t = lds_table();
     lds_clear(t);
     __syncthreads();
@@ -1734,13 +1758,24 @@ k_density_dg0(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp
             const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
             const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
             const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-            if (key >= 0 && tail) lds_add(t, raw, key, (double)(lane - hl + 1));
+            if (key >= 0 && tail) {
+                const uint32_t w = (uint32_t)(key - 2 * wbase);
+                if (w < 2u * DG0_WINDOW)
+                    atomicAdd(&win[w], (uint32_t)(lane - hl + 1));
+                else
+                    lds_add(t, raw, key, (double)(lane - hl + 1));
+            }
         }
         // every 2 iterations (16 groups), one decision for the whole block
         if ((++iter & 1) == 0 && __syncthreads_or(threadIdx.x == 0 && *t.count > LDS_SLOTS * 5 / 8))
             lds_flush(t, raw);
     }
-    lds_flush(t, raw);
+    lds_flush(t, raw);  // synchronizes: the window is complete
+    const int64_t wend = (int64_t)2 * (cmask + 1);
+    for (int k = threadIdx.x; k < 2 * DG0_WINDOW; k += blockDim.x) {
+        const uint32_t c = win[k];
+        if (c != 0u && 2 * (int64_t)wbase + k < wend) unsafeAtomicAdd(&raw[2 * (int64_t)wbase + k], (double)c);
+    }
 }
 
 // CG1 number densities (density_cg1, diagnostics.h:492-555) into
