@@ -634,6 +634,8 @@ struct WalkRec {
 };
 static_assert(sizeof(WalkRec) == 32, "WalkRec layout");
 
+constexpr int STEP_WALK_LDS = 512;  // most walk records a block keeps in LDS
+
 struct PartRegs {
     double x, y, z, vx, vy, vz;
     int32_t cs;
@@ -675,7 +677,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
-            WalkRec* __restrict__ wrec, double* __restrict__ pe_part) {
+            WalkRec* __restrict__ wrec, double* __restrict__ pe_part, const int wl_cap) {
     typedef double v2d __attribute__((ext_vector_type(2)));
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     __shared__ double sh[8];
@@ -686,6 +688,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     SpCoef* coef = step_coef_lds<DMODE>();
     for (int k = threadIdx.x; k < nsp * (int)(sizeof(SpCoef) / 8); k += blockDim.x)
         ((double*)coef)[k] = ((const double*)gcoef)[k];
+    // The block's walk list: its first wl_cap records in LDS (after the
+    // coefficients), the rest in the block's segment of wrec.  Phase 1 writes
+    // it and phase 2 of the same block reads it, so LDS spares the list two
+    // vector-memory stores per wavefront iteration and their reads.
+    WalkRec* wl = (WalkRec*)(coef + nsp);
     const GArrays A = gview(S->cur);
     const int64_t n = S->n;
     int64_t s, e;
@@ -830,7 +837,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 wr.z = z;
                 wr.i = (int32_t)i;
                 wr.cs = p.cs;
-                wrec[s + wb + __popcll(wm & ((1ull << lane) - 1ull))] = wr;
+                const int wk = wb + __popcll(wm & ((1ull << lane) - 1ull));
+                if (wk < wl_cap)
+                    wl[wk] = wr;
+                else
+                    wrec[s + wk] = wr;
             }
         }
         deposit_cg1<DMODE>(M, rho_raw, dkey, 0.0, 0.0, 0.0, K.q, true, dl, ddof);
@@ -861,7 +872,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         int64_t i = 0;
         bool near = false;
         if (j < we) {
-            const WalkRec r = wrec[j];
+            const WalkRec r = j - s < wl_cap ? wl[j - s] : wrec[j];
             i = r.i;
             const int sp = r.cs >> cell_bits;
             const int32_t c = r.cs & cmask;
@@ -2177,15 +2188,21 @@ static int fill_grid(int64_t n);
 template <int PUSH, int MOVE, int DMODE>
 static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
     if (L.nsp < 1 || L.nsp > MAX_SPECIES) return hipErrorInvalidValue;
-    const size_t shmem = (DMODE >= 2 ? LDS_TABLE_BYTES : 0) + (size_t)L.nsp * sizeof(SpCoef);
+    size_t shmem = (DMODE >= 2 ? LDS_TABLE_BYTES : 0) + (size_t)L.nsp * sizeof(SpCoef);
+    // LDS walk records: what fits beside the table and the coefficients in a
+    // quarter of the CU's 160 KB (4 blocks of 256 per CU, the occupancy the
+    // kernel's registers allow), at most STEP_WALK_LDS.
+    const size_t budget = 40 * 1024 - 1024;  // less the static LDS and allocation rounding
+    const int wl_cap = shmem < budget ? (int)std::min<size_t>(STEP_WALK_LDS, (budget - shmem) / sizeof(WalkRec)) : 0;
+    shmem += (size_t)wl_cap * sizeof(WalkRec);
     if (L.pe_part)
         hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, true>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
                            L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
-                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, L.pe_part);
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, L.pe_part, wl_cap);
     else
         hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, false>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
                            L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
-                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, nullptr);
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, nullptr, wl_cap);
     LAUNCH_CHECK();
     return hipSuccess;
 }
