@@ -101,7 +101,7 @@ struct punc_pop_s {
     int dep_mode = 3;
 
     DevState* d_state = nullptr;
-    void* bufs[2][7] = {};
+    void* bufs[2][4] = {};  // per buffer set: the (x, y), (z, vx), (vy, vz) pair arrays and cs
     int32_t* d_holes = nullptr;
     int32_t* d_slot = nullptr;  // compaction meeting slots (capacity), -1 between calls
     int32_t* d_S = nullptr;     // the re-sort's resulting order (capacity)
@@ -573,7 +573,7 @@ static void pop_free(punc_pop_s* P) {
     hipFree(P->d_pair_f);
     if (P->d_state) hipFree(P->d_state);
     for (int b = 0; b < 2; ++b)
-        for (int k = 0; k < 7; ++k) hipFree(P->bufs[b][k]);
+        for (int k = 0; k < 4; ++k) hipFree(P->bufs[b][k]);
     hipFree(P->d_holes);
     hipFree(P->d_slot);
     hipFree(P->d_S);
@@ -672,8 +672,8 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     P->grid = std::min(prop.multiProcessorCount * 128, P->ke_np);
     P->num_cu = prop.multiProcessorCount;
     for (int b = 0; b < 2; ++b) {
-        for (int k = 0; k < 6; ++k) PCHK(dalloc((double**)&P->bufs[b][k], capacity));
-        PCHK(dalloc((int32_t**)&P->bufs[b][6], capacity));
+        for (int k = 0; k < 3; ++k) PCHK(dalloc((double**)&P->bufs[b][k], 2 * capacity));
+        PCHK(dalloc((int32_t**)&P->bufs[b][3], capacity));
     }
     PCHK(dalloc(&P->d_state, 1));
     PCHK(dalloc(&P->d_holes, capacity));
@@ -706,13 +706,10 @@ extern "C" int punc_gpu_pop_create(punc_mesh_t* m, int nspecies, const double* q
     memset(&s, 0, sizeof(s));
     PopArrays* sets[2] = {&s.cur, &s.alt};
     for (int b = 0; b < 2; ++b) {
-        sets[b]->x = (double*)P->bufs[b][0];
-        sets[b]->y = (double*)P->bufs[b][1];
-        sets[b]->z = (double*)P->bufs[b][2];
-        sets[b]->vx = (double*)P->bufs[b][3];
-        sets[b]->vy = (double*)P->bufs[b][4];
-        sets[b]->vz = (double*)P->bufs[b][5];
-        sets[b]->cs = (int32_t*)P->bufs[b][6];
+        sets[b]->xy = (double*)P->bufs[b][0];
+        sets[b]->zvx = (double*)P->bufs[b][1];
+        sets[b]->vyz = (double*)P->bufs[b][2];
+        sets[b]->cs = (int32_t*)P->bufs[b][3];
     }
     s.capacity = capacity;
     PCHK(hipMemcpy(P->d_state, &s, sizeof(s), hipMemcpyHostToDevice));
@@ -825,12 +822,17 @@ static int append_host(punc_pop_s* P, int64_t n, const double* xs, const double*
     int r = read_state(P, s);
     if (r) return r;
     if (s.n + n > P->capacity) return fail("population capacity exceeded");
-    std::vector<double> col((size_t)n);
-    double* dst[6] = {s.cur.x, s.cur.y, s.cur.z, s.cur.vx, s.cur.vy, s.cur.vz};
-    for (int k = 0; k < 6; ++k) {
-        const double* src = k < 3 ? xs : vs;
-        for (int64_t i = 0; i < n; ++i) col[i] = src[i * 3 + (k % 3)];
-        HIPCHK(hipMemcpy(dst[k] + s.n, col.data(), n * 8, hipMemcpyHostToDevice));
+    // the pair arrays: value q (0..5: x, y, z, vx, vy, vz) of particle i is
+    // element 2i + q % 2 of pair array q / 2
+    std::vector<double> col((size_t)(2 * n));
+    double* dst[3] = {s.cur.xy, s.cur.zvx, s.cur.vyz};
+    for (int a = 0; a < 3; ++a) {
+        for (int h = 0; h < 2; ++h) {
+            const int q = 2 * a + h;
+            const double* src = q < 3 ? xs : vs;
+            for (int64_t i = 0; i < n; ++i) col[2 * i + h] = src[i * 3 + (q % 3)];
+        }
+        HIPCHK(hipMemcpy(dst[a] + 2 * s.n, col.data(), n * 16, hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(s.cur.cs + s.n, cs_packed, n * 4, hipMemcpyHostToDevice));
     int64_t nn = s.n + n;
@@ -976,13 +978,19 @@ extern "C" int punc_gpu_pop_get(punc_pop_t* p, double* xs, double* vs, int32_t* 
     if (r) return r;
     const int64_t n = s.n;
     if (n == 0) return 0;
-    std::vector<double> col((size_t)n);
-    const double* src[6] = {s.cur.x, s.cur.y, s.cur.z, s.cur.vx, s.cur.vy, s.cur.vz};
-    for (int k = 0; k < 6; ++k) {
-        double* dst = k < 3 ? xs : vs;
-        if (!dst) continue;
-        HIPCHK(hipMemcpy(col.data(), src[k], n * 8, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < n; ++i) dst[i * 3 + (k % 3)] = col[i];
+    std::vector<double> col((size_t)(2 * n));
+    const double* src[3] = {s.cur.xy, s.cur.zvx, s.cur.vyz};
+    for (int a = 0; a < 3; ++a) {
+        // pair array a holds values 2a and 2a + 1 of (x, y, z, vx, vy, vz)
+        const bool need = a == 0 ? xs != nullptr : a == 2 ? vs != nullptr : (xs != nullptr || vs != nullptr);
+        if (!need) continue;
+        HIPCHK(hipMemcpy(col.data(), src[a], n * 16, hipMemcpyDeviceToHost));
+        for (int h = 0; h < 2; ++h) {
+            const int q = 2 * a + h;
+            double* dst = q < 3 ? xs : vs;
+            if (!dst) continue;
+            for (int64_t i = 0; i < n; ++i) dst[i * 3 + (q % 3)] = col[2 * i + h];
+        }
     }
     if (species || cells) {
         std::vector<int32_t> cs((size_t)n);
@@ -2471,15 +2479,17 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     const int64_t n = s0.n;
     const size_t nab = (size_t)MAX_SPECIES * MAX_BND;
     // snapshot: particle arrays, device state, absorbed counters
-    std::vector<void*> snap(7, nullptr);
+    std::vector<void*> snap(4, nullptr);
     void* snap_ab = nullptr;
     auto release = [&]() {
         for (void* q : snap) hipFree(q);
         hipFree(snap_ab);
     };
-    const void* src[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
-    for (int k = 0; k < 7; ++k) {
-        const size_t bytes = (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4);
+    // bytes per particle of each array: the three pair arrays, then cs
+    auto snap_bytes = [&](int k) { return (size_t)std::max<int64_t>(n, 1) * (k < 3 ? 16 : 4); };
+    const void* src[4] = {s0.cur.xy, s0.cur.zvx, s0.cur.vyz, s0.cur.cs};
+    for (int k = 0; k < 4; ++k) {
+        const size_t bytes = snap_bytes(k);
         hipError_t e = hipMalloc(&snap[k], bytes);
         if (e == hipSuccess) e = hipMemcpyAsync(snap[k], src[k], bytes, hipMemcpyDeviceToDevice, p->stream);
         if (e != hipSuccess) {
@@ -2519,10 +2529,10 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
                 p->block = bl;
                 p->dep_mode = md;
                 // restore the snapshot, one untimed step, then reps timed steps
-                for (int k = 0; k < 7 && !rc; ++k) {
-                    void* dst[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
-                    if (hipMemcpyAsync(dst[k], snap[k], (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4),
-                                       hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+                for (int k = 0; k < 4 && !rc; ++k) {
+                    void* dst[4] = {s0.cur.xy, s0.cur.zvx, s0.cur.vyz, s0.cur.cs};
+                    if (hipMemcpyAsync(dst[k], snap[k], snap_bytes(k), hipMemcpyDeviceToDevice, p->stream) !=
+                        hipSuccess)
                         rc = fail("autotune: restore failed");
                 }
                 if (!rc && hipMemcpyAsync(p->d_state, &s0, sizeof(DevState), hipMemcpyHostToDevice, p->stream) != hipSuccess)
@@ -2550,11 +2560,8 @@ extern "C" int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, cons
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     // restore the state the caller handed in
-    const void* back[7] = {snap[0], snap[1], snap[2], snap[3], snap[4], snap[5], snap[6]};
-    void* dst[7] = {s0.cur.x, s0.cur.y, s0.cur.z, s0.cur.vx, s0.cur.vy, s0.cur.vz, s0.cur.cs};
-    for (int k = 0; k < 7; ++k)
-        hipMemcpyAsync(dst[k], back[k], (size_t)std::max<int64_t>(n, 1) * (k < 6 ? 8 : 4), hipMemcpyDeviceToDevice,
-                       p->stream);
+    void* dst[4] = {s0.cur.xy, s0.cur.zvx, s0.cur.vyz, s0.cur.cs};
+    for (int k = 0; k < 4; ++k) hipMemcpyAsync(dst[k], snap[k], snap_bytes(k), hipMemcpyDeviceToDevice, p->stream);
     hipMemcpyAsync(p->d_state, &s0, sizeof(DevState), hipMemcpyHostToDevice, p->stream);
     hipMemcpyAsync(p->d_absorbed, snap_ab, nab * 8, hipMemcpyDeviceToDevice, p->stream);
     hipMemcpyAsync(p->d_rho, (char*)snap_ab + nab * 8, ndof * 8, hipMemcpyDeviceToDevice, p->stream);

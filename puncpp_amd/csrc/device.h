@@ -11,13 +11,15 @@ constexpr int MAX_BND = 64;       // boundary ids 1..63
 constexpr int MAX_SPECIES = 128;  // species per population
 
 // Particle storage: FP64 structure of arrays plus one packed int32 cell field.
+// Particle state: three arrays of 16-byte pairs, (x, y), (z, vx) and
+// (vy, vz) - 2 doubles per particle each - and the packed species/cell.
+// A pass over all six values is three 16-byte loads (or stores) per lane
+// instead of six 8-byte ones; the particle kernels are bound by how many
+// vector-memory instructions the texture units process, not by bytes.
 struct PopArrays {
-    double* x;
-    double* y;
-    double* z;
-    double* vx;
-    double* vy;
-    double* vz;
+    double* xy;
+    double* zvx;
+    double* vyz;
     int32_t* cs;
 };
 

@@ -27,26 +27,66 @@ namespace punc_gpu {
 // outstanding particle loads and stores.  Viewing them through global-address-
 // space pointers gives plain global_load/global_store.
 #define GLOBAL __attribute__((address_space(1)))
+typedef double v2d __attribute__((ext_vector_type(2)));
 struct GArrays {
-    GLOBAL double* x;
-    GLOBAL double* y;
-    GLOBAL double* z;
-    GLOBAL double* vx;
-    GLOBAL double* vy;
-    GLOBAL double* vz;
+    GLOBAL v2d* xy;   // (x, y)
+    GLOBAL v2d* zvx;  // (z, vx)
+    GLOBAL v2d* vyz;  // (vy, vz)
     GLOBAL int32_t* cs;
 };
 
 __device__ inline GArrays gview(const PopArrays& a) {
     GArrays g;
-    g.x = (GLOBAL double*)a.x;
-    g.y = (GLOBAL double*)a.y;
-    g.z = (GLOBAL double*)a.z;
-    g.vx = (GLOBAL double*)a.vx;
-    g.vy = (GLOBAL double*)a.vy;
-    g.vz = (GLOBAL double*)a.vz;
+    g.xy = (GLOBAL v2d*)a.xy;
+    g.zvx = (GLOBAL v2d*)a.zvx;
+    g.vyz = (GLOBAL v2d*)a.vyz;
     g.cs = (GLOBAL int32_t*)a.cs;
     return g;
+}
+
+__device__ inline v2d mk2(double a, double b) {
+    v2d r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+
+// Particle i's position (two 16-byte loads; the second also holds vx).
+__device__ inline void ld_pos(const GArrays& A, int64_t i, double& x, double& y, double& z) {
+    const v2d a = A.xy[i], b = A.zvx[i];
+    x = a.x;
+    y = a.y;
+    z = b.x;
+}
+// Particle i's velocity (two 16-byte loads; the first also holds z).
+__device__ inline void ld_vel(const GArrays& A, int64_t i, double& vx, double& vy, double& vz) {
+    const v2d b = A.zvx[i], c = A.vyz[i];
+    vx = b.y;
+    vy = c.x;
+    vz = c.y;
+}
+__device__ inline void ld_all(const GArrays& A, int64_t i, double& x, double& y, double& z, double& vx, double& vy,
+                              double& vz) {
+    const v2d a = A.xy[i], b = A.zvx[i], c = A.vyz[i];
+    x = a.x;
+    y = a.y;
+    z = b.x;
+    vx = b.y;
+    vy = c.x;
+    vz = c.y;
+}
+__device__ inline void st_all(const GArrays& A, int64_t i, double x, double y, double z, double vx, double vy,
+                              double vz) {
+    A.xy[i] = mk2(x, y);
+    A.zvx[i] = mk2(z, vx);
+    A.vyz[i] = mk2(vy, vz);
+}
+// Particle s of S to slot d of D, all fields.
+__device__ inline void copy_particle(const GArrays& D, int64_t d, const GArrays& S, int64_t s) {
+    D.xy[d] = S.xy[s];
+    D.zvx[d] = S.zvx[s];
+    D.vyz[d] = S.vyz[s];
+    D.cs[d] = S.cs[s];
 }
 
 // Chunk of the population for this block: XCD x = b % 8 (blocks are
@@ -508,27 +548,18 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         const int32_t cs = A.cs[i];
         const int sp = cs >> cell_bits;
         int32_t c = cs & cmask;
-        double x = A.x[i], y = A.y[i], z = A.z[i];
+        double x, y, z;
         double vx = 0, vy = 0, vz = 0;
-        if (PUSH != 0) {
-            vx = A.vx[i];
-            vy = A.vy[i];
-            vz = A.vz[i];
-        }
-        if (MOVE != 0 && PUSH == 0) {
-            vx = A.vx[i];
-            vy = A.vy[i];
-            vz = A.vz[i];
-        }
+        if (PUSH != 0 || MOVE != 0)
+            ld_all(A, i, x, y, z, vx, vy, vz);
+        else
+            ld_pos(A, i, x, y, z);
         if (PUSH != 0) {
             const CellRec& R = M.rec[c];
             double l[4], ef[12];
             bary4(R.bary, x, y, z, l);
             load_cell_field(cellE, c, ef);
             push_velocity<PUSH>(coef[sp], l, ef, vx, vy, vz, ke);
-            A.vx[i] = vx;
-            A.vy[i] = vy;
-            A.vz[i] = vz;
         }
         if (MOVE != 0) {
             // move / move_periodic (pusher.h:658, 676-677)
@@ -540,10 +571,12 @@ k_particles(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 y -= Ly * floor(y / Ly);
                 z -= Lz * floor(z / Lz);
             }
-            A.x[i] = x;
-            A.y[i] = y;
-            A.z[i] = z;
+            A.xy[i] = mk2(x, y);
         }
+        // the pairs holding what changed: (z, vx) after a push or a move,
+        // (vy, vz) after a push
+        if (PUSH != 0 || MOVE != 0) A.zvx[i] = mk2(z, vx);
+        if (PUSH != 0) A.vyz[i] = mk2(vy, vz);
         bool alive = true;
         if (LOCATE) {
             // Certainly-inside shortcut: if all barycentric coordinates of the
@@ -646,12 +679,7 @@ __device__ inline void load_particle(const GArrays& A, int64_t i, PartRegs& p) {
     // non-temporal ones on the 64M-particle workload; the stores of the
     // results stay non-temporal (ordinary stores were 7% slower).
     p.cs = A.cs[i];
-    p.x = A.x[i];
-    p.y = A.y[i];
-    p.z = A.z[i];
-    p.vx = A.vx[i];
-    p.vy = A.vy[i];
-    p.vz = A.vz[i];
+    ld_all(A, i, p.x, p.y, p.z, p.vx, p.vy, p.vz);
 }
 
 // All four barycentric coordinates above the margin: the point is certainly
@@ -798,12 +826,9 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         }
         if (active) {
             ke += ke_l;
-            __builtin_nontemporal_store(vx, &O.vx[i]);
-            __builtin_nontemporal_store(vy, &O.vy[i]);
-            __builtin_nontemporal_store(vz, &O.vz[i]);
-            __builtin_nontemporal_store(x, &O.x[i]);
-            __builtin_nontemporal_store(y, &O.y[i]);
-            __builtin_nontemporal_store(z, &O.z[i]);
+            __builtin_nontemporal_store(mk2(x, y), &O.xy[i]);
+            __builtin_nontemporal_store(mk2(z, vx), &O.zvx[i]);
+            __builtin_nontemporal_store(mk2(vy, vz), &O.vyz[i]);
             if (permuted) __builtin_nontemporal_store(p.cs, &O.cs[i]);
         }
         double dl[4];
@@ -956,7 +981,8 @@ k_kinetic(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bi
     for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
         const int sp = A.cs[i] >> cell_bits;
         const double hm = coef[sp].half_m;
-        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+        double vx, vy, vz;
+        ld_vel(A, i, vx, vy, vz);
         ke += (hm * vx) * vx;
         ke += (hm * vy) * vy;
         ke += (hm * vz) * vz;
@@ -1093,13 +1119,7 @@ __global__ void k_compact(DevState* __restrict__ S, const int32_t* __restrict__ 
         const int32_t other = atomicExch(&slot[k], idx);
         if (other < 0) return;  // first to arrive
         const int64_t d = is_hole ? idx : other, s = is_hole ? other : idx;
-        A.x[d] = A.x[s];
-        A.y[d] = A.y[s];
-        A.z[d] = A.z[s];
-        A.vx[d] = A.vx[s];
-        A.vy[d] = A.vy[s];
-        A.vz[d] = A.vz[s];
-        A.cs[d] = A.cs[s];
+        copy_particle(A, d, A, s);
         slot[k] = -1;
     };
     for (int64_t b = w0; b < L; b += stride) {
@@ -1317,12 +1337,9 @@ __global__ void __launch_bounds__(256) k_sort_gather(DevState* __restrict__ S) {
     if (j >= n || !S->perm_pending) return;
     const GArrays A = gview(S->cur), B = gview(S->alt);
     const int64_t i = S->perm[j];
-    __builtin_nontemporal_store(A.x[i], &B.x[j]);
-    __builtin_nontemporal_store(A.y[i], &B.y[j]);
-    __builtin_nontemporal_store(A.z[i], &B.z[j]);
-    __builtin_nontemporal_store(A.vx[i], &B.vx[j]);
-    __builtin_nontemporal_store(A.vy[i], &B.vy[j]);
-    __builtin_nontemporal_store(A.vz[i], &B.vz[j]);
+    __builtin_nontemporal_store(A.xy[i], &B.xy[j]);
+    __builtin_nontemporal_store(A.zvx[i], &B.zvx[j]);
+    __builtin_nontemporal_store(A.vyz[i], &B.vyz[j]);
     __builtin_nontemporal_store(A.cs[i], &B.cs[j]);
 }
 
@@ -1385,12 +1402,7 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
             if (!vok) atomicAdd(errs, 1ull);
         }
         const int64_t i = base + k;
-        A.x[i] = p[0];
-        A.y[i] = p[1];
-        A.z[i] = p[2];
-        A.vx[i] = v[0];
-        A.vy[i] = v[1];
-        A.vz[i] = v[2];
+        st_all(A, i, p[0], p[1], p[2], v[0], v[1], v[2]);
         A.cs[i] = (sp << cell_bits) | c;
     }
 }
@@ -1595,12 +1607,7 @@ k_inject_append(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restr
         return;
     }
     const double x = cx[k], y = cx[cand_cap + k], z = cx[2 * cand_cap + k];
-    A.x[d] = x;
-    A.y[d] = y;
-    A.z[d] = z;
-    A.vx[d] = cx[3 * cand_cap + k];
-    A.vy[d] = cx[4 * cand_cap + k];
-    A.vz[d] = cx[5 * cand_cap + k];
+    st_all(A, d, x, y, z, cx[3 * cand_cap + k], cx[4 * cand_cap + k], cx[5 * cand_cap + k]);
     A.cs[d] = cs;
     if (rho_raw) {
         const CellRec& R = M.rec[cs & cmask];
@@ -1671,9 +1678,7 @@ k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __re
         int32_t ncs;
         {
             const int64_t j0 = clampi(s + threadIdx.x);
-            nx = A.x[j0];
-            ny = A.y[j0];
-            nz = A.z[j0];
+            ld_pos(A, j0, nx, ny, nz);
             ncs = A.cs[j0];
         }
         for (int64_t base = s; base < e; base += blockDim.x) {  // block-uniform trip count
@@ -1693,9 +1698,7 @@ k_potential_energy(const DevMesh M, DevState* __restrict__ S, const SpCoef* __re
             __builtin_amdgcn_sched_barrier(0);
             {
                 const int64_t j = clampi(base + threadIdx.x + blockDim.x);
-                nx = A.x[j];
-                ny = A.y[j];
-                nz = A.z[j];
+                ld_pos(A, j, nx, ny, nz);
                 ncs = A.cs[j];
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -1816,9 +1819,7 @@ k_density_cg1(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restric
     int32_t ncs;
     {
         const int64_t j0 = clampi(s + threadIdx.x);
-        nx = A.x[j0];
-        ny = A.y[j0];
-        nz = A.z[j0];
+        ld_pos(A, j0, nx, ny, nz);
         ncs = A.cs[j0];
     }
     int grp = 0;
@@ -1839,9 +1840,7 @@ k_density_cg1(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restric
         __builtin_amdgcn_sched_barrier(0);
         {
             const int64_t j = clampi(i + blockDim.x);
-            nx = A.x[j];
-            ny = A.y[j];
-            nz = A.z[j];
+            ld_pos(A, j, nx, ny, nz);
             ncs = A.cs[j];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1912,7 +1911,8 @@ k_statistics(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell
     for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
         const double q = coef[A.cs[i] >> cell_bits].q;
         if (q == 0.0) continue;
-        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+        double vx, vy, vz;
+        ld_vel(A, i, vx, vy, vz);
         double v = 0;
         v += vx * vx;
         v += vy * vy;
@@ -1999,12 +1999,14 @@ k_pop_digest(DevState* __restrict__ S, int cell_bits, const int32_t* __restrict_
         const int32_t cs = A.cs[i];
         const int sp = cs >> cell_bits;
         const uint32_t cell = (uint32_t)((cs & cmask) - c0[sp]);
-        uint64_t h = mix64((uint64_t)__double_as_longlong(A.x[i]) ^ 0x9e3779b97f4a7c15ull);
-        h = mix64(h ^ (uint64_t)__double_as_longlong(A.y[i]));
-        h = mix64(h ^ (uint64_t)__double_as_longlong(A.z[i]));
-        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vx[i]));
-        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vy[i]));
-        h = mix64(h ^ (uint64_t)__double_as_longlong(A.vz[i]));
+        double px, py, pz, qx, qy, qz;
+        ld_all(A, i, px, py, pz, qx, qy, qz);
+        uint64_t h = mix64((uint64_t)__double_as_longlong(px) ^ 0x9e3779b97f4a7c15ull);
+        h = mix64(h ^ (uint64_t)__double_as_longlong(py));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(pz));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(qx));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(qy));
+        h = mix64(h ^ (uint64_t)__double_as_longlong(qz));
         h = mix64(h ^ (uint64_t)cell);
         atomicAdd(&sh[sp], (unsigned long long)h);
         atomicAdd(&sh[nsp + sp], 1ull);
@@ -2085,9 +2087,10 @@ __global__ void k_pack_records(const DevState* __restrict__ S, const SpCoef* __r
         const int64_t i = first + k;
         const SpCoef& C = coef[A.cs[i] >> cell_bits];
         double2* r = (double2*)(rec + 8 * k);
-        r[0] = make_double2(A.x[i], A.y[i]);
-        r[1] = make_double2(A.z[i], A.vx[i]);
-        r[2] = make_double2(A.vy[i], A.vz[i]);
+        const v2d a = A.xy[i], b = A.zvx[i], c = A.vyz[i];
+        r[0] = make_double2(a.x, a.y);
+        r[1] = make_double2(b.x, b.y);
+        r[2] = make_double2(c.x, c.y);
         r[3] = make_double2(C.q, C.m);
     }
 }
@@ -2125,12 +2128,7 @@ __global__ void k_file_append(DevState* __restrict__ S, int64_t count, int64_t c
         if (cs < 0) continue;
         const int64_t d = n0 + pos[k];
         if (d >= pcap) continue;  // counted by k_file_finish
-        A.x[d] = cx[k];
-        A.y[d] = cx[cap + k];
-        A.z[d] = cx[2 * cap + k];
-        A.vx[d] = cx[3 * cap + k];
-        A.vy[d] = cx[4 * cap + k];
-        A.vz[d] = cx[5 * cap + k];
+        st_all(A, d, cx[k], cx[cap + k], cx[2 * cap + k], cx[3 * cap + k], cx[4 * cap + k], cx[5 * cap + k]);
         A.cs[d] = cs;
     }
 }
