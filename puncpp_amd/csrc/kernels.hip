@@ -768,7 +768,7 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         const GLOBAL v2d* rp = (const GLOBAL v2d*)(M.rec + c);
         const GLOBAL v2d* ep = (const GLOBAL v2d*)(cellE + (int64_t)c * CELL_FIELD_STRIDE);
         double rb[12], ef[12];
-        int32_t rd[8];  // dof[4], nbr[4]
+        int32_t rd[8];  // dof[4], nbr[4] (the neighbours for a walker's first hop)
         for (int m = 0; m < 6; ++m) {
             const v2d t = rp[m];
             rb[2 * m] = t.x;
@@ -847,6 +847,33 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 walker = true;
             }
         }
+        // A walker's first hop, decided here from the coordinates and the
+        // neighbour ids already loaded, as walk_bary decides it at cell c
+        // (the same lambda, the same margin, the first facet whose
+        // coordinate is below -m): phase 2 then starts at that neighbour and
+        // does not fetch this cell's record again.  Undecided cases (a
+        // coordinate within the margin before the first clear exit, a point
+        // outside the margin's coordinate range, an exit through the
+        // boundary) start at c.
+        int32_t wstart = c;
+        bool hopped = false;
+        if (walker && in_range) {
+            const double Sl = (fabs(dl[0]) + fabs(dl[1])) + (fabs(dl[2]) + fabs(dl[3]));
+            const double m = tol * (Sl > 1.0 ? Sl : 1.0);
+            int f = -1;
+            for (int q = 0; q < 4; ++q) {
+                if (dl[q] > m) continue;
+                if (dl[q] < -m) f = q;
+                break;  // decided exit, or within the margin (undecided)
+            }
+            if (f >= 0) {
+                const int32_t nb = f == 0 ? rd[4] : f == 1 ? rd[5] : f == 2 ? rd[6] : rd[7];
+                if (nb >= 0) {
+                    wstart = nb;
+                    hopped = true;
+                }
+            }
+        }
         count_near(S, near);
         // Wavefront-aggregated append to this block's walk-list segment.
         const unsigned long long wm = __ballot(walker);
@@ -860,8 +887,8 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
                 wr.x = x;
                 wr.y = y;
                 wr.z = z;
-                wr.i = (int32_t)i;
-                wr.cs = p.cs;
+                wr.i = hopped ? ~(int32_t)i : (int32_t)i;  // negative: starts at the neighbour
+                wr.cs = (sp << cell_bits) | wstart;
                 const int wk = wb + __popcll(wm & ((1ull << lane) - 1ull));
                 if (wk < wl_cap)
                     wl[wk] = wr;
@@ -898,15 +925,16 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         bool near = false;
         if (j < we) {
             const WalkRec r = j - s < wl_cap ? wl[j - s] : wrec[j];
-            i = r.i;
+            const bool hopped = r.i < 0;  // the walk's first hop was taken in phase 1
+            i = hopped ? ~r.i : r.i;
             const int sp = r.cs >> cell_bits;
-            const int32_t c = r.cs & cmask;
+            const int32_t c = r.cs & cmask;  // where the walk starts
             const int32_t c2 = walk_bary(M, c, r.x, r.y, r.z, dl, ddof);
             if (c2 < 0) {
                 leave_key = (c2 == WALK_FAILED) ? ABS_LOST : sp * MAX_BND + (-c2);
                 O.cs[i] = c2;
             } else {
-                if (c2 != c) O.cs[i] = (sp << cell_bits) | c2;
+                if (hopped || c2 != c) O.cs[i] = (sp << cell_bits) | c2;
                 dkey = c2;
                 dq = coef[sp].q;
                 const bool in_range = tol > 0.0 && fabs(r.x) <= X && fabs(r.y) <= X && fabs(r.z) <= X;
