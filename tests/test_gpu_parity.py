@@ -268,7 +268,9 @@ def test_fused_step_table_overflow(gpu, mode):
     """One block processes the whole population (70 groups) on a mesh with
     more dofs than the LDS deposit table has slots: the table is flushed
     whenever it is 5/8 full at an 8-group boundary, and adds that find it
-    full in between take the direct HBM path; results as the oracle's."""
+    full in between take the direct HBM path.  The block's walk list (a few
+    thousand walkers) also outgrows its LDS part and continues in the
+    block's global segment.  Results as the oracle's."""
     _print_case("fused_step_overflow", mode=mode)
     hm = sphere_mesh(n=14, L=4.0, r=1.0)
     assert hm.ndof > 2048
