@@ -1764,7 +1764,7 @@ extern "C" int punc_gpu_statistics(punc_pop_t* p, double* stats) {
     if (r) return r;
     const int g = diag_grid(p);
     double* out = p->d_diag + 6 * (size_t)p->ke_np;
-    HIPCHK(launch_statistics(p->d_state, coef, p->cell_bits, p->d_diag, out, g, p->stream));
+    HIPCHK(launch_statistics(p->d_state, coef, p->nsp, p->cell_bits, p->d_diag, out, g, p->stream));
     double h[6];
     HIPCHK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));

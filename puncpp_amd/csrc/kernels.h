@@ -157,7 +157,7 @@ hipError_t launch_density(const DevMesh& M, DevState* S, const SpCoef* coef, int
                           double* raw, int grid, hipStream_t st);
 hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale, double we, double wi,
                                 double* ne, double* ni, int64_t n, hipStream_t st);
-hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, double* part, double* out, int grid,
+hipError_t launch_statistics(DevState* S, const SpCoef* coef, int nsp, int cell_bits, double* part, double* out, int grid,
                              hipStream_t st);
 hipError_t launch_ema(const double* f, double* g, double w, int64_t n, hipStream_t st);
 // Per species (nsp <= MAX_SPECIES): out[s] += sum of particle hashes (cell ids

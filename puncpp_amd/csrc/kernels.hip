@@ -1930,27 +1930,45 @@ __device__ inline void welford_merge(Welford& a, const Welford& b) {
 }
 
 __global__ void __launch_bounds__(256)
-k_statistics(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int cell_bits, double* __restrict__ part) {
+k_statistics(DevState* __restrict__ S, const SpCoef* __restrict__ coef, int nsp, int cell_bits,
+             double* __restrict__ part) {
     __shared__ Welford sh[2][256];
+    __shared__ double qs[MAX_SPECIES];
+    for (int k = threadIdx.x; k < nsp; k += blockDim.x) qs[k] = coef[k].q;
+    __syncthreads();
     const GArrays A = gview(S->cur);
     int64_t s, e;
     chunk_range(S->n, s, e);
     Welford w[2] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-        const double q = coef[A.cs[i] >> cell_bits].q;
-        if (q == 0.0) continue;
-        double vx, vy, vz;
-        ld_vel(A, i, vx, vy, vz);
-        double v = 0;
-        v += vx * vx;
-        v += vy * vy;
-        v += vz * vz;
-        v = sqrt(v);
-        Welford& W = w[q < 0 ? 0 : 1];
-        W.n += 1.0;
-        const double old = W.mean;
-        W.mean = old + (v - old) / W.n;
-        W.m2 += (v - old) * (v - W.mean);
+    // STATS_UNROLL particles' loads in flight at once, then the running
+    // updates in the thread's particle order (the same order as one at a time)
+    constexpr int STATS_UNROLL = 4;
+    for (int64_t i0 = s + threadIdx.x; i0 < e; i0 += (int64_t)STATS_UNROLL * blockDim.x) {
+        int32_t cs[STATS_UNROLL];
+        double vx[STATS_UNROLL], vy[STATS_UNROLL], vz[STATS_UNROLL];
+        for (int u = 0; u < STATS_UNROLL; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            cs[u] = -1;
+            if (i < e) {
+                cs[u] = A.cs[i];
+                ld_vel(A, i, vx[u], vy[u], vz[u]);
+            }
+        }
+        for (int u = 0; u < STATS_UNROLL; ++u) {
+            if (cs[u] < 0) continue;
+            const double q = qs[cs[u] >> cell_bits];
+            if (q == 0.0) continue;
+            double v = 0;
+            v += vx[u] * vx[u];
+            v += vy[u] * vy[u];
+            v += vz[u] * vz[u];
+            v = sqrt(v);
+            Welford& W = w[q < 0 ? 0 : 1];
+            W.n += 1.0;
+            const double old = W.mean;
+            W.mean = old + (v - old) / W.n;
+            W.m2 += (v - old) * (v - W.mean);
+        }
     }
     for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] = w[k];
     __syncthreads();
@@ -2526,9 +2544,10 @@ hipError_t launch_density_scale(bool dg0, const double* raw, const double* scale
     return hipSuccess;
 }
 
-hipError_t launch_statistics(DevState* S, const SpCoef* coef, int cell_bits, double* part, double* out, int grid,
-                             hipStream_t st) {
-    hipLaunchKernelGGL(k_statistics, dim3(grid), dim3(256), 0, st, S, coef, cell_bits, part);
+hipError_t launch_statistics(DevState* S, const SpCoef* coef, int nsp, int cell_bits, double* part, double* out,
+                             int grid, hipStream_t st) {
+    if (nsp < 1 || nsp > MAX_SPECIES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_statistics, dim3(grid), dim3(256), 0, st, S, coef, nsp, cell_bits, part);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_statistics_merge, dim3(1), dim3(256), 0, st, part, grid, out);
     LAUNCH_CHECK();
