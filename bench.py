@@ -130,6 +130,8 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
 
     run(max(warmup - 1, 0), sort_every)
     N.check(lib.punc_gpu_synchronize(pop._h))
+    nf = N.C.c_int64()
+    N.check(lib.punc_gpu_near_face(pop._h, N.C.byref(nf)))  # resets the count
     n_start = pop.size()
     log(f"[bench] warmup done, {n_start} particles")
     t1 = time.perf_counter()
@@ -137,6 +139,10 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
     N.check(lib.punc_gpu_synchronize(pop._h))
     t2 = time.perf_counter()
     n_end = pop.size()
+    # particles each timed step left within 1e-12 of a facet plane of their
+    # cell (counted by the relocation, reported only)
+    N.check(lib.punc_gpu_near_face(pop._h, N.C.byref(nf)))
+    near_face = {"particles": int(nf.value), "steps": steps, "per_particle_step": nf.value / max(1.0, 0.5 * (n_start + n_end) * steps)}
     alt = None
     if alt_sort_every and alt_sort_every != sort_every:
         # the same steps with another re-sort interval, from a fresh sort
@@ -208,6 +214,7 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
         f"({achieved / 1e9:.0f} GB/s), injected last step {kin.value}, absorbed {ab.sum()}")
     return {
         "value": value, "ms_per_step": 1e3 * elapsed / steps, "kern_ms": kern_ms, "b_launch": b_launch,
+        "near_face": near_face,
         "achieved": achieved, "n_avg": n_avg, "desc": desc, "cells": mesh.num_cells, "ndof": mesh.ndof,
         "hm": hm, "nps": nps, "vth": vth, "vd": vd, "density": density, "tune": tune, "xfer": xfer,
         "diag": diag, "flux_mc": flux_mc, "field": field, "phases": phases, "alt": alt, "cfg": cfg,
@@ -667,7 +674,7 @@ def main():
                       extras=False)
         probe = {"workload": f"probe: {pr['desc']}", "value": pr["value"], "unit": "particle-steps/s",
                  "ms_per_step": pr["ms_per_step"], "particles": int(pr["n_avg"]), "cells": pr["cells"],
-                 "dofs": pr["ndof"], "sort_every": args.sort_every, "launch": pr["tune"],
+                 "dofs": pr["ndof"], "sort_every": args.sort_every, "launch": pr["tune"], "near_face": pr["near_face"],
                  "roofline": roofline(pr)}
     batched = None if args.no_batched else run_batched(args.batched_instances, args.batched_steps)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(res)
@@ -688,6 +695,7 @@ def main():
                    "particles": int(res["n_avg"]), "sort_every": args.sort_every, "launch": res["tune"],
                    "field": "frozen synthetic CG1 field on device"},
         "roofline": roofline(res),
+        "near_face": res["near_face"],
         "resort_interval_alt": res["alt"],
         "probe": probe,
         "cpu_baseline": cpu,
