@@ -1841,10 +1841,8 @@ extern "C" int punc_gpu_sort(punc_pop_t* p) {
     // goes to the compaction scratch, used only after that step's particle
     // kernel has finished.
     uint32_t* w = (uint32_t*)p->d_walk;
-    L.keys0 = w;
-    L.vals0 = w + p->capacity;
-    L.keys1 = w + 2 * p->capacity;
-    L.vals1 = w + 3 * p->capacity;
+    L.kv0 = (uint64_t*)w;
+    L.kv1 = (uint64_t*)(w + 2 * p->capacity);
     L.perm = (uint32_t*)p->d_S;
     L.counts = p->d_scount;
     L.offsets = p->d_soffset;

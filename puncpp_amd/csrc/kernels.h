@@ -103,10 +103,8 @@ struct SortLaunch {
     DevState* state;
     int cell_bits;
     int64_t capacity;   // particle slots per buffer
-    uint32_t* keys0;    // capacity each: pass scratch, free once the sort has run
-    uint32_t* vals0;
-    uint32_t* keys1;
-    uint32_t* vals1;
+    uint64_t* kv0;      // capacity each: pass scratch ((key << 32) | value), free once
+    uint64_t* kv1;      // the sort has run
     uint32_t* perm;     // capacity: the resulting order; must stay untouched until
                         // the next fused step has read it (that step writes its
                         // walk list, so not in the walk-list buffer)

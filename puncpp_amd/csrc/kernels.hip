@@ -1274,8 +1274,9 @@ __device__ inline unsigned long long match_digit(uint32_t dg) {
 
 // Key of element i in pass `shift`: the first pass reads cells from the
 // particle array, later passes the previous pass's keys.
-__device__ inline uint32_t radix_key(const GLOBAL int32_t* cs, const uint32_t* keys, int32_t cmask, int64_t i) {
-    return cs ? (uint32_t)(cs[i] & cmask) : keys[i];
+// Pass elements after the first: (key << 32) | value, one 8-byte access each.
+__device__ inline uint32_t radix_key(const GLOBAL int32_t* cs, const uint64_t* kv, int32_t cmask, int64_t i) {
+    return cs ? (uint32_t)(cs[i] & cmask) : (uint32_t)(kv[i] >> 32);
 }
 
 // Keys (and values) are loaded RADIX_UNROLL wavefront steps at a time, all
@@ -1285,7 +1286,7 @@ static_assert(RADIX_TILE % (64 * RADIX_UNROLL) == 0, "tiles hold whole unrolled 
 
 // Digit counts of each tile: counts[digit * ntiles + tile].
 __global__ void __launch_bounds__(256)
-k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __restrict__ keys, int shift,
+k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint64_t* __restrict__ keys, int shift,
               int64_t ntiles, int64_t* __restrict__ counts) {
     __shared__ uint32_t cnt[4][RADIX];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1303,10 +1304,18 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __r
                 const int64_t i = base + 64 * u + lane;
                 dg[u] = i < t1 ? (radix_key(cs, keys, cmask, i) >> shift) & (RADIX - 1) : RADIX;
             }
+            // Counting needs no order: each run of consecutive lanes with one
+            // digit adds its length with one LDS atomic (the cell-sorted
+            // input gives long runs), inactive lanes (digit RADIX) add nothing.
             for (int u = 0; u < RADIX_UNROLL; ++u) {
-                const bool act = dg[u] < RADIX;
-                const unsigned long long m = match_digit(dg[u]) & __ballot(act);
-                if (act && lane == __ffsll((long long)m) - 1) cnt[w][dg[u]] += (uint32_t)__popcll(m);
+                const uint32_t d = dg[u];
+                const uint32_t dprev = __shfl_up(d, 1, 64);
+                const bool head = lane == 0 || dprev != d;
+                const unsigned long long heads = __ballot(head);
+                const unsigned long long upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+                const int hl = 63 - __clzll((long long)upto);  // first lane of this lane's run
+                const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+                if (tail && d < RADIX) atomicAdd(&cnt[w][d], (uint32_t)(lane - hl + 1));
             }
         }
         for (int d = lane; d < RADIX; d += 64) counts[(int64_t)d * ntiles + tile] = cnt[w][d];
@@ -1314,19 +1323,20 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint32_t* __r
 }
 
 // Stable scatter of each tile's (key, value) pairs to offsets[digit * ntiles +
-// tile] plus the rank among the tile's earlier keys of that digit.  vals ==
-// nullptr: values are the element indices (first pass).  keys_out == nullptr:
-// the last pass, only values are written.
+// tile] plus the rank among the tile's earlier keys of that digit.  kv ==
+// nullptr: the first pass, keys from the particles' cells and values the
+// element indices.  kv_out == nullptr: the last pass, only the values are
+// written (to vals_out).
 __global__ void __launch_bounds__(256)
-k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint32_t* __restrict__ keys,
-                const uint32_t* __restrict__ vals, int shift, int64_t ntiles, const int64_t* __restrict__ offsets,
-                uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint64_t* __restrict__ kv, int shift,
+                int64_t ntiles, const int64_t* __restrict__ offsets, uint64_t* __restrict__ kv_out,
+                uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t run[4][RADIX];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * 4 + w;
     if (tile >= ntiles) return;  // wavefront-uniform, and no block barrier below
     const int64_t n = S->n;
-    const GLOBAL int32_t* cs = keys ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
+    const GLOBAL int32_t* cs = kv ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
     const int32_t cmask = (1 << cell_bits) - 1;
     for (int d = lane; d < RADIX; d += 64) run[w][d] = (uint32_t)offsets[(int64_t)d * ntiles + tile];
     const int64_t t0 = tile * RADIX_TILE;
@@ -1337,8 +1347,14 @@ k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint32_t* _
         for (int u = 0; u < RADIX_UNROLL; ++u) {
             const int64_t i = base + 64 * u + lane;
             const bool act = i < t1;
-            kk[u] = act ? radix_key(cs, keys, cmask, i) : 0u;
-            vv[u] = act ? (vals ? vals[i] : (uint32_t)i) : 0u;
+            if (kv) {
+                const uint64_t e = act ? kv[i] : 0ull;
+                kk[u] = (uint32_t)(e >> 32);
+                vv[u] = (uint32_t)e;
+            } else {
+                kk[u] = act ? (uint32_t)(cs[i] & cmask) : 0u;
+                vv[u] = (uint32_t)i;
+            }
         }
         for (int u = 0; u < RADIX_UNROLL; ++u) {
             const bool act = base + 64 * u + lane < t1;
@@ -1350,8 +1366,10 @@ k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint32_t* _
             // lane advances it (one wavefront: LDS operations complete in order).
             if (act && lane == __ffsll((long long)m) - 1) run[w][dg] += (uint32_t)__popcll(m);
             if (act) {
-                if (keys_out) keys_out[dst] = kk[u];
-                vals_out[dst] = vv[u];
+                if (kv_out)
+                    kv_out[dst] = ((uint64_t)kk[u] << 32) | vv[u];
+                else
+                    vals_out[dst] = vv[u];
             }
         }
     }
@@ -2402,10 +2420,8 @@ hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
     const unsigned blocks = (unsigned)((ntiles + 3) / 4);
     const int passes = (L.cell_bits + RADIX_BITS - 1) / RADIX_BITS;
     const int64_t ncount = ntiles * RADIX;
-    uint32_t* kbuf[2] = {L.keys0, L.keys1};
-    uint32_t* vbuf[2] = {L.vals0, L.vals1};
-    const uint32_t* kin = nullptr;
-    const uint32_t* vin = nullptr;
+    uint64_t* kvbuf[2] = {L.kv0, L.kv1};
+    const uint64_t* kin = nullptr;
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = ps * RADIX_BITS;
         hipLaunchKernelGGL(k_radix_count, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
@@ -2413,15 +2429,13 @@ hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
         LAUNCH_CHECK();
         hipError_t e = launch_scan(L.counts, L.offsets, ncount, L.bsum, L.total, st);
         if (e != hipSuccess) return e;
-        uint32_t* kout = ps + 1 < passes ? kbuf[ps & 1] : nullptr;
-        uint32_t* vout = ps + 1 < passes ? vbuf[ps & 1] : L.perm;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, vin, shift,
-                           ntiles, L.offsets, kout, vout);
+        uint64_t* kout = ps + 1 < passes ? kvbuf[ps & 1] : nullptr;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
+                           L.offsets, kout, L.perm);
         LAUNCH_CHECK();
         kin = kout;
-        vin = vout;
     }
-    hipLaunchKernelGGL(k_set_perm, dim3(1), dim3(64), 0, st, L.state, vin);
+    hipLaunchKernelGGL(k_set_perm, dim3(1), dim3(64), 0, st, L.state, L.perm);
     LAUNCH_CHECK();
     return hipSuccess;
 }
