@@ -1252,7 +1252,9 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t len, const int64_t
 // Cell re-sort: least-significant-digit radix sort of (cell, index) pairs,
 // RADIX_BITS per pass, then one gather of the particle data into the
 // alternate buffers and a pointer swap.  No global atomics: every wavefront
-// ranks one tile of RADIX_TILE consecutive keys, stably, with ballot-matched
+// ranks one tile of consecutive keys (radix_tile: 1024 up to 16M particle
+// slots, so that small populations still give every SIMD several
+// wavefronts, 4096 beyond), stably, with ballot-matched
 // lane groups and wavefront-private digit counters in LDS; the tile's digit
 // counts, scanned digit-major over all tiles, give each (digit, tile) its
 // output offset.  The result is a stable sort by cell (particles of one cell
@@ -1260,7 +1262,9 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t len, const int64_t
 // ---------------------------------------------------------------------------
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
-constexpr int RADIX_TILE = 4096;  // keys per wavefront tile
+// Keys per wavefront tile for a population of `capacity` slots.
+inline int radix_tile(int64_t capacity) { return capacity <= (16ll << 20) ? 1024 : 4096; }
+constexpr int RADIX_TILE_MIN = 1024;
 
 // Lanes of the wavefront whose digit equals this lane's.
 __device__ inline unsigned long long match_digit(uint32_t dg) {
@@ -1282,12 +1286,12 @@ __device__ inline uint32_t radix_key(const GLOBAL int32_t* cs, const uint64_t* k
 // Keys (and values) are loaded RADIX_UNROLL wavefront steps at a time, all
 // loads in flight together, then ranked step by step in order.
 constexpr int RADIX_UNROLL = 8;
-static_assert(RADIX_TILE % (64 * RADIX_UNROLL) == 0, "tiles hold whole unrolled groups");
+static_assert(RADIX_TILE_MIN % (64 * RADIX_UNROLL) == 0, "tiles hold whole unrolled groups");
 
 // Digit counts of each tile: counts[digit * ntiles + tile].
 __global__ void __launch_bounds__(256)
 k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint64_t* __restrict__ keys, int shift,
-              int64_t ntiles, int64_t* __restrict__ counts) {
+              int64_t ntiles, int tsize, int64_t* __restrict__ counts) {
     __shared__ uint32_t cnt[4][RADIX];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * 4 + w;
@@ -1296,8 +1300,8 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint64_t* __r
     const int32_t cmask = (1 << cell_bits) - 1;
     for (int d = lane; d < RADIX; d += 64) cnt[w][d] = 0;
     if (tile < ntiles) {
-        const int64_t t0 = tile * RADIX_TILE;
-        const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
+        const int64_t t0 = tile * tsize;
+        const int64_t t1 = t0 + tsize < n ? t0 + tsize : n;
         for (int64_t base = t0; base < t1; base += 64 * RADIX_UNROLL) {  // wavefront-uniform
             uint32_t dg[RADIX_UNROLL];
             for (int u = 0; u < RADIX_UNROLL; ++u) {
@@ -1329,7 +1333,7 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint64_t* __r
 // written (to vals_out).
 __global__ void __launch_bounds__(256)
 k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint64_t* __restrict__ kv, int shift,
-                int64_t ntiles, const int64_t* __restrict__ offsets, uint64_t* __restrict__ kv_out,
+                int64_t ntiles, int tsize, const int64_t* __restrict__ offsets, uint64_t* __restrict__ kv_out,
                 uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t run[4][RADIX];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1339,8 +1343,8 @@ k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint64_t* _
     const GLOBAL int32_t* cs = kv ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
     const int32_t cmask = (1 << cell_bits) - 1;
     for (int d = lane; d < RADIX; d += 64) run[w][d] = (uint32_t)offsets[(int64_t)d * ntiles + tile];
-    const int64_t t0 = tile * RADIX_TILE;
-    const int64_t t1 = t0 + RADIX_TILE < n ? t0 + RADIX_TILE : n;
+    const int64_t t0 = tile * tsize;
+    const int64_t t1 = t0 + tsize < n ? t0 + tsize : n;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int64_t base = t0; base < t1; base += 64 * RADIX_UNROLL) {  // wavefront-uniform
         uint32_t kk[RADIX_UNROLL], vv[RADIX_UNROLL];
@@ -2416,7 +2420,8 @@ hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bs
 }
 
 hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
-    const int64_t ntiles = std::max<int64_t>((L.capacity + RADIX_TILE - 1) / RADIX_TILE, 1);
+    const int tsize = radix_tile(L.capacity);
+    const int64_t ntiles = std::max<int64_t>((L.capacity + tsize - 1) / tsize, 1);
     const unsigned blocks = (unsigned)((ntiles + 3) / 4);
     const int passes = (L.cell_bits + RADIX_BITS - 1) / RADIX_BITS;
     const int64_t ncount = ntiles * RADIX;
@@ -2425,13 +2430,13 @@ hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = ps * RADIX_BITS;
         hipLaunchKernelGGL(k_radix_count, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
-                           L.counts);
+                           tsize, L.counts);
         LAUNCH_CHECK();
         hipError_t e = launch_scan(L.counts, L.offsets, ncount, L.bsum, L.total, st);
         if (e != hipSuccess) return e;
         uint64_t* kout = ps + 1 < passes ? kvbuf[ps & 1] : nullptr;
         hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
-                           L.offsets, kout, L.perm);
+                           tsize, L.offsets, kout, L.perm);
         LAUNCH_CHECK();
         kin = kout;
     }
@@ -2456,7 +2461,8 @@ hipError_t launch_step_end(DevState* S, hipStream_t st) {
 }
 
 int64_t sort_count_entries(int64_t capacity) {
-    return std::max<int64_t>((capacity + RADIX_TILE - 1) / RADIX_TILE, 1) * RADIX;
+    const int tsize = radix_tile(capacity);
+    return std::max<int64_t>((capacity + tsize - 1) / tsize, 1) * RADIX;
 }
 
 hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, int sp, int rng_sp,
