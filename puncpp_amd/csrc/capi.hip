@@ -2046,6 +2046,9 @@ static int inject_launch_args(punc_pop_s* P, double dt, uint64_t seed, double* r
     L.kc = P->d_kc;
     L.rho_raw = rho_raw;
     L.errs = P->d_errs + 1;
+    L.all_maxwellian = true;
+    for (int sp = 0; sp < P->nsp; ++sp)
+        if (P->inj_on[sp] && P->spi[sp].kind != VDF_MAXWELLIAN) L.all_maxwellian = false;
     return 0;
 }
 

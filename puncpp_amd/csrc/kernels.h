@@ -48,6 +48,7 @@ struct InjectLaunch {
     int32_t* kc;          // inject_group_count(cand_cap): kept per group of slots
     double* rho_raw;  // nullable: deposit the kept particles
     unsigned long long* errs;
+    bool all_maxwellian;  // every species with injection pairs draws from a Maxwellian
 };
 
 hipError_t launch_particles(const ParticleLaunch& L, hipStream_t st);

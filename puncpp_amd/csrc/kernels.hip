@@ -1480,6 +1480,10 @@ k_load(const DevMesh M, DevState* __restrict__ S, int64_t base, int64_t N, int s
 constexpr uint64_t INJECT_MAX_ATTEMPTS = 1000000ull;
 constexpr int INJECT_PER_WAVE = 16;
 
+// MAXW: every injected species is Maxwellian - the flux pdf without the
+// other distributions' code (fewer registers, so more of its wavefronts run
+// beside the particle kernel); the same draws as the general kernel.
+template <bool MAXW>
 __global__ void __launch_bounds__(256)
 k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
              const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
@@ -1551,7 +1555,8 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
                 const double w0 = lo0 + (hi0 - lo0) * u01(ra.v[0]);
                 const double w1 = lo1 + (hi1 - lo1) * u01(ra.v[1]);
                 const double w2 = lo2 + (hi2 - lo2) * u01(ra.v[2]);
-                const double f = vdf_flux(F, w0, w1, w2, m0, m1, m2);
+                const double f = MAXW ? vdf_flux_maxwellian(F.factor, F.vth2, F.vd, w0, w1, w2, m0, m1, m2)
+                                      : vdf_flux(F, w0, w1, w2, m0, m1, m2);
                 const unsigned long long acc = __ballot(u01(ra.v[3]) * pm < f);
                 if (acc) {
                     const int win = __ffsll((long long)acc) - 1;
@@ -2479,9 +2484,14 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
 hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st) {
     // INJECT_PER_WAVE slots per wavefront, 4 wavefronts per block.
     const int64_t gen_blocks = std::min<int64_t>((L.cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE), 16384);
-    hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st, L.mesh, L.inj,
-                       L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx, L.ccs, L.kc,
-                       L.errs);
+    if (L.all_maxwellian)
+        hipLaunchKernelGGL(k_inject_gen<true>, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+                           L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
+                           L.ccs, L.kc, L.errs);
+    else
+        hipLaunchKernelGGL(k_inject_gen<false>, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+                           L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
+                           L.ccs, L.kc, L.errs);
     LAUNCH_CHECK();
     return hipSuccess;
 }

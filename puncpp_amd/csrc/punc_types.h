@@ -145,6 +145,22 @@ PUNC_HD double vdf_eval(const VdfParams& f, double w0, double w1, double w2) {
     }
 }
 
+// The flux pdf of a Maxwellian only: the same expressions, in the same
+// order, as vdf_flux with f.kind == VDF_MAXWELLIAN (bit-identical), without
+// the other distributions' code.
+PUNC_HD double vdf_flux_maxwellian(double factor, double vth2, const double vd[3], double w0, double w1, double w2,
+                                   double n0, double n1, double n2) {
+    double vn = 0.0;
+    vn += w0 * n0;
+    vn += w1 * n1;
+    vn += w2 * n2;
+    double v2 = 0.0;
+    v2 += (w0 - vd[0]) * (w0 - vd[0]);
+    v2 += (w1 - vd[1]) * (w1 - vd[1]);
+    v2 += (w2 - vd[2]) * (w2 - vd[2]);
+    return (vn > 0.0) * vn * (factor * exp(-0.5 * v2 / vth2));
+}
+
 // Flux pdf (v.n)^+ f(v) (distributions.h:93-102).
 PUNC_HD double vdf_flux(const VdfParams& f, double w0, double w1, double w2, double n0, double n1, double n2) {
     double vn = 0.0;
