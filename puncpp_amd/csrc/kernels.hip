@@ -1484,12 +1484,12 @@ constexpr int INJECT_PER_WAVE = 16;
 // other distributions' code (fewer registers, so more of its wavefronts run
 // beside the particle kernel); the same draws as the general kernel.
 template <bool MAXW>
-__global__ void __launch_bounds__(256)
-k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
-             const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
-             const int32_t* __restrict__ spair, uint64_t seed, double dt, int cell_bits,
-             int64_t cand_cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
-             int32_t* __restrict__ kc, unsigned long long* __restrict__ errs) {
+__device__ __forceinline__ void inject_gen_body(const DevMesh& M, const DevInject& I, const DevState* __restrict__ S,
+                                                const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
+                                                const int32_t* __restrict__ spair, uint64_t seed, double dt,
+                                                int cell_bits, int64_t cand_cap, double* __restrict__ cx,
+                                                int32_t* __restrict__ ccs, int32_t* __restrict__ kc,
+                                                unsigned long long* __restrict__ errs) {
     const uint64_t step = S->step;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -1601,6 +1601,25 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
         const unsigned long long km = __ballot(kept);
         if (lane == 0) kc[g0 / INJECT_PER_WAVE] = __popcll(km);
     }
+}
+
+// The general generator, and the Maxwellian-only one held to 128 VGPRs (4
+// wavefronts per SIMD, no spills; the general one would spill there).
+__global__ void __launch_bounds__(256)
+k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S, const SpInject* __restrict__ spi,
+             const int64_t* __restrict__ soff, const int32_t* __restrict__ spair, uint64_t seed, double dt,
+             int cell_bits, int64_t cand_cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
+             int32_t* __restrict__ kc, unsigned long long* __restrict__ errs) {
+    inject_gen_body<false>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_inject_gen_maxwellian(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
+                        const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
+                        const int32_t* __restrict__ spair, uint64_t seed, double dt, int cell_bits, int64_t cand_cap,
+                        double* __restrict__ cx, int32_t* __restrict__ ccs, int32_t* __restrict__ kc,
+                        unsigned long long* __restrict__ errs) {
+    inject_gen_body<true>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs);
 }
 
 // Append kept candidates at the end of the population, in candidate (slot)
@@ -2485,11 +2504,11 @@ hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st) {
     // INJECT_PER_WAVE slots per wavefront, 4 wavefronts per block.
     const int64_t gen_blocks = std::min<int64_t>((L.cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE), 16384);
     if (L.all_maxwellian)
-        hipLaunchKernelGGL(k_inject_gen<true>, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_inject_gen_maxwellian, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
                            L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
                            L.ccs, L.kc, L.errs);
     else
-        hipLaunchKernelGGL(k_inject_gen<false>, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
                            L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
                            L.ccs, L.kc, L.errs);
     LAUNCH_CHECK();
