@@ -69,8 +69,17 @@ struct U64x4 { uint64_t v[4]; };
 
 PUNC_HD void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    *lo = a * b;
-    *hi = __umul64hi(a, b);
+    // Four 32x32->64 products (v_mad_u64_u32, each adding a carry word that
+    // cannot overflow it); the low word comes from the same partial sums.
+    // a * b and __umul64hi(a, b) separately cost two more 32-bit multiplies,
+    // which dominate the injection sampler's Philox rounds.
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t t = (uint64_t)a0 * b0;
+    const uint64_t u = (uint64_t)a1 * b0 + (t >> 32);
+    const uint64_t v = (uint64_t)a0 * b1 + (uint32_t)u;
+    *hi = (uint64_t)a1 * b1 + (u >> 32) + (v >> 32);
+    *lo = (v << 32) | (uint32_t)t;
 #else
     unsigned __int128 p = (unsigned __int128)a * b;
     *lo = (uint64_t)p;
