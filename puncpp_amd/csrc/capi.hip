@@ -1054,6 +1054,12 @@ static int get_coef(punc_pop_s* P, int kind, double dt, const double* B, const S
     return 0;
 }
 
+// Removal's grid: its loops stride over the holes and the tail, a few
+// thousand to a few hundred thousand per step, so four blocks per compute
+// unit cover a step in one or two passes (the particle grid, up to 32768
+// blocks, mostly launched blocks that had nothing to do).
+static int compact_grid(const punc_pop_s* P) { return std::min(P->grid, 4 * P->num_cu); }
+
 static ParticleLaunch base_launch(punc_pop_s* P) {
     ParticleLaunch L;
     memset(&L, 0, sizeof(L));
@@ -1202,7 +1208,7 @@ extern "C" int punc_gpu_update(punc_pop_t* p, int64_t* absorbed) {
     L.coef = coef;
     L.locate = true;
     HIPCHK(timed_launch(p, L));
-    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_slot, p->grid, false, p->stream));
+    HIPCHK(launch_compact(p->d_state, p->d_holes, p->d_slot, compact_grid(p), false, p->stream));
     r = read_absorbed(p, absorbed);
     if (r) return r;
     return check_errors(p, "update");
@@ -2166,7 +2172,7 @@ static int step_enqueue(punc_pop_s* P, const StepArgs& a, const SpCoef* coef, bo
     // Three kernels close the step: removal, append of the injected
     // particles, and scale + bookkeeping (buffer swap, KE/PE sums, count,
     // step counter) - no single-block kernels in between.
-    HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_slot, P->grid, true, P->stream));
+    HIPCHK(launch_compact(P->d_state, P->d_holes, P->d_slot, compact_grid(P), true, P->stream));
     if (inj) {
         HIPCHK(hipStreamWaitEvent(P->stream, P->ev_join, 0));
         HIPCHK(launch_inject_append(IL, P->stream, false));

@@ -1134,37 +1134,50 @@ __device__ inline PopArrays step_arrays(const DevState* S) { return S->perm_pend
 // hold -1 between calls).  No block waits for another.  step: the arrays
 // after a fused step (before its swap); the count is updated afterwards
 // (k_compact_finish, or the step's k_scale_finish).
-__global__ void k_compact(DevState* __restrict__ S, const int32_t* __restrict__ holes,
-                          int32_t* __restrict__ slot, int step) {
+constexpr int COMPACT_BLOCK = 256;
+
+__global__ void __launch_bounds__(COMPACT_BLOCK)
+k_compact(DevState* __restrict__ S, const int32_t* __restrict__ holes, int32_t* __restrict__ slot, int step) {
+    __shared__ unsigned int wc[COMPACT_BLOCK / 64];
+    __shared__ unsigned long long bbase;
     const int64_t n = S->n, L = S->n_holes, nn = n - L;
     const GArrays A = gview(step ? step_arrays(S) : S->cur);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int lane = threadIdx.x & 63;
-    const int64_t w0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63);  // this wavefront's first index
-    // Tickets for the lanes that want one: one atomic per wavefront.  Called
-    // by all lanes of the wavefront (the loops below are wavefront-uniform).
+    const int64_t stride = (int64_t)gridDim.x * COMPACT_BLOCK;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // Tickets for the lanes that want one: one atomic per block (the
+    // counters are single addresses, so per-wavefront atomics from a few
+    // thousand wavefronts serialize).  Called by the whole block (the loops
+    // below are block-uniform).
     auto meet = [&](unsigned long long* ctr, bool want, int32_t idx, bool is_hole) {
         const unsigned long long m = __ballot(want);
-        if (m == 0ull) return;
-        const int leader = __ffsll((long long)m) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (!want) return;
-        const int64_t k = (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
-        const int32_t other = atomicExch(&slot[k], idx);
-        if (other < 0) return;  // first to arrive
-        const int64_t d = is_hole ? idx : other, s = is_hole ? other : idx;
-        copy_particle(A, d, A, s);
-        slot[k] = -1;
+        if (lane == 0) wc[w] = (unsigned int)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (int v = 0; v < COMPACT_BLOCK / 64; ++v) tot += wc[v];
+            bbase = tot ? atomicAdd(ctr, tot) : 0ull;
+        }
+        __syncthreads();
+        unsigned long long base = bbase;
+        for (int v = 0; v < w; ++v) base += wc[v];
+        if (want) {
+            const int64_t k = (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+            const int32_t other = atomicExch(&slot[k], idx);
+            if (other >= 0) {  // second to arrive: move the survivor into the hole
+                const int64_t d = is_hole ? idx : other, s = is_hole ? other : idx;
+                copy_particle(A, d, A, s);
+                slot[k] = -1;
+            }
+        }
+        __syncthreads();  // wc and bbase are reused by the next call
     };
-    for (int64_t b = w0; b < L; b += stride) {
-        const int64_t k = b + lane;
+    for (int64_t b = blockIdx.x * (int64_t)COMPACT_BLOCK; b < L; b += stride) {
+        const int64_t k = b + threadIdx.x;
         const int32_t h = k < L ? holes[k] : INT32_MAX;
         meet((unsigned long long*)&S->nH, h < nn, h, true);
     }
-    for (int64_t b = nn + w0; b < n; b += stride) {
-        const int64_t j = b + lane;
+    for (int64_t b = nn + blockIdx.x * (int64_t)COMPACT_BLOCK; b < n; b += stride) {
+        const int64_t j = b + threadIdx.x;
         meet((unsigned long long*)&S->nS, j < n && A.cs[j] >= 0, (int32_t)j, false);
     }
 }
@@ -1649,16 +1662,13 @@ k_inject_append(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restr
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // kept candidates in the slots before this block
     const int64_t g_end = (int64_t)blockIdx.x * (APPEND_BLOCK / INJECT_PER_WAVE);
-    int64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    // eight independent loads in flight per thread (integer sums: any order)
+    int64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t g = threadIdx.x;
-    for (; g + 3 * APPEND_BLOCK < g_end; g += 4 * APPEND_BLOCK) {
-        a0 += kc[g];
-        a1 += kc[g + APPEND_BLOCK];
-        a2 += kc[g + 2 * APPEND_BLOCK];
-        a3 += kc[g + 3 * APPEND_BLOCK];
-    }
-    for (; g < g_end; g += APPEND_BLOCK) a0 += kc[g];
-    int64_t part = (a0 + a1) + (a2 + a3);
+    for (; g + 7 * APPEND_BLOCK < g_end; g += 8 * APPEND_BLOCK)
+        for (int u = 0; u < 8; ++u) a[u] += kc[g + u * APPEND_BLOCK];
+    for (; g < g_end; g += APPEND_BLOCK) a[0] += kc[g];
+    int64_t part = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
     const int64_t k = (int64_t)blockIdx.x * APPEND_BLOCK + threadIdx.x;
     const int32_t cs = k < cand_cap ? ccs[k] : -1;
@@ -2411,7 +2421,7 @@ hipError_t launch_scale_rho(double* raw, const double* dv_inv, double* rho, int6
 
 hipError_t launch_compact(DevState* S, const int32_t* holes, int32_t* slot, int grid, bool after_step,
                           hipStream_t st) {
-    hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, st, S, holes, slot, after_step ? 1 : 0);
+    hipLaunchKernelGGL(k_compact, dim3(grid), dim3(COMPACT_BLOCK), 0, st, S, holes, slot, after_step ? 1 : 0);
     LAUNCH_CHECK();
     if (!after_step) {
         hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(64), 0, st, S);
