@@ -139,6 +139,16 @@ __global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ col, in
 
 // Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
 // dofs, phi_D on Dirichlet dofs; the initial iterate x0 = phi_D there.
+// phi_D: the exterior dofs take ext (0 without it), object j's dofs its
+// potential objpot[j - 1] (0 without them), the free dofs 0.
+__global__ void k_dirichlet_values(const int32_t* __restrict__ kind, const double* __restrict__ ext,
+                                   const double* __restrict__ objpot, double* __restrict__ phid, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = kind[i];
+        phid[i] = k == 0 ? (ext ? ext[i] : 0.0) : (k >= 1 && objpot ? objpot[k - 1] : 0.0);
+    }
+}
+
 __global__ void k_rhs_dirichlet(const double* __restrict__ b, const double* __restrict__ kphid,
                                 const double* __restrict__ phid, const uint8_t* __restrict__ isd,
                                 const double* __restrict__ prev, double* __restrict__ rhs, double* __restrict__ x,
@@ -468,6 +478,10 @@ struct FieldSolver {
     int grid = 1;  // one row / element per thread, grid-stride
     int w = 0;     // ELL width
     std::vector<int32_t> dir_kind;  // per dof: -1 free, 0 exterior, 1+j object j
+    int32_t* d_kind = nullptr;      // the same on the device
+    double* d_objpot = nullptr;     // object potentials of the current solve
+    double* h_objpot = nullptr;     // page-locked staging for them
+    CgState* h_state = nullptr;     // page-locked read-back of the CG state
     // device
     int32_t* col = nullptr;  // ELL columns (w * n)
     double *kfull = nullptr, *mass = nullptr, *grad[3] = {nullptr, nullptr, nullptr};
@@ -560,6 +574,10 @@ static void free_all(FieldSolver* F) {
     hipFree(F->b);
     hipFree(F->t);
     hipFree(F->phid);
+    hipFree(F->d_kind);
+    hipFree(F->d_objpot);
+    if (F->h_objpot) hipHostFree(F->h_objpot);
+    if (F->h_state) hipHostFree(F->h_state);
     hipFree(F->rhs);
     hipFree(F->x);
     hipFree(F->r);
@@ -842,6 +860,10 @@ FieldSolver* field_create(const HostMesh& m, const MeshTopo& t, double eps0, boo
     up(&F->dinv_k, dk);
     up(&F->dinv_m, dm);
     up(&F->isd, isd);
+    up(&F->d_kind, kind);
+    if (e == hipSuccess) e = dmalloc(&F->d_objpot, (size_t)std::max(n_objects, 1));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&F->h_objpot, (size_t)std::max(n_objects, 1) * 8);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&F->h_state, sizeof(CgState));
     double** vecs[] = {&F->b, &F->t, &F->phid, &F->phi};
     for (double** v : vecs)
         if (e == hipSuccess) e = dmalloc(v, (size_t)n);
@@ -1107,8 +1129,9 @@ static bool run_cg(FieldSolver* F, const CgOp& op, const double* dinv, const dou
     CgState h;
     for (;;) {
         FCHK(hipGraphLaunch(exec, F->st));
-        FCHK(hipMemcpyAsync(&h, F->S, sizeof(h), hipMemcpyDeviceToHost, F->st));
+        FCHK(hipMemcpyAsync(F->h_state, F->S, sizeof(CgState), hipMemcpyDeviceToHost, F->st));
         FCHK(hipStreamSynchronize(F->st));
+        h = *F->h_state;
         if (h.done || h.base >= F->maxit) break;
     }
     if (iterations) *iterations = (int)(h.done ? h.it : h.base);
@@ -1124,14 +1147,23 @@ bool field_poisson(FieldSolver* F, const double* d_rho, const double* ext_values
     FCHK(hipSetDevice(F->device));
     const int64_t n = F->n;
     const dim3 g(F->grid), blk(FB);
-    // phi_D on the Dirichlet dofs (exterior values, object potentials)
-    std::vector<double> pd((size_t)n, 0.0);
-    for (int64_t i = 0; i < n; ++i) {
-        const int32_t k = F->dir_kind[i];
-        if (k == 0) pd[i] = ext_values ? ext_values[i] : 0.0;
-        if (k >= 1) pd[i] = object_potentials ? object_potentials[k - 1] : 0.0;
+    // phi_D on the Dirichlet dofs (exterior values, object potentials), filled
+    // on the device: the caller's exterior values are staged in rhs (written
+    // only later, by k_rhs_dirichlet), the object potentials through
+    // page-locked memory (the stream is synchronized before this returns, so
+    // the staging buffer is free again for the next call)
+    const double* d_ext = nullptr;
+    if (ext_values) {
+        FCHK(hipMemcpyAsync(F->rhs, ext_values, n * 8, hipMemcpyHostToDevice, F->st));
+        d_ext = F->rhs;
     }
-    FCHK(hipMemcpyAsync(F->phid, pd.data(), n * 8, hipMemcpyHostToDevice, F->st));
+    const double* d_obj = nullptr;
+    if (object_potentials && F->nobj > 0) {
+        memcpy(F->h_objpot, object_potentials, (size_t)F->nobj * 8);
+        FCHK(hipMemcpyAsync(F->d_objpot, F->h_objpot, (size_t)F->nobj * 8, hipMemcpyHostToDevice, F->st));
+        d_obj = F->d_objpot;
+    }
+    hipLaunchKernelGGL(k_dirichlet_values, g, blk, 0, F->st, F->d_kind, d_ext, d_obj, F->phid, n);
     hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->mass, d_rho, F->b, n);     // b = M rho
     hipLaunchKernelGGL(k_spmv, g, blk, 0, F->st, F->col, F->w, F->kfull, F->phid, F->t, n);  // K phi_D
     hipLaunchKernelGGL(k_rhs_dirichlet, g, blk, 0, F->st, F->b, F->t, F->phid, F->isd,
