@@ -137,8 +137,6 @@ __global__ void __launch_bounds__(FB) k_spmv(const int32_t* __restrict__ col, in
         y[i] = row_dot(col, av, w, n, x, i);
 }
 
-// Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
-// dofs, phi_D on Dirichlet dofs; the initial iterate x0 = phi_D there.
 // phi_D: the exterior dofs take ext (0 without it), object j's dofs its
 // potential objpot[j - 1] (0 without them), the free dofs 0.
 __global__ void k_dirichlet_values(const int32_t* __restrict__ kind, const double* __restrict__ ext,
@@ -149,6 +147,8 @@ __global__ void k_dirichlet_values(const int32_t* __restrict__ kind, const doubl
     }
 }
 
+// Poisson right-hand side of the eliminated system: b_i - (K phi_D)_i on free
+// dofs, phi_D on Dirichlet dofs; the initial iterate x0 = phi_D there.
 __global__ void k_rhs_dirichlet(const double* __restrict__ b, const double* __restrict__ kphid,
                                 const double* __restrict__ phid, const uint8_t* __restrict__ isd,
                                 const double* __restrict__ prev, double* __restrict__ rhs, double* __restrict__ x,
