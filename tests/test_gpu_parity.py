@@ -297,6 +297,40 @@ def test_fused_step_table_overflow(gpu, mode):
         assert np.all(np.abs(rho_g - rho_o) <= 1e-11 * rho_scale(o, hm, dv))
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_mass_exit_removal(gpu, fused):
+    """Most of a 400k-particle population leaves through the exterior in one
+    step (thermal speed 30 per component, dt 0.2, box half-width 4): some
+    350k holes, more than one pass of the removal's grid-stride loops (four
+    blocks of 256 threads per compute unit), where holes and survivors meet
+    through per-block tickets.  Survivors and absorbed counts as the
+    oracle's, through update and through the fused step."""
+    _print_case("mass_exit", fused=fused)
+    hm = sphere_mesh()
+    o, mesh, pop = loaded_pair(hm, (200_000, 200_000), vth=(30.0, 30.0))
+    dt = 0.2
+    if fused:
+        E = np.zeros((hm.ndof, 3))
+        dv = P.element_volume(mesh)
+        N.check(N.lib().punc_gpu_set_field(pop._h, N.dptr(E)))
+        N.check(N.lib().punc_gpu_distribute(pop._h, N.dptr(dv), None))
+        B = np.array([0.0, 0.0, 1.0])
+        N.check(N.lib().punc_gpu_step(pop._h, 1, dt, N.dptr(B), dt, None, 0, 5, 0))
+        ab_g = np.zeros((2, 64), dtype=np.int64)
+        N.check(N.lib().punc_gpu_step_results(pop._h, None, N.i64ptr(ab_g), None))
+        o.accel(E, dt)
+    else:
+        P.move(pop, dt)
+        ab_g = pop.update([P.Object(2)], dt)
+    o.move(dt)
+    ab_o, _, lost = o.update()
+    assert lost == 0
+    np.testing.assert_array_equal(ab_g, ab_o)
+    left = int(ab_o.sum())
+    assert left > 300_000, f"only {left} particles left"
+    assert_same_sets(pop.get_particles(), o.get_particles(), rel=0.0, what="mass exit")
+
+
 def test_graph_replay_matches_eager(gpu):
     """A captured step replayed one step at a time (with host work and
     synchronization between replays) gives what the eager step gives."""
