@@ -782,13 +782,19 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             rb[2 * m] = t.x;
             rb[2 * m + 1] = t.y;
         }
-        for (int m = 0; m < 2; ++m) {
+        // The potential-energy variant loads no neighbour ids (no first hop
+        // below): its four more field values would otherwise take it past
+        // 128 VGPRs, three wavefronts per SIMD instead of four (measured:
+        // 2.20 -> 2.03 ms per step on 64M particles with the hop dropped).
+        for (int m = 0; m < (PE ? 1 : 2); ++m) {
             const v4i t = ((const GLOBAL v4i*)(rp + 6))[m];
             rd[4 * m] = t.x;
             rd[4 * m + 1] = t.y;
             rd[4 * m + 2] = t.z;
             rd[4 * m + 3] = t.w;
         }
+        if (PE)
+            for (int m = 4; m < 8; ++m) rd[m] = -1;
         for (int m = 0; m < 6; ++m) {
             const v2d t = ep[m];
             ef[2 * m] = t.x;
@@ -862,10 +868,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
         // does not fetch this cell's record again.  Undecided cases (a
         // coordinate within the margin before the first clear exit, a point
         // outside the margin's coordinate range, an exit through the
-        // boundary) start at c.
+        // boundary) start at c, and so does every walker of the
+        // potential-energy variant.
         int32_t wstart = c;
         bool hopped = false;
-        if (walker && in_range) {
+        if (!PE && walker && in_range) {
             const double Sl = (fabs(dl[0]) + fabs(dl[1])) + (fabs(dl[2]) + fabs(dl[3]));
             const double m = tol * (Sl > 1.0 ? Sl : 1.0);
             int f = -1;
