@@ -1355,14 +1355,15 @@ k_radix_count(const DevState* __restrict__ S, int cell_bits, const uint64_t* __r
 }
 
 // Stable scatter of each tile's (key, value) pairs to offsets[digit * ntiles +
-// tile] plus the rank among the tile's earlier keys of that digit.  kv ==
+// tile] plus the rank among the tile's earlier keys of that digit; offsets
+// holds the scan within blocks of 1024 entries, bsum the blocks' bases.  kv ==
 // nullptr: the first pass, keys from the particles' cells and values the
 // element indices.  kv_out == nullptr: the last pass, only the values are
 // written (to vals_out).
 __global__ void __launch_bounds__(256)
 k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint64_t* __restrict__ kv, int shift,
-                int64_t ntiles, int tsize, const int64_t* __restrict__ offsets, uint64_t* __restrict__ kv_out,
-                uint32_t* __restrict__ vals_out) {
+                int64_t ntiles, int tsize, const int64_t* __restrict__ offsets, const int64_t* __restrict__ bsum,
+                uint64_t* __restrict__ kv_out, uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t run[4][RADIX];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * 4 + w;
@@ -1370,7 +1371,10 @@ k_radix_scatter(const DevState* __restrict__ S, int cell_bits, const uint64_t* _
     const int64_t n = S->n;
     const GLOBAL int32_t* cs = kv ? nullptr : (const GLOBAL int32_t*)S->cur.cs;
     const int32_t cmask = (1 << cell_bits) - 1;
-    for (int d = lane; d < RADIX; d += 64) run[w][d] = (uint32_t)offsets[(int64_t)d * ntiles + tile];
+    for (int d = lane; d < RADIX; d += 64) {
+        const int64_t k = (int64_t)d * ntiles + tile;
+        run[w][d] = (uint32_t)(offsets[k] + bsum[k >> 10]);
+    }
     const int64_t t0 = tile * tsize;
     const int64_t t1 = t0 + tsize < n ? t0 + tsize : n;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -2453,8 +2457,10 @@ hipError_t launch_count_species(DevState* S, int cell_bits, int nsp, unsigned lo
     return hipSuccess;
 }
 
-hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
-                       hipStream_t st) {
+// The first two kernels of launch_scan: out[i] + bsum[i / 1024] is then the
+// exclusive scan (a consumer that adds the block sum itself saves the third).
+static hipError_t launch_scan_blocks(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum,
+                                     int64_t* total, hipStream_t st) {
     const int64_t nb = (len + 1023) / 1024;
     if (nb == 0) {
         return launch_fill_u64((unsigned long long*)total, 1, st);
@@ -2463,6 +2469,13 @@ hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bs
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(256), 0, st, bsum, nb, total);
     LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_scan(const int64_t* in, int64_t* out, int64_t len, int64_t* bsum, int64_t* total,
+                       hipStream_t st) {
+    hipError_t e = launch_scan_blocks(in, out, len, bsum, total, st);
+    if (e != hipSuccess || len <= 0) return e;
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, out, len, bsum);
     LAUNCH_CHECK();
     return hipSuccess;
@@ -2481,11 +2494,11 @@ hipError_t launch_sort(const SortLaunch& L, hipStream_t st) {
         hipLaunchKernelGGL(k_radix_count, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
                            tsize, L.counts);
         LAUNCH_CHECK();
-        hipError_t e = launch_scan(L.counts, L.offsets, ncount, L.bsum, L.total, st);
+        hipError_t e = launch_scan_blocks(L.counts, L.offsets, ncount, L.bsum, L.total, st);
         if (e != hipSuccess) return e;
         uint64_t* kout = ps + 1 < passes ? kvbuf[ps & 1] : nullptr;
         hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(256), 0, st, L.state, L.cell_bits, kin, shift, ntiles,
-                           tsize, L.offsets, kout, L.perm);
+                           tsize, L.offsets, L.bsum, kout, L.perm);
         LAUNCH_CHECK();
         kin = kout;
     }
