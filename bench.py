@@ -107,14 +107,21 @@ def run_gpu(cfg, steps, warmup, sort_every, seed=0, autotune=True, launch=None, 
     # Launch shape from the autotuner (runs on a snapshot; the state is kept).
     tune = None
     if launch is not None:
-        N.check(lib.punc_gpu_set_launch(pop._h, *launch))
+        N.check(lib.punc_gpu_set_launch(pop._h, *launch[:3]))
+        N.check(lib.punc_gpu_set_inject_in_step(pop._h, launch[3] if len(launch) > 3 else 0))
         tune = {"grid": launch[0], "block": launch[1], "deposit_mode": launch[2], "step_ms": None}
     elif autotune:
         g, b, md, ms = N.C.c_int32(), N.C.c_int32(), N.C.c_int32(), N.C.c_double()
         N.check(lib.punc_gpu_autotune(pop._h, 2, dt, N.dptr(B), dt, None, 1, seed, 3, N.C.byref(g),
                                       N.C.byref(b), N.C.byref(md), N.C.byref(ms)))
         tune = {"grid": g.value, "block": b.value, "deposit_mode": md.value, "step_ms": ms.value}
-        log(f"[bench] autotune: {tune}")
+    if tune is not None:
+        # where the Maxwellian injection generator runs: inside the step
+        # kernel's grid, or on a second stream beside it
+        ins = N.C.c_int32()
+        N.check(lib.punc_gpu_get_inject_in_step(pop._h, N.C.byref(ins)))
+        tune["inject_in_step"] = bool(ins.value)
+        log(f"[bench] launch: {tune}")
     # Step 0 is the half push (interaction.cpp:386,388); later steps replay a
     # captured graph of the full step; the cell re-sort runs between replays.
     N.check(lib.punc_gpu_step(pop._h, 2, 0.5 * dt, N.dptr(B), dt, None, 1, seed, 0))
@@ -547,6 +554,9 @@ def run_batched(n_inst=64, steps=20, warmup=3, base_particles=2_000_000, seed=1)
     N.check(lib.punc_gpu_autotune(bp.pop._h, 2, dt, N.dptr(B), dt, None, 1, 0, 3, N.C.byref(g), N.C.byref(b),
                                   N.C.byref(md), N.C.byref(ms)))
     tune = {"grid": g.value, "block": b.value, "deposit_mode": md.value, "step_ms": ms.value}
+    ins = N.C.c_int32()
+    N.check(lib.punc_gpu_get_inject_in_step(bp.pop._h, N.C.byref(ins)))
+    tune["inject_in_step"] = bool(ins.value)
     log(f"[bench] batched: populations loaded, batched autotune {tune} ({time.time() - t0:.1f}s)")
     ke = N.C.c_double()
     absorbed = np.zeros((2, 64), dtype=np.int64)
@@ -647,7 +657,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the diagnostics, flux Monte Carlo and field-solve timings (profiling runs)")
     ap.add_argument("--launch", default=None,
-                    help="GRID,BLOCK,MODE: fixed launch shape instead of the autotuner (profiling runs)")
+                    help="GRID,BLOCK,MODE[,INSTEP]: fixed launch shape instead of the autotuner (profiling "
+                         "runs); INSTEP 1: the injection generator runs inside the step kernel's grid")
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",

@@ -444,11 +444,23 @@ int punc_gpu_adjoint_backward_q(punc_adj_t* a, const double* g, const double* gq
  * 128 x CUs, 256, mode 3. */
 int punc_gpu_set_launch(punc_pop_t* p, int grid, int block, int deposit_mode);
 int punc_gpu_get_launch(punc_pop_t* p, int* grid, int* block, int* deposit_mode);
+/* Where a fused step with Maxwellian-only injection generates its injected
+ * particles: 1 - the generator's blocks run first in the step kernel's own
+ * grid; 0 (default) - a generator kernel on a second stream beside the step
+ * kernel, joined before the append.  The particles drawn are the same either
+ * way.  A change drops a captured step graph (its launch would then fail
+ * until the step is captured again).  No counterpart in the reference. */
+int punc_gpu_set_inject_in_step(punc_pop_t* p, int on);
+int punc_gpu_get_inject_in_step(punc_pop_t* p, int* on);
 /* Autotuner: times `reps` fused steps (arguments as punc_gpu_step) for every
  * launch shape (grid 4..128 x CUs, block 128/256, deposit mode 0..4) on a
  * snapshot of the population, restores the snapshot (particles, counts,
- * absorbed counters, rho, KE), and keeps the fastest shape.  A captured step
- * graph is dropped.  No counterpart in the reference. */
+ * absorbed counters, rho, KE), and keeps the fastest shape.  With injection
+ * from Maxwellians only, the fastest shape is also timed with the injection
+ * generator's blocks inside the step kernel's grid instead of in a kernel on
+ * a second stream, and the faster of the two is kept (it pays on small
+ * populations, where the join of the streams is a visible part of a step).
+ * A captured step graph is dropped.  No counterpart in the reference. */
 int punc_gpu_autotune(punc_pop_t* p, int pusher, double push_dt, const double B[3], double dt,
                       const double* Ld, int inject, uint64_t seed, int reps, int* best_grid,
                       int* best_block, int* best_mode, double* best_ms);

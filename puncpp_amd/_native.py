@@ -112,6 +112,8 @@ _SIGS = {
     "punc_gpu_step_potential_energy": (C.c_int, [vp, d_p]),
     "punc_gpu_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "punc_gpu_get_launch": (C.c_int, [vp, i32_p, i32_p, i32_p]),
+    "punc_gpu_set_inject_in_step": (C.c_int, [vp, C.c_int]),
+    "punc_gpu_get_inject_in_step": (C.c_int, [vp, i32_p]),
     "punc_gpu_autotune": (C.c_int, [vp, C.c_int, C.c_double, d_p, C.c_double, d_p, C.c_int, C.c_uint64,
                                     C.c_int, i32_p, i32_p, i32_p, d_p]),
     "punc_gpu_timing": (C.c_int, [vp, C.c_int]),

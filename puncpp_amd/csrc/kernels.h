@@ -6,6 +6,8 @@
 
 namespace punc_gpu {
 
+struct InjectLaunch;
+
 struct ParticleLaunch {
     DevMesh mesh;
     DevState* state;
@@ -28,6 +30,9 @@ struct ParticleLaunch {
     double* pe_part;  // fused step: per-block potential energy (nullptr: not computed)
     int grid;
     int block;
+    // Fused step: the Maxwellian injection generator's blocks run first in
+    // the same grid (nullptr: no generation in the step kernel).
+    const InjectLaunch* gen;
 };
 
 struct InjectLaunch {

@@ -101,18 +101,22 @@ __device__ inline void copy_particle(const GArrays& D, int64_t d, const GArrays&
 //   among them.  The fused step uses it except when it reads a fresh cell
 //   order (measured: 2-3% faster per step on an ordered population, 9% slower
 //   on the step that gathers through a new permutation).
-__device__ inline int64_t logical_block(bool xcd) {
-    const int64_t G = gridDim.x, b = blockIdx.x;
+__device__ inline int64_t logical_block(bool xcd, int64_t b, int64_t G) {
     if (xcd && G % 8 == 0) return (b % 8) * (G / 8) + (G / 8 - 1 - b / 8);
     return G - 1 - b;
 }
 
-__device__ inline void chunk_range(int64_t n, int64_t& s, int64_t& e, bool xcd = true) {
-    const int64_t G = gridDim.x;
+// Block b of G particle blocks (the fused step's grid also holds the
+// injection generator's blocks).
+__device__ inline void chunk_range(int64_t n, int64_t& s, int64_t& e, bool xcd, int64_t b, int64_t G) {
     int64_t per = (n + G - 1) / G;
     per = (per + blockDim.x - 1) / blockDim.x * blockDim.x;
-    s = logical_block(xcd) * per;
+    s = logical_block(xcd, b, G) * per;
     e = s + per < n ? s + per : n;
+}
+
+__device__ inline void chunk_range(int64_t n, int64_t& s, int64_t& e, bool xcd = true) {
+    chunk_range(n, s, e, xcd, blockIdx.x, gridDim.x);
 }
 
 // Barycentric coordinates (reference population.h:885-893).
@@ -703,20 +707,55 @@ __device__ inline SpCoef* step_coef_lds() {
     return (SpCoef*)(dyn_smem + (DMODE >= 2 ? LDS_TABLE_BYTES / 8 : 0));
 }
 
+// The injection generator (defined with the injection kernels below).
+template <bool MAXW>
+__device__ __forceinline__ void inject_gen_body(const DevMesh& M, const DevInject& I, const DevState* __restrict__ S,
+                                                const SpInject* __restrict__ spi, const int64_t* __restrict__ soff,
+                                                const int32_t* __restrict__ spair, uint64_t seed, double dt,
+                                                int cell_bits, int64_t cand_cap, double* __restrict__ cx,
+                                                int32_t* __restrict__ ccs, int32_t* __restrict__ kc,
+                                                unsigned long long* __restrict__ errs, int64_t blk, int64_t nblk);
+
+// The Maxwellian generator's arguments when its blocks share the fused
+// step's grid: blocks [0, blocks) generate, the others are the step's
+// particle blocks in order.  One kernel instead of a kernel on a second
+// stream spares the step the join of the two streams (about 6 us per step,
+// which pays on small populations).
+struct GenArgs {
+    DevInject I;
+    const SpInject* spi;
+    const int64_t* soff;
+    const int32_t* spair;
+    uint64_t seed;
+    double dt;
+    int64_t cand_cap;
+    double* cx;
+    int32_t* ccs;
+    int32_t* kc;
+    unsigned long long* errs;
+    int blocks;
+};
+
 // PE: also the particle potential energy at the step's starting positions
 // (particle_potential_energy_cg1, diagnostics.h:379-412), phi read from
 // slots 12..15 of the cell's field row: per-block sums into pe_part.
 template <int PUSH, int MOVE, int DMODE, bool PE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict__ gcoef, const int nsp,
             const double* __restrict__ cellE, const double dt, const double Lx, const double Ly,
             const double Lz, const int cell_bits, double* __restrict__ rho_raw,
             double* __restrict__ ke_part, int32_t* __restrict__ holes, unsigned long long* __restrict__ absorbed,
-            WalkRec* __restrict__ wrec, double* __restrict__ pe_part, const int wl_cap) {
+            WalkRec* __restrict__ wrec, double* __restrict__ pe_part, const int wl_cap, const GenArgs G) {
     typedef double v2d __attribute__((ext_vector_type(2)));
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     __shared__ double sh[8];
     __shared__ int32_t wcnt;
+    if ((int64_t)blockIdx.x < G.blocks) {  // block-uniform
+        inject_gen_body<true>(M, G.I, S, G.spi, G.soff, G.spair, G.seed, G.dt, cell_bits, G.cand_cap, G.cx, G.ccs,
+                              G.kc, G.errs, blockIdx.x, G.blocks);
+        return;
+    }
+    const int64_t blk = blockIdx.x - G.blocks, nblk = gridDim.x - G.blocks;
     // Species coefficients in LDS: read per lane (species differ within a
     // wavefront) without adding to the vector-memory queue that the
     // prefetched particle loads sit in.
@@ -735,11 +774,11 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
     // That step takes the XCD-range chunk order, every other one the plain.
     const bool permuted = S->perm_pending != 0;
     int64_t s, e;
-    chunk_range(n, s, e, permuted);
+    chunk_range(n, s, e, permuted, blk, nblk);
     if (s >= e) {  // block-uniform: nothing in this block's chunk
         if (threadIdx.x == 0) {
-            ke_part[blockIdx.x] = 0.0;
-            if (PE) pe_part[blockIdx.x] = 0.0;
+            ke_part[blk] = 0.0;
+            if (PE) pe_part[blk] = 0.0;
         }
         return;
     }
@@ -921,10 +960,10 @@ k_step_main(const DevMesh M, DevState* __restrict__ S, const SpCoef* __restrict_
             lds_flush(lds_table(), rho_raw);
     }
     const double t = block_sum(ke, sh);  // synchronizes: wcnt and the segment are complete
-    if (threadIdx.x == 0) ke_part[blockIdx.x] = t;
+    if (threadIdx.x == 0) ke_part[blk] = t;
     if (PE) {
         const double tp = block_sum(pe, sh);
-        if (threadIdx.x == 0) pe_part[blockIdx.x] = tp;
+        if (threadIdx.x == 0) pe_part[blk] = tp;
     }
     // Phase 2: the walks of this block's list.  A.cs is rewritten only when
     // the cell changes.
@@ -1521,11 +1560,11 @@ __device__ __forceinline__ void inject_gen_body(const DevMesh& M, const DevInjec
                                                 const int32_t* __restrict__ spair, uint64_t seed, double dt,
                                                 int cell_bits, int64_t cand_cap, double* __restrict__ cx,
                                                 int32_t* __restrict__ ccs, int32_t* __restrict__ kc,
-                                                unsigned long long* __restrict__ errs) {
+                                                unsigned long long* __restrict__ errs, int64_t blk, int64_t nblk) {
     const uint64_t step = S->step;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = (blk * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = (nblk * (int64_t)blockDim.x) >> 6;
     for (int64_t g0 = wave * INJECT_PER_WAVE; g0 < cand_cap; g0 += nwaves * INJECT_PER_WAVE) {
         const int64_t k = g0 + lane;
         const bool owner = lane < INJECT_PER_WAVE;
@@ -1642,7 +1681,8 @@ k_inject_gen(const DevMesh M, const DevInject I, const DevState* __restrict__ S,
              const int64_t* __restrict__ soff, const int32_t* __restrict__ spair, uint64_t seed, double dt,
              int cell_bits, int64_t cand_cap, double* __restrict__ cx, int32_t* __restrict__ ccs,
              int32_t* __restrict__ kc, unsigned long long* __restrict__ errs) {
-    inject_gen_body<false>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs);
+    inject_gen_body<false>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs, blockIdx.x,
+                           gridDim.x);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
@@ -1651,7 +1691,8 @@ k_inject_gen_maxwellian(const DevMesh M, const DevInject I, const DevState* __re
                         const int32_t* __restrict__ spair, uint64_t seed, double dt, int cell_bits, int64_t cand_cap,
                         double* __restrict__ cx, int32_t* __restrict__ ccs, int32_t* __restrict__ kc,
                         unsigned long long* __restrict__ errs) {
-    inject_gen_body<true>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs);
+    inject_gen_body<true>(M, I, S, spi, soff, spair, seed, dt, cell_bits, cand_cap, cx, ccs, kc, errs, blockIdx.x,
+                          gridDim.x);
 }
 
 // Append kept candidates at the end of the population, in candidate (slot)
@@ -2304,6 +2345,13 @@ static hipError_t launch_pm(const ParticleLaunch& L, hipStream_t st) {
 
 static int fill_grid(int64_t n);
 
+// Blocks of the injection generator: INJECT_PER_WAVE slots per wavefront, 4
+// wavefronts per block of 256 (a grid-stride loop covers any shortfall).
+static int inject_gen_blocks(int64_t cand_cap) {
+    const int64_t b = (cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE);
+    return (int)std::max<int64_t>(std::min<int64_t>(b, 16384), 1);
+}
+
 template <int PUSH, int MOVE, int DMODE>
 static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
     if (L.nsp < 1 || L.nsp > MAX_SPECIES) return hipErrorInvalidValue;
@@ -2314,14 +2362,33 @@ static hipError_t launch_step_t(const ParticleLaunch& L, hipStream_t st) {
     const size_t budget = 40 * 1024 - 1024;  // less the static LDS and allocation rounding
     const int wl_cap = shmem < budget ? (int)std::min<size_t>(STEP_WALK_LDS, (budget - shmem) / sizeof(WalkRec)) : 0;
     shmem += (size_t)wl_cap * sizeof(WalkRec);
+    GenArgs G{};  // no generator blocks
+    int grid = L.grid;
+    if (L.gen) {
+        const InjectLaunch& J = *L.gen;
+        if (!J.all_maxwellian) return hipErrorInvalidValue;
+        G.I = J.inj;
+        G.spi = J.spi;
+        G.soff = J.soff;
+        G.spair = J.spair;
+        G.seed = J.seed;
+        G.dt = J.dt;
+        G.cand_cap = J.cand_cap;
+        G.cx = J.cx;
+        G.ccs = J.ccs;
+        G.kc = J.kc;
+        G.errs = J.errs;
+        G.blocks = inject_gen_blocks(J.cand_cap);
+        grid += G.blocks;
+    }
     if (L.pe_part)
-        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, true>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
+        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, true>), dim3(grid), dim3(L.block), shmem, st, L.mesh,
                            L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
-                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, L.pe_part, wl_cap);
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, L.pe_part, wl_cap, G);
     else
-        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, false>), dim3(L.grid), dim3(L.block), shmem, st, L.mesh,
+        hipLaunchKernelGGL((k_step_main<PUSH, MOVE, DMODE, false>), dim3(grid), dim3(L.block), shmem, st, L.mesh,
                            L.state, L.coef, L.nsp, L.cellE, L.dt, L.Ld[0], L.Ld[1], L.Ld[2], L.cell_bits, L.rho_raw,
-                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, nullptr, wl_cap);
+                           L.ke_part, L.holes, L.absorbed, (WalkRec*)L.walk, nullptr, wl_cap, G);
     LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -2539,14 +2606,13 @@ hipError_t launch_load(const DevMesh& M, DevState* S, int64_t base, int64_t N, i
 }
 
 hipError_t launch_inject_gen(const InjectLaunch& L, hipStream_t st) {
-    // INJECT_PER_WAVE slots per wavefront, 4 wavefronts per block.
-    const int64_t gen_blocks = std::min<int64_t>((L.cand_cap + 4 * INJECT_PER_WAVE - 1) / (4 * INJECT_PER_WAVE), 16384);
+    const int gen_blocks = inject_gen_blocks(L.cand_cap);
     if (L.all_maxwellian)
-        hipLaunchKernelGGL(k_inject_gen_maxwellian, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_inject_gen_maxwellian, dim3((unsigned)gen_blocks), dim3(256), 0, st,
                            L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
                            L.ccs, L.kc, L.errs);
     else
-        hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)std::max<int64_t>(gen_blocks, 1)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_inject_gen, dim3((unsigned)gen_blocks), dim3(256), 0, st,
                            L.mesh, L.inj, L.state, L.spi, L.soff, L.spair, L.seed, L.dt, L.cell_bits, L.cand_cap, L.cx,
                            L.ccs, L.kc, L.errs);
     LAUNCH_CHECK();

@@ -426,14 +426,17 @@ def test_fused_step_after_sort(gpu, mode, graph):
     assert_same_sets(got, o.get_particles(), rel=0.0, what="sorted read")
 
 
+@pytest.mark.parametrize("in_step", [False, True])
 @pytest.mark.parametrize("graph", [False, True])
-def test_steps_with_injection_and_resort_match_oracle(gpu, graph):
+def test_steps_with_injection_and_resort_match_oracle(gpu, graph, in_step):
     """Eight fused steps back to back with no download in between (eager, or
     replays of the captured step), each removing the particles that leave and
     appending the injected ones, with a re-sort after the fifth that the next
     step reads through.  Per step absorbed and injected counts exact and the
-    count balanced; at the end the particle set and rho equal the oracle's."""
-    _print_case("steps_inject_resort", graph=graph)
+    count balanced; at the end the particle set and rho equal the oracle's.
+    in_step: the injection generator runs inside the step kernel's grid
+    instead of on a second stream."""
+    _print_case("steps_inject_resort", graph=graph, in_step=in_step)
     hm = sphere_mesh(n=6, L=3.0, r=1.0)
     o, mesh, pop = loaded_pair(hm, (4000, 4000))
     E = synthetic_field(hm, scale=0.2)
@@ -448,6 +451,10 @@ def test_steps_with_injection_and_resort_match_oracle(gpu, graph):
     for s in range(2):
         o.set_injector(s, vth[s], vd, species[s].n, 5.0)
     L = N
+    L.check(L.lib().punc_gpu_set_inject_in_step(pop._h, int(in_step)))
+    got_in_step = L.C.c_int32()
+    L.check(L.lib().punc_gpu_get_inject_in_step(pop._h, L.C.byref(got_in_step)))
+    assert got_in_step.value == int(in_step)
     L.check(L.lib().punc_gpu_set_field(pop._h, L.dptr(E)))
     L.check(L.lib().punc_gpu_distribute(pop._h, L.dptr(dv), None))
     dt = 0.05
