@@ -53,7 +53,7 @@ def measured_traffic(cfg, tune):
         return None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
-    shape = f"{tune['grid']},{tune['block']},{tune['deposit_mode']}"
+    shape = f"{tune['grid']},{tune['block']},{tune['deposit_mode']}" + (",1" if tune.get("inject_in_step") else "")
     rec = t.get("runs", {}).get(f"{cfg}:{shape}")
     return rec["traffic_bytes"] if rec else None
 
